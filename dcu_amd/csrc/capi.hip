@@ -1,0 +1,794 @@
+// C ABI (spai_mi355x.h) and the reference-shaped C++ entry points
+// (spai_multi_dcu_api.h) on top of the SPAI builder and the BiCGSTAB solver.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "device_common.hpp"
+#include "krylov.hpp"
+#include "problems.hpp"
+#include "spai_build.hpp"
+#include "spai_mi355x.h"
+#include "spai_multi_dcu_api.h"
+
+using namespace spai;
+
+namespace {
+thread_local std::string g_last_error;
+
+int fail(int status, const std::string &msg) {
+    g_last_error = msg;
+    return status;
+}
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        f();
+        return SPAI_OK;
+    } catch (const Error &e) {
+        return fail(e.status, e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(SPAI_ERR_NOMEM, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(SPAI_ERR_INVALID, e.what());
+    }
+}
+
+__global__ void k_fill_f64(double *p, double v, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_f32(float *p, float v, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void fill(double *p, long long n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, 0.0, n);
+    SPAI_CHECK_LAUNCH();
+}
+void fill(float *p, long long n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, 0.0f, n);
+    SPAI_CHECK_LAUNCH();
+}
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    hipStream_t st;
+    explicit Timer(hipStream_t s) : st(s) {
+        SPAI_HIP(hipEventCreate(&a));
+        SPAI_HIP(hipEventCreate(&b));
+        SPAI_HIP(hipEventRecord(a, st));
+    }
+    float stop() {
+        SPAI_HIP(hipEventRecord(b, st));
+        SPAI_HIP(hipEventSynchronize(b));
+        float ms = 0.0f;
+        SPAI_HIP(hipEventElapsedTime(&ms, a, b));
+        return ms;
+    }
+    ~Timer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+// Host-side structural checks of a CSC matrix before it is uploaded.
+void validate_host_csc(const CSC_Matrix *A) {
+    require(A != nullptr, "matrix is NULL");
+    int n = A->nCol;
+    require(n > 0, "matrix has no columns");
+    require(A->nRow == n, "matrix is not square (nRow != nCol)");
+    require(A->mPtr != nullptr, "mPtr is NULL");
+    require(A->mPtr[0] == 0, "mPtr[0] must be 0");
+    for (int c = 0; c < n; ++c) require(A->mPtr[c + 1] >= A->mPtr[c], "mPtr is not non-decreasing");
+    int nnz = A->mPtr[n];
+    require(A->nonzeroes == nnz, "nonzeroes does not match mPtr[nCol]");
+    require(nnz == 0 || (A->mIndex && A->mData), "mIndex/mData are NULL");
+    for (int p = 0; p < nnz; ++p) require(A->mIndex[p] >= 0 && A->mIndex[p] < n, "row index out of range");
+}
+}  // namespace
+
+struct spai_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cus = 0;
+    std::string name;
+    SpaiBuilder builder;
+    BiCGStab solver;
+    BiCGStab adjoint_solver;
+};
+
+struct spai_system {
+    spai_context *ctx = nullptr;
+    int n = 0, nnz = 0, nsys = 1;
+    std::vector<int> h_row_off;
+    std::vector<long long> h_nnz_off;
+    DeviceArray<int> colptr, rowidx, rowptr, colidx, perm, n1;
+    DeviceArray<double> cscval, csrval;
+    DeviceArray<double> m64_csc, m64_csr;
+    DeviceArray<float> m32_csc, m32_csr;
+    int m_precision = -1;
+    TileMap tiles, tiles_t;
+    bool tiles_t_ready = false;
+    DeviceArray<double> stage_b, stage_x, stage_g, stage_dA, stage_db;
+    float setup_ms = 0.0f;
+    std::vector<SysState> last_states;
+};
+
+struct spai_problem {
+    Problem p;
+};
+
+// ----------------------------------------------------------- helpers ----
+namespace {
+void finish_setup(spai_system *S, hipStream_t st) {
+    S->rowptr.resize((size_t)S->n + 1);
+    S->colidx.resize((size_t)std::max(S->nnz, 1));
+    S->csrval.resize((size_t)std::max(S->nnz, 1));
+    S->perm.resize((size_t)std::max(S->nnz, 1));
+    csc_to_csr_device(S->n, S->nnz, S->colptr.get(), S->rowidx.get(), S->cscval.get(), S->rowptr.get(),
+                      S->colidx.get(), S->csrval.get(), S->perm.get(), st);
+    build_tiles(S->tiles, S->rowptr.get(), S->h_row_off, S->h_nnz_off, st);
+}
+
+spai_system *create_from_host(spai_context *ctx, const std::vector<const CSC_Matrix *> &parts) {
+    auto S = std::make_unique<spai_system>();
+    S->ctx = ctx;
+    S->nsys = (int)parts.size();
+    S->h_row_off.assign((size_t)S->nsys + 1, 0);
+    S->h_nnz_off.assign((size_t)S->nsys + 1, 0);
+    for (int s = 0; s < S->nsys; ++s) {
+        validate_host_csc(parts[(size_t)s]);
+        S->h_row_off[(size_t)s + 1] = S->h_row_off[(size_t)s] + parts[(size_t)s]->nCol;
+        S->h_nnz_off[(size_t)s + 1] = S->h_nnz_off[(size_t)s] + parts[(size_t)s]->nonzeroes;
+    }
+    require(S->h_nnz_off.back() < (1LL << 31), "batch has more than 2^31-1 nonzeros");
+    S->n = S->h_row_off.back();
+    S->nnz = (int)S->h_nnz_off.back();
+    hipStream_t st = ctx->stream;
+    SPAI_HIP(hipSetDevice(ctx->device));
+    // block-diagonal concatenation with global indices
+    std::vector<int> cp((size_t)S->n + 1), ri((size_t)std::max(S->nnz, 1));
+    std::vector<double> va((size_t)std::max(S->nnz, 1));
+    for (int s = 0; s < S->nsys; ++s) {
+        const CSC_Matrix *A = parts[(size_t)s];
+        int r0 = S->h_row_off[(size_t)s];
+        long long z0 = S->h_nnz_off[(size_t)s];
+        for (int c = 0; c < A->nCol; ++c) cp[(size_t)r0 + c] = (int)(z0 + A->mPtr[c]);
+        for (int p = 0; p < A->nonzeroes; ++p) {
+            ri[(size_t)(z0 + p)] = A->mIndex[p] + r0;
+            va[(size_t)(z0 + p)] = A->mData[p];
+        }
+    }
+    cp[(size_t)S->n] = S->nnz;
+    Timer tm(st);
+    S->colptr.resize((size_t)S->n + 1);
+    S->rowidx.resize((size_t)std::max(S->nnz, 1));
+    S->cscval.resize((size_t)std::max(S->nnz, 1));
+    SPAI_HIP(hipMemcpyAsync(S->colptr.get(), cp.data(), sizeof(int) * cp.size(), hipMemcpyHostToDevice, st));
+    SPAI_HIP(hipMemcpyAsync(S->rowidx.get(), ri.data(), sizeof(int) * (size_t)S->nnz, hipMemcpyHostToDevice, st));
+    SPAI_HIP(hipMemcpyAsync(S->cscval.get(), va.data(), sizeof(double) * (size_t)S->nnz, hipMemcpyHostToDevice, st));
+    finish_setup(S.get(), st);
+    S->setup_ms = tm.stop();
+    return S.release();
+}
+
+void ensure_transposed_tiles(spai_system *S) {
+    if (S->tiles_t_ready) return;
+    build_tiles(S->tiles_t, S->colptr.get(), S->h_row_off, S->h_nnz_off, S->ctx->stream);
+    S->tiles_t_ready = true;
+}
+
+SolveOptions to_options(const spai_solve_options *o) {
+    SolveOptions s;
+    if (o) {
+        s.rel_tol = o->rel_tol;
+        s.abs_tol = o->abs_tol;
+        s.max_iter = o->max_iter;
+        s.use_graph = o->use_graph;
+        s.graph_iters = o->graph_iters;
+        s.check_every = o->check_every;
+    }
+    return s;
+}
+
+void fill_report(spai_solve_report *r, const SolveResult &res) {
+    if (!r) return;
+    r->ms = res.ms;
+    r->iterations_max = res.iterations_max;
+    r->iterations_sum = res.iterations_sum;
+    r->host_checks = res.host_checks;
+    r->all_converged = res.all_converged;
+}
+
+OperatorView m_view(spai_system *S, bool transposed) {
+    OperatorView M;
+    M.ptr = transposed ? S->colptr.get() : S->rowptr.get();
+    M.idx = transposed ? S->rowidx.get() : S->colidx.get();
+    if (S->m_precision == SPAI_PRECISION_MIXED) {
+        M.val = transposed ? (const void *)S->m32_csc.get() : (const void *)S->m32_csr.get();
+        M.fp32 = 1;
+    } else {
+        M.val = transposed ? (const void *)S->m64_csc.get() : (const void *)S->m64_csr.get();
+        M.fp32 = 0;
+    }
+    return M;
+}
+}  // namespace
+
+// ============================================================ C ABI ====
+extern "C" {
+
+const char *spai_status_string(int status) {
+    switch (status) {
+    case SPAI_OK: return "ok";
+    case SPAI_ERR_INVALID: return "invalid argument";
+    case SPAI_ERR_HIP: return "HIP runtime error";
+    case SPAI_ERR_NOMEM: return "out of memory";
+    case SPAI_ERR_UNSUPPORTED: return "unsupported input";
+    case SPAI_ERR_BREAKDOWN: return "BiCGSTAB breakdown";
+    case SPAI_ERR_NOT_BUILT: return "preconditioner not built";
+    default: return "unknown status";
+    }
+}
+
+const char *spai_last_error(void) { return g_last_error.c_str(); }
+
+const char *spai_version(void) { return "spai_mi355x 0.1 (gfx950)"; }
+
+int spai_context_create(int device, spai_context **out) {
+    if (!out) return fail(SPAI_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    return guarded([&] {
+        int count = 0;
+        SPAI_HIP(hipGetDeviceCount(&count));
+        require(device >= 0 && device < count, "no HIP device " + std::to_string(device));
+        auto c = std::make_unique<spai_context>();
+        c->device = device;
+        SPAI_HIP(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        SPAI_HIP(hipGetDeviceProperties(&prop, device));
+        c->name = prop.name;
+        c->cus = prop.multiProcessorCount;
+        SPAI_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        *out = c.release();
+    });
+}
+
+int spai_context_destroy(spai_context *ctx) {
+    if (!ctx) return SPAI_OK;
+    return guarded([&] {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        hipStream_t st = ctx->stream;
+        delete ctx;
+        (void)hipStreamDestroy(st);
+    });
+}
+
+int spai_context_device_info(spai_context *ctx, char *name, int name_len, int *cus) {
+    if (!ctx) return fail(SPAI_ERR_INVALID, "ctx is NULL");
+    if (name && name_len > 0) {
+        std::strncpy(name, ctx->name.c_str(), (size_t)name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (cus) *cus = ctx->cus;
+    return SPAI_OK;
+}
+
+int spai_system_create(spai_context *ctx, const CSC_Matrix *host_A, spai_system **out) {
+    if (!ctx || !out) return fail(SPAI_ERR_INVALID, "ctx/out is NULL");
+    *out = nullptr;
+    return guarded([&] { *out = create_from_host(ctx, {host_A}); });
+}
+
+int spai_system_create_batched(spai_context *ctx, int nsys, const CSC_Matrix *host_systems, spai_system **out) {
+    if (!ctx || !out || nsys < 1 || !host_systems) return fail(SPAI_ERR_INVALID, "bad batched arguments");
+    *out = nullptr;
+    return guarded([&] {
+        std::vector<const CSC_Matrix *> parts;
+        for (int s = 0; s < nsys; ++s) parts.push_back(host_systems + s);
+        *out = create_from_host(ctx, parts);
+    });
+}
+
+int spai_system_create_device(spai_context *ctx, const CSC_Matrix *dev_A, spai_system **out) {
+    if (!ctx || !out || !dev_A) return fail(SPAI_ERR_INVALID, "ctx/out/A is NULL");
+    *out = nullptr;
+    return guarded([&] {
+        require(dev_A->nCol > 0 && dev_A->nRow == dev_A->nCol, "device matrix must be square and non-empty");
+        require(dev_A->mPtr && dev_A->mIndex && dev_A->mData, "device matrix pointers are NULL");
+        SPAI_HIP(hipSetDevice(ctx->device));
+        auto S = std::make_unique<spai_system>();
+        S->ctx = ctx;
+        S->n = dev_A->nCol;
+        hipStream_t st = ctx->stream;
+        int nnz = 0;
+        SPAI_HIP(hipMemcpyAsync(&nnz, dev_A->mPtr + S->n, sizeof(int), hipMemcpyDeviceToHost, st));
+        SPAI_HIP(hipStreamSynchronize(st));
+        require(nnz >= 0 && nnz == dev_A->nonzeroes, "device mPtr[n] does not match nonzeroes");
+        S->nnz = nnz;
+        S->h_row_off = {0, S->n};
+        S->h_nnz_off = {0, (long long)nnz};
+        Timer tm(st);
+        S->colptr.resize((size_t)S->n + 1);
+        S->rowidx.resize((size_t)std::max(nnz, 1));
+        S->cscval.resize((size_t)std::max(nnz, 1));
+        SPAI_HIP(hipMemcpyAsync(S->colptr.get(), dev_A->mPtr, sizeof(int) * ((size_t)S->n + 1),
+                                hipMemcpyDeviceToDevice, st));
+        SPAI_HIP(hipMemcpyAsync(S->rowidx.get(), dev_A->mIndex, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToDevice,
+                                st));
+        SPAI_HIP(hipMemcpyAsync(S->cscval.get(), dev_A->mData, sizeof(double) * (size_t)nnz,
+                                hipMemcpyDeviceToDevice, st));
+        finish_setup(S.get(), st);
+        S->setup_ms = tm.stop();
+        *out = S.release();
+    });
+}
+
+int spai_system_destroy(spai_system *sys) {
+    if (!sys) return SPAI_OK;
+    return guarded([&] {
+        (void)hipSetDevice(sys->ctx->device);
+        (void)hipStreamSynchronize(sys->ctx->stream);
+        delete sys;
+    });
+}
+
+int spai_system_get_info(const spai_system *sys, spai_system_info *info) {
+    if (!sys || !info) return fail(SPAI_ERR_INVALID, "sys/info is NULL");
+    info->n = sys->n;
+    info->nnz = sys->nnz;
+    info->nsys = sys->nsys;
+    info->m_precision = sys->m_precision;
+    info->setup_ms = sys->setup_ms;
+    return SPAI_OK;
+}
+
+int spai_system_row_offsets(const spai_system *sys, int *row_offsets) {
+    if (!sys || !row_offsets) return fail(SPAI_ERR_INVALID, "sys/row_offsets is NULL");
+    std::memcpy(row_offsets, sys->h_row_off.data(), sizeof(int) * sys->h_row_off.size());
+    return SPAI_OK;
+}
+
+int spai_system_set_values(spai_system *sys, const double *values, int values_on_device) {
+    if (!sys || !values) return fail(SPAI_ERR_INVALID, "sys/values is NULL");
+    return guarded([&] {
+        hipStream_t st = sys->ctx->stream;
+        SPAI_HIP(hipSetDevice(sys->ctx->device));
+        SPAI_HIP(hipMemcpyAsync(sys->cscval.get(), values, sizeof(double) * (size_t)sys->nnz,
+                                values_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+        // CSR values follow through the permutation: csr[perm[p]] = csc[p]
+        std::vector<int> hp((size_t)sys->nnz);
+        std::vector<double> hv((size_t)sys->nnz), hc((size_t)sys->nnz);
+        SPAI_HIP(hipMemcpyAsync(hp.data(), sys->perm.get(), sizeof(int) * (size_t)sys->nnz, hipMemcpyDeviceToHost,
+                                st));
+        SPAI_HIP(hipMemcpyAsync(hv.data(), sys->cscval.get(), sizeof(double) * (size_t)sys->nnz,
+                                hipMemcpyDeviceToHost, st));
+        SPAI_HIP(hipStreamSynchronize(st));
+        for (int p = 0; p < sys->nnz; ++p) hc[(size_t)hp[(size_t)p]] = hv[(size_t)p];
+        SPAI_HIP(hipMemcpyAsync(sys->csrval.get(), hc.data(), sizeof(double) * (size_t)sys->nnz,
+                                hipMemcpyHostToDevice, st));
+        SPAI_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int spai_system_build(spai_system *sys, int precision, int schedule, int col_begin, int col_end,
+                      spai_build_report *report) {
+    if (!sys) return fail(SPAI_ERR_INVALID, "sys is NULL");
+    if (precision != SPAI_PRECISION_FP64 && precision != SPAI_PRECISION_MIXED)
+        return fail(SPAI_ERR_INVALID, "unknown precision");
+    if (schedule != SPAI_SCHEDULE_BALANCED_COLUMNS && schedule != SPAI_SCHEDULE_BALANCED_NNZ)
+        return fail(SPAI_ERR_INVALID, "unknown schedule");
+    return guarded([&] {
+        hipStream_t st = sys->ctx->stream;
+        SPAI_HIP(hipSetDevice(sys->ctx->device));
+        size_t nz = (size_t)std::max(sys->nnz, 1);
+        if (precision == SPAI_PRECISION_MIXED) {
+            sys->m32_csc.resize(nz);
+            sys->m32_csr.resize(nz);
+        } else {
+            sys->m64_csc.resize(nz);
+            sys->m64_csr.resize(nz);
+        }
+        sys->n1.resize((size_t)sys->n);
+        int cb = std::max(0, col_begin), ce = col_end < 0 ? sys->n : std::min(col_end, sys->n);
+        require(cb <= ce, "column range is inverted");
+        sys->m_precision = -1;
+        Timer tm(st);
+        bool partial = cb > 0 || ce < sys->n;
+        if (partial) {
+            if (precision == SPAI_PRECISION_MIXED) {
+                fill(sys->m32_csc.get(), sys->nnz, st);
+                fill(sys->m32_csr.get(), sys->nnz, st);
+            } else {
+                fill(sys->m64_csc.get(), sys->nnz, st);
+                fill(sys->m64_csr.get(), sys->nnz, st);
+            }
+        }
+        SpaiBuildParams p;
+        p.n = sys->n;
+        p.colptr = sys->colptr.get();
+        p.rowidx = sys->rowidx.get();
+        p.val = sys->cscval.get();
+        p.perm = sys->perm.get();
+        p.precision = precision;
+        p.schedule = schedule;
+        p.col_begin = cb;
+        p.col_end = ce;
+        p.m_csc = precision == SPAI_PRECISION_MIXED ? (void *)sys->m32_csc.get() : (void *)sys->m64_csc.get();
+        p.m_csr = precision == SPAI_PRECISION_MIXED ? (void *)sys->m32_csr.get() : (void *)sys->m64_csr.get();
+        p.n1_out = sys->n1.get();
+        SpaiBuildStats stats;
+        sys->ctx->builder.build(p, st, &stats);
+        float ms = tm.stop();
+        sys->m_precision = precision;
+        if (report) {
+            report->ms = ms;
+            for (int t = 0; t < 5; ++t) report->tier_columns[t] = stats.tier_columns[t];
+            report->max_gathered = stats.max_gathered;
+            report->max_n2 = stats.max_n2;
+            report->empty_columns = stats.empty_columns;
+            report->sum_gathered = stats.sum_gathered;
+            report->static_tier = stats.static_tier;
+        }
+    });
+}
+
+int spai_system_get_m(spai_system *sys, double *values, int values_on_device) {
+    if (!sys || !values) return fail(SPAI_ERR_INVALID, "sys/values is NULL");
+    if (sys->m_precision < 0) return fail(SPAI_ERR_NOT_BUILT, "M has not been built");
+    return guarded([&] {
+        hipStream_t st = sys->ctx->stream;
+        SPAI_HIP(hipSetDevice(sys->ctx->device));
+        size_t bytes = sizeof(double) * (size_t)sys->nnz;
+        const double *src = sys->m64_csc.get();
+        DeviceArray<double> tmp;
+        if (sys->m_precision == SPAI_PRECISION_MIXED) {
+            tmp.resize((size_t)std::max(sys->nnz, 1));
+            convert_f32_to_f64(sys->m32_csc.get(), tmp.get(), sys->nnz, st);
+            src = tmp.get();
+        }
+        SPAI_HIP(hipMemcpyAsync(values, src, bytes, values_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                st));
+        SPAI_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int spai_system_get_row_set_sizes(spai_system *sys, int *n1) {
+    if (!sys || !n1) return fail(SPAI_ERR_INVALID, "sys/n1 is NULL");
+    if (sys->m_precision < 0) return fail(SPAI_ERR_NOT_BUILT, "M has not been built");
+    return guarded([&] {
+        SPAI_HIP(hipSetDevice(sys->ctx->device));
+        SPAI_HIP(hipMemcpyAsync(n1, sys->n1.get(), sizeof(int) * (size_t)sys->n, hipMemcpyDeviceToHost,
+                                sys->ctx->stream));
+        SPAI_HIP(hipStreamSynchronize(sys->ctx->stream));
+    });
+}
+
+void spai_solve_options_default(spai_solve_options *opt) {
+    if (!opt) return;
+    opt->rel_tol = 1e-8;
+    opt->abs_tol = 0.0;
+    opt->max_iter = 1000;
+    opt->use_graph = 0;
+    opt->graph_iters = 8;
+    opt->check_every = 1;
+    opt->precondition = 1;
+}
+
+int spai_system_solve(spai_system *sys, const double *b, double *x, int on_device, const spai_solve_options *opt,
+                      spai_solve_report *report) {
+    if (!sys || !b || !x) return fail(SPAI_ERR_INVALID, "sys/b/x is NULL");
+    spai_solve_options dflt;
+    spai_solve_options_default(&dflt);
+    if (!opt) opt = &dflt;
+    if (opt->precondition && sys->m_precision < 0) return fail(SPAI_ERR_NOT_BUILT, "M has not been built");
+    return guarded([&] {
+        hipStream_t st = sys->ctx->stream;
+        SPAI_HIP(hipSetDevice(sys->ctx->device));
+        const double *db = b;
+        double *dx = x;
+        if (!on_device) {
+            sys->stage_b.reserve((size_t)sys->n);
+            sys->stage_x.reserve((size_t)sys->n);
+            SPAI_HIP(hipMemcpyAsync(sys->stage_b.get(), b, sizeof(double) * sys->n, hipMemcpyHostToDevice, st));
+            SPAI_HIP(hipMemcpyAsync(sys->stage_x.get(), x, sizeof(double) * sys->n, hipMemcpyHostToDevice, st));
+            db = sys->stage_b.get();
+            dx = sys->stage_x.get();
+        }
+        OperatorView A{sys->rowptr.get(), sys->colidx.get(), sys->csrval.get(), 0};
+        OperatorView M = opt->precondition ? m_view(sys, false) : OperatorView{};
+        SolveResult res;
+        sys->ctx->solver.solve(A, M, sys->tiles, sys->n, sys->nsys, db, dx, to_options(opt), st, &res);
+        sys->last_states = sys->ctx->solver.states();
+        if (!on_device) {
+            SPAI_HIP(hipMemcpyAsync(x, dx, sizeof(double) * sys->n, hipMemcpyDeviceToHost, st));
+            SPAI_HIP(hipStreamSynchronize(st));
+        }
+        fill_report(report, res);
+    });
+}
+
+int spai_system_outcomes(spai_system *sys, spai_system_outcome *out) {
+    if (!sys || !out) return fail(SPAI_ERR_INVALID, "sys/out is NULL");
+    if ((int)sys->last_states.size() != sys->nsys) return fail(SPAI_ERR_INVALID, "no solve has run");
+    for (int s = 0; s < sys->nsys; ++s) {
+        const SysState &S = sys->last_states[(size_t)s];
+        out[s].status = S.status;
+        out[s].iterations = S.iters;
+        out[s].residual = std::sqrt(S.rr);
+        out[s].residual0 = S.err0;
+        out[s].true_residual = std::sqrt(S.true_rr);
+        out[s].tol = S.tol;
+    }
+    return SPAI_OK;
+}
+
+int spai_system_adjoint(spai_system *sys, const double *x, const double *g, double *dA, double *db, int on_device,
+                        const spai_solve_options *opt, spai_solve_report *report) {
+    if (!sys || !x || !g || !dA || !db) return fail(SPAI_ERR_INVALID, "sys/x/g/dA/db is NULL");
+    spai_solve_options dflt;
+    spai_solve_options_default(&dflt);
+    if (!opt) opt = &dflt;
+    if (opt->precondition && sys->m_precision < 0) return fail(SPAI_ERR_NOT_BUILT, "M has not been built");
+    return guarded([&] {
+        hipStream_t st = sys->ctx->stream;
+        SPAI_HIP(hipSetDevice(sys->ctx->device));
+        ensure_transposed_tiles(sys);
+        const size_t n = (size_t)sys->n, nz = (size_t)std::max(sys->nnz, 1);
+        const double *dx = x, *dg = g;
+        double *ddA = dA, *ddb = db;
+        if (!on_device) {
+            sys->stage_x.reserve(n);
+            sys->stage_g.reserve(n);
+            sys->stage_dA.reserve(nz);
+            sys->stage_db.reserve(n);
+            SPAI_HIP(hipMemcpyAsync(sys->stage_x.get(), x, sizeof(double) * n, hipMemcpyHostToDevice, st));
+            SPAI_HIP(hipMemcpyAsync(sys->stage_g.get(), g, sizeof(double) * n, hipMemcpyHostToDevice, st));
+            dx = sys->stage_x.get();
+            dg = sys->stage_g.get();
+            ddA = sys->stage_dA.get();
+            ddb = sys->stage_db.get();
+        }
+        fill(ddb, (long long)n, st);  // lambda_0 = 0
+        OperatorView At{sys->colptr.get(), sys->rowidx.get(), sys->cscval.get(), 0};
+        OperatorView Mt = opt->precondition ? m_view(sys, true) : OperatorView{};
+        SolveResult res;
+        sys->ctx->adjoint_solver.solve(At, Mt, sys->tiles_t, sys->n, sys->nsys, dg, ddb, to_options(opt), st, &res);
+        sys->last_states = sys->ctx->adjoint_solver.states();
+        adjoint_scatter(sys->n, sys->colptr.get(), sys->rowidx.get(), ddb, dx, ddA, st);
+        if (!on_device) {
+            SPAI_HIP(hipMemcpyAsync(dA, ddA, sizeof(double) * (size_t)sys->nnz, hipMemcpyDeviceToHost, st));
+            SPAI_HIP(hipMemcpyAsync(db, ddb, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        }
+        SPAI_HIP(hipStreamSynchronize(st));
+        fill_report(report, res);
+    });
+}
+
+int spai_system_time_spmv(spai_system *sys, int which, int reps, float *ms) {
+    if (!sys || !ms || reps < 1) return fail(SPAI_ERR_INVALID, "bad arguments");
+    if (which == 1 && sys->m_precision < 0) return fail(SPAI_ERR_NOT_BUILT, "M has not been built");
+    return guarded([&] {
+        SPAI_HIP(hipSetDevice(sys->ctx->device));
+        OperatorView A{sys->rowptr.get(), sys->colidx.get(), sys->csrval.get(), 0};
+        OperatorView M = which == 1 ? m_view(sys, false) : OperatorView{};
+        *ms = sys->ctx->solver.time_kernel(which, A, M, sys->tiles, sys->n, sys->nsys, reps, sys->ctx->stream);
+    });
+}
+
+int spai_csc_to_csr_device(int m, int n, int nnz, const double *cscData, const int *cscIndex, const int *cscPtr,
+                           double *csrData, int *csrIndex, int *csrPtr) {
+    if (m <= 0 || n <= 0 || m != n || nnz < 0 || !cscIndex || !cscPtr || !csrIndex || !csrPtr)
+        return fail(SPAI_ERR_INVALID, "csc_to_csr: bad arguments (square matrices only)");
+    return guarded([&] {
+        DeviceArray<int> perm((size_t)std::max(nnz, 1));
+        csc_to_csr_device(n, nnz, cscPtr, cscIndex, cscData, csrPtr, csrIndex, csrData, perm.get(), nullptr);
+    });
+}
+
+// ---------------------------------------------------------- problems ----
+int spai_problem_generate(int config, uint64_t seed, spai_problem **out) {
+    if (!out) return fail(SPAI_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    return guarded([&] {
+        auto P = std::make_unique<spai_problem>();
+        P->p = generate_config(config, seed);
+        *out = P.release();
+    });
+}
+
+int spai_problem_generate_power_law(int n, uint64_t seed, spai_problem **out) {
+    if (!out || n < 2) return fail(SPAI_ERR_INVALID, "bad power-law arguments");
+    *out = nullptr;
+    return guarded([&] {
+        auto P = std::make_unique<spai_problem>();
+        P->p = generate_power_law(seed, n);
+        *out = P.release();
+    });
+}
+
+int spai_problem_generate_batched(int nsys, uint64_t seed, spai_problem **out) {
+    if (!out || nsys < 1) return fail(SPAI_ERR_INVALID, "bad batched arguments");
+    *out = nullptr;
+    return guarded([&] {
+        auto P = std::make_unique<spai_problem>();
+        P->p = generate_batched(seed, nsys);
+        *out = P.release();
+    });
+}
+
+int spai_problem_convection_diffusion(int m, double peclet, double vx, double vy, spai_problem **out) {
+    if (!out || m < 1 || !(peclet > 0)) return fail(SPAI_ERR_INVALID, "bad grid arguments");
+    *out = nullptr;
+    return guarded([&] {
+        auto P = std::make_unique<spai_problem>();
+        P->p.name = "convdiff" + std::to_string(m);
+        P->p.systems.push_back(convection_diffusion_2d(m, peclet, vx, vy));
+        P->p.rhs.emplace_back((size_t)m * m, 1.0);
+        P->p.target.emplace_back();
+        *out = P.release();
+    });
+}
+
+int spai_problem_read_mtx(const char *path, spai_problem **out) {
+    if (!out || !path) return fail(SPAI_ERR_INVALID, "path/out is NULL");
+    *out = nullptr;
+    return guarded([&] {
+        auto P = std::make_unique<spai_problem>();
+        P->p.name = path;
+        P->p.systems.push_back(read_matrix_market(path));
+        P->p.rhs.emplace_back((size_t)P->p.systems[0].n, 1.0);
+        P->p.target.emplace_back();
+        *out = P.release();
+    });
+}
+
+int spai_problem_free(spai_problem *p) {
+    delete p;
+    return SPAI_OK;
+}
+
+int spai_problem_num_systems(const spai_problem *p) { return p ? (int)p->p.systems.size() : 0; }
+
+int spai_problem_system(const spai_problem *p, int s, CSC_Matrix *out) {
+    if (!p || !out || s < 0 || s >= (int)p->p.systems.size()) return fail(SPAI_ERR_INVALID, "bad system index");
+    const HostCSC &A = p->p.systems[(size_t)s];
+    out->n = out->nRow = out->nCol = A.n;
+    out->nonzeroes = (int)A.nnz();
+    out->mPtr = const_cast<int *>(A.colptr.data());
+    out->mIndex = const_cast<int *>(A.rowidx.data());
+    out->mData = const_cast<double *>(A.val.data());
+    return SPAI_OK;
+}
+
+const double *spai_problem_rhs(const spai_problem *p, int s) {
+    if (!p || s < 0 || s >= (int)p->p.rhs.size() || p->p.rhs[(size_t)s].empty()) return nullptr;
+    return p->p.rhs[(size_t)s].data();
+}
+
+const double *spai_problem_target(const spai_problem *p, int s) {
+    if (!p || s < 0 || s >= (int)p->p.target.size() || p->p.target[(size_t)s].empty()) return nullptr;
+    return p->p.target[(size_t)s].data();
+}
+
+const char *spai_problem_name(const spai_problem *p) { return p ? p->p.name.c_str() : ""; }
+const char *spai_problem_notes(const spai_problem *p) { return p ? p->p.notes.c_str() : ""; }
+
+}  // extern "C"
+
+// ===================================================== C++ drop-ins ====
+struct MultiDCU_Context {
+    spai_context *ctx = nullptr;
+    int numDCUs = 1;
+};
+
+namespace {
+[[noreturn]] void die(const char *where) {
+    std::printf("HIP error in %s: %s\n", where, spai_last_error());
+    std::exit(EXIT_FAILURE);
+}
+
+float run_spai_api(MultiDCU_Context *ctx, const CSC_Matrix *CSC_A, CSC_Matrix *devM, int cs, int ce, int schedule) {
+    if (!ctx || !CSC_A || !devM) return 0.0f;
+    if (cs < 0 || ce > CSC_A->nCol || cs >= ce) return 0.0f;  // the reference returns 0 for bad ranges
+    spai_system *S = nullptr;
+    if (spai_system_create(ctx->ctx, CSC_A, &S) != SPAI_OK) die("RunSPAI (upload)");
+    spai_build_report rep{};
+    if (spai_system_build(S, SPAI_PRECISION_FP64, schedule, cs, ce, &rep) != SPAI_OK) die("RunSPAI (build)");
+    int n = CSC_A->nCol, nnz = CSC_A->nonzeroes;
+    devM->n = devM->nRow = devM->nCol = n;
+    devM->nonzeroes = nnz;
+    if (hipMalloc((void **)&devM->mPtr, sizeof(int) * ((size_t)n + 1)) != hipSuccess ||
+        hipMalloc((void **)&devM->mIndex, sizeof(int) * (size_t)std::max(nnz, 1)) != hipSuccess ||
+        hipMalloc((void **)&devM->mData, sizeof(double) * (size_t)std::max(nnz, 1)) != hipSuccess) {
+        g_last_error = "hipMalloc for M failed";
+        die("RunSPAI (allocate M)");
+    }
+    if (hipMemcpy(devM->mPtr, S->colptr.get(), sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToDevice) !=
+            hipSuccess ||
+        hipMemcpy(devM->mIndex, S->rowidx.get(), sizeof(int) * (size_t)nnz, hipMemcpyDeviceToDevice) != hipSuccess) {
+        g_last_error = "copy of M's pattern failed";
+        die("RunSPAI (M pattern)");
+    }
+    if (spai_system_get_m(S, devM->mData, 1) != SPAI_OK) die("RunSPAI (M values)");
+    spai_system_destroy(S);
+    return rep.ms;
+}
+
+spai_context *default_context() {
+    static spai_context *ctx = nullptr;
+    if (!ctx) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        if (spai_context_create(dev, &ctx) != SPAI_OK) die("default context");
+    }
+    return ctx;
+}
+}  // namespace
+
+MultiDCU_Context *initMultiDCU(int requestedDCUs) {
+    (void)requestedDCUs;
+    auto *c = new MultiDCU_Context();
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (spai_context_create(dev, &c->ctx) != SPAI_OK) die("initMultiDCU");
+    c->numDCUs = 1;
+    return c;
+}
+
+void cleanupMultiDCU(MultiDCU_Context *ctx) {
+    if (!ctx) return;
+    spai_context_destroy(ctx->ctx);
+    delete ctx;
+}
+
+float RunStaticSPAI_MultiDCU_MPI(MultiDCU_Context *ctx, const CSC_Matrix *CSC_A, CSC_Matrix *devCSC_M_global,
+                                 int globalColStart, int globalColEnd) {
+    return run_spai_api(ctx, CSC_A, devCSC_M_global, globalColStart, globalColEnd,
+                        SPAI_SCHEDULE_BALANCED_COLUMNS);
+}
+
+float RunDynamicSPAI_MultiDCU_MPI(MultiDCU_Context *ctx, const CSC_Matrix *CSC_A, CSC_Matrix *devCSC_M_global,
+                                  int globalColStart, int globalColEnd) {
+    return run_spai_api(ctx, CSC_A, devCSC_M_global, globalColStart, globalColEnd, SPAI_SCHEDULE_BALANCED_NNZ);
+}
+
+int cublas2_pbicgstabv2(CSR_Matrix *devA, CSR_Matrix *devM, double *dev_b, double *dev_x, double TOL, int MAX_ITER) {
+    if (!devA || !dev_b || !dev_x) return 0;
+    spai_context *ctx = default_context();
+    int n = devA->n;
+    int status = guarded([&] {
+        hipStream_t st = ctx->stream;
+        TileMap ta;
+        std::vector<int> roff = {0, n};
+        build_tiles(ta, devA->mPtr, roff, {0, (long long)devA->nonzeroes}, st);
+        OperatorView A{devA->mPtr, devA->mIndex, devA->mData, 0};
+        OperatorView M{};
+        // M may have its own pattern: the kernels read each operator's own
+        // row pointers inside the shared row ranges
+        if (devM) M = OperatorView{devM->mPtr, devM->mIndex, devM->mData, 0};
+        SolveOptions o;
+        o.abs_tol = TOL;
+        o.max_iter = MAX_ITER <= 0 ? n : MAX_ITER;
+        SolveResult res;
+        ctx->solver.solve(A, M, ta, n, 1, dev_b, dev_x, o, st, &res);
+    });
+    if (status != SPAI_OK) die("cublas2_pbicgstabv2");
+    return ctx->solver.states()[0].iters;
+}
+
+void cuCSC2CSR(int m, int n, int nnz, double *cscData, int *cscIndex, int *cscPtr, double *csrData, int *csrIndex,
+               int *csrPtr) {
+    if (spai_csc_to_csr_device(m, n, nnz, cscData, cscIndex, cscPtr, csrData, csrIndex, csrPtr) != SPAI_OK)
+        die("cuCSC2CSR");
+}

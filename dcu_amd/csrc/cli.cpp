@@ -1,0 +1,166 @@
+// spai_mi355x: command-line driver with the reference's flags
+// (src/spai_multi_dcu.cpp:2386-2440) and its machine-readable report line
+// (src/spai_multi_dcu.cpp:2572-2574):
+//
+//   [ALGO_REPORT] mode=%s, dcu_units=%d, partition=%s, preconditioner_ms=%.4f, iteration_count=%d
+//
+//   spai_mi355x [dcu_count] [--algo static|dynamic]
+//               [--partition balanced_columns|balanced_nnz]
+//               [--matrix <path>] [--precondition-only] [--help]
+//   additional: --gen <1..5> [--seed S]    synthetic workload instead of a file
+//               --precision fp64|mixed     (default fp64)
+//               --tol T                    absolute ||r|| threshold (reference: 1e-7)
+//               --rel-tol R                relative threshold (overrides --tol)
+//               --max-iter K               (reference: 10000)
+//               --x0 ones|zero, --rhs ones|problem
+//               --graph                    replay iterations from a hipGraph
+//
+// With --matrix the solve defaults follow the reference driver
+// (src/spai_multi_dcu.cpp:1920-1934: b = 1, x0 = 1, TOL = 1e-7 absolute,
+// MAX_ITER = 10000); with --gen they follow the workload definitions
+// (b from the generator, x0 = 0, relative 1e-8, 1000 iterations).
+// preconditioner_ms is the device time of the whole SPAI build.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "spai_mi355x.h"
+
+static void usage(const char *prog) {
+    std::printf("usage: %s [dcu_count] [--algo static|dynamic] [--partition balanced_columns|balanced_nnz]\n", prog);
+    std::printf("           [--matrix <path> | --gen <1..5> [--seed S]] [--precondition-only]\n");
+    std::printf("           [--precision fp64|mixed] [--tol T | --rel-tol R] [--max-iter K]\n");
+    std::printf("           [--x0 ones|zero] [--rhs ones|problem] [--graph] [--help]\n");
+}
+
+static void check(int st, const char *what) {
+    if (st != SPAI_OK) {
+        std::printf("error: %s: %s (%s)\n", what, spai_status_string(st), spai_last_error());
+        std::exit(EXIT_FAILURE);
+    }
+}
+
+int main(int argc, char **argv) {
+    std::string matrix = "matrices/circuit_2.mtx";
+    std::string algo = "static";
+    bool nnz_balance = false, partition_given = false, precondition_only = false, use_graph = false;
+    int gen = 0, max_iter = -1, precision = SPAI_PRECISION_FP64;
+    unsigned long long seed = 0;
+    double tol = -1.0, rel_tol = -1.0;
+    std::string x0_mode, rhs_mode;
+    int requested = -1;
+    for (int i = 1; i < argc; ++i) {
+        std::string a = argv[i];
+        auto next = [&]() -> const char * {
+            if (i + 1 >= argc) {
+                std::printf("missing value for %s\n", a.c_str());
+                std::exit(EXIT_FAILURE);
+            }
+            return argv[++i];
+        };
+        if (a == "--help" || a == "-h") {
+            usage(argv[0]);
+            return 0;
+        } else if (a == "--matrix") matrix = next();
+        else if (a == "--algo") algo = next();
+        else if (a == "--partition") {
+            partition_given = true;
+            nnz_balance = std::strcmp(next(), "balanced_nnz") == 0;
+        } else if (a == "--precondition-only") precondition_only = true;
+        else if (a == "--gen") gen = std::atoi(next());
+        else if (a == "--seed") seed = std::strtoull(next(), nullptr, 10);
+        else if (a == "--precision") precision = std::strcmp(next(), "mixed") == 0 ? SPAI_PRECISION_MIXED : SPAI_PRECISION_FP64;
+        else if (a == "--tol") tol = std::atof(next());
+        else if (a == "--rel-tol") rel_tol = std::atof(next());
+        else if (a == "--max-iter") max_iter = std::atoi(next());
+        else if (a == "--x0") x0_mode = next();
+        else if (a == "--rhs") rhs_mode = next();
+        else if (a == "--graph") use_graph = true;
+        else if (a[0] != '-') requested = std::atoi(a.c_str());
+    }
+    if (!partition_given) nnz_balance = (algo == "dynamic");
+    (void)requested;  // one device: dcu_units is always 1
+
+    spai_problem *P = nullptr;
+    if (gen > 0) {
+        check(spai_problem_generate(gen, seed ? seed : (unsigned long long)gen, &P), "generate");
+        if (x0_mode.empty()) x0_mode = "zero";
+        if (rhs_mode.empty()) rhs_mode = "problem";
+        if (max_iter < 0) max_iter = 1000;
+        if (tol < 0 && rel_tol < 0) rel_tol = 1e-8;
+    } else {
+        check(spai_problem_read_mtx(matrix.c_str(), &P), "read matrix");
+        if (x0_mode.empty()) x0_mode = "ones";
+        if (rhs_mode.empty()) rhs_mode = "ones";
+        if (max_iter < 0) max_iter = 10000;
+        if (tol < 0 && rel_tol < 0) tol = 1e-7;
+    }
+    int nsys = spai_problem_num_systems(P);
+    std::vector<CSC_Matrix> parts((size_t)nsys);
+    for (int s = 0; s < nsys; ++s) check(spai_problem_system(P, s, &parts[(size_t)s]), "system");
+    std::printf("matrix: %s  systems: %d  n(first): %d  nnz(first): %d\n", spai_problem_name(P), nsys, parts[0].n,
+                parts[0].nonzeroes);
+
+    spai_context *ctx = nullptr;
+    check(spai_context_create(0, &ctx), "context");
+    char name[128];
+    int cus = 0;
+    spai_context_device_info(ctx, name, sizeof name, &cus);
+    std::printf("device: %s (%d CUs)\n", name, cus);
+
+    spai_system *S = nullptr;
+    if (nsys == 1) check(spai_system_create(ctx, &parts[0], &S), "upload");
+    else check(spai_system_create_batched(ctx, nsys, parts.data(), &S), "upload");
+    spai_system_info info;
+    spai_system_get_info(S, &info);
+
+    spai_build_report br{};
+    check(spai_system_build(S, precision, nnz_balance ? SPAI_SCHEDULE_BALANCED_NNZ : SPAI_SCHEDULE_BALANCED_COLUMNS,
+                            0, -1, &br),
+          "SPAI build");
+    std::printf("SPAI build: %.4f ms  max|J|=%d  max gathered=%d  tiers=[%d %d %d %d]\n", br.ms, br.max_n2,
+                br.max_gathered, br.tier_columns[0], br.tier_columns[1], br.tier_columns[2], br.tier_columns[3]);
+
+    int iterations = 0;
+    if (!precondition_only) {
+        std::vector<double> b((size_t)info.n), x((size_t)info.n);
+        std::vector<int> off((size_t)nsys + 1);
+        spai_system_row_offsets(S, off.data());
+        for (int s = 0; s < nsys; ++s) {
+            const double *r = spai_problem_rhs(P, s);
+            for (int i = off[(size_t)s]; i < off[(size_t)s + 1]; ++i)
+                b[(size_t)i] = (rhs_mode == "problem" && r) ? r[i - off[(size_t)s]] : 1.0;
+        }
+        for (auto &v : x) v = (x0_mode == "ones") ? 1.0 : 0.0;
+        spai_solve_options o;
+        spai_solve_options_default(&o);
+        o.max_iter = max_iter;
+        o.use_graph = use_graph ? 1 : 0;
+        if (rel_tol > 0) {
+            o.rel_tol = rel_tol;
+            o.abs_tol = 0.0;
+        } else {
+            o.abs_tol = tol;
+        }
+        spai_solve_report sr{};
+        check(spai_system_solve(S, b.data(), x.data(), 0, &o, &sr), "BiCGSTAB");
+        std::vector<spai_system_outcome> oc((size_t)nsys);
+        spai_system_outcomes(S, oc.data());
+        iterations = sr.iterations_max;
+        double worst = 0.0;
+        for (auto &q : oc) {
+            double bn = q.tol > 0 && rel_tol > 0 ? q.tol / rel_tol : 1.0;
+            worst = std::max(worst, q.true_residual / bn);
+        }
+        std::printf("BiCGSTAB: %.4f ms  iterations(max)=%d  converged=%d  true residual %s=%.3e\n", sr.ms,
+                    sr.iterations_max, sr.all_converged, rel_tol > 0 ? "(rel, worst)" : "(abs, worst)", worst);
+    }
+    std::printf("[ALGO_REPORT] mode=%s, dcu_units=%d, partition=%s, preconditioner_ms=%.4f, iteration_count=%d\n",
+                algo.c_str(), 1, nnz_balance ? "balanced_nnz" : "balanced_columns", br.ms, iterations);
+    spai_system_destroy(S);
+    spai_context_destroy(ctx);
+    spai_problem_free(P);
+    return 0;
+}

@@ -1,0 +1,133 @@
+// Shared HIP helpers for gfx950 (CDNA4, 64-wide wavefronts).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace spai {
+
+// Error carried from the HIP runtime or an argument check up to the C-ABI,
+// which turns it into a status code.
+struct Error : std::runtime_error {
+    int status;
+    Error(int st, const std::string &msg) : std::runtime_error(msg), status(st) {}
+};
+
+enum Status {
+    kOk = 0,
+    kErrInvalid = -1,
+    kErrHip = -2,
+    kErrNoMem = -3,
+    kErrUnsupported = -4,
+    kErrBreakdown = -5,
+    kErrNotBuilt = -6,
+};
+
+#define SPAI_HIP(call)                                                                          \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            throw ::spai::Error(e_ == hipErrorOutOfMemory ? ::spai::kErrNoMem : ::spai::kErrHip, \
+                                std::string(#call) + ": " + hipGetErrorString(e_) + " at " +    \
+                                    __FILE__ + ":" + std::to_string(__LINE__));                 \
+    } while (0)
+
+#define SPAI_CHECK_LAUNCH() SPAI_HIP(hipGetLastError())
+
+inline void require(bool ok, const std::string &msg, int st = kErrInvalid) {
+    if (!ok) throw Error(st, msg);
+}
+
+// Owning device allocation.
+template <class T>
+struct DeviceArray {
+    T *ptr = nullptr;
+    size_t count = 0;
+    DeviceArray() = default;
+    explicit DeviceArray(size_t n) { resize(n); }
+    DeviceArray(const DeviceArray &) = delete;
+    DeviceArray &operator=(const DeviceArray &) = delete;
+    DeviceArray(DeviceArray &&o) noexcept : ptr(o.ptr), count(o.count) { o.ptr = nullptr; o.count = 0; }
+    DeviceArray &operator=(DeviceArray &&o) noexcept {
+        if (this != &o) { release(); ptr = o.ptr; count = o.count; o.ptr = nullptr; o.count = 0; }
+        return *this;
+    }
+    ~DeviceArray() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    void resize(size_t n) {
+        if (n == count && ptr) return;
+        release();
+        if (n) SPAI_HIP(hipMalloc((void **)&ptr, n * sizeof(T)));
+        count = n;
+    }
+    // grow-only reservation for workspaces reused across calls
+    void reserve(size_t n) {
+        if (n > count) resize(n);
+    }
+    size_t bytes() const { return count * sizeof(T); }
+    T *get() const { return ptr; }
+};
+
+// ------------------------------------------------------- device helpers ----
+#ifdef __HIP_DEVICE_COMPILE__
+#define SPAI_DEV __device__ __forceinline__
+#else
+#define SPAI_DEV __device__ inline
+#endif
+
+// Sum across a power-of-two group of W lanes (W <= 64) inside one wavefront;
+// every lane of the group receives the total.
+template <int W, class T>
+SPAI_DEV T group_sum(T v) {
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
+    return v;
+}
+
+template <int W>
+SPAI_DEV int group_max(int v) {
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) {
+        int t = __shfl_xor(v, o, W);
+        v = t > v ? t : v;
+    }
+    return v;
+}
+
+// Inclusive scan across W lanes (Hillis-Steele with shuffles).
+template <int W>
+SPAI_DEV int group_inclusive_scan(int v, int lane) {
+#pragma unroll
+    for (int o = 1; o < W; o <<= 1) {
+        int t = __shfl_up(v, o, W);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Orders LDS traffic between lanes of one wavefront: the compiler may not move
+// LDS accesses across it and waits for outstanding LDS operations.
+SPAI_DEV void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// 64-bit load that bypasses the non-coherent vector L1 (used for data written
+// by other workgroups of the same launch).
+template <class T>
+SPAI_DEV T load_coherent(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace spai

@@ -1,0 +1,848 @@
+// Sparse formats, row tiles and the fused right-preconditioned BiCGSTAB.
+//
+// The recurrences are those of include/bicgstab/bicgstab_solver.h:106-232
+// (one iteration: y = M p, a = A y, alpha = rho / (r*, a), s = r - alpha a,
+// z = M s, t = A z, omega = (t, s) / (t, t), x += alpha y + omega z,
+// r = s - omega t, rho' = (r*, r), beta = (rho'/rho)(alpha/omega),
+// p = r + beta (p - omega a), stop on ||r||_2 < TOL).  The reference issues
+// 18 hipSPARSE/hipBLAS calls and 5 blocking scalar reads per iteration; here
+// one iteration is five kernels and every scalar stays in device memory:
+//
+//   K1  y = M p'   with p' = beta p + r - beta omega a formed while gathering
+//                  (the p update of the previous iteration, fused), p' stored
+//   K2  a = A y,   (r*, a) reduced, alpha = rho / (r*, a) by the last tile
+//   K3  z = M s    with s = r - alpha a formed while gathering, s stored
+//   K4  t = A z,   (t, s), (t, t) reduced, omega by the last tile
+//   K5  x += alpha y + omega z, r = s - omega t, (r*, r), (r, r) reduced,
+//       beta, convergence and the iteration count by the last tile
+//
+// Every kernel walks row tiles (one workgroup per tile); a tile's nonzeros
+// are streamed with fully coalesced loads into LDS as products, then each
+// thread sums its rows in storage order (deterministic).  Reductions are
+// per-tile partials summed in a fixed order by the last tile of each system,
+// so results are reproducible run to run.  All systems of a batch share the
+// launches; tiles of a converged system return at once.
+#include "krylov.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace spai {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kLdsProducts = 4096;  // products staged per tile (32 KB)
+
+// ------------------------------------------------------------ transpose ----
+__global__ void k_fill_int(int *p, int v, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_count_rows(const int *rowidx, int nnz, int *cnt) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nnz) atomicAdd(&cnt[rowidx[p]], 1);
+}
+
+// Exclusive scan, three passes: per-block scan, scan of block sums (one
+// block, sequential carry), add offsets.
+constexpr int kScanBlock = 1024;
+__global__ __launch_bounds__(kScanBlock) void k_scan_blocks(const int *in, int *out, int n, int *bsum) {
+    __shared__ int sh[kScanBlock];
+    int i = blockIdx.x * kScanBlock + threadIdx.x;
+    int v = i < n ? in[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < kScanBlock; o <<= 1) {
+        int t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    if (i < n) out[i] = sh[threadIdx.x] - v;
+    if (threadIdx.x == kScanBlock - 1) bsum[blockIdx.x] = sh[kScanBlock - 1];
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_sums(int *bsum, int nb, int *total) {
+    __shared__ int sh[kScanBlock];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += kScanBlock) {
+        int i = base + threadIdx.x;
+        int v = i < nb ? bsum[i] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < kScanBlock; o <<= 1) {
+            int t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nb) bsum[i] = carry + sh[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += sh[kScanBlock - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_add(int *out, int n, const int *bsum, const int *total, int *last) {
+    int i = blockIdx.x * kScanBlock + threadIdx.x;
+    if (i < n) out[i] += bsum[blockIdx.x];
+    if (i == 0 && last) *last = *total;
+}
+
+// out[0..n) = exclusive scan of in[0..n), out[n] = total
+void exclusive_scan(const int *in, int *out, int n, DeviceArray<int> &tmp, hipStream_t st) {
+    int nb = std::max(1, ceil_div(n, kScanBlock));
+    tmp.reserve((size_t)nb + 1);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanBlock), 0, st, in, out, n, tmp.get());
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, st, tmp.get(), nb, tmp.get() + nb);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScanBlock), 0, st, out, n, tmp.get(), tmp.get() + nb, out + n);
+    SPAI_CHECK_LAUNCH();
+}
+
+// One thread per column: append (column, csc position) into each row's slots.
+__global__ void k_fill_rows(const int *colptr, const int *rowidx, int n, const int *rowptr, int *cursor, int *colidx,
+                            int *src) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    for (int p = colptr[c]; p < colptr[c + 1]; ++p) {
+        int r = rowidx[p];
+        int q = rowptr[r] + atomicAdd(&cursor[r], 1);
+        colidx[q] = c;
+        src[q] = p;
+    }
+}
+
+// One thread per row: order the row by column (insertion sort up to 32
+// entries, shell sort beyond), then write perm and the CSR values.
+__global__ void k_sort_rows(int n, const int *rowptr, int *colidx, int *src, const double *cscval, double *csrval,
+                            int *perm) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    int b = rowptr[r], e = rowptr[r + 1], len = e - b;
+    int *ci = colidx + b;
+    int *sp = src + b;
+    if (len <= 32) {
+        for (int i = 1; i < len; ++i) {
+            int kc = ci[i], ks = sp[i], j = i - 1;
+            while (j >= 0 && ci[j] > kc) {
+                ci[j + 1] = ci[j];
+                sp[j + 1] = sp[j];
+                --j;
+            }
+            ci[j + 1] = kc;
+            sp[j + 1] = ks;
+        }
+    } else {
+        int gap = 1;
+        while (gap < len / 3) gap = 3 * gap + 1;
+        for (; gap > 0; gap /= 3) {
+            for (int i = gap; i < len; ++i) {
+                int kc = ci[i], ks = sp[i], j = i;
+                while (j >= gap && ci[j - gap] > kc) {
+                    ci[j] = ci[j - gap];
+                    sp[j] = sp[j - gap];
+                    j -= gap;
+                }
+                ci[j] = kc;
+                sp[j] = ks;
+            }
+        }
+    }
+    for (int q = b; q < e; ++q) {
+        perm[src[q]] = q;
+        if (csrval) csrval[q] = cscval[src[q]];
+    }
+}
+
+// ---------------------------------------------------------------- tiles ----
+// Tile t of system s starts at the first row of s whose nonzeros begin at or
+// after nnz_off[s] + (t - tile_off[s]) * target (the first tile at the
+// system's first row).
+__global__ void k_tile_rows(const int *ptr, int ntiles, const int *tile_sys, const int *sys_tiles,
+                            const int *sys_row_off, int target, int *tile_row, int n) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > ntiles) return;
+    if (t == ntiles) {
+        tile_row[t] = n;
+        return;
+    }
+    int s = tile_sys[t];
+    int local = t - sys_tiles[s];
+    int r0 = sys_row_off[s], r1 = sys_row_off[s + 1];
+    if (local == 0) {
+        tile_row[t] = r0;
+        return;
+    }
+    long long want = (long long)ptr[r0] + (long long)local * target;
+    int lo = r0, hi = r1;  // first row in [r0, r1] with ptr >= want
+    while (lo < hi) {
+        int mid = lo + ((hi - lo) >> 1);
+        if ((long long)ptr[mid] >= want) hi = mid; else lo = mid + 1;
+    }
+    tile_row[t] = lo;
+}
+
+// ----------------------------------------------------------- reductions ----
+template <int NRED>
+__device__ __forceinline__ void block_reduce(double (&v)[NRED], double *sh) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NRED; ++q) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+        if (lane == 0) sh[q * 4 + wave] = v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NRED; ++q) v[q] = (sh[q * 4 + 0] + sh[q * 4 + 1]) + (sh[q * 4 + 2] + sh[q * 4 + 3]);
+    }
+    __syncthreads();
+}
+
+struct Ctx {
+    SysState *st;
+    const int *sys_tiles;
+    int *counter;    // [nsys] + n_active at [nsys]
+    double *partials;
+    int nsys;
+};
+
+// Publishes this tile's partials; the last tile of the system sums all of
+// them in tile order and calls op.finish.  Returns nothing; op.finish runs on
+// thread 0 of the last tile only.
+template <int NRED, class Op>
+__device__ __forceinline__ void reduce_system(const Ctx &c, int tile, int s, double (&v)[NRED], Op &op, double *sh) {
+    block_reduce<NRED>(v, sh);
+    __shared__ int am_last;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NRED; ++q) c.partials[(size_t)tile * NRED + q] = v[q];
+        __threadfence();
+        int nt = c.sys_tiles[s + 1] - c.sys_tiles[s];
+        int prev = atomicAdd(&c.counter[s], 1);
+        am_last = (prev == nt - 1);
+    }
+    __syncthreads();
+    if (!am_last) return;
+    __threadfence();
+    const int t0 = c.sys_tiles[s], t1 = c.sys_tiles[s + 1];
+    double acc[NRED];
+#pragma unroll
+    for (int q = 0; q < NRED; ++q) acc[q] = 0.0;
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < NRED; ++q) acc[q] += load_coherent(&c.partials[(size_t)t * NRED + q]);
+    }
+    block_reduce<NRED>(acc, sh);
+    if (threadIdx.x == 0) {
+        c.counter[s] = 0;
+        op.finish(c, s, acc);
+    }
+}
+
+// ------------------------------------------------------- tile kernels ----
+// y_i = sum_p val[p] * op.gather(idx[p]) for the rows of one tile, then
+// op.row(i, y_i, acc).  HAS_MATRIX = false skips the product (streaming ops).
+template <bool HAS_MATRIX, class MT, class Op>
+__global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr, const int *__restrict__ idx,
+                                                    const MT *__restrict__ val, const int *__restrict__ tile_row,
+                                                    const int *__restrict__ tile_sys, Ctx c, Op op) {
+    constexpr int NRED = Op::NRED;
+    __shared__ double prod[HAS_MATRIX ? kLdsProducts : 1];
+    __shared__ double sh[(NRED > 0 ? NRED : 1) * 4];
+    const int tile = blockIdx.x;
+    const int s = tile_sys[tile];
+    if (!op.enabled(c.st[s])) return;
+    op.load(c.st[s]);
+    const int r0 = tile_row[tile], r1 = tile_row[tile + 1];
+    double acc[NRED > 0 ? NRED : 1];
+#pragma unroll
+    for (int q = 0; q < (NRED > 0 ? NRED : 1); ++q) acc[q] = 0.0;
+
+    if (!HAS_MATRIX) {
+        for (int i = r0 + (int)threadIdx.x; i < r1; i += kThreads) op.row(i, 0.0, acc);
+    } else {
+        const int p0 = ptr[r0], p1 = ptr[r1];
+        const int cnt = p1 - p0;
+        if (cnt <= kLdsProducts) {
+            for (int q = threadIdx.x; q < cnt; q += kThreads) {
+                int j = idx[p0 + q];
+                prod[q] = (double)val[p0 + q] * op.gather(j);
+            }
+            __syncthreads();
+            for (int i = r0 + (int)threadIdx.x; i < r1; i += kThreads) {
+                int b = ptr[i] - p0, e = ptr[i + 1] - p0;
+                double sum = 0.0;
+                for (int q = b; q < e; ++q) sum += prod[q];
+                op.row(i, sum, acc);
+            }
+        } else {
+            // long rows: the whole workgroup reduces one row at a time
+            for (int i = r0; i < r1; ++i) {
+                int b = ptr[i], e = ptr[i + 1];
+                double part[1] = {0.0};
+                for (int q = b + (int)threadIdx.x; q < e; q += kThreads) part[0] += (double)val[q] * op.gather(idx[q]);
+                block_reduce<1>(part, sh);
+                if (threadIdx.x == 0) op.row(i, part[0], acc);
+            }
+        }
+    }
+    if constexpr (NRED > 0) {
+        double v[NRED];
+#pragma unroll
+        for (int q = 0; q < NRED; ++q) v[q] = acc[q];
+        reduce_system<NRED>(c, tile, s, v, op, sh);
+    }
+}
+
+// ------------------------------------------------------------- the ops ----
+struct OpInit {  // r = b - A x0, r* = r, p = a = 0; rho = rr = (r, r); tol
+    static constexpr int NRED = 2;
+    const double *x, *b;
+    double *r, *rs, *p0, *p1, *a;
+    double rel_tol, abs_tol;
+    __device__ bool enabled(const SysState &) const { return true; }
+    __device__ void load(const SysState &) {}
+    __device__ double gather(int j) const { return x[j]; }
+    __device__ void row(int i, double ax, double (&acc)[2]) const {
+        double ri = -ax + b[i];
+        r[i] = ri;
+        rs[i] = ri;
+        p0[i] = 0.0;
+        p1[i] = 0.0;
+        a[i] = 0.0;
+        acc[0] += ri * ri;
+        acc[1] += b[i] * b[i];
+    }
+    __device__ void finish(const Ctx &c, int s, const double (&tot)[2]) const {
+        SysState &S = c.st[s];
+        S.rho = tot[0];
+        S.rr = tot[0];
+        S.err0 = sqrt(tot[0]);
+        S.tol = abs_tol > 0.0 ? abs_tol : rel_tol * sqrt(tot[1]);
+        S.alpha = 0.0;
+        S.omega = 0.0;
+        S.beta = 0.0;
+        S.iters = 0;
+        S.true_rr = 0.0;
+        if (S.err0 < S.tol) {
+            // r0 already below the threshold: the reference would divide 0/0
+            S.status = kConvergedAtStart;
+            S.active = 0;
+            atomicSub(&c.counter[c.nsys], 1);
+        }
+    }
+};
+
+template <class MT>
+struct OpMapplyP {  // K1: y = M p', p' = beta p + r - beta omega a
+    static constexpr int NRED = 0;
+    const double *r, *pin, *a;
+    double *pout, *y;
+    double beta, nbw;
+    __device__ bool enabled(const SysState &S) const { return S.active; }
+    __device__ void load(const SysState &S) {
+        beta = S.beta;
+        nbw = -S.beta * S.omega;
+    }
+    __device__ double pval(int j) const { return (beta * pin[j] + r[j]) + nbw * a[j]; }
+    __device__ double gather(int j) const { return pval(j); }
+    __device__ void row(int i, double mx, double (&)[1]) const {
+        y[i] = mx;
+        pout[i] = pval(i);
+    }
+    __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
+};
+
+struct OpSpmvAlpha {  // K2: a = A y, alpha = rho / (r*, a)
+    static constexpr int NRED = 1;
+    const double *y, *rs;
+    double *a;
+    __device__ bool enabled(const SysState &S) const { return S.active; }
+    __device__ void load(const SysState &) {}
+    __device__ double gather(int j) const { return y[j]; }
+    __device__ void row(int i, double ax, double (&acc)[1]) const {
+        a[i] = ax;
+        acc[0] += rs[i] * ax;
+    }
+    __device__ void finish(const Ctx &c, int s, const double (&tot)[1]) const {
+        SysState &S = c.st[s];
+        S.alpha = S.rho / tot[0];
+    }
+};
+
+template <class MT>
+struct OpMapplyS {  // K3: z = M s, s = r - alpha a
+    static constexpr int NRED = 0;
+    const double *r, *a;
+    double *s, *z;
+    double nalpha;
+    __device__ bool enabled(const SysState &S) const { return S.active; }
+    __device__ void load(const SysState &S) { nalpha = -S.alpha; }
+    __device__ double sval(int j) const { return r[j] + nalpha * a[j]; }
+    __device__ double gather(int j) const { return sval(j); }
+    __device__ void row(int i, double mx, double (&)[1]) const {
+        z[i] = mx;
+        s[i] = sval(i);
+    }
+    __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
+};
+
+struct OpSpmvOmega {  // K4: t = A z, omega = (t, s) / (t, t)
+    static constexpr int NRED = 2;
+    const double *z, *s;
+    double *t;
+    __device__ bool enabled(const SysState &S) const { return S.active; }
+    __device__ void load(const SysState &) {}
+    __device__ double gather(int j) const { return z[j]; }
+    __device__ void row(int i, double ax, double (&acc)[2]) const {
+        t[i] = ax;
+        acc[0] += ax * s[i];
+        acc[1] += ax * ax;
+    }
+    __device__ void finish(const Ctx &c, int s_, const double (&tot)[2]) const {
+        SysState &S = c.st[s_];
+        S.omega = tot[0] / tot[1];
+    }
+};
+
+struct OpUpdate {  // K5: x, r update; rho', beta, ||r||, stop test
+    static constexpr int NRED = 2;
+    double *x, *r;
+    const double *y, *z, *s, *t, *rs;
+    double alpha, omega;
+    __device__ bool enabled(const SysState &S) const { return S.active; }
+    __device__ void load(const SysState &S) {
+        alpha = S.alpha;
+        omega = S.omega;
+    }
+    __device__ double gather(int) const { return 0.0; }
+    __device__ void row(int i, double, double (&acc)[2]) const {
+        x[i] = (x[i] + alpha * y[i]) + omega * z[i];
+        double ri = s[i] + (-omega) * t[i];
+        r[i] = ri;
+        acc[0] += rs[i] * ri;
+        acc[1] += ri * ri;
+    }
+    __device__ void finish(const Ctx &c, int s, const double (&tot)[2]) const {
+        SysState &S = c.st[s];
+        double rho_new = tot[0];
+        S.beta = (rho_new / S.rho) * (S.alpha / S.omega);
+        S.rho = rho_new;
+        S.rr = tot[1];
+        S.iters += 1;
+        double err = sqrt(tot[1]);
+        bool stop = false;
+        if (err < S.tol) {
+            S.status = kConverged;
+            stop = true;
+        } else if (!(err == err) || !isfinite(S.beta)) {
+            S.status = kBreakdown;
+            stop = true;
+        } else if (S.iters >= S.max_iter - 1) {
+            // the reference returns MAX_ITER after MAX_ITER - 1 iterations
+            S.status = kMaxIter;
+            S.iters = S.max_iter;
+            stop = true;
+        }
+        if (stop) {
+            S.active = 0;
+            atomicSub(&c.counter[c.nsys], 1);
+        }
+    }
+};
+
+struct OpTrueResidual {  // ||b - A x||^2 per system
+    static constexpr int NRED = 1;
+    const double *x, *b;
+    __device__ bool enabled(const SysState &) const { return true; }
+    __device__ void load(const SysState &) {}
+    __device__ double gather(int j) const { return x[j]; }
+    __device__ void row(int i, double ax, double (&acc)[1]) const {
+        double d = b[i] - ax;
+        acc[0] += d * d;
+    }
+    __device__ void finish(const Ctx &c, int s, const double (&tot)[1]) const { c.st[s].true_rr = tot[0]; }
+};
+
+struct OpSpmvPlain {  // y = A x (timing of the bare product)
+    static constexpr int NRED = 0;
+    const double *x;
+    double *y;
+    __device__ bool enabled(const SysState &) const { return true; }
+    __device__ void load(const SysState &) {}
+    __device__ double gather(int j) const { return x[j]; }
+    __device__ void row(int i, double ax, double (&)[1]) const { y[i] = ax; }
+    __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
+};
+
+struct OpCopyP {  // K1 without a preconditioner: y = p'
+    static constexpr int NRED = 0;
+    const double *r, *pin, *a;
+    double *pout, *y;
+    double beta, nbw;
+    __device__ bool enabled(const SysState &S) const { return S.active; }
+    __device__ void load(const SysState &S) {
+        beta = S.beta;
+        nbw = -S.beta * S.omega;
+    }
+    __device__ double gather(int) const { return 0.0; }
+    __device__ void row(int i, double, double (&)[1]) const {
+        double v = (beta * pin[i] + r[i]) + nbw * a[i];
+        pout[i] = v;
+        y[i] = v;
+    }
+    __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
+};
+
+struct OpCopyS {  // K3 without a preconditioner: z = s
+    static constexpr int NRED = 0;
+    const double *r, *a;
+    double *s, *z;
+    double nalpha;
+    __device__ bool enabled(const SysState &S) const { return S.active; }
+    __device__ void load(const SysState &S) { nalpha = -S.alpha; }
+    __device__ double gather(int) const { return 0.0; }
+    __device__ void row(int i, double, double (&)[1]) const {
+        double v = r[i] + nalpha * a[i];
+        s[i] = v;
+        z[i] = v;
+    }
+    __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
+};
+
+__global__ void k_state_init(SysState *st, int nsys, int max_iter, int *counter) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsys) {
+        SysState z{};
+        z.active = 1;
+        z.status = kRunning;
+        z.max_iter = max_iter;
+        st[s] = z;
+        counter[s] = 0;
+    }
+    if (s == 0) counter[nsys] = nsys;
+}
+
+__global__ void k_adjoint_scatter(int n, const int *colptr, const int *rowidx, const double *lambda, const double *x,
+                                  double *dA) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    double xc = x[c];
+    for (int p = colptr[c]; p < colptr[c + 1]; ++p) dA[p] = -lambda[rowidx[p]] * xc;
+}
+
+__global__ void k_f32_to_f64(const float *in, double *out, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (double)in[i];
+}
+
+template <bool HM, class MT, class Op>
+void launch_tiles(const OperatorView &O, const TileMap &tm, const Ctx &c, const Op &op, hipStream_t st) {
+    hipLaunchKernelGGL((k_tiles<HM, MT, Op>), dim3(tm.ntiles), dim3(kThreads), 0, st, O.ptr, O.idx,
+                       (const MT *)O.val, tm.row.get(), tm.sys.get(), c, op);
+}
+
+}  // namespace
+
+// ================================================================ host ====
+void csc_to_csr_device(int n, int nnz, const int *colptr, const int *rowidx, const double *cscval, int *rowptr,
+                       int *colidx, double *csrval, int *perm, hipStream_t st) {
+    require(n >= 0 && nnz >= 0, "csc_to_csr: negative size");
+    DeviceArray<int> cnt((size_t)n + 1), src((size_t)std::max(nnz, 1)), tmp;
+    const int B = 256;
+    hipLaunchKernelGGL(k_fill_int, dim3(std::max(1, ceil_div(n + 1, B))), dim3(B), 0, st, cnt.get(), 0,
+                       (long long)n + 1);
+    if (nnz > 0) hipLaunchKernelGGL(k_count_rows, dim3(ceil_div(nnz, B)), dim3(B), 0, st, rowidx, nnz, cnt.get());
+    exclusive_scan(cnt.get(), rowptr, n, tmp, st);
+    hipLaunchKernelGGL(k_fill_int, dim3(std::max(1, ceil_div(n + 1, B))), dim3(B), 0, st, cnt.get(), 0,
+                       (long long)n + 1);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_fill_rows, dim3(ceil_div(n, B)), dim3(B), 0, st, colptr, rowidx, n, rowptr, cnt.get(),
+                           colidx, src.get());
+        hipLaunchKernelGGL(k_sort_rows, dim3(ceil_div(n, B)), dim3(B), 0, st, n, rowptr, colidx, src.get(), cscval,
+                           csrval, perm);
+    }
+    SPAI_CHECK_LAUNCH();
+    SPAI_HIP(hipStreamSynchronize(st));  // temporaries are freed on return
+}
+
+void build_tiles(TileMap &tm, const int *ptr, const std::vector<int> &h_row_off,
+                 const std::vector<long long> &h_nnz_off, hipStream_t st) {
+    const int nsys = (int)h_row_off.size() - 1;
+    require(nsys >= 1, "build_tiles: no systems");
+    long long total = h_nnz_off[(size_t)nsys];
+    // aim for >= ~2048 tiles on large inputs, 256..2048 nonzeros each
+    long long t = total / 2048;
+    int target = 256;
+    while (target < 2048 && target < t) target <<= 1;
+    tm.target_nnz = target;
+    tm.h_sys_tiles.assign((size_t)nsys + 1, 0);
+    for (int s = 0; s < nsys; ++s) {
+        long long nz = h_nnz_off[(size_t)s + 1] - h_nnz_off[(size_t)s];
+        int rows = h_row_off[(size_t)s + 1] - h_row_off[(size_t)s];
+        int k = (int)std::max<long long>(1, (nz + target - 1) / target);
+        if (rows == 0) k = 1;
+        tm.h_sys_tiles[(size_t)s + 1] = tm.h_sys_tiles[(size_t)s] + k;
+    }
+    tm.ntiles = tm.h_sys_tiles[(size_t)nsys];
+    std::vector<int> hsys((size_t)tm.ntiles);
+    for (int s = 0; s < nsys; ++s)
+        for (int i = tm.h_sys_tiles[(size_t)s]; i < tm.h_sys_tiles[(size_t)s + 1]; ++i) hsys[(size_t)i] = s;
+    tm.row.resize((size_t)tm.ntiles + 1);
+    tm.sys.resize((size_t)tm.ntiles);
+    tm.sys_tiles.resize((size_t)nsys + 1);
+    DeviceArray<int> roff((size_t)nsys + 1);
+    SPAI_HIP(hipMemcpyAsync(tm.sys.get(), hsys.data(), sizeof(int) * hsys.size(), hipMemcpyHostToDevice, st));
+    SPAI_HIP(hipMemcpyAsync(tm.sys_tiles.get(), tm.h_sys_tiles.data(), sizeof(int) * (nsys + 1),
+                            hipMemcpyHostToDevice, st));
+    SPAI_HIP(hipMemcpyAsync(roff.get(), h_row_off.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_tile_rows, dim3(ceil_div(tm.ntiles + 1, 256)), dim3(256), 0, st, ptr, tm.ntiles,
+                       tm.sys.get(), tm.sys_tiles.get(), roff.get(), target, tm.row.get(),
+                       h_row_off[(size_t)nsys]);
+    SPAI_CHECK_LAUNCH();
+    SPAI_HIP(hipStreamSynchronize(st));
+}
+
+void adjoint_scatter(int n, const int *colptr, const int *rowidx, const double *lambda, const double *x, double *dA,
+                     hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_adjoint_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, colptr, rowidx, lambda, x,
+                       dA);
+    SPAI_CHECK_LAUNCH();
+}
+
+void convert_f32_to_f64(const float *in, double *out, long long n, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_f32_to_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, out, n);
+    SPAI_CHECK_LAUNCH();
+}
+
+BiCGStab::BiCGStab() = default;
+BiCGStab::~BiCGStab() { drop_graph(); }
+
+void BiCGStab::drop_graph() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+    for (auto &k : graph_key_) k = nullptr;
+    graph_iters_ = 0;
+}
+
+void BiCGStab::ensure(int n, int nsys, int ntiles) {
+    if (n != n_) {
+        drop_graph();
+        vec_.resize((size_t)9 * std::max(n, 1));
+        double *v = vec_.get();
+        size_t N = (size_t)std::max(n, 1);
+        r_ = v;
+        rs_ = v + N;
+        p_[0] = v + 2 * N;
+        p_[1] = v + 3 * N;
+        y_ = v + 4 * N;
+        a_ = v + 5 * N;
+        s_ = v + 6 * N;
+        z_ = v + 7 * N;
+        t_ = v + 8 * N;
+        n_ = n;
+    }
+    if (nsys != nsys_) {
+        drop_graph();
+        state_.resize((size_t)nsys);
+        counter_.resize((size_t)nsys + 1);
+        nsys_ = nsys;
+    }
+    if (ntiles != ntiles_) {
+        drop_graph();
+        partials_.resize((size_t)2 * std::max(ntiles, 1));
+        ntiles_ = ntiles;
+    }
+}
+
+void BiCGStab::enqueue_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tm, int parity,
+                                 hipStream_t st) {
+    Ctx c{state_.get(), tm.sys_tiles.get(), counter_.get(), partials_.get(), nsys_};
+    double *pin = p_[parity], *pout = p_[parity ^ 1];
+    if (M.val) {
+        if (M.fp32) {
+            OpMapplyP<float> op{r_, pin, a_, pout, y_, 0.0, 0.0};
+            launch_tiles<true, float>(M, tm, c, op, st);
+        } else {
+            OpMapplyP<double> op{r_, pin, a_, pout, y_, 0.0, 0.0};
+            launch_tiles<true, double>(M, tm, c, op, st);
+        }
+    } else {
+        // no preconditioner: y = p' through the streaming path
+        OpCopyP op{r_, pin, a_, pout, y_, 0.0, 0.0};
+        launch_tiles<false, double>(A, tm, c, op, st);
+    }
+    {
+        OpSpmvAlpha op{y_, rs_, a_};
+        launch_tiles<true, double>(A, tm, c, op, st);
+    }
+    if (M.val) {
+        if (M.fp32) {
+            OpMapplyS<float> op{r_, a_, s_, z_, 0.0};
+            launch_tiles<true, float>(M, tm, c, op, st);
+        } else {
+            OpMapplyS<double> op{r_, a_, s_, z_, 0.0};
+            launch_tiles<true, double>(M, tm, c, op, st);
+        }
+    } else {
+        OpCopyS op{r_, a_, s_, z_, 0.0};
+        launch_tiles<false, double>(A, tm, c, op, st);
+    }
+    {
+        OpSpmvOmega op{z_, s_, t_};
+        launch_tiles<true, double>(A, tm, c, op, st);
+    }
+    {
+        OpUpdate op{x_, r_, y_, z_, s_, t_, rs_, 0.0, 0.0};
+        launch_tiles<false, double>(A, tm, c, op, st);
+    }
+    SPAI_CHECK_LAUNCH();
+}
+
+void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap &tm, int n, int nsys,
+                     const double *b, double *x, const SolveOptions &opt, hipStream_t st, SolveResult *res) {
+    require(n > 0 && nsys >= 1 && A.ptr && A.idx && A.val && b && x, "bicgstab: bad arguments");
+    require(tm.ntiles > 0 && (int)tm.h_sys_tiles.size() == nsys + 1, "bicgstab: tiles do not match the systems");
+    require(opt.max_iter >= 1, "bicgstab: max_iter must be >= 1");
+    ensure(n, nsys, tm.ntiles);
+    b_ = b;
+    x_ = x;
+    hipEvent_t e0, e1;
+    SPAI_HIP(hipEventCreate(&e0));
+    SPAI_HIP(hipEventCreate(&e1));
+    SPAI_HIP(hipEventRecord(e0, st));
+    Ctx c{state_.get(), tm.sys_tiles.get(), counter_.get(), partials_.get(), nsys};
+    hipLaunchKernelGGL(k_state_init, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys, opt.max_iter,
+                       counter_.get());
+    {
+        OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, opt.rel_tol, opt.abs_tol};
+        launch_tiles<true, double>(A, tm, c, op, st);
+    }
+    SPAI_CHECK_LAUNCH();
+    int active = nsys;
+    int checks = 0;
+    int *h_active = nullptr;
+    SPAI_HIP(hipHostMalloc((void **)&h_active, sizeof(int), hipHostMallocDefault));
+    auto read_active = [&]() {
+        SPAI_HIP(hipMemcpyAsync(h_active, counter_.get() + nsys, sizeof(int), hipMemcpyDeviceToHost, st));
+        SPAI_HIP(hipStreamSynchronize(st));
+        ++checks;
+        return *h_active;
+    };
+    const int max_launch_iters = opt.max_iter;  // every system stops by then
+    int done_iters = 0;
+    try {
+        active = read_active();
+        if (opt.use_graph) {
+            int gi = std::max(2, opt.graph_iters + (opt.graph_iters & 1));
+            const void *key[6] = {A.val, M.val, tm.row.get(), b, x, (const void *)(size_t)M.fp32};
+            bool same = graph_exec_ && gi == graph_iters_;
+            for (int q = 0; q < 6 && same; ++q) same = key[q] == graph_key_[q];
+            if (!same) {
+                drop_graph();
+                hipStream_t cap;
+                SPAI_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+                SPAI_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+                for (int k = 0; k < gi; ++k) enqueue_iteration(A, M, tm, k & 1, cap);
+                SPAI_HIP(hipStreamEndCapture(cap, &graph_));
+                SPAI_HIP(hipStreamDestroy(cap));
+                SPAI_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+                graph_iters_ = gi;
+                for (int q = 0; q < 6; ++q) graph_key_[q] = key[q];
+            }
+            while (active > 0 && done_iters < max_launch_iters) {
+                SPAI_HIP(hipGraphLaunch(graph_exec_, st));
+                done_iters += gi;
+                active = read_active();
+            }
+        } else {
+            int every = std::max(1, opt.check_every);
+            int parity = 0;
+            while (active > 0 && done_iters < max_launch_iters) {
+                for (int k = 0; k < every; ++k) {
+                    enqueue_iteration(A, M, tm, parity, st);
+                    parity ^= 1;
+                }
+                done_iters += every;
+                active = read_active();
+            }
+        }
+        {
+            OpTrueResidual op{x, b};
+            launch_tiles<true, double>(A, tm, c, op, st);
+            SPAI_CHECK_LAUNCH();
+        }
+        SPAI_HIP(hipEventRecord(e1, st));
+        SPAI_HIP(hipEventSynchronize(e1));
+    } catch (...) {
+        (void)hipHostFree(h_active);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipHostFree(h_active);
+    float ms = 0.0f;
+    SPAI_HIP(hipEventElapsedTime(&ms, e0, e1));
+    SPAI_HIP(hipEventDestroy(e0));
+    SPAI_HIP(hipEventDestroy(e1));
+    h_states_.resize((size_t)nsys);
+    SPAI_HIP(hipMemcpy(h_states_.data(), state_.get(), sizeof(SysState) * nsys, hipMemcpyDeviceToHost));
+    if (res) {
+        res->ms = ms;
+        res->host_checks = checks;
+        res->iterations_max = 0;
+        res->iterations_sum = 0;
+        res->all_converged = 1;
+        for (auto &S : h_states_) {
+            res->iterations_max = std::max(res->iterations_max, S.iters);
+            res->iterations_sum += S.iters;
+            if (!(S.status == kConverged || S.status == kConvergedAtStart)) res->all_converged = 0;
+        }
+    }
+}
+
+float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView &M, const TileMap &tm, int n,
+                            int nsys, int reps, hipStream_t st) {
+    ensure(n, nsys, tm.ntiles);
+    Ctx c{state_.get(), tm.sys_tiles.get(), counter_.get(), partials_.get(), nsys};
+    hipEvent_t e0, e1;
+    SPAI_HIP(hipEventCreate(&e0));
+    SPAI_HIP(hipEventCreate(&e1));
+    auto once = [&]() {
+        if (which == 0) {
+            OpSpmvPlain op{p_[0], y_};
+            launch_tiles<true, double>(A, tm, c, op, st);
+        } else if (M.fp32) {
+            OpSpmvPlain op{p_[0], y_};
+            launch_tiles<true, float>(M, tm, c, op, st);
+        } else {
+            OpSpmvPlain op{p_[0], y_};
+            launch_tiles<true, double>(M, tm, c, op, st);
+        }
+    };
+    once();
+    SPAI_HIP(hipEventRecord(e0, st));
+    for (int i = 0; i < reps; ++i) once();
+    SPAI_HIP(hipEventRecord(e1, st));
+    SPAI_HIP(hipEventSynchronize(e1));
+    SPAI_CHECK_LAUNCH();
+    float ms = 0.0f;
+    SPAI_HIP(hipEventElapsedTime(&ms, e0, e1));
+    SPAI_HIP(hipEventDestroy(e0));
+    SPAI_HIP(hipEventDestroy(e1));
+    return ms / std::max(1, reps);
+}
+
+}  // namespace spai

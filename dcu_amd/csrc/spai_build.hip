@@ -1,0 +1,873 @@
+// Static SPAI construction on gfx950: one least-squares problem per column,
+//
+//     min || A(I_k, J_k) m_k - e_k(I_k) ||_2,   pattern(M) = pattern(A),
+//
+// the semantics of StaticSPAIv20 (src/spai_single_dcu.cpp:1224-2687) computed
+// in one pass per column instead of the reference's seven passes over padded
+// n*n1max*n2max workspaces:
+//
+//   gather   J_k = A(:,k)'s rows, then the (row, source) pairs of A(:,J_k)
+//            straight from CSC into registers (no HBM tile);
+//   I_k      a bitonic sort of the gathered rows across the lanes of the
+//            column's group, duplicates collapsed by a ballot prefix count,
+//            giving I_k sorted ascending (computeIShared_Symbol_SpMMv1,
+//            src/spai_single_dcu.cpp:510-615) and E_k = position of k
+//            (ComputeTildeE_SSPAIv10, :973-992);
+//   A(I,J)   scattered once into LDS, each lane then owns rows of the tile;
+//   QR       Householder in registers (the reference uses MGS,
+//            QR_RShared_SSPAIv10 :1057-1121; both give the same least-squares
+//            solution, checked to 1e-10 against the MGS oracle);
+//   solve    Q^T e_E and back substitution (Sol_SSPAIv10 :1016-1055), x = 0
+//            when E_k = -1;
+//   store    M's values at A's CSC positions (modifyData4_SSPAIv10
+//            :1123-1140) and at the matching CSR positions for the M-apply.
+//
+// Mixed precision: the Householder QR runs in FP32 and is followed by one
+// FP64 corrected-semi-normal-equations sweep (r = e - A x, g = A^T r,
+// R^T R d = g, x += d) before M is stored in FP32.
+//
+// Scheduling: the planner classifies every column by (T_k = entries gathered,
+// n2_k = |J_k|) into size tiers.  balanced_columns runs every column through
+// the one tier that fits the largest column, in contiguous column blocks per
+// wavefront; balanced_nnz runs each tier's column list through its own
+// kernel with an atomic work queue (largest tier first).
+#include "device_common.hpp"
+#include "spai_build.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace spai {
+
+namespace {
+
+constexpr int kBlock = 256;  // threads per workgroup for the column kernels
+
+// Per-tier group shapes: W lanes per column, EPL gathered entries (and tile
+// rows) per lane, NJ = largest |J_k| the tier holds in registers.
+template <int W_, int EPL_, int NJ_>
+struct RegTier {
+    static constexpr int W = W_;
+    static constexpr int EPL = EPL_;
+    static constexpr int NJ = NJ_;
+    static constexpr int ROWS = W * EPL;     // largest |I_k|
+    static constexpr int TMAX = W * EPL;     // largest T_k
+    static constexpr int TS = NJ + 1;        // LDS tile row stride (doubles)
+    static constexpr int GROUPS = kBlock / W;
+    static constexpr int LDS_DOUBLES = ROWS * TS + NJ * NJ + NJ;
+};
+
+using Tier0 = RegTier<16, 2, 8>;
+using Tier1 = RegTier<32, 2, 8>;
+using Tier2 = RegTier<64, 2, 16>;
+
+// The LDS tier: one wavefront per column, tile in LDS, FP64 arithmetic.
+constexpr int kLdsTierT = 1024;        // largest T_k
+constexpr int kLdsTierEPL = kLdsTierT / 64;
+constexpr int kLdsTierNJ = 64;         // largest |J_k|
+constexpr int kLdsTierDoubles = 7168;  // tile + R + y (56 KB)
+
+struct ColumnArgs {
+    const int *colptr;
+    const int *rowidx;
+    const double *val;
+    const int *perm;  // CSC position -> CSR position
+    void *m_csc;      // MT[nnz]
+    void *m_csr;      // MT[nnz]
+    int *n1_out;      // optional per-column |I_k|
+    // scheduling
+    const int *list;  // dynamic: columns of this tier; static: nullptr
+    const int *count; // dynamic: number of columns in list
+    int *head;        // dynamic: queue head
+    int col_begin, col_end;  // static range
+};
+
+// 64-bit shuffle helpers on top of the 32-bit cross-lane permute.
+template <int W>
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+    unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+    lo = __shfl_xor(lo, m, W);
+    hi = __shfl_xor(hi, m, W);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Locate, for gathered entry e (0 <= e < T), the J index j with
+// off_j <= e < off_j + c_j.  off is the exclusive prefix held by lane j;
+// the search runs the same number of steps on every lane.
+template <int W, int NJ>
+__device__ __forceinline__ int locate_j(int e, int off, int n2) {
+    int lo = 0;
+#pragma unroll
+    for (int step = NJ / 2; step >= 1; step >>= 1) {
+        int cand = lo + step;
+        int oc = __shfl(off, cand < W ? cand : W - 1, W);
+        if (cand < n2 && oc <= e) lo = cand;
+    }
+    return lo;
+}
+
+// Bitonic sort (ascending) of W*EPL 64-bit keys, lane l holding elements
+// l*EPL .. l*EPL+EPL-1.
+template <int W, int EPL>
+__device__ __forceinline__ void bitonic_sort(uint64_t (&key)[EPL], int lane) {
+    constexpr int N = W * EPL;
+#pragma unroll
+    for (int size = 2; size <= N; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            if (stride >= EPL) {
+                const int ls = stride / EPL;
+#pragma unroll
+                for (int i = 0; i < EPL; ++i) {
+                    uint64_t pk = shfl_xor_u64<W>(key[i], ls);
+                    int e = lane * EPL + i;
+                    bool up = (e & size) == 0;
+                    bool lower = (e & stride) == 0;
+                    bool take_min = (lower == up);
+                    uint64_t mn = pk < key[i] ? pk : key[i];
+                    uint64_t mx = pk < key[i] ? key[i] : pk;
+                    key[i] = take_min ? mn : mx;
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < EPL; ++i) {
+                    if ((i & stride) == 0) {
+                        const int i2 = i | stride;
+                        int e = lane * EPL + i;
+                        bool up = (e & size) == 0;
+                        uint64_t a = key[i], b = key[i2];
+                        bool sw = up ? (a > b) : (a < b);
+                        key[i] = sw ? b : a;
+                        key[i2] = sw ? a : b;
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <class MT>
+__device__ __forceinline__ void store_m(const ColumnArgs &a, int pos, double x) {
+    MT *csc = (MT *)a.m_csc;
+    MT *csr = (MT *)a.m_csr;
+    csc[pos] = (MT)x;
+    csr[a.perm[pos]] = (MT)x;
+}
+
+// ---------------------------------------------------------------------------
+// Register tier: one group of W lanes per column.
+// ---------------------------------------------------------------------------
+template <class TR, class QT, bool REFINE, class MT>
+__device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *lds) {
+    constexpr int W = TR::W, EPL = TR::EPL, NJ = TR::NJ, TS = TR::TS;
+    constexpr int RPL = EPL;
+    double *tile = lds;                       // ROWS x TS
+    double *Rs = lds + TR::ROWS * TS;         // NJ x NJ
+    double *ys = Rs + NJ * NJ;                // NJ
+
+    const int c0 = a.colptr[k];
+    const int n2 = a.colptr[k + 1] - c0;
+    if (n2 <= 0) return;
+
+    // J_k and the extent of each column A(:, J_k[j])
+    int cs = 0, cl = 0;
+    if (lane < n2) {
+        int jr = a.rowidx[c0 + lane];
+        cs = a.colptr[jr];
+        cl = a.colptr[jr + 1] - cs;
+    }
+    const int incl = group_inclusive_scan<W>(cl, lane);
+    const int T = __shfl(incl, W - 1, W);
+    const int off = incl - cl;
+
+    // gather (row, entry) keys, blocked layout e = lane*EPL + i
+    uint64_t key[EPL];
+#pragma unroll
+    for (int i = 0; i < EPL; ++i) {
+        int e = lane * EPL + i;
+        int j = locate_j<W, NJ>(e, off, n2);
+        int src = __shfl(cs, j, W) + (e - __shfl(off, j, W));
+        key[i] = ~0ull;
+        if (e < T) key[i] = ((uint64_t)(unsigned)a.rowidx[src] << 32) | (unsigned)e;
+    }
+    bitonic_sort<W, EPL>(key, lane);
+
+    // collapse duplicate rows: pos = (#first occurrences up to here) - 1
+    int rowv[EPL];
+    bool first[EPL];
+    int cnt = 0;
+    int prev = __shfl_up((int)(key[EPL - 1] >> 32), 1, W);
+    if (lane == 0) prev = -1;
+#pragma unroll
+    for (int i = 0; i < EPL; ++i) {
+        bool valid = key[i] != ~0ull;
+        rowv[i] = (int)(key[i] >> 32);
+        first[i] = valid && (rowv[i] != prev);
+        prev = rowv[i];
+        cnt += first[i] ? 1 : 0;
+    }
+    const int cincl = group_inclusive_scan<W>(cnt, lane);
+    const int n1 = __shfl(cincl, W - 1, W);
+    int run = cincl - cnt;
+    int pos[EPL];
+    int e_pos = -1;
+#pragma unroll
+    for (int i = 0; i < EPL; ++i) {
+        run += first[i] ? 1 : 0;
+        pos[i] = run - 1;
+        if (key[i] != ~0ull && rowv[i] == k) e_pos = pos[i];
+    }
+    const int E = group_max<W>(e_pos);
+
+    // zero the rows this lane owns, then scatter A(I_k, J_k) into LDS
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+        int rho = lane + W * r;
+        if (rho < n1) {
+#pragma unroll
+            for (int c = 0; c < NJ; ++c) tile[rho * TS + c] = 0.0;
+        }
+    }
+    wave_lds_fence();
+#pragma unroll
+    for (int i = 0; i < EPL; ++i) {
+        // shuffles run on every lane of the group; only the store is masked
+        const bool valid = key[i] != ~0ull;
+        const int e = valid ? (int)(unsigned)key[i] : 0;
+        const int j = locate_j<W, NJ>(e, off, n2);
+        const int src = __shfl(cs, j, W) + (e - __shfl(off, j, W));
+        if (valid) tile[pos[i] * TS + j] = a.val[src];
+    }
+    wave_lds_fence();
+
+    // rows rho = lane + W*r into registers
+    QT A[RPL][NJ];
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+        int rho = lane + W * r;
+#pragma unroll
+        for (int c = 0; c < NJ; ++c) A[r][c] = (rho < n1 && c < n2) ? (QT)tile[rho * TS + c] : (QT)0;
+    }
+
+    // Householder QR: H_j = I - tau_j v_j v_j^T, v_j[j] = 1 (v below the
+    // diagonal kept in A), R on and above the diagonal.
+    QT tau[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        tau[j] = 0;
+        if (j < n2) {
+            QT alpha = __shfl(A[j / W][j], j % W, W);
+            QT sl = 0;
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                int rho = lane + W * r;
+                sl += (rho > j) ? A[r][j] * A[r][j] : (QT)0;
+            }
+            QT sigma = group_sum<W>(sl);
+            QT beta = alpha, t = 0, scale = 0;
+            if (sigma != (QT)0) {
+                QT nrm = sqrt(alpha * alpha + sigma);
+                beta = alpha >= (QT)0 ? -nrm : nrm;
+                t = (beta - alpha) / beta;
+                scale = (QT)1 / (alpha - beta);
+            }
+            tau[j] = t;
+            QT v[RPL];
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                int rho = lane + W * r;
+                v[r] = (rho > j) ? A[r][j] * scale : (rho == j ? (QT)1 : (QT)0);
+                if (rho > j) A[r][j] = v[r];
+                if (rho == j) A[r][j] = beta;
+            }
+            QT w[NJ];
+#pragma unroll
+            for (int c = j + 1; c < NJ; ++c) {
+                QT s = 0;
+#pragma unroll
+                for (int r = 0; r < RPL; ++r) s += v[r] * A[r][c];
+                w[c] = s;
+            }
+#pragma unroll
+            for (int o = W / 2; o > 0; o >>= 1) {
+#pragma unroll
+                for (int c = j + 1; c < NJ; ++c) w[c] += __shfl_xor(w[c], o, W);
+            }
+#pragma unroll
+            for (int c = j + 1; c < NJ; ++c) {
+                if (c < n2) {
+                    QT wc = w[c] * t;
+#pragma unroll
+                    for (int r = 0; r < RPL; ++r) A[r][c] -= wc * v[r];
+                }
+            }
+        }
+    }
+
+    // y = Q^T e_E
+    QT y[RPL];
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) y[r] = (lane + W * r == E) ? (QT)1 : (QT)0;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        if (j < n2) {
+            QT s = 0;
+            QT v[RPL];
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                int rho = lane + W * r;
+                v[r] = (rho > j) ? A[r][j] : (rho == j ? (QT)1 : (QT)0);
+                s += v[r] * y[r];
+            }
+            s = group_sum<W>(s) * tau[j];
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) y[r] -= s * v[r];
+        }
+    }
+
+    // R and y to LDS (rows < n2 only), then a uniform back substitution
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+        int rho = lane + W * r;
+        if (rho < n2) {
+#pragma unroll
+            for (int c = 0; c < NJ; ++c) Rs[rho * NJ + c] = (double)A[r][c];
+            ys[rho] = (double)y[r];
+        }
+    }
+    wave_lds_fence();
+    double x[NJ];
+#pragma unroll
+    for (int c = NJ - 1; c >= 0; --c) {
+        x[c] = 0.0;
+        if (c < n2) {
+            double t = ys[c];
+#pragma unroll
+            for (int c2 = c + 1; c2 < NJ; ++c2)
+                if (c2 < n2) t -= Rs[c * NJ + c2] * x[c2];
+            x[c] = t / Rs[c * NJ + c];
+        }
+    }
+    if (E < 0) {
+#pragma unroll
+        for (int c = 0; c < NJ; ++c) x[c] = 0.0;
+    }
+
+    if (REFINE && E >= 0) {
+        // one FP64 corrected-semi-normal-equations sweep with the FP64 tile
+        double res[RPL];
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            int rho = lane + W * r;
+            double t = (rho == E) ? 1.0 : 0.0;
+            if (rho < n1) {
+#pragma unroll
+                for (int c = 0; c < NJ; ++c)
+                    if (c < n2) t -= tile[rho * TS + c] * x[c];
+            }
+            res[r] = (rho < n1) ? t : 0.0;
+        }
+        double g[NJ];
+#pragma unroll
+        for (int c = 0; c < NJ; ++c) {
+            double s = 0.0;
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                int rho = lane + W * r;
+                if (rho < n1 && c < n2) s += tile[rho * TS + c] * res[r];
+            }
+            g[c] = s;
+        }
+#pragma unroll
+        for (int o = W / 2; o > 0; o >>= 1) {
+#pragma unroll
+            for (int c = 0; c < NJ; ++c) g[c] += __shfl_xor(g[c], o, W);
+        }
+        // R^T u = g (forward), R d = u (backward), x += d
+#pragma unroll
+        for (int c = 0; c < NJ; ++c) {
+            if (c < n2) {
+                double t = g[c];
+#pragma unroll
+                for (int c2 = 0; c2 < c; ++c2) t -= Rs[c2 * NJ + c] * g[c2];
+                g[c] = t / Rs[c * NJ + c];
+            }
+        }
+#pragma unroll
+        for (int c = NJ - 1; c >= 0; --c) {
+            if (c < n2) {
+                double t = g[c];
+#pragma unroll
+                for (int c2 = c + 1; c2 < NJ; ++c2)
+                    if (c2 < n2) t -= Rs[c * NJ + c2] * g[c2];
+                g[c] = t / Rs[c * NJ + c];
+                x[c] += g[c];
+            }
+        }
+    }
+
+    // lane c writes x[c]
+    double mine = 0.0;
+#pragma unroll
+    for (int c = 0; c < NJ; ++c)
+        if (lane == c) mine = x[c];
+    if (lane < n2) store_m<MT>(a, c0 + lane, mine);
+    if (a.n1_out && lane == 0) a.n1_out[k] = n1;
+    wave_lds_fence();  // LDS reused by this group's next column
+}
+
+template <class TR, class QT, bool REFINE, class MT, bool DYNAMIC>
+__global__ __launch_bounds__(kBlock) void spai_reg_tier_kernel(ColumnArgs a) {
+    constexpr int W = TR::W;
+    constexpr int G = 64 / W;    // groups per wavefront
+    constexpr int CH = 4;        // columns per group per queue grab
+    __shared__ double lds_all[TR::GROUPS * TR::LDS_DOUBLES];
+    const int tid = threadIdx.x;
+    const int lane = tid % W;
+    const int grp = tid / W;
+    const int wgrp = (tid % 64) / W;  // group index inside the wavefront
+    double *lds = lds_all + grp * TR::LDS_DOUBLES;
+
+    if (DYNAMIC) {
+        const int count = *a.count;
+        for (;;) {
+            int base = 0;
+            if ((tid & 63) == 0) base = atomicAdd(a.head, G * CH);
+            base = __shfl(base, 0, 64);
+            if (base >= count) break;
+            for (int it = 0; it < CH; ++it) {
+                int idx = base + it * G + wgrp;
+                if (idx < count) solve_column_reg<TR, QT, REFINE, MT>(a, a.list[idx], lane, lds);
+            }
+        }
+    } else {
+        const int nwaves = gridDim.x * (kBlock / 64);
+        const int wave = blockIdx.x * (kBlock / 64) + tid / 64;
+        const int ncols = a.col_end - a.col_begin;
+        const int chunk = (ncols + nwaves - 1) / nwaves;
+        const int start = a.col_begin + wave * chunk;
+        const int stop = min(start + chunk, a.col_end);
+        for (int base = start; base < stop; base += G) {
+            int k = base + wgrp;
+            if (k < stop) solve_column_reg<TR, QT, REFINE, MT>(a, k, lane, lds);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LDS tier: one wavefront per column, tile n1 x n2 (row-major) in LDS, FP64.
+// ---------------------------------------------------------------------------
+template <class MT>
+__device__ void solve_column_lds(const ColumnArgs &a, int k, int lane, double *lds, int *ilds) {
+    constexpr int W = 64, EPL = kLdsTierEPL, NJ = kLdsTierNJ;
+    const int c0 = a.colptr[k];
+    const int n2 = a.colptr[k + 1] - c0;
+    if (n2 <= 0) return;
+    int *offs = ilds;        // NJ
+    int *starts = ilds + NJ; // NJ
+    int cs = 0, cl = 0;
+    if (lane < n2) {
+        int jr = a.rowidx[c0 + lane];
+        cs = a.colptr[jr];
+        cl = a.colptr[jr + 1] - cs;
+    }
+    const int incl = group_inclusive_scan<W>(cl, lane);
+    const int T = __shfl(incl, W - 1, W);
+    const int off = incl - cl;
+    if (lane < n2) {
+        offs[lane] = off;
+        starts[lane] = cs;
+    }
+    wave_lds_fence();
+    auto find_j = [&](int e) {
+        int lo = 0, hi = n2 - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (offs[mid] <= e) lo = mid; else hi = mid - 1;
+        }
+        return lo;
+    };
+    uint64_t key[EPL];
+#pragma unroll
+    for (int i = 0; i < EPL; ++i) {
+        int e = lane * EPL + i;
+        key[i] = ~0ull;
+        if (e < T) {
+            int j = find_j(e);
+            key[i] = ((uint64_t)(unsigned)a.rowidx[starts[j] + e - offs[j]] << 32) | (unsigned)e;
+        }
+    }
+    bitonic_sort<W, EPL>(key, lane);
+    int cnt = 0;
+    int prev = __shfl_up((int)(key[EPL - 1] >> 32), 1, W);
+    if (lane == 0) prev = -1;
+    bool first[EPL];
+#pragma unroll
+    for (int i = 0; i < EPL; ++i) {
+        int rv = (int)(key[i] >> 32);
+        first[i] = key[i] != ~0ull && rv != prev;
+        prev = rv;
+        cnt += first[i] ? 1 : 0;
+    }
+    const int cincl = group_inclusive_scan<W>(cnt, lane);
+    const int n1 = __shfl(cincl, W - 1, W);
+    double *tile = lds;                       // n1 x n2
+    double *Rs = lds + (size_t)n1 * n2;       // n2 x n2
+    double *ys = Rs + (size_t)n2 * n2;        // n2
+    for (int idx = lane; idx < n1 * n2; idx += W) tile[idx] = 0.0;
+    wave_lds_fence();
+    int run = cincl - cnt, e_pos = -1;
+#pragma unroll
+    for (int i = 0; i < EPL; ++i) {
+        run += first[i] ? 1 : 0;
+        if (key[i] != ~0ull) {
+            int e = (int)(unsigned)key[i];
+            int j = find_j(e);
+            int src = starts[j] + e - offs[j];
+            tile[(run - 1) * n2 + j] = a.val[src];
+            if ((int)(key[i] >> 32) == k) e_pos = run - 1;
+        }
+    }
+    const int E = group_max<W>(e_pos);
+    wave_lds_fence();
+    // Householder on the LDS tile; lane owns rows lane, lane+64, ...
+    double taus_last = 0.0;
+    for (int j = 0; j < n2; ++j) {
+        double alpha = tile[j * n2 + j];
+        double sl = 0.0;
+        for (int rho = lane; rho < n1; rho += W)
+            if (rho > j) sl += tile[rho * n2 + j] * tile[rho * n2 + j];
+        double sigma = group_sum<W>(sl);
+        double beta = alpha, t = 0.0, scale = 0.0;
+        if (sigma != 0.0) {
+            double nrm = sqrt(alpha * alpha + sigma);
+            beta = alpha >= 0.0 ? -nrm : nrm;
+            t = (beta - alpha) / beta;
+            scale = 1.0 / (alpha - beta);
+        }
+        wave_lds_fence();
+        for (int rho = lane; rho < n1; rho += W) {
+            if (rho > j) tile[rho * n2 + j] *= scale;
+            if (rho == j) tile[rho * n2 + j] = beta;
+        }
+        if (lane == 0) ys[j] = t;  // tau_j parked in ys until Q^T e is formed
+        wave_lds_fence();
+        for (int c = j + 1; c < n2; ++c) {
+            double s = 0.0;
+            for (int rho = lane; rho < n1; rho += W) {
+                double v = rho > j ? tile[rho * n2 + j] : (rho == j ? 1.0 : 0.0);
+                s += v * tile[rho * n2 + c];
+            }
+            s = group_sum<W>(s) * t;
+            for (int rho = lane; rho < n1; rho += W) {
+                double v = rho > j ? tile[rho * n2 + j] : (rho == j ? 1.0 : 0.0);
+                if (rho >= j) tile[rho * n2 + c] -= s * v;
+            }
+        }
+        wave_lds_fence();
+        taus_last = t;
+    }
+    (void)taus_last;
+    // y = Q^T e_E kept distributed: lane holds y for rows lane + 64 i
+    constexpr int MAXR = 16;  // rows per lane for n1 <= 1024
+    double y[MAXR];
+#pragma unroll
+    for (int i = 0; i < MAXR; ++i) y[i] = (lane + W * i == E) ? 1.0 : 0.0;
+    for (int j = 0; j < n2; ++j) {
+        double t = ys[j];
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < MAXR; ++i) {
+            int rho = lane + W * i;
+            if (rho < n1 && rho >= j) {
+                double v = rho > j ? tile[rho * n2 + j] : 1.0;
+                s += v * y[i];
+            }
+        }
+        s = group_sum<W>(s) * t;
+#pragma unroll
+        for (int i = 0; i < MAXR; ++i) {
+            int rho = lane + W * i;
+            if (rho < n1 && rho >= j) {
+                double v = rho > j ? tile[rho * n2 + j] : 1.0;
+                y[i] -= s * v;
+            }
+        }
+    }
+    wave_lds_fence();
+#pragma unroll
+    for (int i = 0; i < MAXR; ++i) {
+        int rho = lane + W * i;
+        if (rho < n2) ys[rho] = y[i];
+    }
+    wave_lds_fence();
+    // back substitution: lane 0 solves, then shares through LDS
+    if (lane == 0) {
+        for (int c = n2 - 1; c >= 0; --c) {
+            double t = ys[c];
+            for (int c2 = c + 1; c2 < n2; ++c2) t -= tile[c * n2 + c2] * ys[c2];
+            ys[c] = (E < 0) ? 0.0 : t / tile[c * n2 + c];
+        }
+    }
+    wave_lds_fence();
+    for (int c = lane; c < n2; c += W) store_m<MT>(a, c0 + c, ys[c]);
+    if (a.n1_out && lane == 0) a.n1_out[k] = n1;
+    wave_lds_fence();
+    (void)Rs;
+}
+
+template <class MT, bool DYNAMIC>
+__global__ __launch_bounds__(64) void spai_lds_tier_kernel(ColumnArgs a) {
+    __shared__ double lds[kLdsTierDoubles];
+    __shared__ int ilds[2 * kLdsTierNJ];
+    const int lane = threadIdx.x;
+    if (DYNAMIC) {
+        const int count = *a.count;
+        for (;;) {
+            int idx = 0;
+            if (lane == 0) idx = atomicAdd(a.head, 1);
+            idx = __shfl(idx, 0, 64);
+            if (idx >= count) break;
+            solve_column_lds<MT>(a, a.list[idx], lane, lds, ilds);
+        }
+    } else {
+        for (int k = a.col_begin + blockIdx.x; k < a.col_end; k += gridDim.x)
+            solve_column_lds<MT>(a, k, lane, lds, ilds);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Planner
+// ---------------------------------------------------------------------------
+// counters layout (ints): [0..kTiers) column count per tier,
+// [kTiers..2kTiers) queue heads, [2kTiers] max T, [2kTiers+1] max n2,
+// [2kTiers+2] empty-column count; stats64[0] = sum of T_k.
+__device__ __forceinline__ int classify(int T, int n2, int lds_fit) {
+    if (n2 <= Tier0::NJ && T <= Tier0::TMAX) return 0;
+    if (n2 <= Tier1::NJ && T <= Tier1::TMAX) return 1;
+    if (n2 <= Tier2::NJ && T <= Tier2::TMAX) return 2;
+    if (n2 <= kLdsTierNJ && T <= kLdsTierT && lds_fit) return 3;
+    return 4;
+}
+
+__device__ __forceinline__ bool lds_tier_fits(int T, int n2) {
+    // n1 <= T, tile + R + y must fit the LDS tier's buffer
+    long long need = (long long)T * n2 + (long long)n2 * n2 + n2;
+    return need <= kLdsTierDoubles;
+}
+
+__global__ void spai_zero_counters(int *c, int n, unsigned long long *s64, int n64) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) c[i] = 0;
+    if (i < n64) s64[i] = 0ull;
+}
+
+// One thread per column: n2_k, T_k = sum_j |A(:,J_k[j])|, tier; wave-level
+// aggregation keeps the atomics at one per wavefront per tier.
+template <bool DYNAMIC>
+__global__ __launch_bounds__(256) void spai_plan_kernel(const int *colptr, const int *rowidx, int col_begin,
+                                                        int col_end, int *counters,
+                                                        unsigned long long *stats64, int *lists,
+                                                        int list_stride) {
+    const int k = col_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int T = 0, n2 = 0, tier = -1;
+    if (k < col_end) {
+        int c0 = colptr[k];
+        n2 = colptr[k + 1] - c0;
+        for (int j = 0; j < n2; ++j) {
+            int jr = rowidx[c0 + j];
+            T += colptr[jr + 1] - colptr[jr];
+        }
+        tier = n2 > 0 ? classify(T, n2, lds_tier_fits(T, n2)) : -1;
+    }
+    int mT = T, mN = n2;
+    unsigned long long sT = (unsigned long long)T;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        int a = __shfl_xor(mT, o, 64), b = __shfl_xor(mN, o, 64);
+        mT = a > mT ? a : mT;
+        mN = b > mN ? b : mN;
+        sT += __shfl_xor(sT, o, 64);
+    }
+    const unsigned long long empty_mask = __ballot(k < col_end && n2 == 0);
+    if (lane == 0) {
+        atomicMax(&counters[2 * kTiers], mT);
+        atomicMax(&counters[2 * kTiers + 1], mN);
+        if (empty_mask) atomicAdd(&counters[2 * kTiers + 2], (int)__popcll(empty_mask));
+        atomicAdd(&stats64[0], sT);
+    }
+#pragma unroll
+    for (int t = 0; t < kTiers; ++t) {
+        const unsigned long long m = __ballot(tier == t);
+        if (m) {
+            const int leader = __ffsll((long long)m) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(&counters[t], (int)__popcll(m));
+            base = __shfl(base, leader, 64);
+            if (DYNAMIC && tier == t) {
+                int rank = (int)__popcll(m & ((1ull << lane) - 1ull));
+                lists[(size_t)t * list_stride + base + rank] = k;
+            }
+        }
+    }
+}
+
+template <class KernelT>
+int persistent_grid(KernelT kernel, int block, int device_cus) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    return per_cu * device_cus;
+}
+
+template <class MT, bool REFINE>
+void launch_tiers(const ColumnArgs &base, bool dynamic, int static_tier, int *counters, int *lists,
+                  int list_stride, int cus, hipStream_t st) {
+    using QT = typename std::conditional<REFINE, float, double>::type;
+    ColumnArgs a = base;
+    auto setup = [&](int t) {
+        a = base;
+        if (dynamic) {
+            a.list = lists + (size_t)t * list_stride;
+            a.count = counters + t;
+            a.head = counters + kTiers + t;
+        }
+    };
+    auto run_reg = [&](auto tier_tag, int t) {
+        using TR = decltype(tier_tag);
+        setup(t);
+        if (dynamic) {
+            auto k = spai_reg_tier_kernel<TR, QT, REFINE, MT, true>;
+            hipLaunchKernelGGL(k, dim3(persistent_grid(k, kBlock, cus)), dim3(kBlock), 0, st, a);
+        } else {
+            auto k = spai_reg_tier_kernel<TR, QT, REFINE, MT, false>;
+            int ncols = base.col_end - base.col_begin;
+            int waves = ceil_div(ncols, 64 / TR::W * 4);  // ~4 columns per group
+            int grid = std::max(1, std::min(ceil_div(waves, kBlock / 64), persistent_grid(k, kBlock, cus) * 4));
+            hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, st, a);
+        }
+        SPAI_CHECK_LAUNCH();
+    };
+    auto run_lds = [&](int t) {
+        setup(t);
+        if (dynamic) {
+            auto k = spai_lds_tier_kernel<MT, true>;
+            hipLaunchKernelGGL(k, dim3(persistent_grid(k, 64, cus)), dim3(64), 0, st, a);
+        } else {
+            auto k = spai_lds_tier_kernel<MT, false>;
+            int ncols = base.col_end - base.col_begin;
+            hipLaunchKernelGGL(k, dim3(std::max(1, std::min(ncols, persistent_grid(k, 64, cus) * 8))), dim3(64), 0,
+                               st, a);
+        }
+        SPAI_CHECK_LAUNCH();
+    };
+    if (dynamic) {
+        // largest tier first so the long columns start early
+        run_lds(3);
+        run_reg(Tier2{}, 2);
+        run_reg(Tier1{}, 1);
+        run_reg(Tier0{}, 0);
+    } else {
+        switch (static_tier) {
+        case 0: run_reg(Tier0{}, 0); break;
+        case 1: run_reg(Tier1{}, 1); break;
+        case 2: run_reg(Tier2{}, 2); break;
+        default: run_lds(3); break;
+        }
+    }
+}
+
+}  // namespace
+
+SpaiBuilder::SpaiBuilder() = default;
+SpaiBuilder::~SpaiBuilder() = default;
+
+void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats *stats) {
+    require(p.n >= 0 && p.colptr && p.rowidx && p.val && p.perm && p.m_csc && p.m_csr, "spai build: bad arguments");
+    int cb = std::max(0, p.col_begin), ce = std::min(p.n, p.col_end < 0 ? p.n : p.col_end);
+    require(cb <= ce, "spai build: empty or inverted column range");
+    int dev = 0;
+    SPAI_HIP(hipGetDevice(&dev));
+    if (cus_ == 0) SPAI_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
+    const bool dynamic = p.schedule == 1;
+    const int nc = 2 * kTiers + 3;
+    counters_.reserve(nc);
+    stats64_.reserve(2);
+    const int list_stride = std::max(1, ce - cb);
+    if (dynamic) lists_.reserve((size_t)kTiers * list_stride);
+    hipLaunchKernelGGL(spai_zero_counters, dim3(1), dim3(64), 0, st, counters_.get(), nc, stats64_.get(), 2);
+    SPAI_CHECK_LAUNCH();
+    if (ce > cb) {
+        int grid = ceil_div(ce - cb, 256);
+        if (dynamic)
+            hipLaunchKernelGGL(spai_plan_kernel<true>, dim3(grid), dim3(256), 0, st, p.colptr, p.rowidx, cb, ce,
+                               counters_.get(), stats64_.get(), lists_.get(), list_stride);
+        else
+            hipLaunchKernelGGL(spai_plan_kernel<false>, dim3(grid), dim3(256), 0, st, p.colptr, p.rowidx, cb, ce,
+                               counters_.get(), stats64_.get(), (int *)nullptr, list_stride);
+        SPAI_CHECK_LAUNCH();
+    }
+    int hc[2 * kTiers + 3] = {0};
+    unsigned long long h64[2] = {0, 0};
+    auto fetch = [&]() {
+        SPAI_HIP(hipMemcpyAsync(hc, counters_.get(), sizeof(int) * nc, hipMemcpyDeviceToHost, st));
+        SPAI_HIP(hipMemcpyAsync(h64, stats64_.get(), sizeof(h64), hipMemcpyDeviceToHost, st));
+        SPAI_HIP(hipStreamSynchronize(st));
+    };
+    ColumnArgs a{};
+    a.colptr = p.colptr;
+    a.rowidx = p.rowidx;
+    a.val = p.val;
+    a.perm = p.perm;
+    a.m_csc = p.m_csc;
+    a.m_csr = p.m_csr;
+    a.n1_out = p.n1_out;
+    a.col_begin = cb;
+    a.col_end = ce;
+    int static_tier = 0;
+    if (!dynamic && ce > cb) {
+        // the reference reads n2max back before choosing its kernels
+        // (src/spai_single_dcu.cpp:1245-1252); static mode does the same for
+        // the largest column's tier
+        fetch();
+        int maxT = hc[2 * kTiers], maxN = hc[2 * kTiers + 1];
+        static_tier = classify_host(maxT, maxN);
+        require(static_tier < kTiers - 1,
+                "spai build: a column exceeds the largest supported tile (T=" + std::to_string(maxT) +
+                    ", n2=" + std::to_string(maxN) + ")",
+                kErrUnsupported);
+    }
+    if (ce > cb) {
+        if (p.precision == 1)
+            launch_tiers<float, true>(a, dynamic, static_tier, counters_.get(), lists_.get(), list_stride, cus_, st);
+        else
+            launch_tiers<double, false>(a, dynamic, static_tier, counters_.get(), lists_.get(), list_stride, cus_,
+                                        st);
+    }
+    fetch();
+    require(hc[kTiers - 1] == 0 || !dynamic,
+            "spai build: " + std::to_string(hc[kTiers - 1]) + " column(s) exceed the largest supported tile",
+            kErrUnsupported);
+    if (stats) {
+        for (int t = 0; t < kTiers; ++t) stats->tier_columns[t] = dynamic ? hc[t] : 0;
+        if (!dynamic && ce > cb) stats->tier_columns[static_tier] = ce - cb - hc[2 * kTiers + 2];
+        stats->max_gathered = hc[2 * kTiers];
+        stats->max_n2 = hc[2 * kTiers + 1];
+        stats->empty_columns = hc[2 * kTiers + 2];
+        stats->sum_gathered = (long long)h64[0];
+        stats->static_tier = dynamic ? -1 : static_tier;
+    }
+}
+
+int classify_host(int T, int n2) {
+    if (n2 <= 8 && T <= 32) return 0;
+    if (n2 <= 8 && T <= 64) return 1;
+    if (n2 <= 16 && T <= 128) return 2;
+    long long need = (long long)T * n2 + (long long)n2 * n2 + n2;
+    if (n2 <= kLdsTierNJ && T <= kLdsTierT && need <= kLdsTierDoubles) return 3;
+    return 4;
+}
+
+}  // namespace spai

@@ -1,0 +1,54 @@
+// Host interface of the SPAI column kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+
+namespace spai {
+
+// Size tiers: 0..2 register tiers (16, 32, 64 lanes per column), 3 the LDS
+// wavefront tier, 4 = larger than any tier (rejected).
+constexpr int kTiers = 5;
+
+struct SpaiBuildParams {
+    int n = 0;
+    const int *colptr = nullptr;  // device CSC of A
+    const int *rowidx = nullptr;
+    const double *val = nullptr;
+    const int *perm = nullptr;    // CSC position -> CSR position
+    int precision = 0;            // 0: FP64 QR, FP64 M; 1: FP32 QR + FP64 sweep, FP32 M
+    int schedule = 0;             // 0: balanced_columns (static), 1: balanced_nnz (queues)
+    int col_begin = 0, col_end = -1;
+    void *m_csc = nullptr;        // double[nnz] or float[nnz] (precision 1)
+    void *m_csr = nullptr;        // same type, CSR order
+    int *n1_out = nullptr;        // optional per-column |I_k|
+};
+
+struct SpaiBuildStats {
+    int tier_columns[kTiers] = {0, 0, 0, 0, 0};
+    int max_gathered = 0;          // max_k T_k
+    int max_n2 = 0;                // max_k |J_k|
+    int empty_columns = 0;
+    long long sum_gathered = 0;    // sum_k T_k (entries of A read by the gather)
+    int static_tier = -1;
+};
+
+int classify_host(int T, int n2);
+
+class SpaiBuilder {
+public:
+    SpaiBuilder();
+    ~SpaiBuilder();
+    // Enqueues the build on `st` and waits for it (the planner's counters are
+    // read back at the end); throws spai::Error on failure.
+    void build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats *stats);
+
+private:
+    DeviceArray<int> counters_;
+    DeviceArray<unsigned long long> stats64_;
+    DeviceArray<int> lists_;
+    int cus_ = 0;
+};
+
+}  // namespace spai

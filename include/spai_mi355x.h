@@ -1,0 +1,207 @@
+/*
+ * libspai_mi355x: static SPAI preconditioner construction and the
+ * SPAI-preconditioned BiCGSTAB solve on one AMD Instinct MI355X (gfx950).
+ *
+ * C ABI: plain pointers and sizes, int status codes (0 = success, negative =
+ * error, message via spai_last_error()), no exit() on error.  Every entry
+ * point is synchronous with respect to the host: when it returns, its device
+ * work is complete.  Calls on one spai_context must not overlap (a context
+ * owns one HIP stream and its workspaces); distinct contexts are independent.
+ *
+ * Reference interfaces these entry points replace (file:line in the
+ * reference tree):
+ *   - spai_system_build            StaticSPAIv20 (src/spai_single_dcu.cpp:1224)
+ *                                  and StaticSPAIv20_MultiDCU
+ *                                  (src/spai_multi_dcu.cpp:1952) at one device
+ *   - spai_system_solve            cublas2_pbicgstabv2
+ *                                  (include/bicgstab/bicgstab_solver.h:33)
+ *   - spai_csc_to_csr_device       cuCSC2CSR
+ *                                  (include/common/cuFormatConversion.h:453)
+ *   - spai_problem_read_mtx        readMatrixToCSC (include/common/read.h:1296)
+ *   - the C++ drop-ins (initMultiDCU, RunStaticSPAI_MultiDCU_MPI, ...) are
+ *     declared in spai_multi_dcu_api.h.
+ */
+#ifndef SPAI_MI355X_H
+#define SPAI_MI355X_H
+
+#include <stdint.h>
+
+#include "spai_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status ---- */
+#define SPAI_OK 0
+#define SPAI_ERR_INVALID (-1)     /* bad argument */
+#define SPAI_ERR_HIP (-2)         /* HIP runtime error */
+#define SPAI_ERR_NOMEM (-3)       /* device or host allocation failed */
+#define SPAI_ERR_UNSUPPORTED (-4) /* e.g. a column larger than every tile tier */
+#define SPAI_ERR_BREAKDOWN (-5)   /* BiCGSTAB produced a non-finite scalar */
+#define SPAI_ERR_NOT_BUILT (-6)   /* solve before spai_system_build */
+
+/* precision of the SPAI build and of the stored M */
+#define SPAI_PRECISION_FP64 0     /* FP64 Householder QR, FP64 M */
+#define SPAI_PRECISION_MIXED 1    /* FP32 QR + one FP64 refinement sweep, FP32 M */
+
+/* column scheduling (the reference's --partition values) */
+#define SPAI_SCHEDULE_BALANCED_COLUMNS 0 /* static contiguous column blocks */
+#define SPAI_SCHEDULE_BALANCED_NNZ 1     /* size-tiered atomic work queues */
+
+/* per-system solver outcome */
+#define SPAI_SOLVE_RUNNING 0
+#define SPAI_SOLVE_CONVERGED 1
+#define SPAI_SOLVE_MAX_ITER 2
+#define SPAI_SOLVE_BREAKDOWN 3
+#define SPAI_SOLVE_CONVERGED_AT_START 4
+
+const char *spai_status_string(int status);
+/* Message of the last failing call on this thread. */
+const char *spai_last_error(void);
+/* Library version string. */
+const char *spai_version(void);
+
+/* ------------------------------------------------------------ context --- */
+typedef struct spai_context spai_context;
+
+/* Creates a context on HIP device `device` with its own stream. */
+int spai_context_create(int device, spai_context **out);
+int spai_context_destroy(spai_context *ctx);
+/* Name of the device (e.g. "AMD Instinct MI355X") and its CU count. */
+int spai_context_device_info(spai_context *ctx, char *name, int name_len, int *cus);
+
+/* ------------------------------------------------------------- system --- */
+/* One square sparse matrix, or a batch of independent square matrices
+ * stored block-diagonally, resident on the context's device with everything
+ * the build and the solves need: CSC and CSR of A, the CSC->CSR permutation,
+ * row tiles, and (after spai_system_build) M in CSC and CSR order. */
+typedef struct spai_system spai_system;
+
+/* From a host CSC (0-based, square; rows inside a column in any order). */
+int spai_system_create(spai_context *ctx, const CSC_Matrix *host_A, spai_system **out);
+/* From `nsys` host CSC matrices; system s occupies rows/columns
+ * [row_offset[s], row_offset[s+1]) of the block-diagonal batch. */
+int spai_system_create_batched(spai_context *ctx, int nsys, const CSC_Matrix *host_systems, spai_system **out);
+/* From a device CSC (pointers on the context's device); copied. */
+int spai_system_create_device(spai_context *ctx, const CSC_Matrix *dev_A, spai_system **out);
+int spai_system_destroy(spai_system *sys);
+
+typedef struct {
+    int n;                   /* total order */
+    int nnz;
+    int nsys;
+    int m_precision;         /* -1 before a build */
+    float setup_ms;          /* upload + CSC->CSR + tiles (device time) */
+} spai_system_info;
+int spai_system_get_info(const spai_system *sys, spai_system_info *info);
+/* Row offsets of the systems (nsys + 1 ints). */
+int spai_system_row_offsets(const spai_system *sys, int *row_offsets);
+
+/* Replaces A's values (same pattern, CSC order, host or device pointer). */
+int spai_system_set_values(spai_system *sys, const double *values, int values_on_device);
+
+/* ------------------------------------------------------------- build ---- */
+typedef struct {
+    float ms;                /* device time: planning + column kernels + M in CSC/CSR */
+    int tier_columns[5];     /* columns per size tier (16/32/64-lane registers, LDS wave, too large) */
+    int max_gathered;        /* max_k sum_{j in J_k} |A(:,j)| */
+    int max_n2;              /* max_k |J_k| (the reference's n2max) */
+    int empty_columns;
+    long long sum_gathered;  /* entries of A gathered over all columns */
+    int static_tier;         /* tier used by balanced_columns, -1 for balanced_nnz */
+} spai_build_report;
+
+/* Builds M with pattern(M) = pattern(A) for columns [col_begin, col_end)
+ * (col_end < 0 means n); other columns of M are zero. */
+int spai_system_build(spai_system *sys, int precision, int schedule, int col_begin, int col_end,
+                      spai_build_report *report);
+
+/* M's values in A's CSC storage order, widened to double. */
+int spai_system_get_m(spai_system *sys, double *values, int values_on_device);
+/* |I_k| per column from the last build (n ints), for diagnostics. */
+int spai_system_get_row_set_sizes(spai_system *sys, int *n1);
+
+/* ------------------------------------------------------------- solve ---- */
+typedef struct {
+    double rel_tol;          /* stop when ||r||_2 < rel_tol * ||b||_2 (per system) */
+    double abs_tol;          /* > 0: stop when ||r||_2 < abs_tol instead (the reference's TOL) */
+    int max_iter;            /* reference semantics: max_iter - 1 iterations at most */
+    int use_graph;           /* 1: iterations replayed from a hipGraph */
+    int graph_iters;         /* iterations per graph launch */
+    int check_every;         /* iterations between host convergence checks without a graph */
+    int precondition;        /* 0: no preconditioner (M = I) */
+} spai_solve_options;
+
+void spai_solve_options_default(spai_solve_options *opt);
+
+typedef struct {
+    float ms;                /* device time of the solve incl. the true residual */
+    int iterations_max;
+    long long iterations_sum;
+    int host_checks;
+    int all_converged;
+} spai_solve_report;
+
+typedef struct {
+    int status;              /* SPAI_SOLVE_* */
+    int iterations;          /* the reference's iteration count */
+    double residual;         /* final recursive ||r||_2 */
+    double residual0;        /* ||r_0||_2 */
+    double true_residual;    /* ||b - A x||_2 */
+    double tol;              /* absolute threshold used */
+} spai_system_outcome;
+
+/* Solves A x = b for every system of the batch.  x holds x0 on entry.
+ * `on_device` = 1: b and x are device pointers (n doubles each). */
+int spai_system_solve(spai_system *sys, const double *b, double *x, int on_device, const spai_solve_options *opt,
+                      spai_solve_report *report);
+/* Per-system outcome of the last solve (nsys entries). */
+int spai_system_outcomes(spai_system *sys, spai_system_outcome *out);
+
+/* Adjoint of the solve for L(x): given the converged x and g = dL/dx, solves
+ * A^T lambda = g with BiCGSTAB right-preconditioned by M^T (CSC(M) used as
+ * CSR(M^T)), then dL/db = lambda and dL/dA_ij = -lambda_i x_j on A's CSC
+ * pattern (nnz doubles in CSC order). */
+int spai_system_adjoint(spai_system *sys, const double *x, const double *g, double *dA, double *db, int on_device,
+                        const spai_solve_options *opt, spai_solve_report *report);
+
+/* Average device time (ms) of one bare product y = A x (which = 0) or
+ * y = M x (which = 1) with the solver's tile kernel, over `reps` launches. */
+int spai_system_time_spmv(spai_system *sys, int which, int reps, float *ms);
+
+/* ------------------------------------------------- format conversion ---- */
+/* Device CSC -> CSR with rows ordered by column (drop-in for cuCSC2CSR);
+ * all pointers on the current device, caller-allocated. */
+int spai_csc_to_csr_device(int m, int n, int nnz, const double *cscData, const int *cscIndex, const int *cscPtr,
+                           double *csrData, int *csrIndex, int *csrPtr);
+
+/* -------------------------------------------------- host problems ------- */
+typedef struct spai_problem spai_problem;
+
+/* Seeded synthetic problem: config 1..5 (random 4096, conv-diff 512^2,
+ * power-law 1e6, batch of 64, 64^3 GRF diffusion). */
+int spai_problem_generate(int config, uint64_t seed, spai_problem **out);
+/* Power-law generator at a chosen size (tests). */
+int spai_problem_generate_power_law(int n, uint64_t seed, spai_problem **out);
+/* Batch generator with a chosen system count (tests). */
+int spai_problem_generate_batched(int nsys, uint64_t seed, spai_problem **out);
+/* 2D conv-diff on an m x m grid / 3D diffusion with given cell kappa. */
+int spai_problem_convection_diffusion(int m, double peclet, double vx, double vy, spai_problem **out);
+/* MatrixMarket file with the reference's reader semantics; b = ones. */
+int spai_problem_read_mtx(const char *path, spai_problem **out);
+int spai_problem_free(spai_problem *p);
+int spai_problem_num_systems(const spai_problem *p);
+/* Host CSC of system s; pointers stay owned by the problem. */
+int spai_problem_system(const spai_problem *p, int s, CSC_Matrix *out);
+/* Right-hand side / target of system s (NULL when absent). */
+const double *spai_problem_rhs(const spai_problem *p, int s);
+const double *spai_problem_target(const spai_problem *p, int s);
+const char *spai_problem_name(const spai_problem *p);
+const char *spai_problem_notes(const spai_problem *p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,27 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+REFERENCE = "/root/reference"
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP library")
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    import dcu_amd as D
+
+    c = D.Context(0)
+    yield c
+    c.close()
+
+
+def reference_available() -> bool:
+    return os.path.isdir(os.path.join(REFERENCE, "matrices"))

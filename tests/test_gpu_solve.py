@@ -1,0 +1,131 @@
+"""GPU parity of the fused BiCGSTAB (bicgstab_solver.h:106-232 recurrences)
+against the CPU oracle.
+
+Bar: iteration counts within +-1 of the oracle run with the oracle's own M,
+true relative residual ||b - A x|| / ||b|| <= 1e-8 (times a 1.5 slack for the
+difference between the recursive and the true residual), solutions within
+1e-6 relative of the oracle's (both are 1e-8-residual solutions).
+"""
+import numpy as np
+import pytest
+
+import dcu_amd as D
+from oracle import binding as O
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_solve(A, m_vals, b, rel_tol=1e-8, max_iter=1000, x0=None):
+    rp, ci, cv, _ = O.csc2csr(A.n, A.colptr, A.rowidx, A.val)
+    M = None
+    if m_vals is not None:
+        mrp, mci, mcv, _ = O.csc2csr(A.n, A.colptr, A.rowidx, m_vals)
+        M = (mrp, mci, mcv)
+    x0 = np.zeros(A.n) if x0 is None else x0
+    x, it, info = O.bicgstab((rp, ci, cv), M, b, x0, rel_tol * np.linalg.norm(b), max_iter)
+    return x, it, info, (rp, ci, cv)
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_random4096_solve(ctx, use_graph):
+    P = D.Problem.generate(1, 1)
+    A = P.system(0)
+    b = P.rhs(0)
+    S = D.System(ctx, A)
+    S.build(D.PRECISION_FP64, D.SCHEDULE_BALANCED_COLUMNS)
+    x, rep, oc = S.solve(b, rel_tol=1e-8, max_iter=1000, use_graph=use_graph)
+    m_ref = O.spai(A.n, A.colptr, A.rowidx, A.val, 0)
+    xr, it, info, Acsr = oracle_solve(A, m_ref, b)
+    true_rel = O.true_residual(Acsr, b, x) / np.linalg.norm(b)
+    print(f"gpu iters {oc[0]['iterations']} oracle {it} true_rel {true_rel:.3e} ms {rep['ms']:.3f}", flush=True)
+    assert oc[0]["status"] == D.SOLVE_CONVERGED
+    assert abs(oc[0]["iterations"] - it) <= 1
+    assert true_rel <= 1.5e-8
+    assert abs(oc[0]["true_residual"] / np.linalg.norm(b) - true_rel) <= 1e-10
+    assert np.linalg.norm(x - xr) <= 1e-6 * np.linalg.norm(xr)
+
+
+def test_graph_and_loop_agree_bitwise(ctx):
+    P = D.Problem.generate(1, 2)
+    A = P.system(0)
+    b = P.rhs(0)
+    S = D.System(ctx, A)
+    S.build(D.PRECISION_FP64, D.SCHEDULE_BALANCED_NNZ)
+    x1, _, o1 = S.solve(b, use_graph=False)
+    x2, _, o2 = S.solve(b, use_graph=True, graph_iters=4)
+    x3, _, o3 = S.solve(b, use_graph=True, graph_iters=4)  # graph replayed from the cache
+    assert o1[0]["iterations"] == o2[0]["iterations"] == o3[0]["iterations"]
+    assert np.array_equal(x1, x2) and np.array_equal(x2, x3)
+
+
+def test_unpreconditioned_matches_oracle(ctx):
+    P = D.Problem.convection_diffusion(40, 5.0)
+    A = P.system(0)
+    b = np.random.default_rng(3).uniform(size=A.n)
+    S = D.System(ctx, A)
+    x, rep, oc = S.solve(b, rel_tol=1e-8, max_iter=2000, precondition=False)
+    xr, it, info, Acsr = oracle_solve(A, None, b, max_iter=2000)
+    assert abs(oc[0]["iterations"] - it) <= 1
+    assert O.true_residual(Acsr, b, x) / np.linalg.norm(b) <= 1.5e-8
+
+
+def test_mixed_precision_solve(ctx):
+    P = D.Problem.generate(1, 4)
+    A = P.system(0)
+    b = P.rhs(0)
+    S = D.System(ctx, A)
+    S.build(D.PRECISION_MIXED, D.SCHEDULE_BALANCED_NNZ)
+    x, rep, oc = S.solve(b, rel_tol=1e-8)
+    m_ref = O.spai(A.n, A.colptr, A.rowidx, A.val, 0)
+    xr, it, info, Acsr = oracle_solve(A, m_ref, b)
+    true_rel = O.true_residual(Acsr, b, x) / np.linalg.norm(b)
+    print(f"mixed: gpu iters {oc[0]['iterations']} oracle(fp64 M) {it} true_rel {true_rel:.3e}", flush=True)
+    assert oc[0]["status"] == D.SOLVE_CONVERGED
+    assert abs(oc[0]["iterations"] - it) <= 1
+    assert true_rel <= 1.5e-8
+
+
+def test_batched_solve_per_system(ctx):
+    P = D.Problem.batched(6, 5)
+    mats = P.systems()
+    S = D.System(ctx, mats)
+    S.build(D.PRECISION_FP64, D.SCHEDULE_BALANCED_NNZ)
+    b = np.concatenate([P.rhs(s) for s in range(len(mats))])
+    for use_graph in (False, True):
+        x, rep, oc = S.solve(b, rel_tol=1e-8, use_graph=use_graph)
+        off = S.row_offsets
+        for s, A in enumerate(mats):
+            bs = b[off[s]:off[s + 1]]
+            m_ref = O.spai(A.n, A.colptr, A.rowidx, A.val, 0)
+            xr, it, info, Acsr = oracle_solve(A, m_ref, bs)
+            xs = x[off[s]:off[s + 1]]
+            true_rel = O.true_residual(Acsr, bs, xs) / np.linalg.norm(bs)
+            assert oc[s]["status"] == D.SOLVE_CONVERGED, (s, oc[s])
+            assert abs(oc[s]["iterations"] - it) <= 1, (s, oc[s]["iterations"], it)
+            assert true_rel <= 1.5e-8, (s, true_rel)
+
+
+def test_max_iter_semantics(ctx):
+    # the reference returns MAX_ITER after MAX_ITER - 1 unconverged iterations
+    P = D.Problem.convection_diffusion(60, 10.0)
+    A = P.system(0)
+    b = np.ones(A.n)
+    S = D.System(ctx, A)
+    S.build(D.PRECISION_FP64, D.SCHEDULE_BALANCED_NNZ)
+    x, rep, oc = S.solve(b, rel_tol=1e-14, max_iter=5)
+    m_ref = O.spai(A.n, A.colptr, A.rowidx, A.val, 0)
+    xr, it, info, _ = oracle_solve(A, m_ref, b, rel_tol=1e-14, max_iter=5)
+    assert it == 5 and info[2] == 0
+    assert oc[0]["status"] == D.SOLVE_MAX_ITER and oc[0]["iterations"] == 5
+    assert np.linalg.norm(x - xr) <= 1e-10 * np.linalg.norm(xr)
+
+
+def test_converged_at_start(ctx):
+    P = D.Problem.generate(1, 1)
+    A = P.system(0)
+    S = D.System(ctx, A)
+    S.build(D.PRECISION_FP64, D.SCHEDULE_BALANCED_NNZ)
+    b = P.rhs(0)  # b = A * ones, so x0 = ones is exact
+    x, rep, oc = S.solve(b, x0=np.ones(A.n), rel_tol=1e-8)
+    assert oc[0]["status"] == D.SOLVE_CONVERGED_AT_START and oc[0]["iterations"] == 0
+    assert np.array_equal(x, np.ones(A.n))

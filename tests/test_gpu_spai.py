@@ -1,0 +1,139 @@
+"""GPU parity of the SPAI build against the CPU oracle (MGS restatement of
+StaticSPAIv20, src/spai_single_dcu.cpp:1224-2687).
+
+Bar: FP64 M within 1e-10 (max-norm, relative to max |M| per column) of the
+oracle; pattern(M) = pattern(A) by construction; |I_k| bit-exact.  Mixed
+precision (FP32 QR + one FP64 sweep, FP32 M) is compared to the FP64 oracle
+with the FP32 storage bound 2^-23 (relative, per column) and the error is
+printed.
+"""
+import numpy as np
+import pytest
+
+import dcu_amd as D
+from oracle import binding as O
+
+pytestmark = pytest.mark.gpu
+
+
+def column_rel_err(A, m_gpu, m_ref):
+    """max over columns of max|dM| / max|M_ref| within the column."""
+    worst = 0.0
+    cp = A.colptr
+    d = np.abs(m_gpu - m_ref)
+    mx = np.maximum.reduceat(np.abs(m_ref), cp[:-1][np.diff(cp) > 0])
+    dx = np.maximum.reduceat(d, cp[:-1][np.diff(cp) > 0])
+    ok = mx > 0
+    if ok.any():
+        worst = float(np.max(dx[ok] / mx[ok]))
+    return worst
+
+
+def build_and_compare(ctx, A, precision, schedule, tol):
+    S = D.System(ctx, A)
+    rep = S.build(precision, schedule)
+    m_gpu = S.m_values()
+    m_ref = O.spai(A.n, A.colptr, A.rowidx, A.val, 0)
+    err = column_rel_err(A, m_gpu, m_ref)
+    print(f"n={A.n} nnz={A.nnz} prec={precision} sched={schedule} build={rep['ms']:.3f} ms "
+          f"tiers={rep['tier_columns']} err={err:.3e}", flush=True)
+    assert np.all(np.isfinite(m_gpu))
+    assert err <= tol, err
+    return S, rep, m_gpu, m_ref
+
+
+@pytest.mark.parametrize("schedule", [D.SCHEDULE_BALANCED_COLUMNS, D.SCHEDULE_BALANCED_NNZ])
+def test_random4096_fp64(ctx, schedule):
+    P = D.Problem.generate(1, 1)
+    A = P.system(0)
+    S, rep, _, _ = build_and_compare(ctx, A, D.PRECISION_FP64, schedule, 1e-10)
+    n1_gpu = S.row_set_sizes()
+    n1_ref, n2_ref, t_ref, _ = O.column_sets(A.n, A.colptr, A.rowidx)
+    assert np.array_equal(n1_gpu, n1_ref)
+    assert rep["max_n2"] == n2_ref.max()
+    assert rep["max_gathered"] == t_ref.max()
+    assert rep["sum_gathered"] == int(t_ref.sum())
+
+
+def test_random4096_mixed(ctx):
+    P = D.Problem.generate(1, 1)
+    A = P.system(0)
+    build_and_compare(ctx, A, D.PRECISION_MIXED, D.SCHEDULE_BALANCED_NNZ, 2.0 ** -23 * 4)
+
+
+@pytest.mark.parametrize("precision,tol", [(D.PRECISION_FP64, 1e-10), (D.PRECISION_MIXED, 2.0 ** -23 * 4)])
+def test_power_law_all_tiers(ctx, precision, tol):
+    # 60k rows of the power-law generator reach the 16/32/64-lane tiers and
+    # the LDS tier
+    P = D.Problem.power_law(60000, 7)
+    A = P.system(0)
+    S, rep, _, _ = build_and_compare(ctx, A, precision, D.SCHEDULE_BALANCED_NNZ, tol)
+    assert rep["tier_columns"][0] > 0 and rep["tier_columns"][1] > 0 and rep["tier_columns"][2] > 0
+
+
+def test_power_law_lds_tier_static(ctx):
+    # static scheduling puts every column through the tier of the largest one
+    P = D.Problem.power_law(60000, 7)
+    A = P.system(0)
+    S, rep, _, _ = build_and_compare(ctx, A, D.PRECISION_FP64, D.SCHEDULE_BALANCED_COLUMNS, 1e-10)
+    assert rep["static_tier"] >= 2
+
+
+def test_batched_small(ctx):
+    P = D.Problem.batched(6, 11)
+    mats = P.systems()
+    S = D.System(ctx, mats)
+    rep = S.build(D.PRECISION_FP64, D.SCHEDULE_BALANCED_NNZ)
+    m = S.m_values()
+    off = 0
+    for A in mats:
+        m_ref = O.spai(A.n, A.colptr, A.rowidx, A.val, 0)
+        err = column_rel_err(A, m[off:off + A.nnz], m_ref)
+        assert err <= 1e-10, err
+        off += A.nnz
+
+
+def test_column_range_partial(ctx):
+    P = D.Problem.generate(1, 3)
+    A = P.system(0)
+    S = D.System(ctx, A)
+    S.build(D.PRECISION_FP64, D.SCHEDULE_BALANCED_NNZ, 1000, 2500)
+    m = S.m_values()
+    m_ref = O.spai(A.n, A.colptr, A.rowidx, A.val, 0, 1000, 2500)
+    lo, hi = A.colptr[1000], A.colptr[2500]
+    assert np.all(m[:lo] == 0) and np.all(m[hi:] == 0)
+    assert np.max(np.abs(m[lo:hi] - m_ref[lo:hi])) <= 1e-10 * np.max(np.abs(m_ref[lo:hi]))
+
+
+def test_diagonal_matrix_closed_form(ctx):
+    n = 5000
+    d = np.linspace(0.5, 3.0, n)
+    A = D.CSC(n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), d)
+    S = D.System(ctx, A)
+    S.build(D.PRECISION_FP64, D.SCHEDULE_BALANCED_COLUMNS)
+    assert np.max(np.abs(S.m_values() * d - 1.0)) <= 1e-15
+
+
+def test_missing_diagonal_gives_zero_column(ctx):
+    # column 0 of A has no entry in row 0 and row 0 appears in no column of
+    # J_0's union: E_0 = -1 and M(:,0) = 0 (Sol_SSPAIv10)
+    n = 4
+    dense = np.array([[0.0, 0.0, 0.0, 0.0],
+                      [2.0, 3.0, 0.0, 0.0],
+                      [0.0, 0.0, 4.0, 1.0],
+                      [0.0, 0.0, 1.0, 5.0]])
+    cp = [0]
+    ri, va = [], []
+    for c in range(n):
+        for r in range(n):
+            if dense[r, c] != 0:
+                ri.append(r)
+                va.append(dense[r, c])
+        cp.append(len(ri))
+    A = D.CSC(n, np.array(cp, dtype=np.int32), np.array(ri, dtype=np.int32), np.array(va))
+    S = D.System(ctx, A)
+    S.build(D.PRECISION_FP64, D.SCHEDULE_BALANCED_NNZ)
+    m = S.m_values()
+    m_ref = O.spai(n, A.colptr, A.rowidx, A.val, 0)
+    assert np.all(m[A.colptr[0]:A.colptr[1]] == 0)
+    assert np.allclose(m, m_ref, rtol=1e-12, atol=1e-14)
