@@ -1,0 +1,261 @@
+"""Benchmark: SPAI build + SPAI-preconditioned BiCGSTAB on one MI355X.
+
+Default workload (one "step"): the batch of 64 independent systems (sizes
+log-uniform in [2e3, 2e5]; even: 2D conv-diff, odd: 3D lognormal diffusion),
+per-system relative tolerance 1e-8, x0 = 0:
+  - batched mixed path: FP32 SPAI QR + FP64 sweep, FP32 M, balanced_nnz
+    queues, all 64 systems in each launch, BiCGSTAB replayed from a hipGraph;
+  - plain FP64 path (timed beside it): the same 64 systems one at a time,
+    FP64 SPAI and M, one launch set per iteration with a host convergence
+    check per iteration.
+value = end-to-end systems solved per second of the batched path
+(SPAI build + solve, inputs resident in HBM).
+
+Usage: python bench.py [--steps K] [--warmup W] [--workload batched|convdiff|powerlaw|grf|random]
+                       [--precision mixed|fp64] [--no-cpu] [--no-plain]
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (generator config, description)
+    "random": (1, "random4096: n=4096, 4 off-diagonals/row, static columns"),
+    "convdiff": (2, "convdiff512: 512^2 upwind conv-diff, n=262144"),
+    "powerlaw": (3, "powerlaw1e6: Zipf(2)-capped rows, n=1e6"),
+    "batched": (4, "batched64: 64 systems, n in [2e3,2e5] log-uniform, 2D conv-diff / 3D diffusion"),
+    "grf": (5, "grf64: 64^3 lognormal-GRF diffusion, n=262144"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def tile_bytes(kind, n, nnz, m_bytes):
+    """Algorithmic HBM bytes of one launch of a fused solver kernel.
+    A is FP64 CSR (8 B value + 4 B column per nonzero, 4 B row pointer per
+    row); vectors FP64; every vector read or written once."""
+    if kind == "spmv_alpha":      # a = A y; reads A, y (gather, once), r*; writes a
+        return 12 * nnz + 4 * n + 8 * n * 3
+    if kind == "spmv_omega":      # t = A z; reads A, z, s; writes t
+        return 12 * nnz + 4 * n + 8 * n * 3
+    if kind == "mapply_p":        # y = M p'; reads M, r, p, a; writes p', y
+        return (m_bytes + 4) * nnz + 4 * n + 8 * n * 5
+    if kind == "mapply_s":        # z = M s; reads M, r, a; writes s, z
+        return (m_bytes + 4) * nnz + 4 * n + 8 * n * 4
+    raise ValueError(kind)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="batched", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work")
+    ap.add_argument("--graph-iters", type=int, default=8)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the SPAI build and the batched solve run on a single MI355X "
+              "(multi-GPU sharding is not part of this build); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import dcu_amd as D
+
+    cfg, desc = WORKLOADS[args.workload]
+    t0 = time.time()
+    P = D.Problem.generate(cfg, args.seed if args.workload == "batched" else cfg)
+    mats = P.systems()
+    rhs = [P.rhs(s) for s in range(len(mats))]
+    log(f"generated {P.name} ({len(mats)} systems) in {time.time() - t0:.1f}s")
+    nsys = len(mats)
+    n_tot = sum(A.n for A in mats)
+    nnz_tot = sum(A.nnz for A in mats)
+
+    ctx = D.Context(0)
+    dev_name, cus = ctx.device_info()
+    precision = D.PRECISION_MIXED if args.precision == "mixed" else D.PRECISION_FP64
+    batched = nsys > 1
+    S = D.System(ctx, mats if batched else mats[0])
+    dev = torch.device("cuda:0")
+    b_d = torch.from_numpy(np.concatenate(rhs)).to(dev)
+    x_d = torch.zeros(n_tot, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    bnorm = np.array([np.linalg.norm(r) for r in rhs])
+
+    def step():
+        x_d.zero_()
+        torch.cuda.synchronize()
+        rep_b = S.build(precision, D.SCHEDULE_BALANCED_NNZ)
+        rep_s = S.solve_device(b_d.data_ptr(), x_d.data_ptr(), rel_tol=1e-8, max_iter=1000, use_graph=True,
+                               graph_iters=args.graph_iters)
+        return rep_b, rep_s
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    spai_ms, solve_ms, iters = [], [], []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        rb, rs = step()
+        spai_ms.append(rb["ms"])
+        solve_ms.append(rs["ms"])
+        iters.append(rs["iterations_max"])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    oc = S.outcomes()
+    true_rel = [o["true_residual"] / bn for o, bn in zip(oc, bnorm)]
+    converged = all(o["status"] in (D.SOLVE_CONVERGED, D.SOLVE_CONVERGED_AT_START) for o in oc)
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = nsys * args.steps / elapsed
+    log(f"batched {args.precision}: {ms_per_step:.2f} ms/step, spai {np.median(spai_ms):.3f} ms, solve "
+        f"{np.median(solve_ms):.3f} ms, iters(max) {iters[-1]}, worst true rel {max(true_rel):.2e}")
+
+    # ------------------------------------------------------- roofline ----
+    m_bytes = 4 if precision == D.PRECISION_MIXED else 8
+    kernel_ms = {}
+    for which, name in ((2, "spmv_alpha"), (4, "spmv_omega"), (1, "mapply_p"), (3, "mapply_s")):
+        kernel_ms[name] = S.time_spmv(which, 50)
+    dominant = max(kernel_ms, key=kernel_ms.get)
+    kb = tile_bytes(dominant, n_tot, nnz_tot, m_bytes)
+    achieved = kb / (kernel_ms[dominant] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": int(kb), "launch_ms": round(kernel_ms[dominant], 5),
+                "all_kernels_ms": {k: round(v, 5) for k, v in kernel_ms.items()},
+                "all_kernels_frac": {k: round(tile_bytes(k, n_tot, nnz_tot, m_bytes) / (v * 1e-3) / 1e9 /
+                                              HBM_PEAK_GBS, 4) for k, v in kernel_ms.items()}}
+    # whole solve: every system's iterations x its per-iteration algorithmic
+    # bytes (K1..K5 as above plus the 64 n of the update kernel)
+    per_iter = 0.0
+    for A, o in zip(mats, oc):
+        it = o["iterations"] if o["status"] != D.SOLVE_MAX_ITER else o["iterations"] - 1
+        per_iter += it * ((2 * (m_bytes + 4) + 24) * A.nnz + 200 * A.n)
+    solve_gbs = per_iter / (float(np.median(solve_ms)) * 1e-3) / 1e9
+    roofline["solve_achieved"] = round(solve_gbs, 1)
+    roofline["solve_frac"] = round(solve_gbs / HBM_PEAK_GBS, 4)
+    log(f"roofline: {json.dumps(roofline)}")
+
+    # ------------------------------------------------- plain FP64 path ----
+    plain = None
+    if not args.no_plain:
+        systems = [D.System(ctx, A) for A in mats]
+        bs = [torch.from_numpy(r).to(dev) for r in rhs]
+        xs = [torch.zeros(A.n, dtype=torch.float64, device=dev) for A in mats]
+        torch.cuda.synchronize()
+
+        def plain_step():
+            tot_b = tot_s = 0.0
+            for Sk, bk, xk in zip(systems, bs, xs):
+                xk.zero_()
+                torch.cuda.synchronize()
+                rb = Sk.build(D.PRECISION_FP64, D.SCHEDULE_BALANCED_NNZ)
+                rs = Sk.solve_device(bk.data_ptr(), xk.data_ptr(), rel_tol=1e-8, max_iter=1000, use_graph=False,
+                                     check_every=1)
+                tot_b += rb["ms"]
+                tot_s += rs["ms"]
+            return tot_b, tot_s
+
+        plain_step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        pb, ps = [], []
+        for _ in range(args.plain_steps):
+            a, b_ = plain_step()
+            pb.append(a)
+            ps.append(b_)
+        torch.cuda.synchronize()
+        pel = time.perf_counter() - t1
+        ptrue = max(Sk.outcomes()[0]["true_residual"] / bn for Sk, bn in zip(systems, bnorm))
+        plain = {"value": round(nsys * args.plain_steps / pel, 3), "unit": "solves/s",
+                 "ms_per_step": round(1e3 * pel / args.plain_steps, 3), "spai_ms": round(float(np.median(pb)), 3),
+                 "solve_ms": round(float(np.median(ps)), 3), "worst_true_rel_residual": float(ptrue),
+                 "steps": args.plain_steps}
+        log(f"plain fp64: {json.dumps(plain)}")
+        for Sk in systems:
+            Sk.close()
+
+    # --------------------------------------------------- CPU baseline ----
+    cpu = None
+    if not args.no_cpu:
+        from oracle import binding as O
+
+        done, t_cpu = 0, 0.0
+        for A, r in zip(mats, rhs):
+            t2 = time.perf_counter()
+            m = O.spai(A.n, A.colptr, A.rowidx, A.val, 0)
+            Acsr = O.csc2csr(A.n, A.colptr, A.rowidx, A.val)[:3]
+            Mcsr = O.csc2csr(A.n, A.colptr, A.rowidx, m)[:3]
+            O.bicgstab(Acsr, Mcsr, r, np.zeros(A.n), 1e-8 * np.linalg.norm(r), 1000)
+            t_cpu += time.perf_counter() - t2
+            done += 1
+            if t_cpu >= args.cpu_budget:
+                break
+        cpu = {"value": round(done / t_cpu, 4), "unit": "solves/s", "cores": O.num_threads(), "kind": "port",
+               "sample": f"first {done} of {nsys} systems of the same batch ({sum(A.n for A in mats[:done])} "
+                         f"unknowns), oracle MGS SPAI + BiCGSTAB (FP64) on {cpu_model()}, "
+                         f"{t_cpu:.1f}s"}
+        log(f"cpu baseline: {json.dumps(cpu)}")
+
+    line = {
+        "metric": "SPAI build ms + BiCGSTAB solves/sec, mixed/batched vs plain FP64 at 1e-8 resid",
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if precision == D.PRECISION_MIXED else "f64",
+        "data": "synthetic (seeded generators)",
+        "config": {"workload": desc, "systems": nsys, "unknowns": n_tot, "nnz": nnz_tot,
+                   "precision": args.precision, "schedule": "balanced_nnz", "graph_iters": args.graph_iters,
+                   "rel_tol": 1e-8, "seed": args.seed, "device": dev_name, "cus": cus},
+        "spai_ms": round(float(np.median(spai_ms)), 4),
+        "solve_ms": round(float(np.median(solve_ms)), 4),
+        "iterations_max": int(iters[-1]),
+        "all_converged": bool(converged),
+        "worst_true_rel_residual": float(max(true_rel)),
+        "plain_fp64": plain,
+        "speedup_vs_plain_fp64": (round(value / plain["value"], 3) if plain else None),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    S.close()
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -44,10 +44,10 @@ C_SYMBOLS = [
     "spai_system_destroy", "spai_system_get_info", "spai_system_row_offsets",
     "spai_system_set_values", "spai_system_build", "spai_system_get_m",
     "spai_system_get_row_set_sizes", "spai_solve_options_default", "spai_system_solve",
-    "spai_system_outcomes", "spai_system_adjoint", "spai_system_time_spmv",
+    "spai_system_outcomes", "spai_system_residual_history", "spai_system_adjoint", "spai_system_time_spmv",
     "spai_csc_to_csr_device",
     "spai_problem_generate", "spai_problem_generate_power_law", "spai_problem_generate_batched",
-    "spai_problem_convection_diffusion", "spai_problem_read_mtx", "spai_problem_free",
+    "spai_problem_generate_grf", "spai_problem_convection_diffusion", "spai_problem_read_mtx", "spai_problem_free",
     "spai_problem_num_systems", "spai_problem_system", "spai_problem_rhs", "spai_problem_target",
     "spai_problem_name", "spai_problem_notes",
 ]
@@ -98,7 +98,7 @@ class BuildReport(ctypes.Structure):
 class SolveOptions(ctypes.Structure):
     _fields_ = [("rel_tol", ctypes.c_double), ("abs_tol", ctypes.c_double), ("max_iter", ctypes.c_int),
                 ("use_graph", ctypes.c_int), ("graph_iters", ctypes.c_int), ("check_every", ctypes.c_int),
-                ("precondition", ctypes.c_int)]
+                ("precondition", ctypes.c_int), ("record_history", ctypes.c_int)]
 
 
 class SolveReport(ctypes.Structure):
@@ -125,6 +125,14 @@ def load_library() -> ctypes.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise SpaiError(-6, f"libspai_mi355x.so not built at {LIB_PATH}: run __graft_entry__.build()")
+    # One HIP runtime per process: torch ships its own libamdhip64; loaded
+    # after this library, it would start a second runtime that cannot open the
+    # GPU.  Importing torch first makes this library bind to torch's runtime.
+    if os.environ.get("DCU_AMD_NO_TORCH") != "1":
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
     lib = ctypes.CDLL(LIB_PATH)
     vp, ip, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
     sig = {
@@ -150,6 +158,7 @@ def load_library() -> ctypes.CDLL:
         "spai_system_solve": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.POINTER(SolveOptions),
                                              ctypes.POINTER(SolveReport)]),
         "spai_system_outcomes": (ctypes.c_int, [vp, ctypes.POINTER(Outcome)]),
+        "spai_system_residual_history": (ctypes.c_int, [vp, ctypes.c_int, vp, ctypes.c_int]),
         "spai_system_adjoint": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_int, ctypes.POINTER(SolveOptions),
                                                ctypes.POINTER(SolveReport)]),
         "spai_system_time_spmv": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
@@ -158,6 +167,7 @@ def load_library() -> ctypes.CDLL:
         "spai_problem_generate": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(vp)]),
         "spai_problem_generate_power_law": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(vp)]),
         "spai_problem_generate_batched": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(vp)]),
+        "spai_problem_generate_grf": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(vp)]),
         "spai_problem_convection_diffusion": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                              ctypes.c_double, ctypes.POINTER(vp)]),
         "spai_problem_read_mtx": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(vp)]),
@@ -245,6 +255,13 @@ class Problem:
         return cls(h.value)
 
     @classmethod
+    def grf(cls, m: int, seed: int) -> "Problem":
+        lib = load_library()
+        h = ctypes.c_void_p()
+        _check(lib.spai_problem_generate_grf(int(m), int(seed), ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
     def convection_diffusion(cls, m: int, peclet: float, vx: float = 1.0, vy: float = 0.5) -> "Problem":
         lib = load_library()
         h = ctypes.c_void_p()
@@ -259,9 +276,12 @@ class Problem:
         return cls(h.value)
 
     def __del__(self):
-        if getattr(self, "_h", None) and self._h.value:
-            self._lib.spai_problem_free(self._h)
-            self._h = ctypes.c_void_p()
+        try:
+            if getattr(self, "_h", None) and self._h.value:
+                self._lib.spai_problem_free(self._h)
+                self._h = None
+        except Exception:  # interpreter shutdown
+            pass
 
     @property
     def name(self) -> str:
@@ -328,7 +348,7 @@ class Context:
 
 
 def _options(rel_tol=1e-8, abs_tol=0.0, max_iter=1000, use_graph=False, graph_iters=8, check_every=1,
-             precondition=True) -> SolveOptions:
+             precondition=True, record_history=False) -> SolveOptions:
     o = SolveOptions()
     o.rel_tol = float(rel_tol)
     o.abs_tol = float(abs_tol)
@@ -337,6 +357,7 @@ def _options(rel_tol=1e-8, abs_tol=0.0, max_iter=1000, use_graph=False, graph_it
     o.graph_iters = int(graph_iters)
     o.check_every = int(check_every)
     o.precondition = int(bool(precondition))
+    o.record_history = int(bool(record_history))
     return o
 
 
@@ -425,6 +446,13 @@ class System:
         _check(self._lib.spai_system_outcomes(self._h, arr))
         return [{"status": o.status, "iterations": o.iterations, "residual": o.residual,
                  "residual0": o.residual0, "true_residual": o.true_residual, "tol": o.tol} for o in arr]
+
+    def residual_history(self, s: int = 0, maxlen: int = 100000) -> np.ndarray:
+        out = np.empty(maxlen)
+        k = self._lib.spai_system_residual_history(self._h, int(s), out.ctypes.data, int(maxlen))
+        if k < 0:
+            _check(k)
+        return out[:k]
 
     def solve(self, b: np.ndarray, x0: Optional[np.ndarray] = None, **kw) -> Tuple[np.ndarray, dict, List[dict]]:
         b = np.ascontiguousarray(b, dtype=np.float64)

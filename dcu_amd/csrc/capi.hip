@@ -99,6 +99,10 @@ void validate_host_csc(const CSC_Matrix *A) {
 struct spai_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    // systems hold a pointer to their context: destroying the context while
+    // systems are alive defers the release to the last system's destroy
+    int live_systems = 0;
+    bool release_pending = false;
     int cus = 0;
     std::string name;
     SpaiBuilder builder;
@@ -121,6 +125,7 @@ struct spai_system {
     DeviceArray<double> stage_b, stage_x, stage_g, stage_dA, stage_db;
     float setup_ms = 0.0f;
     std::vector<SysState> last_states;
+    const BiCGStab *last_solver = nullptr;
 };
 
 struct spai_problem {
@@ -196,6 +201,7 @@ SolveOptions to_options(const spai_solve_options *o) {
         s.use_graph = o->use_graph;
         s.graph_iters = o->graph_iters;
         s.check_every = o->check_every;
+        s.record_history = o->record_history;
     }
     return s;
 }
@@ -257,20 +263,31 @@ int spai_context_create(int device, spai_context **out) {
         hipDeviceProp_t prop;
         SPAI_HIP(hipGetDeviceProperties(&prop, device));
         c->name = prop.name;
+        if (c->name.empty()) c->name = prop.gcnArchName;
         c->cus = prop.multiProcessorCount;
         SPAI_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         *out = c.release();
     });
 }
 
+namespace {
+void release_context(spai_context *ctx) {
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    hipStream_t st = ctx->stream;
+    delete ctx;
+    (void)hipStreamDestroy(st);
+}
+}  // namespace
+
 int spai_context_destroy(spai_context *ctx) {
     if (!ctx) return SPAI_OK;
     return guarded([&] {
-        (void)hipSetDevice(ctx->device);
-        (void)hipStreamSynchronize(ctx->stream);
-        hipStream_t st = ctx->stream;
-        delete ctx;
-        (void)hipStreamDestroy(st);
+        if (ctx->live_systems > 0) {
+            ctx->release_pending = true;
+            return;
+        }
+        release_context(ctx);
     });
 }
 
@@ -287,7 +304,10 @@ int spai_context_device_info(spai_context *ctx, char *name, int name_len, int *c
 int spai_system_create(spai_context *ctx, const CSC_Matrix *host_A, spai_system **out) {
     if (!ctx || !out) return fail(SPAI_ERR_INVALID, "ctx/out is NULL");
     *out = nullptr;
-    return guarded([&] { *out = create_from_host(ctx, {host_A}); });
+    return guarded([&] {
+        *out = create_from_host(ctx, {host_A});
+        ctx->live_systems++;
+    });
 }
 
 int spai_system_create_batched(spai_context *ctx, int nsys, const CSC_Matrix *host_systems, spai_system **out) {
@@ -297,6 +317,7 @@ int spai_system_create_batched(spai_context *ctx, int nsys, const CSC_Matrix *ho
         std::vector<const CSC_Matrix *> parts;
         for (int s = 0; s < nsys; ++s) parts.push_back(host_systems + s);
         *out = create_from_host(ctx, parts);
+        ctx->live_systems++;
     });
 }
 
@@ -331,15 +352,18 @@ int spai_system_create_device(spai_context *ctx, const CSC_Matrix *dev_A, spai_s
         finish_setup(S.get(), st);
         S->setup_ms = tm.stop();
         *out = S.release();
+        ctx->live_systems++;
     });
 }
 
 int spai_system_destroy(spai_system *sys) {
     if (!sys) return SPAI_OK;
     return guarded([&] {
-        (void)hipSetDevice(sys->ctx->device);
-        (void)hipStreamSynchronize(sys->ctx->stream);
+        spai_context *ctx = sys->ctx;
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
         delete sys;
+        if (--ctx->live_systems == 0 && ctx->release_pending) release_context(ctx);
     });
 }
 
@@ -483,6 +507,7 @@ void spai_solve_options_default(spai_solve_options *opt) {
     opt->graph_iters = 8;
     opt->check_every = 1;
     opt->precondition = 1;
+    opt->record_history = 0;
 }
 
 int spai_system_solve(spai_system *sys, const double *b, double *x, int on_device, const spai_solve_options *opt,
@@ -510,6 +535,7 @@ int spai_system_solve(spai_system *sys, const double *b, double *x, int on_devic
         SolveResult res;
         sys->ctx->solver.solve(A, M, sys->tiles, sys->n, sys->nsys, db, dx, to_options(opt), st, &res);
         sys->last_states = sys->ctx->solver.states();
+        sys->last_solver = &sys->ctx->solver;
         if (!on_device) {
             SPAI_HIP(hipMemcpyAsync(x, dx, sizeof(double) * sys->n, hipMemcpyDeviceToHost, st));
             SPAI_HIP(hipStreamSynchronize(st));
@@ -531,6 +557,12 @@ int spai_system_outcomes(spai_system *sys, spai_system_outcome *out) {
         out[s].tol = S.tol;
     }
     return SPAI_OK;
+}
+
+int spai_system_residual_history(spai_system *sys, int s, double *out, int maxlen) {
+    if (!sys || !out || maxlen < 0) return fail(SPAI_ERR_INVALID, "sys/out is NULL");
+    if (!sys->last_solver || s < 0 || s >= sys->nsys) return fail(SPAI_ERR_INVALID, "no solve or bad system index");
+    return sys->last_solver->history(s, out, maxlen);
 }
 
 int spai_system_adjoint(spai_system *sys, const double *x, const double *g, double *dA, double *db, int on_device,
@@ -565,6 +597,7 @@ int spai_system_adjoint(spai_system *sys, const double *x, const double *g, doub
         SolveResult res;
         sys->ctx->adjoint_solver.solve(At, Mt, sys->tiles_t, sys->n, sys->nsys, dg, ddb, to_options(opt), st, &res);
         sys->last_states = sys->ctx->adjoint_solver.states();
+        sys->last_solver = &sys->ctx->adjoint_solver;
         adjoint_scatter(sys->n, sys->colptr.get(), sys->rowidx.get(), ddb, dx, ddA, st);
         if (!on_device) {
             SPAI_HIP(hipMemcpyAsync(dA, ddA, sizeof(double) * (size_t)sys->nnz, hipMemcpyDeviceToHost, st));
@@ -577,11 +610,11 @@ int spai_system_adjoint(spai_system *sys, const double *x, const double *g, doub
 
 int spai_system_time_spmv(spai_system *sys, int which, int reps, float *ms) {
     if (!sys || !ms || reps < 1) return fail(SPAI_ERR_INVALID, "bad arguments");
-    if (which == 1 && sys->m_precision < 0) return fail(SPAI_ERR_NOT_BUILT, "M has not been built");
+    if (which != 0 && sys->m_precision < 0) return fail(SPAI_ERR_NOT_BUILT, "M has not been built");
     return guarded([&] {
         SPAI_HIP(hipSetDevice(sys->ctx->device));
         OperatorView A{sys->rowptr.get(), sys->colidx.get(), sys->csrval.get(), 0};
-        OperatorView M = which == 1 ? m_view(sys, false) : OperatorView{};
+        OperatorView M = which != 0 ? m_view(sys, false) : OperatorView{};
         *ms = sys->ctx->solver.time_kernel(which, A, M, sys->tiles, sys->n, sys->nsys, reps, sys->ctx->stream);
     });
 }
@@ -623,6 +656,16 @@ int spai_problem_generate_batched(int nsys, uint64_t seed, spai_problem **out) {
     return guarded([&] {
         auto P = std::make_unique<spai_problem>();
         P->p = generate_batched(seed, nsys);
+        *out = P.release();
+    });
+}
+
+int spai_problem_generate_grf(int m, uint64_t seed, spai_problem **out) {
+    if (!out || m < 2) return fail(SPAI_ERR_INVALID, "bad grid size");
+    *out = nullptr;
+    return guarded([&] {
+        auto P = std::make_unique<spai_problem>();
+        P->p = generate_diffusion_grf_64(seed, m);
         *out = P.release();
     });
 }

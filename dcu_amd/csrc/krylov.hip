@@ -33,7 +33,7 @@ namespace spai {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kLdsProducts = 4096;  // products staged per tile (32 KB)
+constexpr int kLdsProducts = 2048;  // products staged per tile (16 KB)
 
 // ------------------------------------------------------------ transpose ----
 __global__ void k_fill_int(int *p, int v, long long n) {
@@ -208,31 +208,40 @@ __device__ __forceinline__ void block_reduce(double (&v)[NRED], double *sh) {
 
 struct Ctx {
     SysState *st;
-    const int *sys_tiles;
-    int *counter;    // [nsys] + n_active at [nsys]
+    const int *sys_units;  // nsys + 1 offsets of the reducing workgroups (groups) per system
+    int *counter;          // [nsys] + n_active at [nsys]
     double *partials;
     int nsys;
+    double *hist;          // optional ||r_k|| history, hist_stride entries per system
+    int hist_stride;
 };
 
 // Publishes this tile's partials; the last tile of the system sums all of
-// them in tile order and calls op.finish.  Returns nothing; op.finish runs on
-// thread 0 of the last tile only.
+// them in tile order and calls op.finish on its thread 0.
+//
+// No __threadfence(): on gfx950 an agent-scope fence writes back and
+// invalidates the whole L2 (buffer_wbl2 / buffer_inv), which, issued by
+// thousands of tiles, evicts the vectors the other tiles are gathering.
+// Instead the partials are agent-scope atomic stores (coherent across the
+// XCDs' L2s), the thread waits until they are acknowledged (vmcnt(0)) before
+// the relaxed counter increment, and the last tile reads them back with
+// agent-scope atomic loads.
 template <int NRED, class Op>
 __device__ __forceinline__ void reduce_system(const Ctx &c, int tile, int s, double (&v)[NRED], Op &op, double *sh) {
     block_reduce<NRED>(v, sh);
     __shared__ int am_last;
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < NRED; ++q) c.partials[(size_t)tile * NRED + q] = v[q];
-        __threadfence();
-        int nt = c.sys_tiles[s + 1] - c.sys_tiles[s];
-        int prev = atomicAdd(&c.counter[s], 1);
+        for (int q = 0; q < NRED; ++q)
+            __hip_atomic_store(&c.partials[(size_t)tile * NRED + q], v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        int nt = c.sys_units[s + 1] - c.sys_units[s];
+        int prev = __hip_atomic_fetch_add(&c.counter[s], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         am_last = (prev == nt - 1);
     }
     __syncthreads();
     if (!am_last) return;
-    __threadfence();
-    const int t0 = c.sys_tiles[s], t1 = c.sys_tiles[s + 1];
+    const int t0 = c.sys_units[s], t1 = c.sys_units[s + 1];
     double acc[NRED];
 #pragma unroll
     for (int q = 0; q < NRED; ++q) acc[q] = 0.0;
@@ -242,39 +251,72 @@ __device__ __forceinline__ void reduce_system(const Ctx &c, int tile, int s, dou
     }
     block_reduce<NRED>(acc, sh);
     if (threadIdx.x == 0) {
-        c.counter[s] = 0;
+        __hip_atomic_store(&c.counter[s], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         op.finish(c, s, acc);
     }
 }
 
 // ------------------------------------------------------- tile kernels ----
-// y_i = sum_p val[p] * op.gather(idx[p]) for the rows of one tile, then
-// op.row(i, y_i, acc).  HAS_MATRIX = false skips the product (streaming ops).
+struct TileView {
+    const int *row;         // tile row boundaries
+    const int *sys;         // system per tile
+    const int *group_tile;  // first tile per group
+    const int *group_sys;   // system per group
+};
+
+// For every row i of the workgroup's tile(s): y_i = sum_p val[p] *
+// op.gather(idx[p]), then op.row(i, y_i, acc).  HAS_MATRIX = false skips the
+// product (streaming ops).  Ops with reductions run one workgroup per group of
+// tiles (GROUPED) and publish one partial per group.
 template <bool HAS_MATRIX, class MT, class Op>
 __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr, const int *__restrict__ idx,
-                                                    const MT *__restrict__ val, const int *__restrict__ tile_row,
-                                                    const int *__restrict__ tile_sys, Ctx c, Op op) {
+                                                    const MT *__restrict__ val, TileView tv, Ctx c, Op op) {
     constexpr int NRED = Op::NRED;
+    constexpr bool GROUPED = NRED > 0;
     __shared__ double prod[HAS_MATRIX ? kLdsProducts : 1];
     __shared__ double sh[(NRED > 0 ? NRED : 1) * 4];
-    const int tile = blockIdx.x;
-    const int s = tile_sys[tile];
+    const int unit = blockIdx.x;
+    const int s = GROUPED ? tv.group_sys[unit] : tv.sys[unit];
     if (!op.enabled(c.st[s])) return;
     op.load(c.st[s]);
-    const int r0 = tile_row[tile], r1 = tile_row[tile + 1];
+    const int ft0 = GROUPED ? tv.group_tile[unit] : unit;
+    const int ft1 = GROUPED ? tv.group_tile[unit + 1] : unit + 1;
     double acc[NRED > 0 ? NRED : 1];
 #pragma unroll
     for (int q = 0; q < (NRED > 0 ? NRED : 1); ++q) acc[q] = 0.0;
 
-    if (!HAS_MATRIX) {
-        for (int i = r0 + (int)threadIdx.x; i < r1; i += kThreads) op.row(i, 0.0, acc);
-    } else {
+    for (int ft = ft0; ft < ft1; ++ft) {
+        const int r0 = tv.row[ft], r1 = tv.row[ft + 1];
+        if (!HAS_MATRIX) {
+            for (int i = r0 + (int)threadIdx.x; i < r1; i += kThreads) op.row(i, 0.0, acc);
+            continue;
+        }
         const int p0 = ptr[r0], p1 = ptr[r1];
         const int cnt = p1 - p0;
         if (cnt <= kLdsProducts) {
-            for (int q = threadIdx.x; q < cnt; q += kThreads) {
-                int j = idx[p0 + q];
-                prod[q] = (double)val[p0 + q] * op.gather(j);
+            // all index/value loads of a batch first, then all gathers, so the
+            // dependent loads overlap instead of running one after another
+            constexpr int U = 8;
+            for (int base = 0; base < cnt; base += kThreads * U) {
+                int jj[U];
+                MT vv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    int q = base + u * kThreads + (int)threadIdx.x;
+                    jj[u] = q < cnt ? idx[p0 + q] : 0;
+                    vv[u] = q < cnt ? val[p0 + q] : (MT)0;
+                }
+                double gv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    int q = base + u * kThreads + (int)threadIdx.x;
+                    gv[u] = q < cnt ? op.gather(jj[u]) : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    int q = base + u * kThreads + (int)threadIdx.x;
+                    if (q < cnt) prod[q] = (double)vv[u] * gv[u];
+                }
             }
             __syncthreads();
             for (int i = r0 + (int)threadIdx.x; i < r1; i += kThreads) {
@@ -283,6 +325,7 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
                 for (int q = b; q < e; ++q) sum += prod[q];
                 op.row(i, sum, acc);
             }
+            __syncthreads();  // prod is reused by the next tile
         } else {
             // long rows: the whole workgroup reduces one row at a time
             for (int i = r0; i < r1; ++i) {
@@ -298,7 +341,7 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
         double v[NRED];
 #pragma unroll
         for (int q = 0; q < NRED; ++q) v[q] = acc[q];
-        reduce_system<NRED>(c, tile, s, v, op, sh);
+        reduce_system<NRED>(c, unit, s, v, op, sh);
     }
 }
 
@@ -439,6 +482,7 @@ struct OpUpdate {  // K5: x, r update; rho', beta, ||r||, stop test
         S.rr = tot[1];
         S.iters += 1;
         double err = sqrt(tot[1]);
+        if (c.hist && S.iters <= c.hist_stride) c.hist[(size_t)s * c.hist_stride + S.iters - 1] = err;
         bool stop = false;
         if (err < S.tol) {
             S.status = kConverged;
@@ -531,6 +575,15 @@ __global__ void k_state_init(SysState *st, int nsys, int max_iter, int *counter)
     if (s == 0) counter[nsys] = nsys;
 }
 
+// re-enables every system (scalars kept) so single kernels can be timed
+__global__ void k_state_activate(SysState *st, int nsys, int *counter) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsys) {
+        st[s].active = 1;
+        counter[s] = 0;
+    }
+}
+
 __global__ void k_adjoint_scatter(int n, const int *colptr, const int *rowidx, const double *lambda, const double *x,
                                   double *dA) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -546,8 +599,10 @@ __global__ void k_f32_to_f64(const float *in, double *out, long long n) {
 
 template <bool HM, class MT, class Op>
 void launch_tiles(const OperatorView &O, const TileMap &tm, const Ctx &c, const Op &op, hipStream_t st) {
-    hipLaunchKernelGGL((k_tiles<HM, MT, Op>), dim3(tm.ntiles), dim3(kThreads), 0, st, O.ptr, O.idx,
-                       (const MT *)O.val, tm.row.get(), tm.sys.get(), c, op);
+    TileView tv{tm.row.get(), tm.sys.get(), tm.group_tile.get(), tm.group_sys.get()};
+    const int grid = Op::NRED > 0 ? tm.ngroups : tm.ntiles;
+    hipLaunchKernelGGL((k_tiles<HM, MT, Op>), dim3(grid), dim3(kThreads), 0, st, O.ptr, O.idx, (const MT *)O.val,
+                       tv, c, op);
 }
 
 }  // namespace
@@ -579,10 +634,11 @@ void build_tiles(TileMap &tm, const int *ptr, const std::vector<int> &h_row_off,
     const int nsys = (int)h_row_off.size() - 1;
     require(nsys >= 1, "build_tiles: no systems");
     long long total = h_nnz_off[(size_t)nsys];
-    // aim for >= ~2048 tiles on large inputs, 256..2048 nonzeros each
+    // aim for >= ~2048 tiles on large inputs, 256..1024 nonzeros each (a
+    // tile plus its longest row must fit the 2048 staged products)
     long long t = total / 2048;
     int target = 256;
-    while (target < 2048 && target < t) target <<= 1;
+    while (target < 1024 && target < t) target <<= 1;
     tm.target_nnz = target;
     tm.h_sys_tiles.assign((size_t)nsys + 1, 0);
     for (int s = 0; s < nsys; ++s) {
@@ -608,6 +664,25 @@ void build_tiles(TileMap &tm, const int *ptr, const std::vector<int> &h_row_off,
                        tm.sys.get(), tm.sys_tiles.get(), roff.get(), target, tm.row.get(),
                        h_row_off[(size_t)nsys]);
     SPAI_CHECK_LAUNCH();
+    // groups of up to group_size consecutive tiles inside each system
+    tm.group_size = 4;
+    std::vector<int> gtile, gsys, sgroups((size_t)nsys + 1, 0);
+    for (int s = 0; s < nsys; ++s) {
+        for (int t = tm.h_sys_tiles[(size_t)s]; t < tm.h_sys_tiles[(size_t)s + 1]; t += tm.group_size) {
+            gtile.push_back(t);
+            gsys.push_back(s);
+        }
+        sgroups[(size_t)s + 1] = (int)gsys.size();
+    }
+    gtile.push_back(tm.ntiles);
+    tm.ngroups = (int)gsys.size();
+    tm.group_tile.resize(gtile.size());
+    tm.group_sys.resize(gsys.size());
+    tm.sys_groups.resize(sgroups.size());
+    SPAI_HIP(hipMemcpyAsync(tm.group_tile.get(), gtile.data(), sizeof(int) * gtile.size(), hipMemcpyHostToDevice, st));
+    SPAI_HIP(hipMemcpyAsync(tm.group_sys.get(), gsys.data(), sizeof(int) * gsys.size(), hipMemcpyHostToDevice, st));
+    SPAI_HIP(hipMemcpyAsync(tm.sys_groups.get(), sgroups.data(), sizeof(int) * sgroups.size(), hipMemcpyHostToDevice,
+                            st));
     SPAI_HIP(hipStreamSynchronize(st));
 }
 
@@ -669,7 +744,7 @@ void BiCGStab::ensure(int n, int nsys, int ntiles) {
 
 void BiCGStab::enqueue_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tm, int parity,
                                  hipStream_t st) {
-    Ctx c{state_.get(), tm.sys_tiles.get(), counter_.get(), partials_.get(), nsys_};
+    Ctx c{state_.get(), tm.sys_groups.get(), counter_.get(), partials_.get(), nsys_, hist_ptr_, hist_stride_};
     double *pin = p_[parity], *pout = p_[parity ^ 1];
     if (M.val) {
         if (M.fp32) {
@@ -714,16 +789,25 @@ void BiCGStab::enqueue_iteration(const OperatorView &A, const OperatorView &M, c
 void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap &tm, int n, int nsys,
                      const double *b, double *x, const SolveOptions &opt, hipStream_t st, SolveResult *res) {
     require(n > 0 && nsys >= 1 && A.ptr && A.idx && A.val && b && x, "bicgstab: bad arguments");
-    require(tm.ntiles > 0 && (int)tm.h_sys_tiles.size() == nsys + 1, "bicgstab: tiles do not match the systems");
+    require(tm.ntiles > 0 && tm.ngroups > 0 && (int)tm.h_sys_tiles.size() == nsys + 1,
+            "bicgstab: tiles do not match the systems");
     require(opt.max_iter >= 1, "bicgstab: max_iter must be >= 1");
-    ensure(n, nsys, tm.ntiles);
+    ensure(n, nsys, tm.ngroups);
     b_ = b;
     x_ = x;
+    if (opt.record_history) {
+        hist_.reserve((size_t)nsys * (size_t)opt.max_iter);
+        hist_ptr_ = hist_.get();
+        hist_stride_ = opt.max_iter;
+    } else {
+        hist_ptr_ = nullptr;
+        hist_stride_ = 0;
+    }
     hipEvent_t e0, e1;
     SPAI_HIP(hipEventCreate(&e0));
     SPAI_HIP(hipEventCreate(&e1));
     SPAI_HIP(hipEventRecord(e0, st));
-    Ctx c{state_.get(), tm.sys_tiles.get(), counter_.get(), partials_.get(), nsys};
+    Ctx c{state_.get(), tm.sys_groups.get(), counter_.get(), partials_.get(), nsys, hist_ptr_, hist_stride_};
     hipLaunchKernelGGL(k_state_init, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys, opt.max_iter,
                        counter_.get());
     {
@@ -747,9 +831,9 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
         active = read_active();
         if (opt.use_graph) {
             int gi = std::max(2, opt.graph_iters + (opt.graph_iters & 1));
-            const void *key[6] = {A.val, M.val, tm.row.get(), b, x, (const void *)(size_t)M.fp32};
+            const void *key[7] = {A.val, M.val, tm.row.get(), b, x, (const void *)(size_t)M.fp32, hist_ptr_};
             bool same = graph_exec_ && gi == graph_iters_;
-            for (int q = 0; q < 6 && same; ++q) same = key[q] == graph_key_[q];
+            for (int q = 0; q < 7 && same; ++q) same = key[q] == graph_key_[q];
             if (!same) {
                 drop_graph();
                 hipStream_t cap;
@@ -760,7 +844,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                 SPAI_HIP(hipStreamDestroy(cap));
                 SPAI_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
                 graph_iters_ = gi;
-                for (int q = 0; q < 6; ++q) graph_key_[q] = key[q];
+                for (int q = 0; q < 7; ++q) graph_key_[q] = key[q];
             }
             while (active > 0 && done_iters < max_launch_iters) {
                 SPAI_HIP(hipGraphLaunch(graph_exec_, st));
@@ -799,6 +883,11 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     SPAI_HIP(hipEventDestroy(e1));
     h_states_.resize((size_t)nsys);
     SPAI_HIP(hipMemcpy(h_states_.data(), state_.get(), sizeof(SysState) * nsys, hipMemcpyDeviceToHost));
+    h_hist_.clear();
+    if (hist_ptr_) {
+        h_hist_.resize((size_t)nsys * hist_stride_);
+        SPAI_HIP(hipMemcpy(h_hist_.data(), hist_ptr_, sizeof(double) * h_hist_.size(), hipMemcpyDeviceToHost));
+    }
     if (res) {
         res->ms = ms;
         res->host_checks = checks;
@@ -813,23 +902,65 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     }
 }
 
+int BiCGStab::history(int s, double *out, int maxlen) const {
+    if (h_hist_.empty() || s < 0 || s >= (int)h_states_.size()) return 0;
+    int k = std::min(h_states_[(size_t)s].iters, hist_stride_);
+    if (h_states_[(size_t)s].status == kMaxIter) k = std::min(k, hist_stride_ - 1);
+    k = std::min(k, maxlen);
+    for (int i = 0; i < k; ++i) out[i] = h_hist_[(size_t)s * hist_stride_ + i];
+    return k;
+}
+
 float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView &M, const TileMap &tm, int n,
                             int nsys, int reps, hipStream_t st) {
-    ensure(n, nsys, tm.ntiles);
-    Ctx c{state_.get(), tm.sys_tiles.get(), counter_.get(), partials_.get(), nsys};
+    // which: 0 bare A*x, 5 bare M*x, 1..4 the fused kernels K1..K4 with the
+    // scalars and vectors of the last solve (systems re-activated first)
+    require(which >= 0 && which <= 5, "time_kernel: which must be 0..5");
+    require((which != 1 && which != 3 && which != 5) || M.val, "time_kernel: no preconditioner");
+    ensure(n, nsys, tm.ngroups);
+    Ctx c{state_.get(), tm.sys_groups.get(), counter_.get(), partials_.get(), nsys, hist_ptr_, hist_stride_};
+    hipLaunchKernelGGL(k_state_activate, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys,
+                       counter_.get());
     hipEvent_t e0, e1;
     SPAI_HIP(hipEventCreate(&e0));
     SPAI_HIP(hipEventCreate(&e1));
     auto once = [&]() {
-        if (which == 0) {
+        switch (which) {
+        case 0: {
             OpSpmvPlain op{p_[0], y_};
             launch_tiles<true, double>(A, tm, c, op, st);
-        } else if (M.fp32) {
+        } break;
+        case 5: {
             OpSpmvPlain op{p_[0], y_};
-            launch_tiles<true, float>(M, tm, c, op, st);
-        } else {
-            OpSpmvPlain op{p_[0], y_};
-            launch_tiles<true, double>(M, tm, c, op, st);
+            if (M.fp32) launch_tiles<true, float>(M, tm, c, op, st);
+            else launch_tiles<true, double>(M, tm, c, op, st);
+        } break;
+        case 1: {
+            if (M.fp32) {
+                OpMapplyP<float> op{r_, p_[0], a_, p_[1], y_, 0.0, 0.0};
+                launch_tiles<true, float>(M, tm, c, op, st);
+            } else {
+                OpMapplyP<double> op{r_, p_[0], a_, p_[1], y_, 0.0, 0.0};
+                launch_tiles<true, double>(M, tm, c, op, st);
+            }
+        } break;
+        case 2: {
+            OpSpmvAlpha op{y_, rs_, a_};
+            launch_tiles<true, double>(A, tm, c, op, st);
+        } break;
+        case 3: {
+            if (M.fp32) {
+                OpMapplyS<float> op{r_, a_, s_, z_, 0.0};
+                launch_tiles<true, float>(M, tm, c, op, st);
+            } else {
+                OpMapplyS<double> op{r_, a_, s_, z_, 0.0};
+                launch_tiles<true, double>(M, tm, c, op, st);
+            }
+        } break;
+        case 4: {
+            OpSpmvOmega op{z_, s_, t_};
+            launch_tiles<true, double>(A, tm, c, op, st);
+        } break;
         }
     };
     once();

@@ -29,7 +29,10 @@ struct alignas(64) SysState {
 enum SolveStatus { kRunning = 0, kConverged = 1, kMaxIter = 2, kBreakdown = 3, kConvergedAtStart = 4 };
 
 // Row tiles: contiguous row ranges, none crossing a system boundary, each
-// holding about `target_nnz` nonzeros.  One workgroup per tile.
+// holding about `target_nnz` nonzeros (at most one LDS staging buffer plus one
+// row).  Kernels without a reduction run one workgroup per tile; kernels that
+// reduce run one workgroup per group of `group_size` consecutive tiles of one
+// system, so that each workgroup publishes one partial for several tiles.
 struct TileMap {
     int ntiles = 0;
     int target_nnz = 0;
@@ -37,6 +40,11 @@ struct TileMap {
     DeviceArray<int> sys;        // system of each tile
     DeviceArray<int> sys_tiles;  // nsys + 1 tile offsets per system
     std::vector<int> h_sys_tiles;
+    int ngroups = 0;
+    int group_size = 0;
+    DeviceArray<int> group_tile;  // ngroups + 1: first tile of each group
+    DeviceArray<int> group_sys;   // system of each group
+    DeviceArray<int> sys_groups;  // nsys + 1 group offsets per system
 };
 
 // Device CSC -> CSR (rows ordered by column) plus perm[csc_pos] = csr_pos.
@@ -57,6 +65,7 @@ struct SolveOptions {
     int use_graph = 0;       // capture iterations in a hipGraph
     int graph_iters = 8;     // iterations per graph launch (rounded up to even)
     int check_every = 1;     // iterations between host convergence checks (no graph)
+    int record_history = 0;  // keep ||r_k|| per iteration and system
 };
 
 struct SolveResult {
@@ -86,6 +95,9 @@ public:
                double *x, const SolveOptions &opt, hipStream_t st, SolveResult *res);
     // Per-system results of the last solve (host copies).
     const std::vector<SysState> &states() const { return h_states_; }
+    // ||r_k||_2 after iteration k = 1.. of system s in the last solve that
+    // recorded a history; returns the number of values written.
+    int history(int s, double *out, int maxlen) const;
     // Average device time of one A*vector tile launch over `reps` launches on
     // the solver's stream (roofline measurement of the dominant kernel).
     float time_kernel(int which, const OperatorView &A, const OperatorView &M, const TileMap &tiles, int n, int nsys,
@@ -108,9 +120,13 @@ private:
     // graph cache
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
-    const void *graph_key_[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const void *graph_key_[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     int graph_iters_ = 0;
     void drop_graph();
+    DeviceArray<double> hist_;
+    double *hist_ptr_ = nullptr;
+    int hist_stride_ = 0;
+    std::vector<double> h_hist_;
 };
 
 // dL/dA on A's CSC pattern: dA[p] = -lambda[rowidx[p]] * x[col(p)].

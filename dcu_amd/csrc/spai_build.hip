@@ -663,15 +663,26 @@ __global__ void spai_zero_counters(int *c, int n, unsigned long long *s64, int n
     if (i < n64) s64[i] = 0ull;
 }
 
-// One thread per column: n2_k, T_k = sum_j |A(:,J_k[j])|, tier; wave-level
-// aggregation keeps the atomics at one per wavefront per tier.
+// One thread per column: n2_k, T_k = sum_j |A(:,J_k[j])|, tier.  Counts and
+// maxima are combined in LDS first, so a workgroup issues one global atomic
+// per counter instead of one per wavefront (same-address atomics serialise).
 template <bool DYNAMIC>
 __global__ __launch_bounds__(256) void spai_plan_kernel(const int *colptr, const int *rowidx, int col_begin,
                                                         int col_end, int *counters,
                                                         unsigned long long *stats64, int *lists,
                                                         int list_stride) {
-    const int k = col_begin + blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
+    __shared__ int s_cnt[kTiers], s_base[kTiers], s_maxT, s_maxN, s_empty;
+    __shared__ unsigned long long s_sumT;
+    const int tid = threadIdx.x;
+    if (tid < kTiers) s_cnt[tid] = 0;
+    if (tid == 0) {
+        s_maxT = 0;
+        s_maxN = 0;
+        s_empty = 0;
+        s_sumT = 0ull;
+    }
+    __syncthreads();
+    const int k = col_begin + blockIdx.x * blockDim.x + tid;
     int T = 0, n2 = 0, tier = -1;
     if (k < col_end) {
         int c0 = colptr[k];
@@ -682,36 +693,25 @@ __global__ __launch_bounds__(256) void spai_plan_kernel(const int *colptr, const
         }
         tier = n2 > 0 ? classify(T, n2, lds_tier_fits(T, n2)) : -1;
     }
-    int mT = T, mN = n2;
-    unsigned long long sT = (unsigned long long)T;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        int a = __shfl_xor(mT, o, 64), b = __shfl_xor(mN, o, 64);
-        mT = a > mT ? a : mT;
-        mN = b > mN ? b : mN;
-        sT += __shfl_xor(sT, o, 64);
+    // slot of this column inside its tier, local to the workgroup
+    int local = -1;
+    if (tier >= 0) local = atomicAdd(&s_cnt[tier], 1);
+    if (k < col_end) {
+        atomicMax(&s_maxT, T);
+        atomicMax(&s_maxN, n2);
+        atomicAdd(&s_sumT, (unsigned long long)T);
+        if (n2 == 0) atomicAdd(&s_empty, 1);
     }
-    const unsigned long long empty_mask = __ballot(k < col_end && n2 == 0);
-    if (lane == 0) {
-        atomicMax(&counters[2 * kTiers], mT);
-        atomicMax(&counters[2 * kTiers + 1], mN);
-        if (empty_mask) atomicAdd(&counters[2 * kTiers + 2], (int)__popcll(empty_mask));
-        atomicAdd(&stats64[0], sT);
+    __syncthreads();
+    if (tid < kTiers) s_base[tid] = s_cnt[tid] ? atomicAdd(&counters[tid], s_cnt[tid]) : 0;
+    if (tid == 0) {
+        atomicMax(&counters[2 * kTiers], s_maxT);
+        atomicMax(&counters[2 * kTiers + 1], s_maxN);
+        if (s_empty) atomicAdd(&counters[2 * kTiers + 2], s_empty);
+        atomicAdd(&stats64[0], s_sumT);
     }
-#pragma unroll
-    for (int t = 0; t < kTiers; ++t) {
-        const unsigned long long m = __ballot(tier == t);
-        if (m) {
-            const int leader = __ffsll((long long)m) - 1;
-            int base = 0;
-            if (lane == leader) base = atomicAdd(&counters[t], (int)__popcll(m));
-            base = __shfl(base, leader, 64);
-            if (DYNAMIC && tier == t) {
-                int rank = (int)__popcll(m & ((1ull << lane) - 1ull));
-                lists[(size_t)t * list_stride + base + rank] = k;
-            }
-        }
-    }
+    __syncthreads();
+    if (DYNAMIC && tier >= 0) lists[(size_t)tier * list_stride + s_base[tier] + local] = k;
 }
 
 template <class KernelT>

@@ -131,6 +131,7 @@ typedef struct {
     int graph_iters;         /* iterations per graph launch */
     int check_every;         /* iterations between host convergence checks without a graph */
     int precondition;        /* 0: no preconditioner (M = I) */
+    int record_history;      /* 1: keep ||r_k||_2 per iteration (spai_system_residual_history) */
 } spai_solve_options;
 
 void spai_solve_options_default(spai_solve_options *opt);
@@ -158,6 +159,10 @@ int spai_system_solve(spai_system *sys, const double *b, double *x, int on_devic
                       spai_solve_report *report);
 /* Per-system outcome of the last solve (nsys entries). */
 int spai_system_outcomes(spai_system *sys, spai_system_outcome *out);
+/* ||r_k||_2 after iterations k = 1.. of system s in the last solve or adjoint
+ * run with record_history = 1; returns the number of values written (>= 0)
+ * or a negative status. */
+int spai_system_residual_history(spai_system *sys, int s, double *out, int maxlen);
 
 /* Adjoint of the solve for L(x): given the converged x and g = dL/dx, solves
  * A^T lambda = g with BiCGSTAB right-preconditioned by M^T (CSC(M) used as
@@ -166,8 +171,11 @@ int spai_system_outcomes(spai_system *sys, spai_system_outcome *out);
 int spai_system_adjoint(spai_system *sys, const double *x, const double *g, double *dA, double *db, int on_device,
                         const spai_solve_options *opt, spai_solve_report *report);
 
-/* Average device time (ms) of one bare product y = A x (which = 0) or
- * y = M x (which = 1) with the solver's tile kernel, over `reps` launches. */
+/* Average device time (ms) of one launch of a solver kernel over `reps`
+ * launches on the context's stream: which = 0 bare y = A x, 5 bare y = M x,
+ * 1..4 the fused iteration kernels (1: y = M p', 2: a = A y + (r*, a),
+ * 3: z = M s, 4: t = A z + (t, s), (t, t)) on the vectors and scalars of the
+ * last solve. */
 int spai_system_time_spmv(spai_system *sys, int which, int reps, float *ms);
 
 /* ------------------------------------------------- format conversion ---- */
@@ -186,6 +194,8 @@ int spai_problem_generate(int config, uint64_t seed, spai_problem **out);
 int spai_problem_generate_power_law(int n, uint64_t seed, spai_problem **out);
 /* Batch generator with a chosen system count (tests). */
 int spai_problem_generate_batched(int nsys, uint64_t seed, spai_problem **out);
+/* Lognormal-GRF 3D diffusion (the 64^3 generator) on an m^3 grid (tests). */
+int spai_problem_generate_grf(int m, uint64_t seed, spai_problem **out);
 /* 2D conv-diff on an m x m grid / 3D diffusion with given cell kappa. */
 int spai_problem_convection_diffusion(int m, double peclet, double vx, double vy, spai_problem **out);
 /* MatrixMarket file with the reference's reader semantics; b = ones. */

@@ -36,10 +36,13 @@ def lib() -> ctypes.CDLL:
         L.oracle_spmv.argtypes = [i, vp, vp, vp, vp, vp]
         L.oracle_spmv.restype = None
         L.oracle_bicgstab.argtypes = [i, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_double, i, vp]
+        L.oracle_bicgstab_history.argtypes = [i, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_double, i, vp, vp]
         L.oracle_true_residual.argtypes = [i, vp, vp, vp, vp, vp]
         L.oracle_true_residual.restype = ctypes.c_double
         L.oracle_adjoint.argtypes = [i, vp, vp, vp, vp, vp, vp, ctypes.c_double, i, vp, vp, vp]
         L.oracle_num_threads.restype = i
+        L.oracle_set_threads.argtypes = [i]
+        L.oracle_set_threads.restype = None
         _lib = L
     return _lib
 
@@ -73,6 +76,25 @@ def _f64(a):
 
 def num_threads() -> int:
     return lib().oracle_num_threads()
+
+
+def set_threads(t: int) -> None:
+    lib().oracle_set_threads(int(t))
+
+
+def iteration_spread(a_csr, m_csr, b, tol: float, max_iter: int, threads=(1, 2, 3, 4, 5, 6, 7, 8, 12, 16)):
+    """Oracle iteration counts under different reduction orders (thread
+    counts): BiCGSTAB's count is a chaotic function of rounding on
+    non-normal problems, so GPU parity is judged against this spread."""
+    before = num_threads()
+    its = []
+    try:
+        for t in threads:
+            set_threads(t)
+            its.append(bicgstab(a_csr, m_csr, b, np.zeros(len(b)), tol, max_iter)[1])
+    finally:
+        set_threads(before)
+    return its
 
 
 def column_sets(n, colptr, rowidx):
@@ -114,8 +136,9 @@ def spmv(rp, ci, v, x) -> np.ndarray:
     return y
 
 
-def bicgstab(a_csr, m_csr, b, x0, tol: float, max_iter: int):
-    """Returns (x, iterations, info) with info = (||r||, ||r0||, converged)."""
+def bicgstab(a_csr, m_csr, b, x0, tol: float, max_iter: int, history: bool = False):
+    """Returns (x, iterations, info) with info = (||r||, ||r0||, converged);
+    with history=True also the per-iteration ||r_k|| as a fourth value."""
     rp, ci, v = (_i32(a_csr[0]), _i32(a_csr[1]), _f64(a_csr[2]))
     b = _f64(b)
     x = np.array(x0, dtype=np.float64, copy=True)
@@ -126,8 +149,13 @@ def bicgstab(a_csr, m_csr, b, x0, tol: float, max_iter: int):
     else:
         mrp, mci, mv = _i32(m_csr[0]), _i32(m_csr[1]), _f64(m_csr[2])
         mp = (mrp.ctypes.data, mci.ctypes.data, mv.ctypes.data)
-    it = lib().oracle_bicgstab(len(rp) - 1, rp.ctypes.data, ci.ctypes.data, v.ctypes.data, mp[0], mp[1], mp[2],
-                               b.ctypes.data, x.ctypes.data, float(tol), int(max_iter), info.ctypes.data)
+    hist = np.zeros(max(int(max_iter), 1)) if history else None
+    it = lib().oracle_bicgstab_history(len(rp) - 1, rp.ctypes.data, ci.ctypes.data, v.ctypes.data, mp[0], mp[1],
+                                       mp[2], b.ctypes.data, x.ctypes.data, float(tol), int(max_iter),
+                                       info.ctypes.data, None if hist is None else hist.ctypes.data)
+    if history:
+        k = it if info[2] else it - 1
+        return x, it, info, hist[:k]
     return x, it, info
 
 

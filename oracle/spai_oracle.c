@@ -325,8 +325,20 @@ static double dot(int n, const double *a, const double *b) {
  *   info[0] = final recursive ||r||_2, info[1] = ||r0||_2,
  *   info[2] = 1 if the stop test fired, else 0.
  * M = NULL means no preconditioner (y = p). */
+int oracle_bicgstab_history(int n, const int *arp, const int *aci, const double *av, const int *mrp,
+                            const int *mci, const double *mv, const double *b, double *x, double tol, int max_iter,
+                            double *info, double *hist);
+
 int oracle_bicgstab(int n, const int *arp, const int *aci, const double *av, const int *mrp, const int *mci,
                     const double *mv, const double *b, double *x, double tol, int max_iter, double *info) {
+    return oracle_bicgstab_history(n, arp, aci, av, mrp, mci, mv, b, x, tol, max_iter, info, NULL);
+}
+
+/* As oracle_bicgstab; hist (max_iter doubles, may be NULL) receives ||r_k||
+ * after iteration k = 1, 2, ... */
+int oracle_bicgstab_history(int n, const int *arp, const int *aci, const double *av, const int *mrp,
+                            const int *mci, const double *mv, const double *b, double *x, double tol, int max_iter,
+                            double *info, double *hist) {
     if (n <= 0 || !arp || !aci || !av || !b || !x) return ORACLE_ERR_ARG;
     size_t sz = sizeof(double) * (size_t)n;
     double *r = (double *)malloc(sz), *rs = (double *)malloc(sz), *p = (double *)malloc(sz);
@@ -367,6 +379,7 @@ int oracle_bicgstab(int n, const int *arp, const int *aci, const double *av, con
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < n; ++i) p[i] = (beta * p[i] + r[i]) + (-beta * w) * ay[i]; /* scal, axpy, axpy */
         err = sqrt(dot(n, r, r));
+        if (hist) hist[it - 1] = err;
         if (err < tol) { converged = 1; break; }
         it++;
     }
@@ -411,6 +424,16 @@ int oracle_adjoint(int n, const int *colptr, const int *rowidx, const double *va
         for (int p = colptr[j]; p < colptr[j + 1]; ++p) dA[p] = -db[rowidx[p]] * x[j];
     if (info) { info[0] = inf[0]; info[1] = inf[1]; info[2] = inf[2]; info[3] = it; }
     return it;
+}
+
+/* Threads used by the parallel loops (changes the reduction order of the
+ * dot products, used to measure how sensitive iteration counts are to it). */
+void oracle_set_threads(int t) {
+#ifdef _OPENMP
+    if (t > 0) omp_set_num_threads(t);
+#else
+    (void)t;
+#endif
 }
 
 int oracle_num_threads(void) {

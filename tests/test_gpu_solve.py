@@ -59,13 +59,22 @@ def test_graph_and_loop_agree_bitwise(ctx):
 
 
 def test_unpreconditioned_matches_oracle(ctx):
+    # Unpreconditioned BiCGSTAB on this conv-diff converges erratically: the
+    # oracle alone needs 101..105 iterations depending on its OpenMP reduction
+    # order (1, 2, 8 threads).  Parity is therefore checked on the iterate
+    # after a fixed 20 iterations, and the full solve within 10% of the count.
     P = D.Problem.convection_diffusion(40, 5.0)
     A = P.system(0)
     b = np.random.default_rng(3).uniform(size=A.n)
     S = D.System(ctx, A)
+    x20, _, oc20 = S.solve(b, rel_tol=0.0, abs_tol=1e-300, max_iter=21, precondition=False)
+    xr20, it20, _, Acsr = oracle_solve(A, None, b, rel_tol=0.0, max_iter=21)
+    assert oc20[0]["iterations"] == it20 == 21
+    assert np.linalg.norm(x20 - xr20) <= 1e-9 * np.linalg.norm(xr20)
     x, rep, oc = S.solve(b, rel_tol=1e-8, max_iter=2000, precondition=False)
     xr, it, info, Acsr = oracle_solve(A, None, b, max_iter=2000)
-    assert abs(oc[0]["iterations"] - it) <= 1
+    assert oc[0]["status"] == D.SOLVE_CONVERGED
+    assert abs(oc[0]["iterations"] - it) <= 0.1 * it
     assert O.true_residual(Acsr, b, x) / np.linalg.norm(b) <= 1.5e-8
 
 
@@ -86,22 +95,39 @@ def test_mixed_precision_solve(ctx):
 
 
 def test_batched_solve_per_system(ctx):
+    # The conv-diff systems of the batch converge irregularly: the oracle's
+    # own count moves by up to +-60 iterations between OpenMP thread counts
+    # (a different summation order in the dot products).  Parity: the first
+    # 10 iterates agree to 1e-8, and the final count lies inside the oracle's
+    # spread over reduction orders, widened by 1.
     P = D.Problem.batched(6, 5)
     mats = P.systems()
     S = D.System(ctx, mats)
     S.build(D.PRECISION_FP64, D.SCHEDULE_BALANCED_NNZ)
     b = np.concatenate([P.rhs(s) for s in range(len(mats))])
+    off = S.row_offsets
+    refs = []
+    for s, A in enumerate(mats):
+        bs = b[off[s]:off[s + 1]]
+        m_ref = O.spai(A.n, A.colptr, A.rowidx, A.val, 0)
+        Acsr = O.csc2csr(A.n, A.colptr, A.rowidx, A.val)[:3]
+        Mcsr = O.csc2csr(A.n, A.colptr, A.rowidx, m_ref)[:3]
+        x10 = O.bicgstab(Acsr, Mcsr, bs, np.zeros(A.n), 0.0, 11)[0]
+        spread = O.iteration_spread(Acsr, Mcsr, bs, 1e-8 * np.linalg.norm(bs), 1000)
+        refs.append((bs, Acsr, x10, spread))
+    x10g, _, oc10 = S.solve(b, rel_tol=0.0, abs_tol=1e-300, max_iter=11)
+    for s, (bs, Acsr, x10, spread) in enumerate(refs):
+        assert oc10[s]["iterations"] == 11
+        xs = x10g[off[s]:off[s + 1]]
+        assert np.linalg.norm(xs - x10) <= 1e-8 * np.linalg.norm(x10), s
     for use_graph in (False, True):
         x, rep, oc = S.solve(b, rel_tol=1e-8, use_graph=use_graph)
-        off = S.row_offsets
-        for s, A in enumerate(mats):
-            bs = b[off[s]:off[s + 1]]
-            m_ref = O.spai(A.n, A.colptr, A.rowidx, A.val, 0)
-            xr, it, info, Acsr = oracle_solve(A, m_ref, bs)
+        for s, (bs, Acsr, x10, spread) in enumerate(refs):
             xs = x[off[s]:off[s + 1]]
             true_rel = O.true_residual(Acsr, bs, xs) / np.linalg.norm(bs)
+            print(f"system {s}: gpu {oc[s]['iterations']} oracle spread {spread}", flush=True)
             assert oc[s]["status"] == D.SOLVE_CONVERGED, (s, oc[s])
-            assert abs(oc[s]["iterations"] - it) <= 1, (s, oc[s]["iterations"], it)
+            assert min(spread) - 1 <= oc[s]["iterations"] <= max(spread) + 1, (s, oc[s]["iterations"], spread)
             assert true_rel <= 1.5e-8, (s, true_rel)
 
 
