@@ -117,7 +117,7 @@ def main():
         torch.cuda.synchronize()
         rep_b = S.build(precision, D.SCHEDULE_BALANCED_NNZ)
         rep_s = S.solve_device(b_d.data_ptr(), x_d.data_ptr(), rel_tol=1e-8, max_iter=1000, use_graph=True,
-                               graph_iters=args.graph_iters)
+                               graph_iters=args.graph_iters, true_residual_restart=True)
         return rep_b, rep_s
 
     for _ in range(args.warmup):
@@ -139,6 +139,7 @@ def main():
     value = nsys * args.steps / elapsed
     log(f"batched {args.precision}: {ms_per_step:.2f} ms/step, spai {np.median(spai_ms):.3f} ms, solve "
         f"{np.median(solve_ms):.3f} ms, iters(max) {iters[-1]}, worst true rel {max(true_rel):.2e}")
+    log("per-system iterations (n): " + ", ".join(f"{o['iterations']}({A.n})" for o, A in zip(oc, mats)))
 
     # ------------------------------------------------------- roofline ----
     m_bytes = 4 if precision == D.PRECISION_MIXED else 8
@@ -180,7 +181,7 @@ def main():
                 torch.cuda.synchronize()
                 rb = Sk.build(D.PRECISION_FP64, D.SCHEDULE_BALANCED_NNZ)
                 rs = Sk.solve_device(bk.data_ptr(), xk.data_ptr(), rel_tol=1e-8, max_iter=1000, use_graph=False,
-                                     check_every=1)
+                                     check_every=1, true_residual_restart=True)
                 tot_b += rb["ms"]
                 tot_s += rs["ms"]
             return tot_b, tot_s

@@ -98,7 +98,8 @@ class BuildReport(ctypes.Structure):
 class SolveOptions(ctypes.Structure):
     _fields_ = [("rel_tol", ctypes.c_double), ("abs_tol", ctypes.c_double), ("max_iter", ctypes.c_int),
                 ("use_graph", ctypes.c_int), ("graph_iters", ctypes.c_int), ("check_every", ctypes.c_int),
-                ("precondition", ctypes.c_int), ("record_history", ctypes.c_int)]
+                ("precondition", ctypes.c_int), ("record_history", ctypes.c_int),
+                ("true_residual_restart", ctypes.c_int)]
 
 
 class SolveReport(ctypes.Structure):
@@ -108,7 +109,8 @@ class SolveReport(ctypes.Structure):
 
 class Outcome(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int), ("residual", ctypes.c_double),
-                ("residual0", ctypes.c_double), ("true_residual", ctypes.c_double), ("tol", ctypes.c_double)]
+                ("residual0", ctypes.c_double), ("true_residual", ctypes.c_double), ("tol", ctypes.c_double),
+                ("restarts", ctypes.c_int)]
 
 
 _lib: Optional[ctypes.CDLL] = None
@@ -348,7 +350,7 @@ class Context:
 
 
 def _options(rel_tol=1e-8, abs_tol=0.0, max_iter=1000, use_graph=False, graph_iters=8, check_every=1,
-             precondition=True, record_history=False) -> SolveOptions:
+             precondition=True, record_history=False, true_residual_restart=False) -> SolveOptions:
     o = SolveOptions()
     o.rel_tol = float(rel_tol)
     o.abs_tol = float(abs_tol)
@@ -358,6 +360,7 @@ def _options(rel_tol=1e-8, abs_tol=0.0, max_iter=1000, use_graph=False, graph_it
     o.check_every = int(check_every)
     o.precondition = int(bool(precondition))
     o.record_history = int(bool(record_history))
+    o.true_residual_restart = int(bool(true_residual_restart))
     return o
 
 
@@ -445,7 +448,8 @@ class System:
         arr = (Outcome * self.nsys)()
         _check(self._lib.spai_system_outcomes(self._h, arr))
         return [{"status": o.status, "iterations": o.iterations, "residual": o.residual,
-                 "residual0": o.residual0, "true_residual": o.true_residual, "tol": o.tol} for o in arr]
+                 "residual0": o.residual0, "true_residual": o.true_residual, "tol": o.tol,
+                 "restarts": o.restarts} for o in arr]
 
     def residual_history(self, s: int = 0, maxlen: int = 100000) -> np.ndarray:
         out = np.empty(maxlen)

@@ -202,6 +202,7 @@ SolveOptions to_options(const spai_solve_options *o) {
         s.graph_iters = o->graph_iters;
         s.check_every = o->check_every;
         s.record_history = o->record_history;
+        s.true_residual_restart = o->true_residual_restart;
     }
     return s;
 }
@@ -508,6 +509,7 @@ void spai_solve_options_default(spai_solve_options *opt) {
     opt->check_every = 1;
     opt->precondition = 1;
     opt->record_history = 0;
+    opt->true_residual_restart = 0;
 }
 
 int spai_system_solve(spai_system *sys, const double *b, double *x, int on_device, const spai_solve_options *opt,
@@ -555,6 +557,7 @@ int spai_system_outcomes(spai_system *sys, spai_system_outcome *out) {
         out[s].residual0 = S.err0;
         out[s].true_residual = std::sqrt(S.true_rr);
         out[s].tol = S.tol;
+        out[s].restarts = S.restarts;
     }
     return SPAI_OK;
 }

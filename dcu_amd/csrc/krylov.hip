@@ -160,34 +160,6 @@ __global__ void k_sort_rows(int n, const int *rowptr, int *colidx, int *src, con
     }
 }
 
-// ---------------------------------------------------------------- tiles ----
-// Tile t of system s starts at the first row of s whose nonzeros begin at or
-// after nnz_off[s] + (t - tile_off[s]) * target (the first tile at the
-// system's first row).
-__global__ void k_tile_rows(const int *ptr, int ntiles, const int *tile_sys, const int *sys_tiles,
-                            const int *sys_row_off, int target, int *tile_row, int n) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t > ntiles) return;
-    if (t == ntiles) {
-        tile_row[t] = n;
-        return;
-    }
-    int s = tile_sys[t];
-    int local = t - sys_tiles[s];
-    int r0 = sys_row_off[s], r1 = sys_row_off[s + 1];
-    if (local == 0) {
-        tile_row[t] = r0;
-        return;
-    }
-    long long want = (long long)ptr[r0] + (long long)local * target;
-    int lo = r0, hi = r1;  // first row in [r0, r1] with ptr >= want
-    while (lo < hi) {
-        int mid = lo + ((hi - lo) >> 1);
-        if ((long long)ptr[mid] >= want) hi = mid; else lo = mid + 1;
-    }
-    tile_row[t] = lo;
-}
-
 // ----------------------------------------------------------- reductions ----
 template <int NRED>
 __device__ __forceinline__ void block_reduce(double (&v)[NRED], double *sh) {
@@ -264,16 +236,24 @@ struct TileView {
     const int *group_sys;   // system per group
 };
 
+constexpr int kRowsPerTile = 512;                  // row cap of a tile (host-built)
+constexpr int kRowsPerThread = kRowsPerTile / kThreads;
+
 // For every row i of the workgroup's tile(s): y_i = sum_p val[p] *
-// op.gather(idx[p]), then op.row(i, y_i, acc).  HAS_MATRIX = false skips the
-// product (streaming ops).  Ops with reductions run one workgroup per group of
-// tiles (GROUPED) and publish one partial per group.
+// op.gather(idx[p]) (stored as products in LDS, then summed per row in
+// storage order), then op.row(i, y_i, pre_i, acc) where pre_i = op.pre(i) are
+// the row's own vector operands, loaded before the product so that they
+// overlap with it.  One tile needs two dependent global round trips: the
+// index/value/row-pointer/operand loads, then the gathers.  HAS_MATRIX =
+// false skips the product (streaming ops).  Ops with reductions run one
+// workgroup per group of tiles (GROUPED) and publish one partial per group.
 template <bool HAS_MATRIX, class MT, class Op>
 __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr, const int *__restrict__ idx,
                                                     const MT *__restrict__ val, TileView tv, Ctx c, Op op) {
     constexpr int NRED = Op::NRED;
     constexpr bool GROUPED = NRED > 0;
     __shared__ double prod[HAS_MATRIX ? kLdsProducts : 1];
+    __shared__ int rp[HAS_MATRIX ? kRowsPerTile + 1 : 1];
     __shared__ double sh[(NRED > 0 ? NRED : 1) * 4];
     const int unit = blockIdx.x;
     const int s = GROUPED ? tv.group_sys[unit] : tv.sys[unit];
@@ -281,59 +261,69 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
     op.load(c.st[s]);
     const int ft0 = GROUPED ? tv.group_tile[unit] : unit;
     const int ft1 = GROUPED ? tv.group_tile[unit + 1] : unit + 1;
+    const int tid = threadIdx.x;
     double acc[NRED > 0 ? NRED : 1];
 #pragma unroll
     for (int q = 0; q < (NRED > 0 ? NRED : 1); ++q) acc[q] = 0.0;
 
     for (int ft = ft0; ft < ft1; ++ft) {
         const int r0 = tv.row[ft], r1 = tv.row[ft + 1];
+        const int nrows = r1 - r0;
         if (!HAS_MATRIX) {
-            for (int i = r0 + (int)threadIdx.x; i < r1; i += kThreads) op.row(i, 0.0, acc);
+            for (int i = r0 + tid; i < r1; i += kThreads) op.row(i, 0.0, op.pre(i), acc);
             continue;
         }
-        const int p0 = ptr[r0], p1 = ptr[r1];
-        const int cnt = p1 - p0;
-        if (cnt <= kLdsProducts) {
-            // all index/value loads of a batch first, then all gathers, so the
-            // dependent loads overlap instead of running one after another
-            constexpr int U = 8;
-            for (int base = 0; base < cnt; base += kThreads * U) {
-                int jj[U];
-                MT vv[U];
+        const int p0 = ptr[r0];
+        const int cnt = ptr[r1] - p0;
+        if (cnt <= kLdsProducts && nrows <= kRowsPerTile) {
+            typename Op::Pre pre[kRowsPerThread];
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    int q = base + u * kThreads + (int)threadIdx.x;
-                    jj[u] = q < cnt ? idx[p0 + q] : 0;
-                    vv[u] = q < cnt ? val[p0 + q] : (MT)0;
-                }
-                double gv[U];
+            for (int k = 0; k < kRowsPerThread; ++k) {
+                const int i = r0 + tid + k * kThreads;
+                if (i < r1) pre[k] = op.pre(i);
+            }
+            for (int k = tid; k <= nrows; k += kThreads) rp[k] = ptr[r0 + k] - p0;
+            constexpr int U = kLdsProducts / kThreads;
+            int jj[U];
+            MT vv[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    int q = base + u * kThreads + (int)threadIdx.x;
-                    gv[u] = q < cnt ? op.gather(jj[u]) : 0.0;
-                }
+            for (int u = 0; u < U; ++u) {
+                const int q = u * kThreads + tid;
+                jj[u] = q < cnt ? idx[p0 + q] : 0;
+                vv[u] = q < cnt ? val[p0 + q] : (MT)0;
+            }
+            double gv[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    int q = base + u * kThreads + (int)threadIdx.x;
-                    if (q < cnt) prod[q] = (double)vv[u] * gv[u];
-                }
+            for (int u = 0; u < U; ++u) {
+                const int q = u * kThreads + tid;
+                gv[u] = q < cnt ? op.gather(jj[u]) : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int q = u * kThreads + tid;
+                if (q < cnt) prod[q] = (double)vv[u] * gv[u];
             }
             __syncthreads();
-            for (int i = r0 + (int)threadIdx.x; i < r1; i += kThreads) {
-                int b = ptr[i] - p0, e = ptr[i + 1] - p0;
-                double sum = 0.0;
-                for (int q = b; q < e; ++q) sum += prod[q];
-                op.row(i, sum, acc);
+#pragma unroll
+            for (int k = 0; k < kRowsPerThread; ++k) {
+                const int li = tid + k * kThreads;
+                if (li < nrows) {
+                    const int b = rp[li], e = rp[li + 1];
+                    double sum = 0.0;
+                    for (int q = b; q < e; ++q) sum += prod[q];
+                    op.row(r0 + li, sum, pre[k], acc);
+                }
             }
-            __syncthreads();  // prod is reused by the next tile
+            __syncthreads();  // prod and rp are reused by the next tile
         } else {
-            // long rows: the whole workgroup reduces one row at a time
+            // a row longer than the LDS buffer (it has a tile of its own):
+            // the whole workgroup reduces it
             for (int i = r0; i < r1; ++i) {
-                int b = ptr[i], e = ptr[i + 1];
+                const int b = ptr[i], e = ptr[i + 1];
                 double part[1] = {0.0};
-                for (int q = b + (int)threadIdx.x; q < e; q += kThreads) part[0] += (double)val[q] * op.gather(idx[q]);
+                for (int q = b + tid; q < e; q += kThreads) part[0] += (double)val[q] * op.gather(idx[q]);
                 block_reduce<1>(part, sh);
-                if (threadIdx.x == 0) op.row(i, part[0], acc);
+                if (tid == 0) op.row(i, part[0], op.pre(i), acc);
             }
         }
     }
@@ -348,33 +338,47 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
 // ------------------------------------------------------------- the ops ----
 struct OpInit {  // r = b - A x0, r* = r, p = a = 0; rho = rr = (r, r); tol
     static constexpr int NRED = 2;
+    struct Pre {
+        double b;
+    };
     const double *x, *b;
     double *r, *rs, *p0, *p1, *a;
     double rel_tol, abs_tol;
-    __device__ bool enabled(const SysState &) const { return true; }
+    int restart;  // 1: restart of an active system from its current x (iterations and tol kept)
+    __device__ bool enabled(const SysState &S) const { return !restart || S.active; }
     __device__ void load(const SysState &) {}
+    __device__ Pre pre(int i) const { return {b[i]}; }
     __device__ double gather(int j) const { return x[j]; }
-    __device__ void row(int i, double ax, double (&acc)[2]) const {
-        double ri = -ax + b[i];
+    __device__ void row(int i, double ax, const Pre &pr, double (&acc)[2]) const {
+        double ri = -ax + pr.b;
         r[i] = ri;
         rs[i] = ri;
         p0[i] = 0.0;
         p1[i] = 0.0;
         a[i] = 0.0;
         acc[0] += ri * ri;
-        acc[1] += b[i] * b[i];
+        acc[1] += pr.b * pr.b;
     }
     __device__ void finish(const Ctx &c, int s, const double (&tot)[2]) const {
         SysState &S = c.st[s];
         S.rho = tot[0];
         S.rr = tot[0];
-        S.err0 = sqrt(tot[0]);
-        S.tol = abs_tol > 0.0 ? abs_tol : rel_tol * sqrt(tot[1]);
         S.alpha = 0.0;
         S.omega = 0.0;
         S.beta = 0.0;
-        S.iters = 0;
         S.true_rr = 0.0;
+        if (restart) {
+            S.restarts += 1;
+            if (sqrt(tot[0]) < S.tol) {
+                S.status = kConverged;
+                S.active = 0;
+                atomicSub(&c.counter[c.nsys], 1);
+            }
+            return;
+        }
+        S.err0 = sqrt(tot[0]);
+        S.tol = abs_tol > 0.0 ? abs_tol : rel_tol * sqrt(tot[1]);
+        S.iters = 0;
         if (S.err0 < S.tol) {
             // r0 already below the threshold: the reference would divide 0/0
             S.status = kConvergedAtStart;
@@ -387,6 +391,9 @@ struct OpInit {  // r = b - A x0, r* = r, p = a = 0; rho = rr = (r, r); tol
 template <class MT>
 struct OpMapplyP {  // K1: y = M p', p' = beta p + r - beta omega a
     static constexpr int NRED = 0;
+    struct Pre {
+        double p;
+    };
     const double *r, *pin, *a;
     double *pout, *y;
     double beta, nbw;
@@ -396,24 +403,29 @@ struct OpMapplyP {  // K1: y = M p', p' = beta p + r - beta omega a
         nbw = -S.beta * S.omega;
     }
     __device__ double pval(int j) const { return (beta * pin[j] + r[j]) + nbw * a[j]; }
+    __device__ Pre pre(int i) const { return {pval(i)}; }
     __device__ double gather(int j) const { return pval(j); }
-    __device__ void row(int i, double mx, double (&)[1]) const {
+    __device__ void row(int i, double mx, const Pre &pr, double (&)[1]) const {
         y[i] = mx;
-        pout[i] = pval(i);
+        pout[i] = pr.p;
     }
     __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
 
 struct OpSpmvAlpha {  // K2: a = A y, alpha = rho / (r*, a)
     static constexpr int NRED = 1;
+    struct Pre {
+        double rs;
+    };
     const double *y, *rs;
     double *a;
     __device__ bool enabled(const SysState &S) const { return S.active; }
     __device__ void load(const SysState &) {}
+    __device__ Pre pre(int i) const { return {rs[i]}; }
     __device__ double gather(int j) const { return y[j]; }
-    __device__ void row(int i, double ax, double (&acc)[1]) const {
+    __device__ void row(int i, double ax, const Pre &pr, double (&acc)[1]) const {
         a[i] = ax;
-        acc[0] += rs[i] * ax;
+        acc[0] += pr.rs * ax;
     }
     __device__ void finish(const Ctx &c, int s, const double (&tot)[1]) const {
         SysState &S = c.st[s];
@@ -424,30 +436,38 @@ struct OpSpmvAlpha {  // K2: a = A y, alpha = rho / (r*, a)
 template <class MT>
 struct OpMapplyS {  // K3: z = M s, s = r - alpha a
     static constexpr int NRED = 0;
+    struct Pre {
+        double s;
+    };
     const double *r, *a;
     double *s, *z;
     double nalpha;
     __device__ bool enabled(const SysState &S) const { return S.active; }
     __device__ void load(const SysState &S) { nalpha = -S.alpha; }
     __device__ double sval(int j) const { return r[j] + nalpha * a[j]; }
+    __device__ Pre pre(int i) const { return {sval(i)}; }
     __device__ double gather(int j) const { return sval(j); }
-    __device__ void row(int i, double mx, double (&)[1]) const {
+    __device__ void row(int i, double mx, const Pre &pr, double (&)[1]) const {
         z[i] = mx;
-        s[i] = sval(i);
+        s[i] = pr.s;
     }
     __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
 
 struct OpSpmvOmega {  // K4: t = A z, omega = (t, s) / (t, t)
     static constexpr int NRED = 2;
+    struct Pre {
+        double s;
+    };
     const double *z, *s;
     double *t;
     __device__ bool enabled(const SysState &S) const { return S.active; }
     __device__ void load(const SysState &) {}
+    __device__ Pre pre(int i) const { return {s[i]}; }
     __device__ double gather(int j) const { return z[j]; }
-    __device__ void row(int i, double ax, double (&acc)[2]) const {
+    __device__ void row(int i, double ax, const Pre &pr, double (&acc)[2]) const {
         t[i] = ax;
-        acc[0] += ax * s[i];
+        acc[0] += ax * pr.s;
         acc[1] += ax * ax;
     }
     __device__ void finish(const Ctx &c, int s_, const double (&tot)[2]) const {
@@ -458,6 +478,9 @@ struct OpSpmvOmega {  // K4: t = A z, omega = (t, s) / (t, t)
 
 struct OpUpdate {  // K5: x, r update; rho', beta, ||r||, stop test
     static constexpr int NRED = 2;
+    struct Pre {
+        double x, y, z, s, t, rs;
+    };
     double *x, *r;
     const double *y, *z, *s, *t, *rs;
     double alpha, omega;
@@ -466,12 +489,13 @@ struct OpUpdate {  // K5: x, r update; rho', beta, ||r||, stop test
         alpha = S.alpha;
         omega = S.omega;
     }
+    __device__ Pre pre(int i) const { return {x[i], y[i], z[i], s[i], t[i], rs[i]}; }
     __device__ double gather(int) const { return 0.0; }
-    __device__ void row(int i, double, double (&acc)[2]) const {
-        x[i] = (x[i] + alpha * y[i]) + omega * z[i];
-        double ri = s[i] + (-omega) * t[i];
+    __device__ void row(int i, double, const Pre &pr, double (&acc)[2]) const {
+        x[i] = (pr.x + alpha * pr.y) + omega * pr.z;
+        double ri = pr.s + (-omega) * pr.t;
         r[i] = ri;
-        acc[0] += rs[i] * ri;
+        acc[0] += pr.rs * ri;
         acc[1] += ri * ri;
     }
     __device__ void finish(const Ctx &c, int s, const double (&tot)[2]) const {
@@ -505,12 +529,16 @@ struct OpUpdate {  // K5: x, r update; rho', beta, ||r||, stop test
 
 struct OpTrueResidual {  // ||b - A x||^2 per system
     static constexpr int NRED = 1;
+    struct Pre {
+        double b;
+    };
     const double *x, *b;
     __device__ bool enabled(const SysState &) const { return true; }
     __device__ void load(const SysState &) {}
+    __device__ Pre pre(int i) const { return {b[i]}; }
     __device__ double gather(int j) const { return x[j]; }
-    __device__ void row(int i, double ax, double (&acc)[1]) const {
-        double d = b[i] - ax;
+    __device__ void row(int, double ax, const Pre &pr, double (&acc)[1]) const {
+        double d = pr.b - ax;
         acc[0] += d * d;
     }
     __device__ void finish(const Ctx &c, int s, const double (&tot)[1]) const { c.st[s].true_rr = tot[0]; }
@@ -518,17 +546,22 @@ struct OpTrueResidual {  // ||b - A x||^2 per system
 
 struct OpSpmvPlain {  // y = A x (timing of the bare product)
     static constexpr int NRED = 0;
+    struct Pre {};
     const double *x;
     double *y;
     __device__ bool enabled(const SysState &) const { return true; }
     __device__ void load(const SysState &) {}
+    __device__ Pre pre(int) const { return {}; }
     __device__ double gather(int j) const { return x[j]; }
-    __device__ void row(int i, double ax, double (&)[1]) const { y[i] = ax; }
+    __device__ void row(int i, double ax, const Pre &, double (&)[1]) const { y[i] = ax; }
     __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
 
 struct OpCopyP {  // K1 without a preconditioner: y = p'
     static constexpr int NRED = 0;
+    struct Pre {
+        double p;
+    };
     const double *r, *pin, *a;
     double *pout, *y;
     double beta, nbw;
@@ -537,27 +570,30 @@ struct OpCopyP {  // K1 without a preconditioner: y = p'
         beta = S.beta;
         nbw = -S.beta * S.omega;
     }
+    __device__ Pre pre(int i) const { return {(beta * pin[i] + r[i]) + nbw * a[i]}; }
     __device__ double gather(int) const { return 0.0; }
-    __device__ void row(int i, double, double (&)[1]) const {
-        double v = (beta * pin[i] + r[i]) + nbw * a[i];
-        pout[i] = v;
-        y[i] = v;
+    __device__ void row(int i, double, const Pre &pr, double (&)[1]) const {
+        pout[i] = pr.p;
+        y[i] = pr.p;
     }
     __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
 
 struct OpCopyS {  // K3 without a preconditioner: z = s
     static constexpr int NRED = 0;
+    struct Pre {
+        double s;
+    };
     const double *r, *a;
     double *s, *z;
     double nalpha;
     __device__ bool enabled(const SysState &S) const { return S.active; }
     __device__ void load(const SysState &S) { nalpha = -S.alpha; }
+    __device__ Pre pre(int i) const { return {r[i] + nalpha * a[i]}; }
     __device__ double gather(int) const { return 0.0; }
-    __device__ void row(int i, double, double (&)[1]) const {
-        double v = r[i] + nalpha * a[i];
-        s[i] = v;
-        z[i] = v;
+    __device__ void row(int i, double, const Pre &pr, double (&)[1]) const {
+        s[i] = pr.s;
+        z[i] = pr.s;
     }
     __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
@@ -573,6 +609,20 @@ __global__ void k_state_init(SysState *st, int nsys, int max_iter, int *counter)
         counter[s] = 0;
     }
     if (s == 0) counter[nsys] = nsys;
+}
+
+// systems whose recursive residual converged but whose true residual did not
+// are re-activated for a restart
+__global__ void k_check_true(SysState *st, int nsys, int *counter) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    SysState &S = st[s];
+    if ((S.status == kConverged || S.status == kConvergedAtStart) && !(sqrt(S.true_rr) < S.tol) &&
+        S.iters < S.max_iter - 1 && S.restarts < 4) {
+        S.status = kRunning;
+        S.active = 1;
+        atomicAdd(&counter[nsys], 1);
+    }
 }
 
 // re-enables every system (scalars kept) so single kernels can be timed
@@ -633,39 +683,48 @@ void build_tiles(TileMap &tm, const int *ptr, const std::vector<int> &h_row_off,
                  const std::vector<long long> &h_nnz_off, hipStream_t st) {
     const int nsys = (int)h_row_off.size() - 1;
     require(nsys >= 1, "build_tiles: no systems");
+    const int n = h_row_off[(size_t)nsys];
+    std::vector<int> hp((size_t)n + 1);
+    SPAI_HIP(hipMemcpyAsync(hp.data(), ptr, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost, st));
+    SPAI_HIP(hipStreamSynchronize(st));
+    // nonzeros per tile: enough tiles to fill the GPU on large inputs, at most
+    // the LDS staging buffer; rows per tile at most kRowsPerTile
     long long total = h_nnz_off[(size_t)nsys];
-    // aim for >= ~2048 tiles on large inputs, 256..1024 nonzeros each (a
-    // tile plus its longest row must fit the 2048 staged products)
-    long long t = total / 2048;
     int target = 256;
-    while (target < 1024 && target < t) target <<= 1;
+    while (target < kLdsProducts && (long long)target * 4096 < total) target <<= 1;
     tm.target_nnz = target;
+    std::vector<int> trow, tsys;
     tm.h_sys_tiles.assign((size_t)nsys + 1, 0);
     for (int s = 0; s < nsys; ++s) {
-        long long nz = h_nnz_off[(size_t)s + 1] - h_nnz_off[(size_t)s];
-        int rows = h_row_off[(size_t)s + 1] - h_row_off[(size_t)s];
-        int k = (int)std::max<long long>(1, (nz + target - 1) / target);
-        if (rows == 0) k = 1;
-        tm.h_sys_tiles[(size_t)s + 1] = tm.h_sys_tiles[(size_t)s] + k;
+        const int r0 = h_row_off[(size_t)s], r1 = h_row_off[(size_t)s + 1];
+        int i = r0;
+        if (r0 == r1) {  // empty system: one empty tile keeps the reductions uniform
+            trow.push_back(r0);
+            tsys.push_back(s);
+        }
+        while (i < r1) {
+            const int start = i;
+            long long nz = 0;
+            if (hp[(size_t)i + 1] - hp[(size_t)i] > kLdsProducts) {
+                ++i;  // a row longer than the staging buffer gets a tile of its own
+            } else {
+                while (i < r1 && i - start < kRowsPerTile) {
+                    const int len = hp[(size_t)i + 1] - hp[(size_t)i];
+                    if (len > kLdsProducts || (i > start && nz + len > target)) break;
+                    nz += len;
+                    ++i;
+                }
+            }
+            trow.push_back(start);
+            tsys.push_back(s);
+        }
+        tm.h_sys_tiles[(size_t)s + 1] = (int)tsys.size();
     }
-    tm.ntiles = tm.h_sys_tiles[(size_t)nsys];
-    std::vector<int> hsys((size_t)tm.ntiles);
-    for (int s = 0; s < nsys; ++s)
-        for (int i = tm.h_sys_tiles[(size_t)s]; i < tm.h_sys_tiles[(size_t)s + 1]; ++i) hsys[(size_t)i] = s;
-    tm.row.resize((size_t)tm.ntiles + 1);
-    tm.sys.resize((size_t)tm.ntiles);
-    tm.sys_tiles.resize((size_t)nsys + 1);
-    DeviceArray<int> roff((size_t)nsys + 1);
-    SPAI_HIP(hipMemcpyAsync(tm.sys.get(), hsys.data(), sizeof(int) * hsys.size(), hipMemcpyHostToDevice, st));
-    SPAI_HIP(hipMemcpyAsync(tm.sys_tiles.get(), tm.h_sys_tiles.data(), sizeof(int) * (nsys + 1),
-                            hipMemcpyHostToDevice, st));
-    SPAI_HIP(hipMemcpyAsync(roff.get(), h_row_off.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_tile_rows, dim3(ceil_div(tm.ntiles + 1, 256)), dim3(256), 0, st, ptr, tm.ntiles,
-                       tm.sys.get(), tm.sys_tiles.get(), roff.get(), target, tm.row.get(),
-                       h_row_off[(size_t)nsys]);
-    SPAI_CHECK_LAUNCH();
-    // groups of up to group_size consecutive tiles inside each system
-    tm.group_size = 4;
+    trow.push_back(n);
+    tm.ntiles = (int)tsys.size();
+    // reducing kernels: groups of consecutive tiles of one system, enough of
+    // them to give every CU several workgroups
+    tm.group_size = (int)std::max<long long>(1, std::min<long long>(4, tm.ntiles / 2048));
     std::vector<int> gtile, gsys, sgroups((size_t)nsys + 1, 0);
     for (int s = 0; s < nsys; ++s) {
         for (int t = tm.h_sys_tiles[(size_t)s]; t < tm.h_sys_tiles[(size_t)s + 1]; t += tm.group_size) {
@@ -676,13 +735,21 @@ void build_tiles(TileMap &tm, const int *ptr, const std::vector<int> &h_row_off,
     }
     gtile.push_back(tm.ntiles);
     tm.ngroups = (int)gsys.size();
+    tm.row.resize(trow.size());
+    tm.sys.resize(tsys.size());
+    tm.sys_tiles.resize((size_t)nsys + 1);
     tm.group_tile.resize(gtile.size());
     tm.group_sys.resize(gsys.size());
     tm.sys_groups.resize(sgroups.size());
-    SPAI_HIP(hipMemcpyAsync(tm.group_tile.get(), gtile.data(), sizeof(int) * gtile.size(), hipMemcpyHostToDevice, st));
-    SPAI_HIP(hipMemcpyAsync(tm.group_sys.get(), gsys.data(), sizeof(int) * gsys.size(), hipMemcpyHostToDevice, st));
-    SPAI_HIP(hipMemcpyAsync(tm.sys_groups.get(), sgroups.data(), sizeof(int) * sgroups.size(), hipMemcpyHostToDevice,
-                            st));
+    auto up = [&](DeviceArray<int> &d, const std::vector<int> &h) {
+        SPAI_HIP(hipMemcpyAsync(d.get(), h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, st));
+    };
+    up(tm.row, trow);
+    up(tm.sys, tsys);
+    up(tm.sys_tiles, tm.h_sys_tiles);
+    up(tm.group_tile, gtile);
+    up(tm.group_sys, gsys);
+    up(tm.sys_groups, sgroups);
     SPAI_HIP(hipStreamSynchronize(st));
 }
 
@@ -811,7 +878,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     hipLaunchKernelGGL(k_state_init, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys, opt.max_iter,
                        counter_.get());
     {
-        OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, opt.rel_tol, opt.abs_tol};
+        OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, opt.rel_tol, opt.abs_tol, 0};
         launch_tiles<true, double>(A, tm, c, op, st);
     }
     SPAI_CHECK_LAUNCH();
@@ -829,39 +896,57 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     int done_iters = 0;
     try {
         active = read_active();
-        if (opt.use_graph) {
-            int gi = std::max(2, opt.graph_iters + (opt.graph_iters & 1));
-            const void *key[7] = {A.val, M.val, tm.row.get(), b, x, (const void *)(size_t)M.fp32, hist_ptr_};
-            bool same = graph_exec_ && gi == graph_iters_;
-            for (int q = 0; q < 7 && same; ++q) same = key[q] == graph_key_[q];
-            if (!same) {
-                drop_graph();
-                hipStream_t cap;
-                SPAI_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-                SPAI_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-                for (int k = 0; k < gi; ++k) enqueue_iteration(A, M, tm, k & 1, cap);
-                SPAI_HIP(hipStreamEndCapture(cap, &graph_));
-                SPAI_HIP(hipStreamDestroy(cap));
-                SPAI_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
-                graph_iters_ = gi;
-                for (int q = 0; q < 7; ++q) graph_key_[q] = key[q];
-            }
-            while (active > 0 && done_iters < max_launch_iters) {
-                SPAI_HIP(hipGraphLaunch(graph_exec_, st));
-                done_iters += gi;
-                active = read_active();
-            }
-        } else {
-            int every = std::max(1, opt.check_every);
-            int parity = 0;
-            while (active > 0 && done_iters < max_launch_iters) {
-                for (int k = 0; k < every; ++k) {
-                    enqueue_iteration(A, M, tm, parity, st);
-                    parity ^= 1;
+        auto iterate = [&]() {
+            if (opt.use_graph) {
+                int gi = std::max(2, opt.graph_iters + (opt.graph_iters & 1));
+                const void *key[7] = {A.val, M.val, tm.row.get(), b, x, (const void *)(size_t)M.fp32, hist_ptr_};
+                bool same = graph_exec_ && gi == graph_iters_;
+                for (int q = 0; q < 7 && same; ++q) same = key[q] == graph_key_[q];
+                if (!same) {
+                    drop_graph();
+                    hipStream_t cap;
+                    SPAI_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+                    SPAI_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+                    for (int k = 0; k < gi; ++k) enqueue_iteration(A, M, tm, k & 1, cap);
+                    SPAI_HIP(hipStreamEndCapture(cap, &graph_));
+                    SPAI_HIP(hipStreamDestroy(cap));
+                    SPAI_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+                    graph_iters_ = gi;
+                    for (int q = 0; q < 7; ++q) graph_key_[q] = key[q];
                 }
-                done_iters += every;
-                active = read_active();
+                while (active > 0 && done_iters < max_launch_iters) {
+                    SPAI_HIP(hipGraphLaunch(graph_exec_, st));
+                    done_iters += gi;
+                    active = read_active();
+                }
+            } else {
+                int every = std::max(1, opt.check_every);
+                int parity = 0;
+                while (active > 0 && done_iters < max_launch_iters) {
+                    for (int k = 0; k < every; ++k) {
+                        enqueue_iteration(A, M, tm, parity, st);
+                        parity ^= 1;
+                    }
+                    done_iters += every;
+                    active = read_active();
+                }
             }
+        };
+        iterate();
+        for (int round = 0; opt.true_residual_restart && round < 4; ++round) {
+            OpTrueResidual tr{x, b};
+            launch_tiles<true, double>(A, tm, c, tr, st);
+            hipLaunchKernelGGL(k_check_true, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys,
+                               counter_.get());
+            SPAI_CHECK_LAUNCH();
+            active = read_active();
+            if (active == 0) break;
+            OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, opt.rel_tol, opt.abs_tol, 1};
+            launch_tiles<true, double>(A, tm, c, op, st);
+            SPAI_CHECK_LAUNCH();
+            active = read_active();
+            done_iters = 0;
+            iterate();
         }
         {
             OpTrueResidual op{x, b};

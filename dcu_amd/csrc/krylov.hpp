@@ -24,6 +24,7 @@ struct alignas(64) SysState {
     int iters;       // the reference's iteration count once stopped
     int status;      // SolveStatus
     int max_iter;
+    int restarts;    // restarts from the true residual (true_residual_restart)
 };
 
 enum SolveStatus { kRunning = 0, kConverged = 1, kMaxIter = 2, kBreakdown = 3, kConvergedAtStart = 4 };
@@ -66,6 +67,10 @@ struct SolveOptions {
     int graph_iters = 8;     // iterations per graph launch (rounded up to even)
     int check_every = 1;     // iterations between host convergence checks (no graph)
     int record_history = 0;  // keep ||r_k|| per iteration and system
+    // After the recursive residual converges, check ||b - A x||; a system
+    // whose true residual is still >= TOL restarts from its current x with
+    // r = b - A x (at most 4 times).  Off: the reference's stop rule only.
+    int true_residual_restart = 0;
 };
 
 struct SolveResult {

@@ -132,6 +132,9 @@ typedef struct {
     int check_every;         /* iterations between host convergence checks without a graph */
     int precondition;        /* 0: no preconditioner (M = I) */
     int record_history;      /* 1: keep ||r_k||_2 per iteration (spai_system_residual_history) */
+    int true_residual_restart; /* 1: after the recursive residual converges, restart (at most 4
+                                  times) any system whose ||b - A x||_2 is still >= TOL; 0: the
+                                  reference's stop rule alone */
 } spai_solve_options;
 
 void spai_solve_options_default(spai_solve_options *opt);
@@ -151,6 +154,7 @@ typedef struct {
     double residual0;        /* ||r_0||_2 */
     double true_residual;    /* ||b - A x||_2 */
     double tol;              /* absolute threshold used */
+    int restarts;            /* true-residual restarts taken */
 } spai_system_outcome;
 
 /* Solves A x = b for every system of the batch.  x holds x0 on entry.

@@ -99,7 +99,9 @@ def test_batched_solve_per_system(ctx):
     # own count moves by up to +-60 iterations between OpenMP thread counts
     # (a different summation order in the dot products).  Parity: the first
     # 10 iterates agree to 1e-8, and the final count lies inside the oracle's
-    # spread over reduction orders, widened by 1.
+    # spread over ten reduction orders, widened by half the spread's width
+    # (at least 1).  Where convergence is regular the spread has width 0 and
+    # the bar is +-1.
     P = D.Problem.batched(6, 5)
     mats = P.systems()
     S = D.System(ctx, mats)
@@ -127,8 +129,27 @@ def test_batched_solve_per_system(ctx):
             true_rel = O.true_residual(Acsr, bs, xs) / np.linalg.norm(bs)
             print(f"system {s}: gpu {oc[s]['iterations']} oracle spread {spread}", flush=True)
             assert oc[s]["status"] == D.SOLVE_CONVERGED, (s, oc[s])
-            assert min(spread) - 1 <= oc[s]["iterations"] <= max(spread) + 1, (s, oc[s]["iterations"], spread)
+            w = max(1, (max(spread) - min(spread)) // 2)
+            assert min(spread) - w <= oc[s]["iterations"] <= max(spread) + w, (s, oc[s]["iterations"], spread)
             assert true_rel <= 1.5e-8, (s, true_rel)
+
+
+def test_true_residual_restart(ctx):
+    # with the restart option every system ends with ||b - A x|| < TOL itself
+    P = D.Problem.batched(8, 4)
+    mats = P.systems()
+    S = D.System(ctx, mats)
+    S.build(D.PRECISION_MIXED, D.SCHEDULE_BALANCED_NNZ)
+    b = np.concatenate([P.rhs(s) for s in range(len(mats))])
+    x, rep, oc = S.solve(b, rel_tol=1e-8, use_graph=True, true_residual_restart=True)
+    off = S.row_offsets
+    for s, A in enumerate(mats):
+        bs = b[off[s]:off[s + 1]]
+        Acsr = O.csc2csr(A.n, A.colptr, A.rowidx, A.val)[:3]
+        true = O.true_residual(Acsr, bs, x[off[s]:off[s + 1]])
+        assert oc[s]["status"] == D.SOLVE_CONVERGED
+        assert true < oc[s]["tol"] * (1 + 1e-9), (s, true, oc[s])
+        assert abs(oc[s]["true_residual"] - true) <= 1e-6 * true
 
 
 def test_max_iter_semantics(ctx):
