@@ -80,6 +80,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work")
     ap.add_argument("--graph-iters", type=int, default=8)
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="after one solve, launch each fused kernel --profile-reps times and a device copy of "
+                         "known size (256 MiB read + 256 MiB written) as many times, for counter collection")
+    ap.add_argument("--profile-reps", type=int, default=20)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -120,6 +124,21 @@ def main():
                                graph_iters=args.graph_iters, true_residual_restart=True)
         return rep_b, rep_s
 
+    if args.profile_kernels:
+        step()
+        for which in (1, 2, 3, 4):
+            S.time_spmv(which, args.profile_reps)
+        copy_bytes = 256 * 1024 * 1024
+        ctx.calibration_copy(copy_bytes, args.profile_reps)
+        m_bytes = 4 if precision == D.PRECISION_MIXED else 8
+        print(json.dumps({"profile_kernels": True, "n": n_tot, "nnz": nnz_tot,
+                          "algorithmic_bytes": {k: tile_bytes(k, n_tot, nnz_tot, m_bytes) for k in
+                                                ("mapply_p", "spmv_alpha", "mapply_s", "spmv_omega")},
+                          "copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes}))
+        S.close()
+        ctx.close()
+        return
+
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -155,6 +174,18 @@ def main():
                 "all_kernels_ms": {k: round(v, 5) for k, v in kernel_ms.items()},
                 "all_kernels_frac": {k: round(tile_bytes(k, n_tot, nnz_tot, m_bytes) / (v * 1e-3) / 1e9 /
                                               HBM_PEAK_GBS, 4) for k, v in kernel_ms.items()}}
+    # HBM traffic per launch from the committed counter summary of this
+    # workload (two rocprofv3 --pmc passes, calibrated against a copy kernel)
+    traffic_file = os.path.join(ROOT, "profiles", "batched64_traffic.md")
+    if args.workload == "batched" and precision == D.PRECISION_MIXED and os.path.exists(traffic_file):
+        txt = open(traffic_file).read()
+        blob = txt[txt.index("```json") + 7: txt.rindex("```")]
+        kern = json.loads(blob)["kernels"].get(dominant)
+        if kern:
+            roofline["traffic"] = kern["traffic_bytes"]
+            roofline["traffic_source"] = "profiles/batched64_traffic.md"
+    copy_ms = ctx.calibration_copy(256 * 1024 * 1024, 10)
+    roofline["copy_kernel_GBs"] = round(2 * 256 * 1024 * 1024 / (copy_ms * 1e-3) / 1e9, 1)
     # whole solve: every system's iterations x its per-iteration algorithmic
     # bytes (K1..K5 as above plus the 64 n of the update kernel)
     per_iter = 0.0

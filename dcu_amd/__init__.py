@@ -45,7 +45,7 @@ C_SYMBOLS = [
     "spai_system_set_values", "spai_system_build", "spai_system_get_m",
     "spai_system_get_row_set_sizes", "spai_solve_options_default", "spai_system_solve",
     "spai_system_outcomes", "spai_system_residual_history", "spai_system_adjoint", "spai_system_time_spmv",
-    "spai_csc_to_csr_device",
+    "spai_calibration_copy", "spai_csc_to_csr_device",
     "spai_problem_generate", "spai_problem_generate_power_law", "spai_problem_generate_batched",
     "spai_problem_generate_grf", "spai_problem_convection_diffusion", "spai_problem_read_mtx", "spai_problem_free",
     "spai_problem_num_systems", "spai_problem_system", "spai_problem_rhs", "spai_problem_target",
@@ -164,6 +164,8 @@ def load_library() -> ctypes.CDLL:
         "spai_system_adjoint": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_int, ctypes.POINTER(SolveOptions),
                                                ctypes.POINTER(SolveReport)]),
         "spai_system_time_spmv": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
+        "spai_calibration_copy": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_int,
+                                                 ctypes.POINTER(ctypes.c_float)]),
         "spai_csc_to_csr_device": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp,
                                                   vp]),
         "spai_problem_generate": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(vp)]),
@@ -341,6 +343,12 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def calibration_copy(self, nbytes: int, reps: int = 10) -> float:
+        """Average ms of a plain device copy reading and writing nbytes."""
+        ms = ctypes.c_float()
+        _check(self._lib.spai_calibration_copy(self._h, int(nbytes), int(reps), ctypes.byref(ms)))
+        return ms.value
 
     def device_info(self) -> Tuple[str, int]:
         buf = ctypes.create_string_buffer(128)

@@ -50,6 +50,14 @@ __global__ void k_fill_f32(float *p, float v, long long n) {
     if (i < n) p[i] = v;
 }
 
+// Plain copy of `n` doubles, 16 bytes per lane per step (calibration kernel).
+__global__ __launch_bounds__(256) void k_copy_calibration(const double2 *__restrict__ src,
+                                                            double2 *__restrict__ dst, long long n2) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n2; i += stride) dst[i] = src[i];
+}
+
 void fill(double *p, long long n, hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, 0.0, n);
     SPAI_CHECK_LAUNCH();
@@ -619,6 +627,26 @@ int spai_system_time_spmv(spai_system *sys, int which, int reps, float *ms) {
         OperatorView A{sys->rowptr.get(), sys->colidx.get(), sys->csrval.get(), 0};
         OperatorView M = which != 0 ? m_view(sys, false) : OperatorView{};
         *ms = sys->ctx->solver.time_kernel(which, A, M, sys->tiles, sys->n, sys->nsys, reps, sys->ctx->stream);
+    });
+}
+
+int spai_calibration_copy(spai_context *ctx, long long bytes, int reps, float *ms) {
+    if (!ctx || bytes < 16 || reps < 1 || !ms) return fail(SPAI_ERR_INVALID, "bad calibration arguments");
+    return guarded([&] {
+        SPAI_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        long long n2 = bytes / 16;
+        DeviceArray<double> a((size_t)n2 * 2), b((size_t)n2 * 2);
+        fill(a.get(), n2 * 2, st);
+        int grid = ctx->cus * 16;
+        hipLaunchKernelGGL(k_copy_calibration, dim3(grid), dim3(256), 0, st, (const double2 *)a.get(),
+                           (double2 *)b.get(), n2);
+        Timer tm(st);
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_copy_calibration, dim3(grid), dim3(256), 0, st, (const double2 *)a.get(),
+                               (double2 *)b.get(), n2);
+        SPAI_CHECK_LAUNCH();
+        *ms = tm.stop() / reps;
     });
 }
 

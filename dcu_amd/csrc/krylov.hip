@@ -30,10 +30,29 @@
 
 namespace spai {
 
+// Reductions are counted in two levels so that no counter sees more than
+// kBucket atomic increments per launch (same-address atomics serialise at
+// the memory side): groups -> buckets of up to kBucket consecutive groups of
+// one system -> the system.
+struct Ctx {
+    SysState *st;
+    const int *group_bucket;  // bucket of each group
+    const int *bucket_group;  // nbuckets + 1: first group of each bucket
+    const int *sys_buckets;   // nsys + 1: first bucket of each system
+    int *counter;             // [nsys] systems, [nsys] = n_active, [nsys + 1 + b] buckets
+    double *partials;         // ngroups * 2 group partials
+    double *bpartials;        // nbuckets * 2 bucket partials
+    int nsys;
+    double *hist;             // optional ||r_k|| history, hist_stride entries per system
+    int hist_stride;
+};
+
+
 namespace {
 
 constexpr int kThreads = 256;
 constexpr int kLdsProducts = 2048;  // products staged per tile (16 KB)
+constexpr int kBucket = 32;         // groups per first-level reduction counter
 
 // ------------------------------------------------------------ transpose ----
 __global__ void k_fill_int(int *p, int v, long long n) {
@@ -178,54 +197,58 @@ __device__ __forceinline__ void block_reduce(double (&v)[NRED], double *sh) {
     __syncthreads();
 }
 
-struct Ctx {
-    SysState *st;
-    const int *sys_units;  // nsys + 1 offsets of the reducing workgroups (groups) per system
-    int *counter;          // [nsys] + n_active at [nsys]
-    double *partials;
-    int nsys;
-    double *hist;          // optional ||r_k|| history, hist_stride entries per system
-    int hist_stride;
-};
-
-// Publishes this tile's partials; the last tile of the system sums all of
-// them in tile order and calls op.finish on its thread 0.
+// Publishes this group's partials; the last group of a bucket sums the
+// bucket, the last bucket of the system sums the buckets (both in a fixed
+// order) and calls op.finish on its thread 0.
 //
 // No __threadfence(): on gfx950 an agent-scope fence writes back and
 // invalidates the whole L2 (buffer_wbl2 / buffer_inv), which, issued by
-// thousands of tiles, evicts the vectors the other tiles are gathering.
-// Instead the partials are agent-scope atomic stores (coherent across the
-// XCDs' L2s), the thread waits until they are acknowledged (vmcnt(0)) before
-// the relaxed counter increment, and the last tile reads them back with
+// thousands of workgroups, evicts the vectors the others are gathering.
+// Instead partials are agent-scope atomic stores (coherent across the XCDs'
+// L2s), the thread waits until they are acknowledged (vmcnt(0)) before the
+// relaxed counter increment, and the last workgroup reads them back with
 // agent-scope atomic loads.
-template <int NRED, class Op>
-__device__ __forceinline__ void reduce_system(const Ctx &c, int tile, int s, double (&v)[NRED], Op &op, double *sh) {
-    block_reduce<NRED>(v, sh);
-    __shared__ int am_last;
+template <int NRED>
+__device__ __forceinline__ bool publish_and_count(double *dst, const double (&v)[NRED], int *counter, int expect,
+                                                  int *flag) {
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < NRED; ++q)
-            __hip_atomic_store(&c.partials[(size_t)tile * NRED + q], v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int q = 0; q < NRED; ++q) __hip_atomic_store(&dst[q], v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        int nt = c.sys_units[s + 1] - c.sys_units[s];
-        int prev = __hip_atomic_fetch_add(&c.counter[s], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        am_last = (prev == nt - 1);
+        int prev = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *flag = (prev == expect - 1);
+        if (*flag) __hip_atomic_store(counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
-    if (!am_last) return;
-    const int t0 = c.sys_units[s], t1 = c.sys_units[s + 1];
-    double acc[NRED];
+    return *flag != 0;
+}
+
+template <int NRED>
+__device__ __forceinline__ void sum_range(const double *src, int first, int last, double (&acc)[NRED], double *sh) {
 #pragma unroll
     for (int q = 0; q < NRED; ++q) acc[q] = 0.0;
-    for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x) {
+    for (int t = first + (int)threadIdx.x; t < last; t += blockDim.x) {
 #pragma unroll
-        for (int q = 0; q < NRED; ++q) acc[q] += load_coherent(&c.partials[(size_t)t * NRED + q]);
+        for (int q = 0; q < NRED; ++q) acc[q] += load_coherent(&src[(size_t)t * NRED + q]);
     }
     block_reduce<NRED>(acc, sh);
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(&c.counter[s], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        op.finish(c, s, acc);
-    }
+}
+
+template <int NRED, class Op>
+__device__ __forceinline__ void reduce_system(const Ctx &c, int group, int s, double (&v)[NRED], Op &op, double *sh) {
+    __shared__ int flag;
+    block_reduce<NRED>(v, sh);
+    const int b = c.group_bucket[group];
+    const int g0 = c.bucket_group[b], g1 = c.bucket_group[b + 1];
+    if (!publish_and_count<NRED>(&c.partials[(size_t)group * NRED], v, &c.counter[c.nsys + 1 + b], g1 - g0, &flag))
+        return;
+    double bsum[NRED];
+    sum_range<NRED>(c.partials, g0, g1, bsum, sh);
+    const int b0 = c.sys_buckets[s], b1 = c.sys_buckets[s + 1];
+    if (!publish_and_count<NRED>(&c.bpartials[(size_t)b * NRED], bsum, &c.counter[s], b1 - b0, &flag)) return;
+    double tot[NRED];
+    sum_range<NRED>(c.bpartials, b0, b1, tot, sh);
+    if (threadIdx.x == 0) op.finish(c, s, tot);
 }
 
 // ------------------------------------------------------- tile kernels ----
@@ -247,6 +270,22 @@ constexpr int kRowsPerThread = kRowsPerTile / kThreads;
 // index/value/row-pointer/operand loads, then the gathers.  HAS_MATRIX =
 // false skips the product (streaming ops).  Ops with reductions run one
 // workgroup per group of tiles (GROUPED) and publish one partial per group.
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Consecutive tiles gather overlapping vector ranges, so within every period
+// of 8 * kXcdRun workgroups, XCD x is given the contiguous run of kXcdRun
+// units starting at x * kXcdRun; a partial last period keeps the identity.
+// Runs (not one block of G/8 units per XCD) keep every system spread over all
+// XCDs when only a few systems are still iterating.
+constexpr int kXcds = 8;
+constexpr int kXcdRun = 8;
+__device__ __forceinline__ int xcd_unit(int b, int G) {
+    constexpr int P = kXcds * kXcdRun;
+    const int p = b / P;
+    if ((p + 1) * P > G) return b;
+    const int i = b - p * P;
+    return p * P + (i % kXcds) * kXcdRun + i / kXcds;
+}
+
 template <bool HAS_MATRIX, class MT, class Op>
 __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr, const int *__restrict__ idx,
                                                     const MT *__restrict__ val, TileView tv, Ctx c, Op op) {
@@ -255,7 +294,7 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
     __shared__ double prod[HAS_MATRIX ? kLdsProducts : 1];
     __shared__ int rp[HAS_MATRIX ? kRowsPerTile + 1 : 1];
     __shared__ double sh[(NRED > 0 ? NRED : 1) * 4];
-    const int unit = blockIdx.x;
+    const int unit = xcd_unit(blockIdx.x, gridDim.x);
     const int s = GROUPED ? tv.group_sys[unit] : tv.sys[unit];
     if (!op.enabled(c.st[s])) return;
     op.load(c.st[s]);
@@ -598,7 +637,7 @@ struct OpCopyS {  // K3 without a preconditioner: z = s
     __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
 
-__global__ void k_state_init(SysState *st, int nsys, int max_iter, int *counter) {
+__global__ void k_state_init(SysState *st, int nsys, int max_iter, int *counter, int nbuckets) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < nsys) {
         SysState z{};
@@ -609,6 +648,7 @@ __global__ void k_state_init(SysState *st, int nsys, int max_iter, int *counter)
         counter[s] = 0;
     }
     if (s == 0) counter[nsys] = nsys;
+    for (int b = s; b < nbuckets; b += gridDim.x * blockDim.x) counter[nsys + 1 + b] = 0;
 }
 
 // systems whose recursive residual converged but whose true residual did not
@@ -626,12 +666,13 @@ __global__ void k_check_true(SysState *st, int nsys, int *counter) {
 }
 
 // re-enables every system (scalars kept) so single kernels can be timed
-__global__ void k_state_activate(SysState *st, int nsys, int *counter) {
+__global__ void k_state_activate(SysState *st, int nsys, int *counter, int nbuckets) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < nsys) {
         st[s].active = 1;
         counter[s] = 0;
     }
+    for (int b = s; b < nbuckets; b += gridDim.x * blockDim.x) counter[nsys + 1 + b] = 0;
 }
 
 __global__ void k_adjoint_scatter(int n, const int *colptr, const int *rowidx, const double *lambda, const double *x,
@@ -735,6 +776,21 @@ void build_tiles(TileMap &tm, const int *ptr, const std::vector<int> &h_row_off,
     }
     gtile.push_back(tm.ntiles);
     tm.ngroups = (int)gsys.size();
+    // buckets of up to kBucket consecutive groups of one system
+    std::vector<int> gbucket((size_t)tm.ngroups), bgroup, sbuckets((size_t)nsys + 1, 0);
+    for (int s = 0; s < nsys; ++s) {
+        for (int g = sgroups[(size_t)s]; g < sgroups[(size_t)s + 1]; g += kBucket) {
+            int bidx = (int)bgroup.size();
+            bgroup.push_back(g);
+            for (int h = g; h < std::min(g + kBucket, sgroups[(size_t)s + 1]); ++h) gbucket[(size_t)h] = bidx;
+        }
+        sbuckets[(size_t)s + 1] = (int)bgroup.size();
+    }
+    bgroup.push_back(tm.ngroups);
+    tm.nbuckets = (int)bgroup.size() - 1;
+    tm.group_bucket.resize(gbucket.size());
+    tm.bucket_group.resize(bgroup.size());
+    tm.sys_buckets.resize(sbuckets.size());
     tm.row.resize(trow.size());
     tm.sys.resize(tsys.size());
     tm.sys_tiles.resize((size_t)nsys + 1);
@@ -750,6 +806,9 @@ void build_tiles(TileMap &tm, const int *ptr, const std::vector<int> &h_row_off,
     up(tm.group_tile, gtile);
     up(tm.group_sys, gsys);
     up(tm.sys_groups, sgroups);
+    up(tm.group_bucket, gbucket);
+    up(tm.bucket_group, bgroup);
+    up(tm.sys_buckets, sbuckets);
     SPAI_HIP(hipStreamSynchronize(st));
 }
 
@@ -779,7 +838,13 @@ void BiCGStab::drop_graph() {
     graph_iters_ = 0;
 }
 
-void BiCGStab::ensure(int n, int nsys, int ntiles) {
+Ctx BiCGStab::make_ctx(const TileMap &tm, int nsys) {
+    return Ctx{state_.get(),     tm.group_bucket.get(), tm.bucket_group.get(), tm.sys_buckets.get(),
+               counter_.get(),   partials_.get(),       bpartials_.get(),      nsys,
+               hist_ptr_,        hist_stride_};
+}
+
+void BiCGStab::ensure(int n, int nsys, int ntiles, int nbuckets) {
     if (n != n_) {
         drop_graph();
         vec_.resize((size_t)9 * std::max(n, 1));
@@ -796,11 +861,13 @@ void BiCGStab::ensure(int n, int nsys, int ntiles) {
         t_ = v + 8 * N;
         n_ = n;
     }
-    if (nsys != nsys_) {
+    if (nsys != nsys_ || nbuckets != nbuckets_) {
         drop_graph();
         state_.resize((size_t)nsys);
-        counter_.resize((size_t)nsys + 1);
+        counter_.resize((size_t)nsys + 1 + nbuckets);
+        bpartials_.resize((size_t)2 * std::max(nbuckets, 1));
         nsys_ = nsys;
+        nbuckets_ = nbuckets;
     }
     if (ntiles != ntiles_) {
         drop_graph();
@@ -811,7 +878,7 @@ void BiCGStab::ensure(int n, int nsys, int ntiles) {
 
 void BiCGStab::enqueue_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tm, int parity,
                                  hipStream_t st) {
-    Ctx c{state_.get(), tm.sys_groups.get(), counter_.get(), partials_.get(), nsys_, hist_ptr_, hist_stride_};
+    Ctx c = make_ctx(tm, nsys_);
     double *pin = p_[parity], *pout = p_[parity ^ 1];
     if (M.val) {
         if (M.fp32) {
@@ -859,7 +926,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     require(tm.ntiles > 0 && tm.ngroups > 0 && (int)tm.h_sys_tiles.size() == nsys + 1,
             "bicgstab: tiles do not match the systems");
     require(opt.max_iter >= 1, "bicgstab: max_iter must be >= 1");
-    ensure(n, nsys, tm.ngroups);
+    ensure(n, nsys, tm.ngroups, tm.nbuckets);
     b_ = b;
     x_ = x;
     if (opt.record_history) {
@@ -874,9 +941,9 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     SPAI_HIP(hipEventCreate(&e0));
     SPAI_HIP(hipEventCreate(&e1));
     SPAI_HIP(hipEventRecord(e0, st));
-    Ctx c{state_.get(), tm.sys_groups.get(), counter_.get(), partials_.get(), nsys, hist_ptr_, hist_stride_};
+    Ctx c = make_ctx(tm, nsys);
     hipLaunchKernelGGL(k_state_init, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys, opt.max_iter,
-                       counter_.get());
+                       counter_.get(), tm.nbuckets);
     {
         OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, opt.rel_tol, opt.abs_tol, 0};
         launch_tiles<true, double>(A, tm, c, op, st);
@@ -1002,10 +1069,10 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
     // scalars and vectors of the last solve (systems re-activated first)
     require(which >= 0 && which <= 5, "time_kernel: which must be 0..5");
     require((which != 1 && which != 3 && which != 5) || M.val, "time_kernel: no preconditioner");
-    ensure(n, nsys, tm.ngroups);
-    Ctx c{state_.get(), tm.sys_groups.get(), counter_.get(), partials_.get(), nsys, hist_ptr_, hist_stride_};
+    ensure(n, nsys, tm.ngroups, tm.nbuckets);
+    Ctx c = make_ctx(tm, nsys);
     hipLaunchKernelGGL(k_state_activate, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys,
-                       counter_.get());
+                       counter_.get(), tm.nbuckets);
     hipEvent_t e0, e1;
     SPAI_HIP(hipEventCreate(&e0));
     SPAI_HIP(hipEventCreate(&e1));

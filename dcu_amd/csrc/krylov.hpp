@@ -46,6 +46,11 @@ struct TileMap {
     DeviceArray<int> group_tile;  // ngroups + 1: first tile of each group
     DeviceArray<int> group_sys;   // system of each group
     DeviceArray<int> sys_groups;  // nsys + 1 group offsets per system
+    // buckets of consecutive groups of one system (two-level reduction count)
+    int nbuckets = 0;
+    DeviceArray<int> group_bucket;  // bucket of each group
+    DeviceArray<int> bucket_group;  // nbuckets + 1: first group of each bucket
+    DeviceArray<int> sys_buckets;   // nsys + 1: first bucket of each system
 };
 
 // Device CSC -> CSR (rows ordered by column) plus perm[csc_pos] = csr_pos.
@@ -90,6 +95,8 @@ struct OperatorView {
     int fp32 = 0;
 };
 
+struct Ctx;  // device-side view of the solver state (krylov.hip)
+
 class BiCGStab {
 public:
     BiCGStab();
@@ -109,12 +116,13 @@ public:
                       int reps, hipStream_t st);
 
 private:
-    void ensure(int n, int nsys, int ntiles);
+    void ensure(int n, int nsys, int ntiles, int nbuckets);
+    Ctx make_ctx(const TileMap &tm, int nsys);
     void enqueue_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tiles, int parity,
                            hipStream_t st);
-    int n_ = 0, nsys_ = 0, ntiles_ = 0;
+    int n_ = 0, nsys_ = 0, ntiles_ = 0, nbuckets_ = -1;
     DeviceArray<double> vec_;       // x-independent work vectors
-    DeviceArray<double> partials_;
+    DeviceArray<double> partials_, bpartials_;
     DeviceArray<SysState> state_;
     DeviceArray<int> counter_;      // per system, + n_active at [nsys]
     std::vector<SysState> h_states_;

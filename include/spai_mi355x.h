@@ -182,6 +182,11 @@ int spai_system_adjoint(spai_system *sys, const double *x, const double *g, doub
  * last solve. */
 int spai_system_time_spmv(spai_system *sys, int which, int reps, float *ms);
 
+/* Average device time (ms) of a plain copy kernel that reads and writes
+ * `bytes` bytes each (the calibration kernel for hardware byte counters and
+ * the practical streaming bandwidth). */
+int spai_calibration_copy(spai_context *ctx, long long bytes, int reps, float *ms);
+
 /* ------------------------------------------------- format conversion ---- */
 /* Device CSC -> CSR with rows ordered by column (drop-in for cuCSC2CSR);
  * all pointers on the current device, caller-allocated. */
