@@ -858,7 +858,13 @@ int cublas2_pbicgstabv2(CSR_Matrix *devA, CSR_Matrix *devM, double *dev_b, doubl
         ctx->solver.solve(A, M, ta, n, 1, dev_b, dev_x, o, st, &res);
     });
     if (status != SPAI_OK) die("cublas2_pbicgstabv2");
-    return ctx->solver.states()[0].iters;
+    const SysState st0 = ctx->solver.states()[0];
+    // The reference has no breakdown test: once a scalar is non-finite its
+    // err < TOL never holds and it returns MAX_ITER
+    // (include/bicgstab/bicgstab_solver.h:138, :228).  x holds non-finite
+    // values in both cases.
+    if (st0.status == kBreakdown) return MAX_ITER <= 0 ? n : MAX_ITER;
+    return st0.iters;
 }
 
 void cuCSC2CSR(int m, int n, int nnz, double *cscData, int *cscIndex, int *cscPtr, double *csrData, int *csrIndex,

@@ -120,8 +120,9 @@ int main(int argc, char **argv) {
     check(spai_system_build(S, precision, nnz_balance ? SPAI_SCHEDULE_BALANCED_NNZ : SPAI_SCHEDULE_BALANCED_COLUMNS,
                             0, -1, &br),
           "SPAI build");
-    std::printf("SPAI build: %.4f ms  max|J|=%d  max gathered=%d  tiers=[%d %d %d %d]\n", br.ms, br.max_n2,
-                br.max_gathered, br.tier_columns[0], br.tier_columns[1], br.tier_columns[2], br.tier_columns[3]);
+    std::printf("SPAI build: %.4f ms  max|J|=%d  max gathered=%d  tiers=[%d %d %d %d %d]\n", br.ms, br.max_n2,
+                br.max_gathered, br.tier_columns[0], br.tier_columns[1], br.tier_columns[2], br.tier_columns[3],
+                br.tier_columns[4]);
 
     int iterations = 0;
     if (!precondition_only) {
@@ -150,6 +151,16 @@ int main(int argc, char **argv) {
         spai_system_outcomes(S, oc.data());
         iterations = sr.iterations_max;
         double worst = 0.0;
+        int breakdowns = 0;
+        for (auto &q : oc) breakdowns += q.status == SPAI_SOLVE_BREAKDOWN;
+        if (breakdowns) {
+            // the reference loop runs on with non-finite scalars until
+            // MAX_ITER and reports that count
+            std::printf("BiCGSTAB: %d system(s) stopped on a non-finite scalar at iteration %d; reporting max_iter "
+                        "as the reference does\n",
+                        breakdowns, sr.iterations_max);
+            iterations = max_iter;
+        }
         for (auto &q : oc) {
             double bn = q.tol > 0 && rel_tol > 0 ? q.tol / rel_tol : 1.0;
             worst = std::max(worst, q.true_residual / bn);

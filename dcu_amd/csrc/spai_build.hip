@@ -638,6 +638,256 @@ __global__ __launch_bounds__(64) void spai_lds_tier_kernel(ColumnArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Global tier: one 512-thread workgroup per very wide column, every buffer in
+// a per-workgroup scratch slot in global memory, FP64 in both precision modes.
+// ---------------------------------------------------------------------------
+constexpr int kGlobThreads = 512;
+constexpr int kGlobCols = 32;                           // columns per pass
+constexpr int kGlobRowGroups = kGlobThreads / kGlobCols;  // 16 row groups
+
+struct GlobalScratch {
+    double *tile;       // slots * tile_cap: the n1 x n2 tile, row-major (stride n2)
+    double *vec;        // slots * vec_cap: y (n1), tau, x, w (n2 each)
+    uint64_t *keys;     // slots * key_cap: sorted (row, entry) keys
+    int *ints;          // slots * int_cap: positions (key_cap + 1), J offsets (n2 + 1)
+    long long tile_cap, vec_cap, key_cap, int_cap;
+    int n1_cap, n2_cap;
+};
+
+__device__ double glob_block_sum(double v, double *red) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < kGlobThreads / 64; ++i) t += red[i];
+    return t;
+}
+
+// Exclusive scan of cnt(i), i < m, into out[0..m] (out[m] = total); returns total.
+template <class F>
+__device__ int glob_block_scan(int m, F cnt, int *out, int *ired) {
+    int carry = 0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int base = 0; base < m; base += kGlobThreads) {
+        const int i = base + (int)threadIdx.x;
+        const int v = i < m ? cnt(i) : 0;
+        int incl = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            int t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        __syncthreads();
+        if (lane == 63) ired[w] = incl;
+        __syncthreads();
+        int before = 0, total = 0;
+#pragma unroll
+        for (int q = 0; q < kGlobThreads / 64; ++q) {
+            before += q < w ? ired[q] : 0;
+            total += ired[q];
+        }
+        if (i < m) out[i] = carry + before + incl - v;
+        carry += total;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) out[m] = carry;
+    __syncthreads();
+    return carry;
+}
+
+__device__ __forceinline__ int glob_find_j(const int *offs, int n2, int e) {
+    int lo = 0, hi = n2 - 1;  // largest j with offs[j] <= e
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (offs[mid] <= e) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <class MT>
+__device__ void solve_column_global(const ColumnArgs &a, int k, const GlobalScratch &g, int slot, double *red,
+                                    int *ired, int *sh) {
+    const int tid = threadIdx.x;
+    const int c0 = a.colptr[k];
+    const int n2 = a.colptr[k + 1] - c0;
+    if (n2 <= 0) return;
+    double *tile = g.tile + (size_t)slot * g.tile_cap;
+    double *yv = g.vec + (size_t)slot * g.vec_cap;
+    double *tau = yv + g.n1_cap;
+    double *xv = tau + g.n2_cap;
+    double *wv = xv + g.n2_cap;
+    uint64_t *keys = g.keys + (size_t)slot * g.key_cap;
+    int *pos = g.ints + (size_t)slot * g.int_cap;  // key_cap + 1 (scan total at [T])
+    int *offs = pos + g.key_cap + 1;                // n2 + 1
+
+    // J offsets: exclusive prefix of |A(:, J_k[j])|
+    const int T = glob_block_scan(
+        n2, [&](int j) { int jr = a.rowidx[c0 + j]; return a.colptr[jr + 1] - a.colptr[jr]; }, offs, ired);
+    int Tp = 1;
+    while (Tp < T) Tp <<= 1;
+    for (int e = tid; e < Tp; e += kGlobThreads) {
+        uint64_t key = ~0ull;
+        if (e < T) {
+            const int j = glob_find_j(offs, n2, e);
+            const int jr = a.rowidx[c0 + j];
+            key = ((uint64_t)(unsigned)a.rowidx[a.colptr[jr] + e - offs[j]] << 32) | (unsigned)e;
+        }
+        keys[e] = key;
+    }
+    __syncthreads();
+    // bitonic sort of Tp keys in global memory
+    for (int size = 2; size <= Tp; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int q = tid; q < Tp / 2; q += kGlobThreads) {
+                const int i = 2 * stride * (q / stride) + (q % stride);
+                const int l = i + stride;
+                const bool up = (i & size) == 0;
+                const uint64_t x = keys[i], y = keys[l];
+                if ((x > y) == up) {
+                    keys[i] = y;
+                    keys[l] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // positions of the unique rows: pos[e] = #first occurrences in [0, e] - 1
+    const int n1 = glob_block_scan(
+        T,
+        [&](int e) { return (e == 0 || (keys[e] >> 32) != (keys[e - 1] >> 32)) ? 1 : 0; }, pos, ired);
+    if (tid == 0) sh[0] = -1;
+    __syncthreads();
+    for (int e = tid; e < T; e += kGlobThreads) {
+        const bool first = (e == 0 || (keys[e] >> 32) != (keys[e - 1] >> 32));
+        if (first && (int)(keys[e] >> 32) == k) sh[0] = pos[e];  // positions are exclusive counts
+    }
+    __syncthreads();
+    const int E = sh[0];
+    // exclusive count of first occurrences -> position of each entry's row:
+    // inclusive count - 1 (a duplicate's run starts earlier, so its exclusive
+    // count already includes its own row)
+    for (int e = tid; e < T; e += kGlobThreads) {
+        const bool first = (e == 0 || (keys[e] >> 32) != (keys[e - 1] >> 32));
+        pos[e] = pos[e] + (first ? 1 : 0) - 1;
+    }
+    __syncthreads();
+    for (long long idx = tid; idx < (long long)n1 * n2; idx += kGlobThreads) tile[idx] = 0.0;
+    __syncthreads();
+    for (int e = tid; e < T; e += kGlobThreads) {
+        const int ee = (int)(unsigned)keys[e];
+        const int j = glob_find_j(offs, n2, ee);
+        const int jr = a.rowidx[c0 + j];
+        tile[(size_t)pos[e] * n2 + j] = a.val[a.colptr[jr] + ee - offs[j]];
+    }
+    __syncthreads();
+    // Householder QR, H_j = I - tau_j v v^T with v_j = 1
+    const int lane32 = tid % kGlobCols, rg = tid / kGlobCols;
+    for (int j = 0; j < n2; ++j) {
+        const double alpha = tile[(size_t)j * n2 + j];
+        double sl = 0.0;
+        for (int i = j + 1 + tid; i < n1; i += kGlobThreads) {
+            const double v = tile[(size_t)i * n2 + j];
+            sl += v * v;
+        }
+        const double sigma = glob_block_sum(sl, red);
+        double beta = alpha, t = 0.0, scale = 0.0;
+        if (sigma != 0.0) {
+            const double nrm = sqrt(alpha * alpha + sigma);
+            beta = alpha >= 0.0 ? -nrm : nrm;
+            t = (beta - alpha) / beta;
+            scale = 1.0 / (alpha - beta);
+        }
+        for (int i = j + 1 + tid; i < n1; i += kGlobThreads) tile[(size_t)i * n2 + j] *= scale;
+        if (tid == 0) {
+            tile[(size_t)j * n2 + j] = beta;
+            tau[j] = t;
+        }
+        __syncthreads();
+        if (t == 0.0) continue;
+        // w_c = tau * (A[j][c] + sum_{i>j} v_i A[i][c]) for c > j, 32 columns a pass
+        for (int cb = j + 1; cb < n2; cb += kGlobCols) {
+            const int c = cb + lane32;
+            double part = 0.0;
+            if (c < n2) {
+                for (int i = j + rg; i < n1; i += kGlobRowGroups) {
+                    const double v = i == j ? 1.0 : tile[(size_t)i * n2 + j];
+                    part += v * tile[(size_t)i * n2 + c];
+                }
+            }
+            red[tid + 64] = part;  // red[64 .. 64 + kGlobThreads)
+            __syncthreads();
+            if (rg == 0 && c < n2) {
+                double w = 0.0;
+                for (int q = 0; q < kGlobRowGroups; ++q) w += red[64 + q * kGlobCols + lane32];
+                wv[c] = w * t;
+            }
+            __syncthreads();
+        }
+        // A[i][c] -= w_c v_i for i >= j, c > j
+        const int wd = n2 - j - 1;
+        for (long long idx = tid; idx < (long long)(n1 - j) * wd; idx += kGlobThreads) {
+            const int i = j + (int)(idx / wd), c = j + 1 + (int)(idx % wd);
+            const double v = i == j ? 1.0 : tile[(size_t)i * n2 + j];
+            tile[(size_t)i * n2 + c] -= wv[c] * v;
+        }
+        __syncthreads();
+    }
+    // y = Q^T e_E
+    for (int i = tid; i < n1; i += kGlobThreads) yv[i] = (i == E) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int j = 0; j < n2 && E >= 0; ++j) {
+        const double t = tau[j];
+        double sl = 0.0;
+        for (int i = j + tid; i < n1; i += kGlobThreads) {
+            const double v = i == j ? 1.0 : tile[(size_t)i * n2 + j];
+            sl += v * yv[i];
+        }
+        const double w = glob_block_sum(sl, red) * t;
+        for (int i = j + tid; i < n1; i += kGlobThreads) {
+            const double v = i == j ? 1.0 : tile[(size_t)i * n2 + j];
+            yv[i] -= w * v;
+        }
+        __syncthreads();
+    }
+    // back substitution, column oriented: x_c = y_c / R_cc, then y_q -= R_qc x_c
+    for (int c = n2 - 1; c >= 0; --c) {
+        const double xc = E < 0 ? 0.0 : yv[c] / tile[(size_t)c * n2 + c];
+        for (int q = tid; q < c; q += kGlobThreads) yv[q] -= tile[(size_t)q * n2 + c] * xc;
+        if (tid == 0) xv[c] = xc;
+        __syncthreads();
+    }
+    for (int c = tid; c < n2; c += kGlobThreads) store_m<MT>(a, c0 + c, xv[c]);
+    if (a.n1_out && tid == 0) a.n1_out[k] = n1;
+    __syncthreads();
+}
+
+template <class MT, bool DYNAMIC>
+__global__ __launch_bounds__(kGlobThreads) void spai_global_tier_kernel(ColumnArgs a, GlobalScratch g) {
+    __shared__ double red[64 + kGlobThreads];
+    __shared__ int ired[kGlobThreads / 64];
+    __shared__ int sh[2];
+    const int slot = blockIdx.x;
+    if (DYNAMIC) {
+        const int count = *a.count;
+        for (;;) {
+            if (threadIdx.x == 0) sh[1] = atomicAdd(a.head, 1);
+            __syncthreads();
+            const int idx = sh[1];
+            __syncthreads();
+            if (idx >= count) break;
+            solve_column_global<MT>(a, a.list[idx], g, slot, red, ired, sh);
+        }
+    } else {
+        for (int k = a.col_begin + blockIdx.x; k < a.col_end; k += gridDim.x)
+            solve_column_global<MT>(a, k, g, slot, red, ired, sh);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Planner
 // ---------------------------------------------------------------------------
 // counters layout (ints): [0..kTiers) column count per tier,
@@ -666,13 +916,15 @@ __global__ void spai_zero_counters(int *c, int n, unsigned long long *s64, int n
 // One thread per column: n2_k, T_k = sum_j |A(:,J_k[j])|, tier.  Counts and
 // maxima are combined in LDS first, so a workgroup issues one global atomic
 // per counter instead of one per wavefront (same-address atomics serialise).
+// stats64: [0] sum T_k, [1] max over global-tier columns of max(min(T_k, n), n2_k) * n2_k
+// (a bound on that column's tile), [2] the same over all columns.
 template <bool DYNAMIC>
-__global__ __launch_bounds__(256) void spai_plan_kernel(const int *colptr, const int *rowidx, int col_begin,
+__global__ __launch_bounds__(256) void spai_plan_kernel(const int *colptr, const int *rowidx, int n, int col_begin,
                                                         int col_end, int *counters,
                                                         unsigned long long *stats64, int *lists,
                                                         int list_stride) {
     __shared__ int s_cnt[kTiers], s_base[kTiers], s_maxT, s_maxN, s_empty;
-    __shared__ unsigned long long s_sumT;
+    __shared__ unsigned long long s_sumT, s_tile4, s_tileAll;
     const int tid = threadIdx.x;
     if (tid < kTiers) s_cnt[tid] = 0;
     if (tid == 0) {
@@ -680,6 +932,8 @@ __global__ __launch_bounds__(256) void spai_plan_kernel(const int *colptr, const
         s_maxN = 0;
         s_empty = 0;
         s_sumT = 0ull;
+        s_tile4 = 0ull;
+        s_tileAll = 0ull;
     }
     __syncthreads();
     const int k = col_begin + blockIdx.x * blockDim.x + tid;
@@ -701,6 +955,11 @@ __global__ __launch_bounds__(256) void spai_plan_kernel(const int *colptr, const
         atomicMax(&s_maxN, n2);
         atomicAdd(&s_sumT, (unsigned long long)T);
         if (n2 == 0) atomicAdd(&s_empty, 1);
+        // rows of the tile: n1 <= min(T, n); a rank-deficient column (n1 < n2)
+        // still touches the diagonal R_jj for every j < n2
+        const unsigned long long tb = (unsigned long long)max(min(T, n), n2) * (unsigned long long)n2;
+        atomicMax(&s_tileAll, tb);
+        if (tier == kTiers - 1) atomicMax(&s_tile4, tb);
     }
     __syncthreads();
     if (tid < kTiers) s_base[tid] = s_cnt[tid] ? atomicAdd(&counters[tid], s_cnt[tid]) : 0;
@@ -709,6 +968,8 @@ __global__ __launch_bounds__(256) void spai_plan_kernel(const int *colptr, const
         atomicMax(&counters[2 * kTiers + 1], s_maxN);
         if (s_empty) atomicAdd(&counters[2 * kTiers + 2], s_empty);
         atomicAdd(&stats64[0], s_sumT);
+        atomicMax(&stats64[1], s_tile4);
+        atomicMax(&stats64[2], s_tileAll);
     }
     __syncthreads();
     if (DYNAMIC && tier >= 0) lists[(size_t)tier * list_stride + s_base[tier] + local] = k;
@@ -794,23 +1055,23 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
     const bool dynamic = p.schedule == 1;
     const int nc = 2 * kTiers + 3;
     counters_.reserve(nc);
-    stats64_.reserve(2);
+    stats64_.reserve(3);
     const int list_stride = std::max(1, ce - cb);
     if (dynamic) lists_.reserve((size_t)kTiers * list_stride);
-    hipLaunchKernelGGL(spai_zero_counters, dim3(1), dim3(64), 0, st, counters_.get(), nc, stats64_.get(), 2);
+    hipLaunchKernelGGL(spai_zero_counters, dim3(1), dim3(64), 0, st, counters_.get(), nc, stats64_.get(), 3);
     SPAI_CHECK_LAUNCH();
     if (ce > cb) {
         int grid = ceil_div(ce - cb, 256);
         if (dynamic)
-            hipLaunchKernelGGL(spai_plan_kernel<true>, dim3(grid), dim3(256), 0, st, p.colptr, p.rowidx, cb, ce,
+            hipLaunchKernelGGL(spai_plan_kernel<true>, dim3(grid), dim3(256), 0, st, p.colptr, p.rowidx, p.n, cb, ce,
                                counters_.get(), stats64_.get(), lists_.get(), list_stride);
         else
-            hipLaunchKernelGGL(spai_plan_kernel<false>, dim3(grid), dim3(256), 0, st, p.colptr, p.rowidx, cb, ce,
-                               counters_.get(), stats64_.get(), (int *)nullptr, list_stride);
+            hipLaunchKernelGGL(spai_plan_kernel<false>, dim3(grid), dim3(256), 0, st, p.colptr, p.rowidx, p.n, cb,
+                               ce, counters_.get(), stats64_.get(), (int *)nullptr, list_stride);
         SPAI_CHECK_LAUNCH();
     }
     int hc[2 * kTiers + 3] = {0};
-    unsigned long long h64[2] = {0, 0};
+    unsigned long long h64[3] = {0, 0, 0};
     auto fetch = [&]() {
         SPAI_HIP(hipMemcpyAsync(hc, counters_.get(), sizeof(int) * nc, hipMemcpyDeviceToHost, st));
         SPAI_HIP(hipMemcpyAsync(h64, stats64_.get(), sizeof(h64), hipMemcpyDeviceToHost, st));
@@ -832,14 +1093,9 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
         // (src/spai_single_dcu.cpp:1245-1252); static mode does the same for
         // the largest column's tier
         fetch();
-        int maxT = hc[2 * kTiers], maxN = hc[2 * kTiers + 1];
-        static_tier = classify_host(maxT, maxN);
-        require(static_tier < kTiers - 1,
-                "spai build: a column exceeds the largest supported tile (T=" + std::to_string(maxT) +
-                    ", n2=" + std::to_string(maxN) + ")",
-                kErrUnsupported);
+        static_tier = classify_host(hc[2 * kTiers], hc[2 * kTiers + 1]);
     }
-    if (ce > cb) {
+    if (ce > cb && static_tier < kTiers - 1) {
         if (p.precision == 1)
             launch_tiers<float, true>(a, dynamic, static_tier, counters_.get(), lists_.get(), list_stride, cus_, st);
         else
@@ -847,9 +1103,52 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
                                         st);
     }
     fetch();
-    require(hc[kTiers - 1] == 0 || !dynamic,
-            "spai build: " + std::to_string(hc[kTiers - 1]) + " column(s) exceed the largest supported tile",
-            kErrUnsupported);
+    const int global_cols = dynamic ? hc[kTiers - 1] : (static_tier == kTiers - 1 ? ce - cb : 0);
+    if (global_cols > 0) {
+        // very wide columns: one workgroup per column with a global scratch
+        // slot sized by the planner's tile bound
+        GlobalScratch g{};
+        const long long maxT = std::max(1, hc[2 * kTiers]);
+        g.n2_cap = std::max(1, hc[2 * kTiers + 1]);
+        g.n1_cap = (int)std::max<long long>(std::min<long long>(maxT, std::max(p.n, 1)), g.n2_cap);
+        g.tile_cap = (long long)std::max<unsigned long long>(1ull, dynamic ? h64[1] : h64[2]);
+        long long kp = 1;
+        while (kp < maxT) kp <<= 1;
+        g.key_cap = kp;
+        g.vec_cap = (long long)g.n1_cap + 3LL * g.n2_cap;
+        g.int_cap = g.key_cap + 1 + g.n2_cap + 1;
+        const long long slot_bytes = 8 * (g.tile_cap + g.vec_cap + g.key_cap) + 4 * g.int_cap;
+        const long long budget = 8LL << 30;  // scratch for concurrent wide columns
+        int slots = (int)std::min<long long>(std::max<long long>(1, budget / slot_bytes),
+                                             std::min<long long>(global_cols, 2LL * cus_));
+        glob_tile_.reserve((size_t)(slots * g.tile_cap));
+        glob_vec_.reserve((size_t)(slots * g.vec_cap));
+        glob_keys_.reserve((size_t)(slots * g.key_cap));
+        glob_ints_.reserve((size_t)(slots * g.int_cap));
+        g.tile = glob_tile_.get();
+        g.vec = glob_vec_.get();
+        g.keys = glob_keys_.get();
+        g.ints = glob_ints_.get();
+        ColumnArgs ga = a;
+        if (dynamic) {
+            ga.list = lists_.get() + (size_t)(kTiers - 1) * list_stride;
+            ga.count = counters_.get() + (kTiers - 1);
+            ga.head = counters_.get() + kTiers + (kTiers - 1);
+        }
+        auto launch = [&](auto mt_tag) {
+            using MT = decltype(mt_tag);
+            if (dynamic)
+                hipLaunchKernelGGL((spai_global_tier_kernel<MT, true>), dim3(slots), dim3(kGlobThreads), 0, st, ga,
+                                   g);
+            else
+                hipLaunchKernelGGL((spai_global_tier_kernel<MT, false>), dim3(slots), dim3(kGlobThreads), 0, st, ga,
+                                   g);
+        };
+        if (p.precision == 1) launch(float{});
+        else launch(double{});
+        SPAI_CHECK_LAUNCH();
+        SPAI_HIP(hipStreamSynchronize(st));
+    }
     if (stats) {
         for (int t = 0; t < kTiers; ++t) stats->tier_columns[t] = dynamic ? hc[t] : 0;
         if (!dynamic && ce > cb) stats->tier_columns[static_tier] = ce - cb - hc[2 * kTiers + 2];

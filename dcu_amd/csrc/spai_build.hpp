@@ -8,7 +8,8 @@
 namespace spai {
 
 // Size tiers: 0..2 register tiers (16, 32, 64 lanes per column), 3 the LDS
-// wavefront tier, 4 = larger than any tier (rejected).
+// wavefront tier, 4 the global-memory tier (one workgroup per very wide
+// column, FP64, tile in a scratch slot).
 constexpr int kTiers = 5;
 
 struct SpaiBuildParams {
@@ -48,6 +49,9 @@ private:
     DeviceArray<int> counters_;
     DeviceArray<unsigned long long> stats64_;
     DeviceArray<int> lists_;
+    DeviceArray<double> glob_tile_, glob_vec_;
+    DeviceArray<uint64_t> glob_keys_;
+    DeviceArray<int> glob_ints_;
     int cus_ = 0;
 };
 
