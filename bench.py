@@ -52,18 +52,26 @@ def cpu_model():
     return platform.processor()
 
 
-def tile_bytes(kind, n, nnz, m_bytes):
-    """Algorithmic HBM bytes of one launch of a fused solver kernel.
-    A is FP64 CSR (8 B value + 4 B column per nonzero, 4 B row pointer per
-    row); vectors FP64; every vector read or written once."""
+KERNELS = ("mapply_p", "spmv_alpha", "mapply_s", "spmv_omega", "update")
+
+
+def kernel_bytes(kind, n, nnz, m_bytes, row_ptr):
+    """Algorithmic HBM bytes of one launch of an iteration kernel with every
+    system active: a 4 B column index and the value (A: 8 B, M: m_bytes) per
+    nonzero, each FP64 vector read or written once, plus 4 B per row for the
+    row pointers when the layout is row tiles (row_ptr).  The padding entries
+    of the SELL-64 layout are not counted (they are overhead)."""
+    rp = 4 * n if row_ptr else 0
     if kind == "spmv_alpha":      # a = A y; reads A, y (gather, once), r*; writes a
-        return 12 * nnz + 4 * n + 8 * n * 3
+        return 12 * nnz + rp + 8 * n * 3
     if kind == "spmv_omega":      # t = A z; reads A, z, s; writes t
-        return 12 * nnz + 4 * n + 8 * n * 3
+        return 12 * nnz + rp + 8 * n * 3
     if kind == "mapply_p":        # y = M p'; reads M, r, p, a; writes p', y
-        return (m_bytes + 4) * nnz + 4 * n + 8 * n * 5
+        return (m_bytes + 4) * nnz + rp + 8 * n * 5
     if kind == "mapply_s":        # z = M s; reads M, r, a; writes s, z
-        return (m_bytes + 4) * nnz + 4 * n + 8 * n * 4
+        return (m_bytes + 4) * nnz + rp + 8 * n * 4
+    if kind == "update":          # x, r; reads x, y, z, s, t, r*; writes x, r
+        return 8 * n * 8
     raise ValueError(kind)
 
 
@@ -124,16 +132,18 @@ def main():
                                graph_iters=args.graph_iters, true_residual_restart=True)
         return rep_b, rep_s
 
+    layout, layout_entries = S.layout()
+    row_ptr = layout != "sell64"
     if args.profile_kernels:
         step()
-        for which in (1, 2, 3, 4):
-            S.time_spmv(which, args.profile_reps)
+        ms5 = S.time_iteration(args.profile_reps)
         copy_bytes = 256 * 1024 * 1024
         ctx.calibration_copy(copy_bytes, args.profile_reps)
         m_bytes = 4 if precision == D.PRECISION_MIXED else 8
-        print(json.dumps({"profile_kernels": True, "n": n_tot, "nnz": nnz_tot,
-                          "algorithmic_bytes": {k: tile_bytes(k, n_tot, nnz_tot, m_bytes) for k in
-                                                ("mapply_p", "spmv_alpha", "mapply_s", "spmv_omega")},
+        print(json.dumps({"profile_kernels": True, "n": n_tot, "nnz": nnz_tot, "layout": layout,
+                          "iteration_kernel_ms": dict(zip(KERNELS, ms5)),
+                          "algorithmic_bytes": {k: kernel_bytes(k, n_tot, nnz_tot, m_bytes, row_ptr)
+                                                for k in KERNELS},
                           "copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes}))
         S.close()
         ctx.close()
@@ -161,19 +171,20 @@ def main():
     log("per-system iterations (n): " + ", ".join(f"{o['iterations']}({A.n})" for o, A in zip(oc, mats)))
 
     # ------------------------------------------------------- roofline ----
+    # per-kernel times of the real iteration sequence (K1..K5 in order, every
+    # system active, HIP events between the kernels on the library's stream)
     m_bytes = 4 if precision == D.PRECISION_MIXED else 8
-    kernel_ms = {}
-    for which, name in ((2, "spmv_alpha"), (4, "spmv_omega"), (1, "mapply_p"), (3, "mapply_s")):
-        kernel_ms[name] = S.time_spmv(which, 50)
+    kernel_ms = dict(zip(KERNELS, S.time_iteration(20)))
     dominant = max(kernel_ms, key=kernel_ms.get)
-    kb = tile_bytes(dominant, n_tot, nnz_tot, m_bytes)
+    kb = kernel_bytes(dominant, n_tot, nnz_tot, m_bytes, row_ptr)
     achieved = kb / (kernel_ms[dominant] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_launch": int(kb), "launch_ms": round(kernel_ms[dominant], 5),
+                "layout": layout, "stored_entries": layout_entries,
                 "all_kernels_ms": {k: round(v, 5) for k, v in kernel_ms.items()},
-                "all_kernels_frac": {k: round(tile_bytes(k, n_tot, nnz_tot, m_bytes) / (v * 1e-3) / 1e9 /
-                                              HBM_PEAK_GBS, 4) for k, v in kernel_ms.items()}}
+                "all_kernels_frac": {k: round(kernel_bytes(k, n_tot, nnz_tot, m_bytes, row_ptr) / (v * 1e-3) /
+                                              1e9 / HBM_PEAK_GBS, 4) for k, v in kernel_ms.items()}}
     # HBM traffic per launch from the committed counter summary of this
     # workload (two rocprofv3 --pmc passes, calibrated against a copy kernel)
     traffic_file = os.path.join(ROOT, "profiles", "batched64_traffic.md")
@@ -187,11 +198,11 @@ def main():
     copy_ms = ctx.calibration_copy(256 * 1024 * 1024, 10)
     roofline["copy_kernel_GBs"] = round(2 * 256 * 1024 * 1024 / (copy_ms * 1e-3) / 1e9, 1)
     # whole solve: every system's iterations x its per-iteration algorithmic
-    # bytes (K1..K5 as above plus the 64 n of the update kernel)
+    # bytes (K1..K5 as above)
     per_iter = 0.0
     for A, o in zip(mats, oc):
         it = o["iterations"] if o["status"] != D.SOLVE_MAX_ITER else o["iterations"] - 1
-        per_iter += it * ((2 * (m_bytes + 4) + 24) * A.nnz + 200 * A.n)
+        per_iter += it * sum(kernel_bytes(k, A.n, A.nnz, m_bytes, row_ptr) for k in KERNELS)
     solve_gbs = per_iter / (float(np.median(solve_ms)) * 1e-3) / 1e9
     roofline["solve_achieved"] = round(solve_gbs, 1)
     roofline["solve_frac"] = round(solve_gbs / HBM_PEAK_GBS, 4)
@@ -273,7 +284,7 @@ def main():
         "data": "synthetic (seeded generators)",
         "config": {"workload": desc, "systems": nsys, "unknowns": n_tot, "nnz": nnz_tot,
                    "precision": args.precision, "schedule": "balanced_nnz", "graph_iters": args.graph_iters,
-                   "rel_tol": 1e-8, "seed": args.seed, "device": dev_name, "cus": cus},
+                   "rel_tol": 1e-8, "seed": args.seed, "device": dev_name, "cus": cus, "layout": layout},
         "spai_ms": round(float(np.median(spai_ms)), 4),
         "solve_ms": round(float(np.median(solve_ms)), 4),
         "iterations_max": int(iters[-1]),

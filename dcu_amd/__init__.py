@@ -22,7 +22,8 @@ from typing import List, Optional, Sequence, Tuple
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_lib", "libspai_mi355x.so")
+# DCU_AMD_LIBRARY may name another build of the same library
+LIB_PATH = os.environ.get("DCU_AMD_LIBRARY") or os.path.join(_HERE, "_lib", "libspai_mi355x.so")
 CLI_PATH = os.path.join(_HERE, "_bin", "spai_mi355x")
 
 PRECISION_FP64 = 0
@@ -45,6 +46,7 @@ C_SYMBOLS = [
     "spai_system_set_values", "spai_system_build", "spai_system_get_m",
     "spai_system_get_row_set_sizes", "spai_solve_options_default", "spai_system_solve",
     "spai_system_outcomes", "spai_system_residual_history", "spai_system_adjoint", "spai_system_time_spmv",
+    "spai_system_time_iteration", "spai_system_layout",
     "spai_calibration_copy", "spai_csc_to_csr_device",
     "spai_problem_generate", "spai_problem_generate_power_law", "spai_problem_generate_batched",
     "spai_problem_generate_grf", "spai_problem_convection_diffusion", "spai_problem_read_mtx", "spai_problem_free",
@@ -164,6 +166,8 @@ def load_library() -> ctypes.CDLL:
         "spai_system_adjoint": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_int, ctypes.POINTER(SolveOptions),
                                                ctypes.POINTER(SolveReport)]),
         "spai_system_time_spmv": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
+        "spai_system_time_iteration": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
+        "spai_system_layout": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)]),
         "spai_calibration_copy": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_int,
                                                  ctypes.POINTER(ctypes.c_float)]),
         "spai_csc_to_csr_device": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp,
@@ -505,6 +509,18 @@ class System:
         ms = ctypes.c_float()
         _check(self._lib.spai_system_time_spmv(self._h, int(which), int(reps), ctypes.byref(ms)))
         return ms.value
+
+    def time_iteration(self, reps: int = 20) -> List[float]:
+        """Average ms of K1..K5 over `reps` consecutive all-active iterations."""
+        ms = (ctypes.c_float * 5)()
+        _check(self._lib.spai_system_time_iteration(self._h, int(reps), ms))
+        return [float(v) for v in ms]
+
+    def layout(self) -> Tuple[str, int]:
+        """("sell64" | "row_tiles", stored entries per operator)."""
+        kind, entries = ctypes.c_int(), ctypes.c_longlong()
+        _check(self._lib.spai_system_layout(self._h, ctypes.byref(kind), ctypes.byref(entries)))
+        return ("sell64" if kind.value == 1 else "row_tiles"), int(entries.value)
 
 
 def _report(rep: SolveReport) -> dict:

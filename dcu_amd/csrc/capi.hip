@@ -51,11 +51,25 @@ __global__ void k_fill_f32(float *p, float v, long long n) {
 }
 
 // Plain copy of `n` doubles, 16 bytes per lane per step (calibration kernel).
-__global__ __launch_bounds__(256) void k_copy_calibration(const double2 *__restrict__ src,
-                                                            double2 *__restrict__ dst, long long n2) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (; i < n2; i += stride) dst[i] = src[i];
+// Streaming copy: 4 x 16 B per thread per step, nontemporal loads and stores
+// (the practical HBM ceiling the solver kernels are compared with).
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(256) void k_copy_calibration(const f64x2 *__restrict__ src, f64x2 *__restrict__ dst,
+                                                            long long n2) {
+    const long long stride = (long long)gridDim.x * blockDim.x * 4;
+    for (long long base = (long long)blockIdx.x * blockDim.x * 4 + threadIdx.x; base < n2; base += stride) {
+        f64x2 v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const long long i = base + (long long)k * blockDim.x;
+            if (i < n2) v[k] = __builtin_nontemporal_load(&src[i]);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const long long i = base + (long long)k * blockDim.x;
+            if (i < n2) __builtin_nontemporal_store(v[k], &dst[i]);
+        }
+    }
 }
 
 void fill(double *p, long long n, hipStream_t st) {
@@ -125,8 +139,11 @@ struct spai_system {
     std::vector<long long> h_nnz_off;
     DeviceArray<int> colptr, rowidx, rowptr, colidx, perm, n1;
     DeviceArray<double> cscval, csrval;
-    DeviceArray<double> m64_csc, m64_csr;
-    DeviceArray<float> m32_csc, m32_csr;
+    // the solver's layout (tiles: CSR order; SELL-64: padded slice order)
+    DeviceArray<int> perm_lay;   // CSC position -> position in the layout's value order
+    DeviceArray<double> sellval;  // A's values in SELL order (SELL only)
+    DeviceArray<double> m64_csc, m64_lay;
+    DeviceArray<float> m32_csc, m32_lay;
     int m_precision = -1;
     TileMap tiles, tiles_t;
     bool tiles_t_ready = false;
@@ -149,7 +166,20 @@ void finish_setup(spai_system *S, hipStream_t st) {
     S->perm.resize((size_t)std::max(S->nnz, 1));
     csc_to_csr_device(S->n, S->nnz, S->colptr.get(), S->rowidx.get(), S->cscval.get(), S->rowptr.get(),
                       S->colidx.get(), S->csrval.get(), S->perm.get(), st);
-    build_tiles(S->tiles, S->rowptr.get(), S->h_row_off, S->h_nnz_off, st);
+    build_layout(S->tiles, S->rowptr.get(), S->colidx.get(), S->h_row_off, S->h_nnz_off, true, st);
+    S->perm_lay.resize((size_t)std::max(S->nnz, 1));
+    layout_perm(S->tiles, S->nnz, S->perm.get(), S->perm_lay.get(), st);
+    if (S->tiles.kind == kLayoutSell) {
+        S->sellval.resize((size_t)S->tiles.sell_len);
+        layout_values<double>(S->tiles, S->nnz, S->csrval.get(), S->sellval.get(), st);
+    }
+    SPAI_HIP(hipStreamSynchronize(st));
+}
+
+// A in the solver's layout
+OperatorView a_view(spai_system *S) {
+    if (S->tiles.kind == kLayoutSell) return OperatorView{nullptr, S->tiles.sell_col.get(), S->sellval.get(), 0};
+    return OperatorView{S->rowptr.get(), S->colidx.get(), S->csrval.get(), 0};
 }
 
 spai_system *create_from_host(spai_context *ctx, const std::vector<const CSC_Matrix *> &parts) {
@@ -196,7 +226,7 @@ spai_system *create_from_host(spai_context *ctx, const std::vector<const CSC_Mat
 
 void ensure_transposed_tiles(spai_system *S) {
     if (S->tiles_t_ready) return;
-    build_tiles(S->tiles_t, S->colptr.get(), S->h_row_off, S->h_nnz_off, S->ctx->stream);
+    build_layout(S->tiles_t, S->colptr.get(), S->rowidx.get(), S->h_row_off, S->h_nnz_off, false, S->ctx->stream);
     S->tiles_t_ready = true;
 }
 
@@ -224,15 +254,17 @@ void fill_report(spai_solve_report *r, const SolveResult &res) {
     r->all_converged = res.all_converged;
 }
 
+// M in the solver's layout (transposed: CSC arrays = CSR of M^T, row tiles)
 OperatorView m_view(spai_system *S, bool transposed) {
     OperatorView M;
-    M.ptr = transposed ? S->colptr.get() : S->rowptr.get();
-    M.idx = transposed ? S->rowidx.get() : S->colidx.get();
+    const bool sell = !transposed && S->tiles.kind == kLayoutSell;
+    M.ptr = transposed ? S->colptr.get() : (sell ? nullptr : S->rowptr.get());
+    M.idx = transposed ? S->rowidx.get() : (sell ? S->tiles.sell_col.get() : S->colidx.get());
     if (S->m_precision == SPAI_PRECISION_MIXED) {
-        M.val = transposed ? (const void *)S->m32_csc.get() : (const void *)S->m32_csr.get();
+        M.val = transposed ? (const void *)S->m32_csc.get() : (const void *)S->m32_lay.get();
         M.fp32 = 1;
     } else {
-        M.val = transposed ? (const void *)S->m64_csc.get() : (const void *)S->m64_csr.get();
+        M.val = transposed ? (const void *)S->m64_csc.get() : (const void *)S->m64_lay.get();
         M.fp32 = 0;
     }
     return M;
@@ -410,6 +442,8 @@ int spai_system_set_values(spai_system *sys, const double *values, int values_on
         for (int p = 0; p < sys->nnz; ++p) hc[(size_t)hp[(size_t)p]] = hv[(size_t)p];
         SPAI_HIP(hipMemcpyAsync(sys->csrval.get(), hc.data(), sizeof(double) * (size_t)sys->nnz,
                                 hipMemcpyHostToDevice, st));
+        if (sys->tiles.kind == kLayoutSell)
+            layout_values<double>(sys->tiles, sys->nnz, sys->csrval.get(), sys->sellval.get(), st);
         SPAI_HIP(hipStreamSynchronize(st));
     });
 }
@@ -425,12 +459,16 @@ int spai_system_build(spai_system *sys, int precision, int schedule, int col_beg
         hipStream_t st = sys->ctx->stream;
         SPAI_HIP(hipSetDevice(sys->ctx->device));
         size_t nz = (size_t)std::max(sys->nnz, 1);
+        const long long lay = std::max<long long>(sys->tiles.value_len(sys->nnz), 1);
+        const bool sell = sys->tiles.kind == kLayoutSell;
         if (precision == SPAI_PRECISION_MIXED) {
             sys->m32_csc.resize(nz);
-            sys->m32_csr.resize(nz);
+            sys->m32_lay.resize((size_t)lay);
+            if (sell) fill(sys->m32_lay.get(), lay, st);  // SELL padding entries stay 0
         } else {
             sys->m64_csc.resize(nz);
-            sys->m64_csr.resize(nz);
+            sys->m64_lay.resize((size_t)lay);
+            if (sell) fill(sys->m64_lay.get(), lay, st);
         }
         sys->n1.resize((size_t)sys->n);
         int cb = std::max(0, col_begin), ce = col_end < 0 ? sys->n : std::min(col_end, sys->n);
@@ -441,10 +479,10 @@ int spai_system_build(spai_system *sys, int precision, int schedule, int col_beg
         if (partial) {
             if (precision == SPAI_PRECISION_MIXED) {
                 fill(sys->m32_csc.get(), sys->nnz, st);
-                fill(sys->m32_csr.get(), sys->nnz, st);
+                if (!sell) fill(sys->m32_lay.get(), sys->nnz, st);
             } else {
                 fill(sys->m64_csc.get(), sys->nnz, st);
-                fill(sys->m64_csr.get(), sys->nnz, st);
+                if (!sell) fill(sys->m64_lay.get(), sys->nnz, st);
             }
         }
         SpaiBuildParams p;
@@ -452,13 +490,13 @@ int spai_system_build(spai_system *sys, int precision, int schedule, int col_beg
         p.colptr = sys->colptr.get();
         p.rowidx = sys->rowidx.get();
         p.val = sys->cscval.get();
-        p.perm = sys->perm.get();
+        p.perm = sys->perm_lay.get();
         p.precision = precision;
         p.schedule = schedule;
         p.col_begin = cb;
         p.col_end = ce;
         p.m_csc = precision == SPAI_PRECISION_MIXED ? (void *)sys->m32_csc.get() : (void *)sys->m64_csc.get();
-        p.m_csr = precision == SPAI_PRECISION_MIXED ? (void *)sys->m32_csr.get() : (void *)sys->m64_csr.get();
+        p.m_csr = precision == SPAI_PRECISION_MIXED ? (void *)sys->m32_lay.get() : (void *)sys->m64_lay.get();
         p.n1_out = sys->n1.get();
         SpaiBuildStats stats;
         sys->ctx->builder.build(p, st, &stats);
@@ -540,7 +578,7 @@ int spai_system_solve(spai_system *sys, const double *b, double *x, int on_devic
             db = sys->stage_b.get();
             dx = sys->stage_x.get();
         }
-        OperatorView A{sys->rowptr.get(), sys->colidx.get(), sys->csrval.get(), 0};
+        OperatorView A = a_view(sys);
         OperatorView M = opt->precondition ? m_view(sys, false) : OperatorView{};
         SolveResult res;
         sys->ctx->solver.solve(A, M, sys->tiles, sys->n, sys->nsys, db, dx, to_options(opt), st, &res);
@@ -624,10 +662,27 @@ int spai_system_time_spmv(spai_system *sys, int which, int reps, float *ms) {
     if (which != 0 && sys->m_precision < 0) return fail(SPAI_ERR_NOT_BUILT, "M has not been built");
     return guarded([&] {
         SPAI_HIP(hipSetDevice(sys->ctx->device));
-        OperatorView A{sys->rowptr.get(), sys->colidx.get(), sys->csrval.get(), 0};
+        OperatorView A = a_view(sys);
         OperatorView M = which != 0 ? m_view(sys, false) : OperatorView{};
         *ms = sys->ctx->solver.time_kernel(which, A, M, sys->tiles, sys->n, sys->nsys, reps, sys->ctx->stream);
     });
+}
+
+int spai_system_time_iteration(spai_system *sys, int reps, float *ms) {
+    if (!sys || !ms || reps < 1) return fail(SPAI_ERR_INVALID, "bad arguments");
+    if (sys->m_precision < 0) return fail(SPAI_ERR_NOT_BUILT, "M has not been built");
+    return guarded([&] {
+        SPAI_HIP(hipSetDevice(sys->ctx->device));
+        sys->ctx->solver.time_iteration(a_view(sys), m_view(sys, false), sys->tiles, sys->n, sys->nsys, reps,
+                                        sys->ctx->stream, ms);
+    });
+}
+
+int spai_system_layout(const spai_system *sys, int *kind, long long *entries) {
+    if (!sys) return fail(SPAI_ERR_INVALID, "sys is NULL");
+    if (kind) *kind = sys->tiles.kind == kLayoutSell ? SPAI_LAYOUT_SELL64 : SPAI_LAYOUT_ROW_TILES;
+    if (entries) *entries = sys->tiles.value_len(sys->nnz);
+    return SPAI_OK;
 }
 
 int spai_calibration_copy(spai_context *ctx, long long bytes, int reps, float *ms) {
@@ -639,12 +694,12 @@ int spai_calibration_copy(spai_context *ctx, long long bytes, int reps, float *m
         DeviceArray<double> a((size_t)n2 * 2), b((size_t)n2 * 2);
         fill(a.get(), n2 * 2, st);
         int grid = ctx->cus * 16;
-        hipLaunchKernelGGL(k_copy_calibration, dim3(grid), dim3(256), 0, st, (const double2 *)a.get(),
-                           (double2 *)b.get(), n2);
+        hipLaunchKernelGGL(k_copy_calibration, dim3(grid), dim3(256), 0, st, (const f64x2 *)a.get(),
+                           (f64x2 *)b.get(), n2);
         Timer tm(st);
         for (int r = 0; r < reps; ++r)
-            hipLaunchKernelGGL(k_copy_calibration, dim3(grid), dim3(256), 0, st, (const double2 *)a.get(),
-                               (double2 *)b.get(), n2);
+            hipLaunchKernelGGL(k_copy_calibration, dim3(grid), dim3(256), 0, st, (const f64x2 *)a.get(),
+                               (f64x2 *)b.get(), n2);
         SPAI_CHECK_LAUNCH();
         *ms = tm.stop() / reps;
     });
@@ -845,7 +900,7 @@ int cublas2_pbicgstabv2(CSR_Matrix *devA, CSR_Matrix *devM, double *dev_b, doubl
         hipStream_t st = ctx->stream;
         TileMap ta;
         std::vector<int> roff = {0, n};
-        build_tiles(ta, devA->mPtr, roff, {0, (long long)devA->nonzeroes}, st);
+        build_layout(ta, devA->mPtr, devA->mIndex, roff, {0, (long long)devA->nonzeroes}, false, st);
         OperatorView A{devA->mPtr, devA->mIndex, devA->mData, 0};
         OperatorView M{};
         // M may have its own pattern: the kernels read each operator's own

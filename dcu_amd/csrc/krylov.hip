@@ -16,12 +16,16 @@
 //   K5  x += alpha y + omega z, r = s - omega t, (r*, r), (r, r) reduced,
 //       beta, convergence and the iteration count by the last tile
 //
-// Every kernel walks row tiles (one workgroup per tile); a tile's nonzeros
-// are streamed with fully coalesced loads into LDS as products, then each
-// thread sums its rows in storage order (deterministic).  Reductions are
-// per-tile partials summed in a fixed order by the last tile of each system,
-// so results are reproducible run to run.  All systems of a batch share the
-// launches; tiles of a converged system return at once.
+// Every kernel runs one workgroup per unit of the row layout (krylov.hpp):
+// four SELL-64 slices, lane = row, the slice's column indices and values
+// streamed with nontemporal coalesced loads and each row summed in storage
+// order in registers; or a row tile whose nonzeros are staged in LDS as
+// products and summed per row in storage order.  Reductions are per-unit
+// partials summed in a fixed order by the last unit of each system, so
+// results are reproducible run to run.  All systems of a batch share the
+// launches; launches inside the iteration loop walk a list of the units of
+// systems still iterating, so converged systems cost nothing once the list
+// has been rebuilt.
 #include "krylov.hpp"
 
 #include <algorithm>
@@ -32,27 +36,27 @@ namespace spai {
 
 // Reductions are counted in two levels so that no counter sees more than
 // kBucket atomic increments per launch (same-address atomics serialise at
-// the memory side): groups -> buckets of up to kBucket consecutive groups of
+// the memory side): units -> buckets of up to kBucket consecutive units of
 // one system -> the system.
 struct Ctx {
     SysState *st;
-    const int *group_bucket;  // bucket of each group
-    const int *bucket_group;  // nbuckets + 1: first group of each bucket
-    const int *sys_buckets;   // nsys + 1: first bucket of each system
-    int *counter;             // [nsys] systems, [nsys] = n_active, [nsys + 1 + b] buckets
-    double *partials;         // ngroups * 2 group partials
-    double *bpartials;        // nbuckets * 2 bucket partials
+    const int *unit_bucket;  // bucket of each unit
+    const int *bucket_unit;  // nbuckets + 1: first unit of each bucket
+    const int *sys_buckets;  // nsys + 1: first bucket of each system
+    int *counter;            // [0, nsys) systems, [nsys] active systems, [nsys + 1] active units,
+                             // [nsys + 2 + b] buckets
+    double *partials;        // nunits * 2 unit partials
+    double *bpartials;       // nbuckets * 2 bucket partials
     int nsys;
-    double *hist;             // optional ||r_k|| history, hist_stride entries per system
+    double *hist;            // optional ||r_k|| history, hist_stride entries per system
     int hist_stride;
 };
-
 
 namespace {
 
 constexpr int kThreads = 256;
 constexpr int kLdsProducts = 2048;  // products staged per tile (16 KB)
-constexpr int kBucket = 32;         // groups per first-level reduction counter
+constexpr int kBucket = 32;         // units per first-level reduction counter
 
 // ------------------------------------------------------------ transpose ----
 __global__ void k_fill_int(int *p, int v, long long n) {
@@ -209,109 +213,131 @@ __device__ __forceinline__ void block_reduce(double (&v)[NRED], double *sh) {
 // relaxed counter increment, and the last workgroup reads them back with
 // agent-scope atomic loads.
 template <int NRED>
-__device__ __forceinline__ bool publish_and_count(double *dst, const double (&v)[NRED], int *counter, int expect,
-                                                  int *flag) {
-    if (threadIdx.x == 0) {
+__device__ __forceinline__ void wave_sum(double (&v)[NRED]) {
+#pragma unroll
+    for (int q = 0; q < NRED; ++q)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+}
+
+// Lane 0 publishes the partials, waits until they are acknowledged, then
+// counts; returns (to the whole wavefront) whether this was the last of
+// `expect` arrivals, in which case the counter is reset for the next launch.
+template <int NRED>
+__device__ __forceinline__ bool publish_and_count(double *dst, const double (&v)[NRED], int *counter, int expect) {
+    int last = 0;
+    if ((threadIdx.x & 63) == 0) {
 #pragma unroll
         for (int q = 0; q < NRED; ++q) __hip_atomic_store(&dst[q], v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        int prev = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        *flag = (prev == expect - 1);
-        if (*flag) __hip_atomic_store(counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int prev = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        last = prev == expect - 1;
+        if (last) __hip_atomic_store(counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __syncthreads();
-    return *flag != 0;
+    return __shfl(last, 0, 64) != 0;
 }
 
+// Sum of src[t * NRED + q] over t in [first, last), by one wavefront, in a
+// fixed order (lane-strided partial sums, then a butterfly).
 template <int NRED>
-__device__ __forceinline__ void sum_range(const double *src, int first, int last, double (&acc)[NRED], double *sh) {
+__device__ __forceinline__ void wave_sum_range(const double *src, int first, int last, double (&acc)[NRED]) {
 #pragma unroll
     for (int q = 0; q < NRED; ++q) acc[q] = 0.0;
-    for (int t = first + (int)threadIdx.x; t < last; t += blockDim.x) {
+    for (int t = first + (int)(threadIdx.x & 63); t < last; t += 64) {
 #pragma unroll
         for (int q = 0; q < NRED; ++q) acc[q] += load_coherent(&src[(size_t)t * NRED + q]);
     }
-    block_reduce<NRED>(acc, sh);
+    wave_sum<NRED>(acc);
 }
 
+// Per-unit partials -> bucket -> system, then op.finish.  After the
+// workgroup's partials are combined in LDS, only wavefront 0 stays: the
+// others retire at once instead of waiting on the publish round trips.
 template <int NRED, class Op>
-__device__ __forceinline__ void reduce_system(const Ctx &c, int group, int s, double (&v)[NRED], Op &op, double *sh) {
-    __shared__ int flag;
-    block_reduce<NRED>(v, sh);
-    const int b = c.group_bucket[group];
-    const int g0 = c.bucket_group[b], g1 = c.bucket_group[b + 1];
-    if (!publish_and_count<NRED>(&c.partials[(size_t)group * NRED], v, &c.counter[c.nsys + 1 + b], g1 - g0, &flag))
-        return;
+__device__ __forceinline__ void reduce_system(const Ctx &c, int unit, int s, double (&v)[NRED], Op &op, double *sh) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int W = kThreads / 64;
+    wave_sum<NRED>(v);
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NRED; ++q) sh[q * W + wave] = v[q];
+    }
+    __syncthreads();
+    if (wave != 0) return;
+    double u[NRED];
+#pragma unroll
+    for (int q = 0; q < NRED; ++q) {
+        u[q] = 0.0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) u[q] += sh[q * W + w];
+    }
+    const int b = c.unit_bucket[unit];
+    const int u0 = c.bucket_unit[b], u1 = c.bucket_unit[b + 1];
+    if (!publish_and_count<NRED>(&c.partials[(size_t)unit * NRED], u, &c.counter[c.nsys + 2 + b], u1 - u0)) return;
     double bsum[NRED];
-    sum_range<NRED>(c.partials, g0, g1, bsum, sh);
+    wave_sum_range<NRED>(c.partials, u0, u1, bsum);
     const int b0 = c.sys_buckets[s], b1 = c.sys_buckets[s + 1];
-    if (!publish_and_count<NRED>(&c.bpartials[(size_t)b * NRED], bsum, &c.counter[s], b1 - b0, &flag)) return;
+    if (!publish_and_count<NRED>(&c.bpartials[(size_t)b * NRED], bsum, &c.counter[s], b1 - b0)) return;
     double tot[NRED];
-    sum_range<NRED>(c.bpartials, b0, b1, tot, sh);
-    if (threadIdx.x == 0) op.finish(c, s, tot);
+    wave_sum_range<NRED>(c.bpartials, b0, b1, tot);
+    if (lane == 0) op.finish(c, s, tot);
 }
 
-// ------------------------------------------------------- tile kernels ----
-struct TileView {
-    const int *row;         // tile row boundaries
-    const int *sys;         // system per tile
-    const int *group_tile;  // first tile per group
-    const int *group_sys;   // system per group
+// ------------------------------------------------------- unit kernels ----
+struct UnitView {
+    const int *row;        // nunits + 1 row boundaries
+    const int *sys;        // system of each unit
+    const int *slice;      // nunits + 1 first slice of each unit (SELL)
+    const int *slice_row;  // nslices + 1 (SELL)
+    const int *slice_blk;  // nslices + 1, offsets in blocks of 64 entries (SELL)
+    const int *list;       // active units, nullptr: every unit (grid = nunits)
+    const int *count;      // number of entries of `list`
 };
 
-constexpr int kRowsPerTile = 512;                  // row cap of a tile (host-built)
+// The workgroup's unit, or -1 past the end of the active list.
+__device__ __forceinline__ int unit_of(const UnitView &uv) {
+    if (!uv.list) return (int)blockIdx.x;
+    const int cnt = *uv.count;
+    const int u = uv.list[blockIdx.x];  // the grid never exceeds the list's capacity
+    return (int)blockIdx.x < cnt ? u : -1;
+}
+
+template <class T>
+__device__ __forceinline__ T stream_load(const T *p) {
+    return __builtin_nontemporal_load(p);
+}
+
+constexpr int kRowsPerTile = 512;  // row cap of a tile (host-built)
 constexpr int kRowsPerThread = kRowsPerTile / kThreads;
 
-// For every row i of the workgroup's tile(s): y_i = sum_p val[p] *
+// Row tiles.  For every row i of the unit's tile: y_i = sum_p val[p] *
 // op.gather(idx[p]) (stored as products in LDS, then summed per row in
 // storage order), then op.row(i, y_i, pre_i, acc) where pre_i = op.pre(i) are
 // the row's own vector operands, loaded before the product so that they
 // overlap with it.  One tile needs two dependent global round trips: the
 // index/value/row-pointer/operand loads, then the gathers.  HAS_MATRIX =
-// false skips the product (streaming ops).  Ops with reductions run one
-// workgroup per group of tiles (GROUPED) and publish one partial per group.
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
-// Consecutive tiles gather overlapping vector ranges, so within every period
-// of 8 * kXcdRun workgroups, XCD x is given the contiguous run of kXcdRun
-// units starting at x * kXcdRun; a partial last period keeps the identity.
-// Runs (not one block of G/8 units per XCD) keep every system spread over all
-// XCDs when only a few systems are still iterating.
-constexpr int kXcds = 8;
-constexpr int kXcdRun = 8;
-__device__ __forceinline__ int xcd_unit(int b, int G) {
-    constexpr int P = kXcds * kXcdRun;
-    const int p = b / P;
-    if ((p + 1) * P > G) return b;
-    const int i = b - p * P;
-    return p * P + (i % kXcds) * kXcdRun + i / kXcds;
-}
-
+// false skips the product (streaming ops).
 template <bool HAS_MATRIX, class MT, class Op>
 __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr, const int *__restrict__ idx,
-                                                    const MT *__restrict__ val, TileView tv, Ctx c, Op op) {
+                                                    const MT *__restrict__ val, UnitView uv, Ctx c, Op op) {
     constexpr int NRED = Op::NRED;
-    constexpr bool GROUPED = NRED > 0;
     __shared__ double prod[HAS_MATRIX ? kLdsProducts : 1];
     __shared__ int rp[HAS_MATRIX ? kRowsPerTile + 1 : 1];
     __shared__ double sh[(NRED > 0 ? NRED : 1) * 4];
-    const int unit = xcd_unit(blockIdx.x, gridDim.x);
-    const int s = GROUPED ? tv.group_sys[unit] : tv.sys[unit];
+    const int unit = unit_of(uv);
+    if (unit < 0) return;
+    const int s = uv.sys[unit];
     if (!op.enabled(c.st[s])) return;
     op.load(c.st[s]);
-    const int ft0 = GROUPED ? tv.group_tile[unit] : unit;
-    const int ft1 = GROUPED ? tv.group_tile[unit + 1] : unit + 1;
     const int tid = threadIdx.x;
     double acc[NRED > 0 ? NRED : 1];
 #pragma unroll
     for (int q = 0; q < (NRED > 0 ? NRED : 1); ++q) acc[q] = 0.0;
-
-    for (int ft = ft0; ft < ft1; ++ft) {
-        const int r0 = tv.row[ft], r1 = tv.row[ft + 1];
-        const int nrows = r1 - r0;
-        if (!HAS_MATRIX) {
-            for (int i = r0 + tid; i < r1; i += kThreads) op.row(i, 0.0, op.pre(i), acc);
-            continue;
-        }
+    const int r0 = uv.row[unit], r1 = uv.row[unit + 1];
+    const int nrows = r1 - r0;
+    if (!HAS_MATRIX) {
+        for (int i = r0 + tid; i < r1; i += kThreads) op.row(i, 0.0, op.pre(i), acc);
+    } else {
         const int p0 = ptr[r0];
         const int cnt = ptr[r1] - p0;
         if (cnt <= kLdsProducts && nrows <= kRowsPerTile) {
@@ -328,8 +354,8 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int q = u * kThreads + tid;
-                jj[u] = q < cnt ? idx[p0 + q] : 0;
-                vv[u] = q < cnt ? val[p0 + q] : (MT)0;
+                jj[u] = q < cnt ? stream_load(&idx[p0 + q]) : 0;
+                vv[u] = q < cnt ? stream_load(&val[p0 + q]) : (MT)0;
             }
             double gv[U];
 #pragma unroll
@@ -353,7 +379,6 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
                     op.row(r0 + li, sum, pre[k], acc);
                 }
             }
-            __syncthreads();  // prod and rp are reused by the next tile
         } else {
             // a row longer than the LDS buffer (it has a tile of its own):
             // the whole workgroup reduces it
@@ -366,12 +391,106 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
             }
         }
     }
-    if constexpr (NRED > 0) {
-        double v[NRED];
+    if constexpr (NRED > 0) reduce_system<NRED>(c, unit, s, acc, op, sh);
+}
+
+// SELL-64.  Wavefront w of the workgroup takes kSellSlicesPerWave
+// consecutive slices of its unit, lane = row.  The slices' column indices and
+// values (column-major, width = the slice's longest row) are streamed with
+// nontemporal loads, kSellUnroll columns of every slice in flight per lane,
+// and each row is summed in storage order in a register.  The first columns
+// are requested before the system's state is read, so that the state load
+// overlaps the matrix stream.  No LDS and no barrier except the reduction's.
+constexpr int kSellUnroll = 8;
+template <bool HAS_MATRIX, class MT, class Op>
+__global__ __launch_bounds__(kThreads) void k_sell(const int *__restrict__ col, const MT *__restrict__ val,
+                                                   UnitView uv, Ctx c, Op op) {
+    constexpr int NRED = Op::NRED;
+    constexpr int SPW = kSellSlicesPerWave;
+    constexpr int U = kSellUnroll;
+    __shared__ double sh[(NRED > 0 ? NRED : 1) * 4];
+    const int unit = unit_of(uv);
+    if (unit < 0) return;
+    const int s = uv.sys[unit];
+    const int lane = threadIdx.x & (kSliceRows - 1);
+    const int sl_end = uv.slice[unit + 1];
+    const int sl0 = uv.slice[unit] + (int)(threadIdx.x / kSliceRows) * SPW;
+    int row[SPW], w[SPW];
+    bool live[SPW];
+    long long base[SPW];
 #pragma unroll
-        for (int q = 0; q < NRED; ++q) v[q] = acc[q];
-        reduce_system<NRED>(c, unit, s, v, op, sh);
+    for (int j = 0; j < SPW; ++j) {
+        const int sl = sl0 + j;
+        const bool have = sl < sl_end;
+        const int r0 = have ? uv.slice_row[sl] : 0, r1 = have ? uv.slice_row[sl + 1] : 0;
+        const int b0 = have ? uv.slice_blk[sl] : 0, b1 = have ? uv.slice_blk[sl + 1] : 0;
+        row[j] = r0 + lane;
+        live[j] = row[j] < r1;
+        w[j] = b1 - b0;
+        base[j] = (long long)b0 * kSliceRows + lane;
     }
+    int jj[SPW][U];
+    MT vv[SPW][U];
+    if (HAS_MATRIX) {
+#pragma unroll
+        for (int j = 0; j < SPW; ++j)
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                const bool ok = k < w[j];
+                const long long q = base[j] + (long long)k * kSliceRows;
+                jj[j][k] = ok ? stream_load(&col[q]) : 0;
+                vv[j][k] = ok ? stream_load(&val[q]) : (MT)0;
+            }
+    }
+    if (!op.enabled(c.st[s])) return;
+    op.load(c.st[s]);
+    double acc[NRED > 0 ? NRED : 1];
+#pragma unroll
+    for (int q = 0; q < (NRED > 0 ? NRED : 1); ++q) acc[q] = 0.0;
+    if (!HAS_MATRIX) {
+#pragma unroll
+        for (int j = 0; j < SPW; ++j)
+            if (live[j]) op.row(row[j], 0.0, op.pre(row[j]), acc);
+    } else {
+        typename Op::Pre pre[SPW];
+#pragma unroll
+        for (int j = 0; j < SPW; ++j) pre[j] = live[j] ? op.pre(row[j]) : typename Op::Pre{};
+        double sum[SPW];
+        int wmax = 0;
+#pragma unroll
+        for (int j = 0; j < SPW; ++j) {
+            sum[j] = 0.0;
+            wmax = max(wmax, w[j]);
+        }
+        for (int k0 = 0;;) {
+            double gv[SPW][U];
+#pragma unroll
+            for (int j = 0; j < SPW; ++j)
+#pragma unroll
+                for (int k = 0; k < U; ++k) gv[j][k] = (k0 + k < w[j]) ? op.gather(jj[j][k]) : 0.0;
+#pragma unroll
+            for (int j = 0; j < SPW; ++j)
+#pragma unroll
+                for (int k = 0; k < U; ++k)
+                    if (k0 + k < w[j]) sum[j] += (double)vv[j][k] * gv[j][k];
+            k0 += U;
+            if (k0 >= wmax) break;
+            // rows longer than U: the next U columns
+#pragma unroll
+            for (int j = 0; j < SPW; ++j)
+#pragma unroll
+                for (int k = 0; k < U; ++k) {
+                    const bool ok = k0 + k < w[j];
+                    const long long q = base[j] + (long long)(k0 + k) * kSliceRows;
+                    jj[j][k] = ok ? stream_load(&col[q]) : 0;
+                    vv[j][k] = ok ? stream_load(&val[q]) : (MT)0;
+                }
+        }
+#pragma unroll
+        for (int j = 0; j < SPW; ++j)
+            if (live[j]) op.row(row[j], sum[j], pre[j], acc);
+    }
+    if constexpr (NRED > 0) reduce_system<NRED>(c, unit, s, acc, op, sh);
 }
 
 // ------------------------------------------------------------- the ops ----
@@ -647,8 +766,11 @@ __global__ void k_state_init(SysState *st, int nsys, int max_iter, int *counter,
         st[s] = z;
         counter[s] = 0;
     }
-    if (s == 0) counter[nsys] = nsys;
-    for (int b = s; b < nbuckets; b += gridDim.x * blockDim.x) counter[nsys + 1 + b] = 0;
+    if (s == 0) {
+        counter[nsys] = nsys;
+        counter[nsys + 1] = 0;
+    }
+    for (int b = s; b < nbuckets; b += gridDim.x * blockDim.x) counter[nsys + 2 + b] = 0;
 }
 
 // systems whose recursive residual converged but whose true residual did not
@@ -672,7 +794,98 @@ __global__ void k_state_activate(SysState *st, int nsys, int *counter, int nbuck
         st[s].active = 1;
         counter[s] = 0;
     }
-    for (int b = s; b < nbuckets; b += gridDim.x * blockDim.x) counter[nsys + 1 + b] = 0;
+    for (int b = s; b < nbuckets; b += gridDim.x * blockDim.x) counter[nsys + 2 + b] = 0;
+}
+
+// Active-unit list, two launches.  k_compact_scan (one workgroup): base[s] =
+// number of units of active systems before s, and the total into *count.
+// k_compact_fill: every unit of an active system writes its id at base[s] +
+// its rank inside the system, so the list keeps the layout's unit order.
+constexpr int kCompactBlock = 1024;
+__global__ __launch_bounds__(kCompactBlock) void k_compact_scan(const SysState *st, const int *sys_units, int nsys,
+                                                                int *base, int *count) {
+    __shared__ int sh[kCompactBlock];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < nsys; b0 += kCompactBlock) {
+        const int s = b0 + (int)threadIdx.x;
+        const int v = (s < nsys && st[s].active) ? sys_units[s + 1] - sys_units[s] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < kCompactBlock; o <<= 1) {
+            const int t = (int)threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (s < nsys) base[s] = carry + sh[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += sh[kCompactBlock - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *count = carry;
+}
+
+__global__ void k_compact_fill(const SysState *st, const int *unit_sys, const int *sys_units, const int *base,
+                               int nunits, int *list) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nunits) return;
+    const int s = unit_sys[u];
+    if (st[s].active) list[base[s] + u - sys_units[s]] = u;
+}
+
+// SELL-64 construction from CSR: one thread per (slice, lane).  Lane l of
+// slice sl holds row slice_row[sl] + l; its entries go to column k of the
+// slice, padding entries are (row, 0) - or (first row of the slice, 0) for a
+// lane past the system's last row - so every padded gather stays in bounds.
+__global__ void k_sell_fill(int nslices, const int *slice_row, const int *slice_blk, const int *rowptr,
+                            const int *colidx, int *sell_col, int *sell_pos) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nslices * kSliceRows) return;
+    const int sl = (int)(t / kSliceRows), lane = (int)(t % kSliceRows);
+    const int r0 = slice_row[sl], row = r0 + lane;
+    const bool live = row < slice_row[sl + 1];
+    const int w = slice_blk[sl + 1] - slice_blk[sl];
+    const long long base = (long long)slice_blk[sl] * kSliceRows + lane;
+    const int p0 = live ? rowptr[row] : 0, len = live ? rowptr[row + 1] - p0 : 0;
+    for (int k = 0; k < w; ++k) {
+        const long long q = base + (long long)k * kSliceRows;
+        if (k < len) {
+            sell_col[q] = colidx[p0 + k];
+            sell_pos[p0 + k] = (int)q;
+        } else {
+            sell_col[q] = live ? row : r0;
+        }
+    }
+}
+
+template <class MT_OUT>
+__global__ void k_scatter_values(long long nnz, const int *pos, const double *in, MT_OUT *out) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nnz) out[pos[p]] = (MT_OUT)in[p];
+}
+
+template <class MT_OUT>
+__global__ void k_convert_values(long long nnz, const double *in, MT_OUT *out) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nnz) out[p] = (MT_OUT)in[p];
+}
+
+template <class T>
+__global__ void k_fill_zero(T *p, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (T)0;
+}
+
+__global__ void k_fill_double(double *p, double v, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_compose_perm(int nnz, const int *perm_csr, const int *sell_pos, int *out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nnz) out[p] = sell_pos[perm_csr[p]];
 }
 
 __global__ void k_adjoint_scatter(int n, const int *colptr, const int *rowidx, const double *lambda, const double *x,
@@ -688,12 +901,25 @@ __global__ void k_f32_to_f64(const float *in, double *out, long long n) {
     if (i < n) out[i] = (double)in[i];
 }
 
+UnitView unit_view(const TileMap &tm, const int *list, const int *count) {
+    return UnitView{tm.unit_row.get(),  tm.unit_sys.get(),  tm.unit_slice.get(), tm.slice_row.get(),
+                    tm.slice_blk.get(), list,               count};
+}
+
+// One launch of op over the layout: every unit (grid = nunits, list unused)
+// or the first `grid` entries of the active list.
 template <bool HM, class MT, class Op>
-void launch_tiles(const OperatorView &O, const TileMap &tm, const Ctx &c, const Op &op, hipStream_t st) {
-    TileView tv{tm.row.get(), tm.sys.get(), tm.group_tile.get(), tm.group_sys.get()};
-    const int grid = Op::NRED > 0 ? tm.ngroups : tm.ntiles;
-    hipLaunchKernelGGL((k_tiles<HM, MT, Op>), dim3(grid), dim3(kThreads), 0, st, O.ptr, O.idx, (const MT *)O.val,
-                       tv, c, op);
+void launch_units(const OperatorView &O, const TileMap &tm, const Ctx &c, const Op &op, const int *list, int grid,
+                  hipStream_t st) {
+    const UnitView uv = unit_view(tm, list, c.counter + c.nsys + 1);
+    const int g = list ? grid : tm.nunits;
+    if (g <= 0) return;
+    if (tm.kind == kLayoutSell)
+        hipLaunchKernelGGL((k_sell<HM, MT, Op>), dim3(g), dim3(kThreads), 0, st, O.idx, (const MT *)O.val, uv, c,
+                           op);
+    else
+        hipLaunchKernelGGL((k_tiles<HM, MT, Op>), dim3(g), dim3(kThreads), 0, st, O.ptr, O.idx, (const MT *)O.val,
+                           uv, c, op);
 }
 
 }  // namespace
@@ -720,96 +946,165 @@ void csc_to_csr_device(int n, int nnz, const int *colptr, const int *rowidx, con
     SPAI_HIP(hipStreamSynchronize(st));  // temporaries are freed on return
 }
 
-void build_tiles(TileMap &tm, const int *ptr, const std::vector<int> &h_row_off,
-                 const std::vector<long long> &h_nnz_off, hipStream_t st) {
+void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector<int> &h_row_off,
+                  const std::vector<long long> &h_nnz_off, bool allow_sell, hipStream_t st) {
     const int nsys = (int)h_row_off.size() - 1;
-    require(nsys >= 1, "build_tiles: no systems");
+    require(nsys >= 1, "build_layout: no systems");
     const int n = h_row_off[(size_t)nsys];
     std::vector<int> hp((size_t)n + 1);
     SPAI_HIP(hipMemcpyAsync(hp.data(), ptr, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost, st));
     SPAI_HIP(hipStreamSynchronize(st));
-    // nonzeros per tile: enough tiles to fill the GPU on large inputs, at most
-    // the LDS staging buffer; rows per tile at most kRowsPerTile
-    long long total = h_nnz_off[(size_t)nsys];
-    int target = 256;
-    while (target < kLdsProducts && (long long)target * 4096 < total) target <<= 1;
-    tm.target_nnz = target;
-    std::vector<int> trow, tsys;
-    tm.h_sys_tiles.assign((size_t)nsys + 1, 0);
-    for (int s = 0; s < nsys; ++s) {
-        const int r0 = h_row_off[(size_t)s], r1 = h_row_off[(size_t)s + 1];
-        int i = r0;
-        if (r0 == r1) {  // empty system: one empty tile keeps the reductions uniform
-            trow.push_back(r0);
-            tsys.push_back(s);
-        }
-        while (i < r1) {
-            const int start = i;
-            long long nz = 0;
-            if (hp[(size_t)i + 1] - hp[(size_t)i] > kLdsProducts) {
-                ++i;  // a row longer than the staging buffer gets a tile of its own
-            } else {
-                while (i < r1 && i - start < kRowsPerTile) {
-                    const int len = hp[(size_t)i + 1] - hp[(size_t)i];
-                    if (len > kLdsProducts || (i > start && nz + len > target)) break;
-                    nz += len;
-                    ++i;
-                }
+    const long long total = h_nnz_off[(size_t)nsys];
+    // SELL-64 candidate: slices of 64 rows inside each system
+    std::vector<int> srow, sblk(1, 0), sys_slices((size_t)nsys + 1, 0);
+    bool sell = false;
+    if (allow_sell && total > 0) {
+        long long blocks = 0;
+        for (int s = 0; s < nsys; ++s) {
+            const int r0 = h_row_off[(size_t)s], r1 = h_row_off[(size_t)s + 1];
+            for (int r = r0; r < r1; r += kSliceRows) {
+                const int e = std::min(r + kSliceRows, r1);
+                int w = 0;
+                for (int i = r; i < e; ++i) w = std::max(w, hp[(size_t)i + 1] - hp[(size_t)i]);
+                srow.push_back(r);
+                blocks += w;
+                sblk.push_back((int)std::min<long long>(blocks, INT32_MAX));
             }
-            trow.push_back(start);
-            tsys.push_back(s);
+            sys_slices[(size_t)s + 1] = (int)srow.size();
         }
-        tm.h_sys_tiles[(size_t)s + 1] = (int)tsys.size();
+        const long long padded = blocks * kSliceRows;
+        // at most 20% padding, and SELL positions must fit in int32
+        sell = padded * 5 <= total * 6 && padded < (long long)INT32_MAX;
     }
-    trow.push_back(n);
-    tm.ntiles = (int)tsys.size();
-    // reducing kernels: groups of consecutive tiles of one system, enough of
-    // them to give every CU several workgroups
-    tm.group_size = (int)std::max<long long>(1, std::min<long long>(4, tm.ntiles / 2048));
-    std::vector<int> gtile, gsys, sgroups((size_t)nsys + 1, 0);
+    std::vector<int> urow, usys, uslice;
+    tm.h_sys_units.assign((size_t)nsys + 1, 0);
+    if (sell) {
+        tm.kind = kLayoutSell;
+        tm.target_nnz = 0;
+        tm.nslices = (int)srow.size();
+        tm.sell_len = (long long)sblk.back() * kSliceRows;
+        for (int s = 0; s < nsys; ++s) {
+            const int k0 = sys_slices[(size_t)s], k1 = sys_slices[(size_t)s + 1];
+            if (k0 == k1) {  // empty system: one empty unit keeps the reductions uniform
+                urow.push_back(h_row_off[(size_t)s]);
+                usys.push_back(s);
+                uslice.push_back(k0);
+            }
+            for (int k = k0; k < k1; k += kSlicesPerUnit) {
+                urow.push_back(srow[(size_t)k]);
+                usys.push_back(s);
+                uslice.push_back(k);
+            }
+            tm.h_sys_units[(size_t)s + 1] = (int)usys.size();
+        }
+        srow.push_back(n);
+        uslice.push_back(tm.nslices);
+    } else {
+        tm.kind = kLayoutTiles;
+        tm.nslices = 0;
+        tm.sell_len = 0;
+        // nonzeros per tile: enough tiles to fill the GPU on large inputs, at
+        // most the LDS staging buffer; rows per tile at most kRowsPerTile
+        int target = 256;
+        while (target < kLdsProducts && (long long)target * 4096 < total) target <<= 1;
+        tm.target_nnz = target;
+        for (int s = 0; s < nsys; ++s) {
+            const int r0 = h_row_off[(size_t)s], r1 = h_row_off[(size_t)s + 1];
+            int i = r0;
+            if (r0 == r1) {  // empty system: one empty tile keeps the reductions uniform
+                urow.push_back(r0);
+                usys.push_back(s);
+            }
+            while (i < r1) {
+                const int start = i;
+                long long nz = 0;
+                if (hp[(size_t)i + 1] - hp[(size_t)i] > kLdsProducts) {
+                    ++i;  // a row longer than the staging buffer gets a tile of its own
+                } else {
+                    while (i < r1 && i - start < kRowsPerTile) {
+                        const int len = hp[(size_t)i + 1] - hp[(size_t)i];
+                        if (len > kLdsProducts || (i > start && nz + len > target)) break;
+                        nz += len;
+                        ++i;
+                    }
+                }
+                urow.push_back(start);
+                usys.push_back(s);
+            }
+            tm.h_sys_units[(size_t)s + 1] = (int)usys.size();
+        }
+        uslice.assign(usys.size() + 1, 0);
+    }
+    urow.push_back(n);
+    tm.nunits = (int)usys.size();
+    // buckets of up to kBucket consecutive units of one system
+    std::vector<int> ubucket((size_t)tm.nunits), bunit, sbuckets((size_t)nsys + 1, 0);
     for (int s = 0; s < nsys; ++s) {
-        for (int t = tm.h_sys_tiles[(size_t)s]; t < tm.h_sys_tiles[(size_t)s + 1]; t += tm.group_size) {
-            gtile.push_back(t);
-            gsys.push_back(s);
+        const int u0 = tm.h_sys_units[(size_t)s], u1 = tm.h_sys_units[(size_t)s + 1];
+        for (int u = u0; u < u1; u += kBucket) {
+            const int bidx = (int)bunit.size();
+            bunit.push_back(u);
+            for (int h = u; h < std::min(u + kBucket, u1); ++h) ubucket[(size_t)h] = bidx;
         }
-        sgroups[(size_t)s + 1] = (int)gsys.size();
+        sbuckets[(size_t)s + 1] = (int)bunit.size();
     }
-    gtile.push_back(tm.ntiles);
-    tm.ngroups = (int)gsys.size();
-    // buckets of up to kBucket consecutive groups of one system
-    std::vector<int> gbucket((size_t)tm.ngroups), bgroup, sbuckets((size_t)nsys + 1, 0);
-    for (int s = 0; s < nsys; ++s) {
-        for (int g = sgroups[(size_t)s]; g < sgroups[(size_t)s + 1]; g += kBucket) {
-            int bidx = (int)bgroup.size();
-            bgroup.push_back(g);
-            for (int h = g; h < std::min(g + kBucket, sgroups[(size_t)s + 1]); ++h) gbucket[(size_t)h] = bidx;
-        }
-        sbuckets[(size_t)s + 1] = (int)bgroup.size();
-    }
-    bgroup.push_back(tm.ngroups);
-    tm.nbuckets = (int)bgroup.size() - 1;
-    tm.group_bucket.resize(gbucket.size());
-    tm.bucket_group.resize(bgroup.size());
-    tm.sys_buckets.resize(sbuckets.size());
-    tm.row.resize(trow.size());
-    tm.sys.resize(tsys.size());
-    tm.sys_tiles.resize((size_t)nsys + 1);
-    tm.group_tile.resize(gtile.size());
-    tm.group_sys.resize(gsys.size());
-    tm.sys_groups.resize(sgroups.size());
+    bunit.push_back(tm.nunits);
+    tm.nbuckets = (int)bunit.size() - 1;
     auto up = [&](DeviceArray<int> &d, const std::vector<int> &h) {
-        SPAI_HIP(hipMemcpyAsync(d.get(), h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, st));
+        d.resize(std::max<size_t>(h.size(), 1));
+        if (!h.empty()) SPAI_HIP(hipMemcpyAsync(d.get(), h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, st));
     };
-    up(tm.row, trow);
-    up(tm.sys, tsys);
-    up(tm.sys_tiles, tm.h_sys_tiles);
-    up(tm.group_tile, gtile);
-    up(tm.group_sys, gsys);
-    up(tm.sys_groups, sgroups);
-    up(tm.group_bucket, gbucket);
-    up(tm.bucket_group, bgroup);
+    up(tm.unit_row, urow);
+    up(tm.unit_sys, usys);
+    up(tm.sys_units, tm.h_sys_units);
+    up(tm.unit_slice, uslice);
+    up(tm.unit_bucket, ubucket);
+    up(tm.bucket_unit, bunit);
     up(tm.sys_buckets, sbuckets);
+    tm.active.resize((size_t)std::max(tm.nunits, 1));
+    tm.sys_base.resize((size_t)nsys);
+    if (sell) {
+        up(tm.slice_row, srow);
+        up(tm.slice_blk, sblk);
+        tm.sell_col.resize((size_t)tm.sell_len);
+        tm.sell_pos.resize((size_t)std::max<long long>(total, 1));
+        const long long threads = (long long)tm.nslices * kSliceRows;
+        hipLaunchKernelGGL(k_sell_fill, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, tm.nslices,
+                           tm.slice_row.get(), tm.slice_blk.get(), ptr, idx, tm.sell_col.get(), tm.sell_pos.get());
+        SPAI_CHECK_LAUNCH();
+    } else {
+        tm.slice_row.resize(1);
+        tm.slice_blk.resize(1);
+    }
     SPAI_HIP(hipStreamSynchronize(st));
+}
+
+template <class MT_OUT>
+void layout_values(const TileMap &tm, long long nnz, const double *in, MT_OUT *out, hipStream_t st) {
+    if (tm.kind == kLayoutSell) {
+        hipLaunchKernelGGL(k_fill_zero<MT_OUT>, dim3((unsigned)((tm.sell_len + 255) / 256)), dim3(256), 0, st, out,
+                           tm.sell_len);
+        if (nnz > 0)
+            hipLaunchKernelGGL(k_scatter_values<MT_OUT>, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, nnz,
+                               tm.sell_pos.get(), in, out);
+    } else if (nnz > 0) {
+        hipLaunchKernelGGL(k_convert_values<MT_OUT>, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, nnz, in,
+                           out);
+    }
+    SPAI_CHECK_LAUNCH();
+}
+template void layout_values<double>(const TileMap &, long long, const double *, double *, hipStream_t);
+template void layout_values<float>(const TileMap &, long long, const double *, float *, hipStream_t);
+
+void layout_perm(const TileMap &tm, int nnz, const int *perm_csr, int *perm_out, hipStream_t st) {
+    if (nnz <= 0) return;
+    if (tm.kind == kLayoutSell) {
+        hipLaunchKernelGGL(k_compose_perm, dim3(ceil_div(nnz, 256)), dim3(256), 0, st, nnz, perm_csr,
+                           tm.sell_pos.get(), perm_out);
+        SPAI_CHECK_LAUNCH();
+    } else {
+        SPAI_HIP(hipMemcpyAsync(perm_out, perm_csr, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToDevice, st));
+    }
 }
 
 void adjoint_scatter(int n, const int *colptr, const int *rowidx, const double *lambda, const double *x, double *dA,
@@ -836,15 +1131,17 @@ void BiCGStab::drop_graph() {
     graph_ = nullptr;
     for (auto &k : graph_key_) k = nullptr;
     graph_iters_ = 0;
+    graph_grid_ = 0;
+    graph_compact_ = 0;
 }
 
 Ctx BiCGStab::make_ctx(const TileMap &tm, int nsys) {
-    return Ctx{state_.get(),     tm.group_bucket.get(), tm.bucket_group.get(), tm.sys_buckets.get(),
-               counter_.get(),   partials_.get(),       bpartials_.get(),      nsys,
+    return Ctx{state_.get(),     tm.unit_bucket.get(), tm.bucket_unit.get(), tm.sys_buckets.get(),
+               counter_.get(),   partials_.get(),      bpartials_.get(),      nsys,
                hist_ptr_,        hist_stride_};
 }
 
-void BiCGStab::ensure(int n, int nsys, int ntiles, int nbuckets) {
+void BiCGStab::ensure(int n, int nsys, int nunits, int nbuckets) {
     if (n != n_) {
         drop_graph();
         vec_.resize((size_t)9 * std::max(n, 1));
@@ -864,69 +1161,81 @@ void BiCGStab::ensure(int n, int nsys, int ntiles, int nbuckets) {
     if (nsys != nsys_ || nbuckets != nbuckets_) {
         drop_graph();
         state_.resize((size_t)nsys);
-        counter_.resize((size_t)nsys + 1 + nbuckets);
+        counter_.resize((size_t)nsys + 2 + nbuckets);
         bpartials_.resize((size_t)2 * std::max(nbuckets, 1));
         nsys_ = nsys;
         nbuckets_ = nbuckets;
     }
-    if (ntiles != ntiles_) {
+    if (nunits != nunits_) {
         drop_graph();
-        partials_.resize((size_t)2 * std::max(ntiles, 1));
-        ntiles_ = ntiles;
+        partials_.resize((size_t)2 * std::max(nunits, 1));
+        nunits_ = nunits;
     }
 }
 
+// Rebuilds the active-unit list from the system states (stream-ordered).
+void BiCGStab::compact(const TileMap &tm, int nsys, hipStream_t st) {
+    hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(kCompactBlock), 0, st, state_.get(), tm.sys_units.get(), nsys,
+                       tm.sys_base.get(), counter_.get() + nsys + 1);
+    hipLaunchKernelGGL(k_compact_fill, dim3(ceil_div(tm.nunits, 256)), dim3(256), 0, st, state_.get(),
+                       tm.unit_sys.get(), tm.sys_units.get(), tm.sys_base.get(), tm.nunits, tm.active.get());
+    SPAI_CHECK_LAUNCH();
+}
+
+// One iteration, five launches.  grid > 0: walk the first `grid` entries of
+// the active-unit list; grid == 0: every unit.
 void BiCGStab::enqueue_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tm, int parity,
-                                 hipStream_t st) {
+                                 int grid, hipStream_t st) {
     Ctx c = make_ctx(tm, nsys_);
+    const int *list = grid > 0 ? tm.active.get() : nullptr;
     double *pin = p_[parity], *pout = p_[parity ^ 1];
     if (M.val) {
         if (M.fp32) {
             OpMapplyP<float> op{r_, pin, a_, pout, y_, 0.0, 0.0};
-            launch_tiles<true, float>(M, tm, c, op, st);
+            launch_units<true, float>(M, tm, c, op, list, grid, st);
         } else {
             OpMapplyP<double> op{r_, pin, a_, pout, y_, 0.0, 0.0};
-            launch_tiles<true, double>(M, tm, c, op, st);
+            launch_units<true, double>(M, tm, c, op, list, grid, st);
         }
     } else {
         // no preconditioner: y = p' through the streaming path
         OpCopyP op{r_, pin, a_, pout, y_, 0.0, 0.0};
-        launch_tiles<false, double>(A, tm, c, op, st);
+        launch_units<false, double>(A, tm, c, op, list, grid, st);
     }
     {
         OpSpmvAlpha op{y_, rs_, a_};
-        launch_tiles<true, double>(A, tm, c, op, st);
+        launch_units<true, double>(A, tm, c, op, list, grid, st);
     }
     if (M.val) {
         if (M.fp32) {
             OpMapplyS<float> op{r_, a_, s_, z_, 0.0};
-            launch_tiles<true, float>(M, tm, c, op, st);
+            launch_units<true, float>(M, tm, c, op, list, grid, st);
         } else {
             OpMapplyS<double> op{r_, a_, s_, z_, 0.0};
-            launch_tiles<true, double>(M, tm, c, op, st);
+            launch_units<true, double>(M, tm, c, op, list, grid, st);
         }
     } else {
         OpCopyS op{r_, a_, s_, z_, 0.0};
-        launch_tiles<false, double>(A, tm, c, op, st);
+        launch_units<false, double>(A, tm, c, op, list, grid, st);
     }
     {
         OpSpmvOmega op{z_, s_, t_};
-        launch_tiles<true, double>(A, tm, c, op, st);
+        launch_units<true, double>(A, tm, c, op, list, grid, st);
     }
     {
         OpUpdate op{x_, r_, y_, z_, s_, t_, rs_, 0.0, 0.0};
-        launch_tiles<false, double>(A, tm, c, op, st);
+        launch_units<false, double>(A, tm, c, op, list, grid, st);
     }
     SPAI_CHECK_LAUNCH();
 }
 
 void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap &tm, int n, int nsys,
                      const double *b, double *x, const SolveOptions &opt, hipStream_t st, SolveResult *res) {
-    require(n > 0 && nsys >= 1 && A.ptr && A.idx && A.val && b && x, "bicgstab: bad arguments");
-    require(tm.ntiles > 0 && tm.ngroups > 0 && (int)tm.h_sys_tiles.size() == nsys + 1,
-            "bicgstab: tiles do not match the systems");
+    require(n > 0 && nsys >= 1 && A.idx && A.val && b && x, "bicgstab: bad arguments");
+    require(tm.kind == kLayoutSell || A.ptr, "bicgstab: row tiles need a pointer array");
+    require(tm.nunits > 0 && (int)tm.h_sys_units.size() == nsys + 1, "bicgstab: layout does not match the systems");
     require(opt.max_iter >= 1, "bicgstab: max_iter must be >= 1");
-    ensure(n, nsys, tm.ngroups, tm.nbuckets);
+    ensure(n, nsys, tm.nunits, tm.nbuckets);
     b_ = b;
     x_ = x;
     if (opt.record_history) {
@@ -946,89 +1255,104 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                        counter_.get(), tm.nbuckets);
     {
         OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, opt.rel_tol, opt.abs_tol, 0};
-        launch_tiles<true, double>(A, tm, c, op, st);
+        launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
     }
     SPAI_CHECK_LAUNCH();
+    // A batch walks the active-unit list, rebuilt after every host check; a
+    // single system walks every unit.
+    const bool use_list = nsys > 1;
     int active = nsys;
+    int units = tm.nunits;
     int checks = 0;
-    int *h_active = nullptr;
-    SPAI_HIP(hipHostMalloc((void **)&h_active, sizeof(int), hipHostMallocDefault));
-    auto read_active = [&]() {
-        SPAI_HIP(hipMemcpyAsync(h_active, counter_.get() + nsys, sizeof(int), hipMemcpyDeviceToHost, st));
+    int *h_counts = nullptr;
+    SPAI_HIP(hipHostMalloc((void **)&h_counts, 2 * sizeof(int), hipHostMallocDefault));
+    auto read_counts = [&]() {
+        SPAI_HIP(hipMemcpyAsync(h_counts, counter_.get() + nsys, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
         SPAI_HIP(hipStreamSynchronize(st));
         ++checks;
-        return *h_active;
+        active = h_counts[0];
+        units = use_list ? h_counts[1] : tm.nunits;
     };
     const int max_launch_iters = opt.max_iter;  // every system stops by then
     int done_iters = 0;
     try {
-        active = read_active();
         auto iterate = [&]() {
+            if (use_list) compact(tm, nsys, st);
+            read_counts();
             if (opt.use_graph) {
-                int gi = std::max(2, opt.graph_iters + (opt.graph_iters & 1));
-                const void *key[7] = {A.val, M.val, tm.row.get(), b, x, (const void *)(size_t)M.fp32, hist_ptr_};
-                bool same = graph_exec_ && gi == graph_iters_;
-                for (int q = 0; q < 7 && same; ++q) same = key[q] == graph_key_[q];
-                if (!same) {
-                    drop_graph();
-                    hipStream_t cap;
-                    SPAI_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-                    SPAI_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-                    for (int k = 0; k < gi; ++k) enqueue_iteration(A, M, tm, k & 1, cap);
-                    SPAI_HIP(hipStreamEndCapture(cap, &graph_));
-                    SPAI_HIP(hipStreamDestroy(cap));
-                    SPAI_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
-                    graph_iters_ = gi;
-                    for (int q = 0; q < 7; ++q) graph_key_[q] = key[q];
-                }
+                const int gi = std::max(2, opt.graph_iters + (opt.graph_iters & 1));
+                const void *key[7] = {A.val, M.val, tm.unit_row.get(), b, x, (const void *)(size_t)M.fp32,
+                                      hist_ptr_};
                 while (active > 0 && done_iters < max_launch_iters) {
+                    // the graph's grid follows the active-unit count: captured
+                    // again when it has fallen below 5/8 of the captured grid
+                    const int grid = use_list ? units : 0;
+                    bool same = graph_exec_ && gi == graph_iters_ && graph_compact_ == (int)use_list;
+                    for (int q = 0; q < 7 && same; ++q) same = key[q] == graph_key_[q];
+                    if (same && use_list) same = grid <= graph_grid_ && grid * 8 >= graph_grid_ * 5;
+                    if (!same) {
+                        drop_graph();
+                        hipStream_t cap;
+                        SPAI_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+                        SPAI_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+                        for (int k = 0; k < gi; ++k) enqueue_iteration(A, M, tm, k & 1, grid, cap);
+                        if (use_list) compact(tm, nsys, cap);
+                        SPAI_HIP(hipStreamEndCapture(cap, &graph_));
+                        SPAI_HIP(hipStreamDestroy(cap));
+                        SPAI_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+                        graph_iters_ = gi;
+                        graph_grid_ = grid;
+                        graph_compact_ = (int)use_list;
+                        for (int q = 0; q < 7; ++q) graph_key_[q] = key[q];
+                    }
                     SPAI_HIP(hipGraphLaunch(graph_exec_, st));
                     done_iters += gi;
-                    active = read_active();
+                    read_counts();
                 }
             } else {
-                int every = std::max(1, opt.check_every);
+                const int every = std::max(1, opt.check_every);
                 int parity = 0;
                 while (active > 0 && done_iters < max_launch_iters) {
+                    const int grid = use_list ? units : 0;
                     for (int k = 0; k < every; ++k) {
-                        enqueue_iteration(A, M, tm, parity, st);
+                        enqueue_iteration(A, M, tm, parity, grid, st);
                         parity ^= 1;
                     }
+                    if (use_list) compact(tm, nsys, st);
                     done_iters += every;
-                    active = read_active();
+                    read_counts();
                 }
             }
         };
         iterate();
         for (int round = 0; opt.true_residual_restart && round < 4; ++round) {
             OpTrueResidual tr{x, b};
-            launch_tiles<true, double>(A, tm, c, tr, st);
+            launch_units<true, double>(A, tm, c, tr, nullptr, 0, st);
             hipLaunchKernelGGL(k_check_true, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys,
                                counter_.get());
             SPAI_CHECK_LAUNCH();
-            active = read_active();
+            read_counts();
             if (active == 0) break;
             OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, opt.rel_tol, opt.abs_tol, 1};
-            launch_tiles<true, double>(A, tm, c, op, st);
+            launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
             SPAI_CHECK_LAUNCH();
-            active = read_active();
             done_iters = 0;
             iterate();
         }
         {
             OpTrueResidual op{x, b};
-            launch_tiles<true, double>(A, tm, c, op, st);
+            launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
             SPAI_CHECK_LAUNCH();
         }
         SPAI_HIP(hipEventRecord(e1, st));
         SPAI_HIP(hipEventSynchronize(e1));
     } catch (...) {
-        (void)hipHostFree(h_active);
+        (void)hipHostFree(h_counts);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         throw;
     }
-    (void)hipHostFree(h_active);
+    (void)hipHostFree(h_counts);
     float ms = 0.0f;
     SPAI_HIP(hipEventElapsedTime(&ms, e0, e1));
     SPAI_HIP(hipEventDestroy(e0));
@@ -1065,11 +1389,9 @@ int BiCGStab::history(int s, double *out, int maxlen) const {
 
 float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView &M, const TileMap &tm, int n,
                             int nsys, int reps, hipStream_t st) {
-    // which: 0 bare A*x, 5 bare M*x, 1..4 the fused kernels K1..K4 with the
-    // scalars and vectors of the last solve (systems re-activated first)
     require(which >= 0 && which <= 5, "time_kernel: which must be 0..5");
     require((which != 1 && which != 3 && which != 5) || M.val, "time_kernel: no preconditioner");
-    ensure(n, nsys, tm.ngroups, tm.nbuckets);
+    ensure(n, nsys, tm.nunits, tm.nbuckets);
     Ctx c = make_ctx(tm, nsys);
     hipLaunchKernelGGL(k_state_activate, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys,
                        counter_.get(), tm.nbuckets);
@@ -1080,38 +1402,38 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
         switch (which) {
         case 0: {
             OpSpmvPlain op{p_[0], y_};
-            launch_tiles<true, double>(A, tm, c, op, st);
+            launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
         } break;
         case 5: {
             OpSpmvPlain op{p_[0], y_};
-            if (M.fp32) launch_tiles<true, float>(M, tm, c, op, st);
-            else launch_tiles<true, double>(M, tm, c, op, st);
+            if (M.fp32) launch_units<true, float>(M, tm, c, op, nullptr, 0, st);
+            else launch_units<true, double>(M, tm, c, op, nullptr, 0, st);
         } break;
         case 1: {
             if (M.fp32) {
                 OpMapplyP<float> op{r_, p_[0], a_, p_[1], y_, 0.0, 0.0};
-                launch_tiles<true, float>(M, tm, c, op, st);
+                launch_units<true, float>(M, tm, c, op, nullptr, 0, st);
             } else {
                 OpMapplyP<double> op{r_, p_[0], a_, p_[1], y_, 0.0, 0.0};
-                launch_tiles<true, double>(M, tm, c, op, st);
+                launch_units<true, double>(M, tm, c, op, nullptr, 0, st);
             }
         } break;
         case 2: {
             OpSpmvAlpha op{y_, rs_, a_};
-            launch_tiles<true, double>(A, tm, c, op, st);
+            launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
         } break;
         case 3: {
             if (M.fp32) {
                 OpMapplyS<float> op{r_, a_, s_, z_, 0.0};
-                launch_tiles<true, float>(M, tm, c, op, st);
+                launch_units<true, float>(M, tm, c, op, nullptr, 0, st);
             } else {
                 OpMapplyS<double> op{r_, a_, s_, z_, 0.0};
-                launch_tiles<true, double>(M, tm, c, op, st);
+                launch_units<true, double>(M, tm, c, op, nullptr, 0, st);
             }
         } break;
         case 4: {
             OpSpmvOmega op{z_, s_, t_};
-            launch_tiles<true, double>(A, tm, c, op, st);
+            launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
         } break;
         }
     };
@@ -1126,6 +1448,89 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
     SPAI_HIP(hipEventDestroy(e0));
     SPAI_HIP(hipEventDestroy(e1));
     return ms / std::max(1, reps);
+}
+
+void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tm, int n, int nsys,
+                              int reps, hipStream_t st, float ms[5]) {
+    require(A.val && M.val, "time_iteration: needs A and M");
+    require(reps >= 1, "time_iteration: reps must be >= 1");
+    ensure(n, nsys, tm.nunits, tm.nbuckets);
+    hist_ptr_ = nullptr;
+    hist_stride_ = 0;
+    tvec_.reserve((size_t)2 * n);
+    double *tb = tvec_.get(), *tx = tvec_.get() + n;
+    hipLaunchKernelGGL(k_fill_double, dim3(ceil_div(n, 256)), dim3(256), 0, st, tb, 1.0, (long long)n);
+    hipLaunchKernelGGL(k_fill_double, dim3(ceil_div(n, 256)), dim3(256), 0, st, tx, 0.0, (long long)n);
+    // no threshold is reached: every system stays active for the timed iterations
+    hipLaunchKernelGGL(k_state_init, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys, reps + 2,
+                       counter_.get(), tm.nbuckets);
+    Ctx c = make_ctx(tm, nsys);
+    {
+        OpInit op{tx, tb, r_, rs_, p_[0], p_[1], a_, 0.0, 1e-300, 0};
+        launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
+    }
+    SPAI_CHECK_LAUNCH();
+    const double *b_keep = b_;
+    double *x_keep = x_;
+    b_ = tb;
+    x_ = tx;
+    std::vector<hipEvent_t> ev((size_t)reps * 5 + 1);
+    for (auto &e : ev) SPAI_HIP(hipEventCreate(&e));
+    try {
+        // the five kernels of enqueue_iteration, each bracketed by events
+        for (int it = 0; it < reps; ++it) {
+            double *pin = p_[it & 1], *pout = p_[(it & 1) ^ 1];
+            SPAI_HIP(hipEventRecord(ev[(size_t)it * 5], st));
+            if (M.fp32) {
+                OpMapplyP<float> op{r_, pin, a_, pout, y_, 0.0, 0.0};
+                launch_units<true, float>(M, tm, c, op, nullptr, 0, st);
+            } else {
+                OpMapplyP<double> op{r_, pin, a_, pout, y_, 0.0, 0.0};
+                launch_units<true, double>(M, tm, c, op, nullptr, 0, st);
+            }
+            SPAI_HIP(hipEventRecord(ev[(size_t)it * 5 + 1], st));
+            {
+                OpSpmvAlpha op{y_, rs_, a_};
+                launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
+            }
+            SPAI_HIP(hipEventRecord(ev[(size_t)it * 5 + 2], st));
+            if (M.fp32) {
+                OpMapplyS<float> op{r_, a_, s_, z_, 0.0};
+                launch_units<true, float>(M, tm, c, op, nullptr, 0, st);
+            } else {
+                OpMapplyS<double> op{r_, a_, s_, z_, 0.0};
+                launch_units<true, double>(M, tm, c, op, nullptr, 0, st);
+            }
+            SPAI_HIP(hipEventRecord(ev[(size_t)it * 5 + 3], st));
+            {
+                OpSpmvOmega op{z_, s_, t_};
+                launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
+            }
+            SPAI_HIP(hipEventRecord(ev[(size_t)it * 5 + 4], st));
+            {
+                OpUpdate op{x_, r_, y_, z_, s_, t_, rs_, 0.0, 0.0};
+                launch_units<false, double>(A, tm, c, op, nullptr, 0, st);
+            }
+        }
+        SPAI_HIP(hipEventRecord(ev.back(), st));
+        SPAI_HIP(hipEventSynchronize(ev.back()));
+        SPAI_CHECK_LAUNCH();
+        for (int k = 0; k < 5; ++k) ms[k] = 0.0f;
+        for (int it = 0; it < reps; ++it)
+            for (int k = 0; k < 5; ++k) {
+                float t = 0.0f;
+                SPAI_HIP(hipEventElapsedTime(&t, ev[(size_t)it * 5 + k], ev[(size_t)it * 5 + k + 1]));
+                ms[k] += t / reps;
+            }
+    } catch (...) {
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        b_ = b_keep;
+        x_ = x_keep;
+        throw;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    b_ = b_keep;
+    x_ = x_keep;
 }
 
 }  // namespace spai

@@ -29,28 +29,58 @@ struct alignas(64) SysState {
 
 enum SolveStatus { kRunning = 0, kConverged = 1, kMaxIter = 2, kBreakdown = 3, kConvergedAtStart = 4 };
 
-// Row tiles: contiguous row ranges, none crossing a system boundary, each
-// holding about `target_nnz` nonzeros (at most one LDS staging buffer plus one
-// row).  Kernels without a reduction run one workgroup per tile; kernels that
-// reduce run one workgroup per group of `group_size` consecutive tiles of one
-// system, so that each workgroup publishes one partial for several tiles.
+// Row layout of the solver's operators.  Every kernel runs one workgroup per
+// unit; a unit's rows are contiguous and belong to one system.
+//
+//  * SELL-64 (kLayoutSell): rows in slices of 64 (one wavefront, lane = row),
+//    each slice stored column-major with the width of its longest row; short
+//    rows are padded with (own row, 0.0).  A unit is up to kSlicesPerUnit
+//    slices of one system.  Chosen when the padding adds at most 20% to the nonzeros (stencil
+//    and other near-uniform rows).  A and M share the column indices (M has
+//    A's pattern); values are stored per operator in SELL order.
+//  * Row tiles (kLayoutTiles): contiguous row ranges holding about
+//    `target_nnz` nonzeros (at most one LDS staging buffer, a longer row gets
+//    a tile of its own), over a CSR (or, for a transposed operator, CSC)
+//    pointer array.  Used for irregular rows and for operators given as CSR.
+//
+// Reducing kernels publish one partial per unit; units are counted in
+// buckets of up to kBucket consecutive units of one system, then per system.
+// `active` lists the units of systems still iterating (rebuilt on the device
+// from the system states); launches that use it exit at once past its count.
+enum LayoutKind { kLayoutTiles = 0, kLayoutSell = 1 };
+#ifndef SPAI_SELL_SLICES_PER_WAVE
+#define SPAI_SELL_SLICES_PER_WAVE 2
+#endif
+constexpr int kSliceRows = 64;
+constexpr int kSellSlicesPerWave = SPAI_SELL_SLICES_PER_WAVE;
+constexpr int kSlicesPerUnit = 4 * kSellSlicesPerWave;  // 4 wavefronts per workgroup
+
 struct TileMap {
-    int ntiles = 0;
+    int kind = kLayoutTiles;
     int target_nnz = 0;
-    DeviceArray<int> row;        // ntiles + 1 row boundaries
-    DeviceArray<int> sys;        // system of each tile
-    DeviceArray<int> sys_tiles;  // nsys + 1 tile offsets per system
-    std::vector<int> h_sys_tiles;
-    int ngroups = 0;
-    int group_size = 0;
-    DeviceArray<int> group_tile;  // ngroups + 1: first tile of each group
-    DeviceArray<int> group_sys;   // system of each group
-    DeviceArray<int> sys_groups;  // nsys + 1 group offsets per system
-    // buckets of consecutive groups of one system (two-level reduction count)
+    int nunits = 0;
+    DeviceArray<int> unit_row;   // nunits + 1 row boundaries
+    DeviceArray<int> unit_sys;   // system of each unit
+    DeviceArray<int> sys_units;  // nsys + 1 unit offsets per system
+    std::vector<int> h_sys_units;
+    // SELL-64 only
+    int nslices = 0;
+    long long sell_len = 0;       // padded entries (a multiple of 64)
+    DeviceArray<int> unit_slice;  // nunits + 1: first slice of each unit
+    DeviceArray<int> slice_row;   // nslices + 1 row boundaries
+    DeviceArray<int> slice_blk;   // nslices + 1 offsets in blocks of 64 entries
+    DeviceArray<int> sell_col;    // sell_len column indices
+    DeviceArray<int> sell_pos;    // nnz: CSR position -> SELL position
+    // two-level reduction count
     int nbuckets = 0;
-    DeviceArray<int> group_bucket;  // bucket of each group
-    DeviceArray<int> bucket_group;  // nbuckets + 1: first group of each bucket
-    DeviceArray<int> sys_buckets;   // nsys + 1: first bucket of each system
+    DeviceArray<int> unit_bucket;  // bucket of each unit
+    DeviceArray<int> bucket_unit;  // nbuckets + 1: first unit of each bucket
+    DeviceArray<int> sys_buckets;  // nsys + 1: first bucket of each system
+    // active-unit list: [0, nunits) unit ids; compaction scratch per system
+    DeviceArray<int> active;
+    DeviceArray<int> sys_base;
+    // entries of an operator's value array in this layout
+    long long value_len(long long nnz) const { return kind == kLayoutSell ? sell_len : nnz; }
 };
 
 // Device CSC -> CSR (rows ordered by column) plus perm[csc_pos] = csr_pos.
@@ -58,11 +88,22 @@ struct TileMap {
 void csc_to_csr_device(int n, int nnz, const int *colptr, const int *rowidx, const double *cscval, int *rowptr,
                        int *colidx, double *csrval, int *perm, hipStream_t st);
 
-// Builds tiles over rows of a CSR-like pointer array `ptr` (n+1 entries, on
-// device) for systems with row offsets h_row_off (nsys+1, host) and nonzero
-// offsets h_nnz_off (nsys+1, host).
-void build_tiles(TileMap &tm, const int *ptr, const std::vector<int> &h_row_off,
-                 const std::vector<long long> &h_nnz_off, hipStream_t st);
+// Builds the row layout over a CSR-like pointer/index pair (`ptr` n+1 and
+// `idx` entries, on the device) for systems with row offsets h_row_off
+// (nsys+1, host) and nonzero offsets h_nnz_off (nsys+1, host).  SELL-64 is
+// considered only when allow_sell is set (the operators then share `idx`'s
+// pattern); otherwise, or when its padding is too large, row tiles.
+void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector<int> &h_row_off,
+                  const std::vector<long long> &h_nnz_off, bool allow_sell, hipStream_t st);
+
+// Values of a CSR-ordered array in the layout's order: out[sell_pos[p]] =
+// in[p] (SELL, padding 0) or a copy (tiles).  MT_OUT is double or float.
+template <class MT_OUT>
+void layout_values(const TileMap &tm, long long nnz, const double *in, MT_OUT *out, hipStream_t st);
+
+// perm_out[p] = position of CSC entry p in the layout's value order, from
+// perm_csr[p] = its CSR position.
+void layout_perm(const TileMap &tm, int nnz, const int *perm_csr, int *perm_out, hipStream_t st);
 
 struct SolveOptions {
     double rel_tol = 1e-8;   // stop when ||r|| < rel_tol * ||b|| (per system)
@@ -86,8 +127,10 @@ struct SolveResult {
     int all_converged = 0;
 };
 
-// Matrix operand of the solver: CSR arrays (or CSC arrays for a transposed
-// operator) with values in double (A) and double or float (M).
+// Matrix operand of the solver in the layout's order: CSR arrays (or CSC
+// arrays for a transposed operator) for row tiles, the shared SELL column
+// indices (ptr unused) for SELL-64; values in double (A) and double or float
+// (M).
 struct OperatorView {
     const int *ptr = nullptr;
     const int *idx = nullptr;
@@ -110,21 +153,29 @@ public:
     // ||r_k||_2 after iteration k = 1.. of system s in the last solve that
     // recorded a history; returns the number of values written.
     int history(int s, double *out, int maxlen) const;
-    // Average device time of one A*vector tile launch over `reps` launches on
-    // the solver's stream (roofline measurement of the dominant kernel).
+    // Average device time of one launch of a single kernel repeated `reps`
+    // times on the solver's stream (which: 0 bare A*x, 5 bare M*x, 1..4 K1..K4
+    // with the vectors and scalars of the last solve, every system active).
     float time_kernel(int which, const OperatorView &A, const OperatorView &M, const TileMap &tiles, int n, int nsys,
                       int reps, hipStream_t st);
+    // Average device time of each of K1..K5 over `reps` consecutive
+    // iterations from x0 = 0, b = 1 with every system active (no threshold is
+    // reached), HIP events between the kernels on the solver's stream.
+    void time_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tiles, int n, int nsys,
+                        int reps, hipStream_t st, float ms[5]);
 
 private:
-    void ensure(int n, int nsys, int ntiles, int nbuckets);
+    void ensure(int n, int nsys, int nunits, int nbuckets);
+    void compact(const TileMap &tm, int nsys, hipStream_t st);
     Ctx make_ctx(const TileMap &tm, int nsys);
-    void enqueue_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tiles, int parity,
+    void enqueue_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tiles, int parity, int grid,
                            hipStream_t st);
-    int n_ = 0, nsys_ = 0, ntiles_ = 0, nbuckets_ = -1;
+    int n_ = 0, nsys_ = 0, nunits_ = 0, nbuckets_ = -1;
     DeviceArray<double> vec_;       // x-independent work vectors
     DeviceArray<double> partials_, bpartials_;
     DeviceArray<SysState> state_;
-    DeviceArray<int> counter_;      // per system, + n_active at [nsys]
+    DeviceArray<int> counter_;      // per system, [nsys] active systems, [nsys + 1] active units, buckets
+    DeviceArray<double> tvec_;      // b = 1 and x = 0 of time_iteration
     std::vector<SysState> h_states_;
     double *r_ = nullptr, *rs_ = nullptr, *p_[2] = {nullptr, nullptr}, *y_ = nullptr, *a_ = nullptr, *s_ = nullptr,
            *z_ = nullptr, *t_ = nullptr;
@@ -135,6 +186,8 @@ private:
     hipGraph_t graph_ = nullptr;
     const void *graph_key_[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     int graph_iters_ = 0;
+    int graph_grid_ = 0;
+    int graph_compact_ = 0;
     void drop_graph();
     DeviceArray<double> hist_;
     double *hist_ptr_ = nullptr;

@@ -182,6 +182,21 @@ int spai_system_adjoint(spai_system *sys, const double *x, const double *g, doub
  * last solve. */
 int spai_system_time_spmv(spai_system *sys, int which, int reps, float *ms);
 
+/* Average device time (ms) of each iteration kernel K1..K5 (ms[0..4]) over
+ * `reps` consecutive iterations from x0 = 0, b = 1, every system active (no
+ * threshold is reached), with HIP events between the kernels on the
+ * context's stream: the per-kernel times of the real iteration sequence.
+ * Overwrites the solver's work vectors; M must be built. */
+int spai_system_time_iteration(spai_system *sys, int reps, float *ms);
+
+/* Row layout chosen for the solver's operators: SPAI_LAYOUT_SELL64 (slices
+ * of 64 rows, column-major, when padding adds at most 20%) or
+ * SPAI_LAYOUT_ROW_TILES; `entries` = stored entries per operator (nnz, or
+ * nnz plus padding). */
+#define SPAI_LAYOUT_ROW_TILES 0
+#define SPAI_LAYOUT_SELL64 1
+int spai_system_layout(const spai_system *sys, int *kind, long long *entries);
+
 /* Average device time (ms) of a plain copy kernel that reads and writes
  * `bytes` bytes each (the calibration kernel for hardware byte counters and
  * the practical streaming bandwidth). */
