@@ -52,7 +52,7 @@ def cpu_model():
     return platform.processor()
 
 
-KERNELS = ("mapply_p", "spmv_alpha", "mapply_s", "spmv_omega", "update")
+KERNELS = ("mapply_p", "spmv_alpha", "mapply_s", "spmv_omega")
 
 
 def kernel_bytes(kind, n, nnz, m_bytes, row_ptr):
@@ -62,16 +62,14 @@ def kernel_bytes(kind, n, nnz, m_bytes, row_ptr):
     row pointers when the layout is row tiles (row_ptr).  The padding entries
     of the SELL-64 layout are not counted (they are overhead)."""
     rp = 4 * n if row_ptr else 0
+    if kind == "mapply_p":        # x, r update + y = M p'; reads M, x, y, z, s, t, p, a; writes x, r, p', y
+        return (m_bytes + 4) * nnz + rp + 8 * n * 11
     if kind == "spmv_alpha":      # a = A y; reads A, y (gather, once), r*; writes a
         return 12 * nnz + rp + 8 * n * 3
-    if kind == "spmv_omega":      # t = A z; reads A, z, s; writes t
-        return 12 * nnz + rp + 8 * n * 3
-    if kind == "mapply_p":        # y = M p'; reads M, r, p, a; writes p', y
-        return (m_bytes + 4) * nnz + rp + 8 * n * 5
     if kind == "mapply_s":        # z = M s; reads M, r, a; writes s, z
         return (m_bytes + 4) * nnz + rp + 8 * n * 4
-    if kind == "update":          # x, r; reads x, y, z, s, t, r*; writes x, r
-        return 8 * n * 8
+    if kind == "spmv_omega":      # t = A z; reads A, z, s, r*; writes t
+        return 12 * nnz + rp + 8 * n * 4
     raise ValueError(kind)
 
 
@@ -171,7 +169,7 @@ def main():
     log("per-system iterations (n): " + ", ".join(f"{o['iterations']}({A.n})" for o, A in zip(oc, mats)))
 
     # ------------------------------------------------------- roofline ----
-    # per-kernel times of the real iteration sequence (K1..K5 in order, every
+    # per-kernel times of the real iteration sequence (K1..K4 in order, every
     # system active, HIP events between the kernels on the library's stream)
     m_bytes = 4 if precision == D.PRECISION_MIXED else 8
     kernel_ms = dict(zip(KERNELS, S.time_iteration(20)))
@@ -198,7 +196,7 @@ def main():
     copy_ms = ctx.calibration_copy(256 * 1024 * 1024, 10)
     roofline["copy_kernel_GBs"] = round(2 * 256 * 1024 * 1024 / (copy_ms * 1e-3) / 1e9, 1)
     # whole solve: every system's iterations x its per-iteration algorithmic
-    # bytes (K1..K5 as above)
+    # bytes (K1..K4 as above)
     per_iter = 0.0
     for A, o in zip(mats, oc):
         it = o["iterations"] if o["status"] != D.SOLVE_MAX_ITER else o["iterations"] - 1

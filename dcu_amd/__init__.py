@@ -511,8 +511,8 @@ class System:
         return ms.value
 
     def time_iteration(self, reps: int = 20) -> List[float]:
-        """Average ms of K1..K5 over `reps` consecutive all-active iterations."""
-        ms = (ctypes.c_float * 5)()
+        """Average ms of K1..K4 over `reps` consecutive all-active iterations."""
+        ms = (ctypes.c_float * 4)()
         _check(self._lib.spai_system_time_iteration(self._h, int(reps), ms))
         return [float(v) for v in ms]
 

@@ -6,15 +6,21 @@
 // r = s - omega t, rho' = (r*, r), beta = (rho'/rho)(alpha/omega),
 // p = r + beta (p - omega a), stop on ||r||_2 < TOL).  The reference issues
 // 18 hipSPARSE/hipBLAS calls and 5 blocking scalar reads per iteration; here
-// one iteration is five kernels and every scalar stays in device memory:
+// one iteration is four kernels and every scalar stays in device memory:
 //
-//   K1  y = M p'   with p' = beta p + r - beta omega a formed while gathering
-//                  (the p update of the previous iteration, fused), p' stored
-//   K2  a = A y,   (r*, a) reduced, alpha = rho / (r*, a) by the last tile
+//   K1  finishes the previous iteration: x += alpha y + omega z and
+//       r = s - omega t per row, ||r||^2 reduced, the stop test and the
+//       iteration count by the last unit; then y = M p' with
+//       p' = beta p + r - beta omega a formed while gathering, p' stored
+//   K2  a = A y,   (r*, a) reduced, alpha = rho / (r*, a) by the last unit
 //   K3  z = M s    with s = r - alpha a formed while gathering, s stored
-//   K4  t = A z,   (t, s), (t, t) reduced, omega by the last tile
-//   K5  x += alpha y + omega z, r = s - omega t, (r*, r), (r, r) reduced,
-//       beta, convergence and the iteration count by the last tile
+//   K4  t = A z,   (t, s), (t, t), (r*, s), (r*, t) reduced; omega,
+//       rho' = (r*, s) - omega (r*, t) = (r*, r), beta by the last unit
+//
+// rho' is the reference's (r*, r) expanded by linearity (r = s - omega t),
+// so that beta is known before K1 needs it; ||r|| for the stop test is
+// reduced from r itself, as the reference computes it.  A system stops in
+// the K1 after its last iteration, with x and r updated.
 //
 // Every kernel runs one workgroup per unit of the row layout (krylov.hpp):
 // four SELL-64 slices, lane = row, the slice's column indices and values
@@ -45,8 +51,8 @@ struct Ctx {
     const int *sys_buckets;  // nsys + 1: first bucket of each system
     int *counter;            // [0, nsys) systems, [nsys] active systems, [nsys + 1] active units,
                              // [nsys + 2 + b] buckets
-    double *partials;        // nunits * 2 unit partials
-    double *bpartials;       // nbuckets * 2 bucket partials
+    double *partials;        // nunits * kMaxRed unit partials
+    double *bpartials;       // nbuckets * kMaxRed bucket partials
     int nsys;
     double *hist;            // optional ||r_k|| history, hist_stride entries per system
     int hist_stride;
@@ -57,6 +63,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kLdsProducts = 2048;  // products staged per tile (16 KB)
 constexpr int kBucket = 32;         // units per first-level reduction counter
+constexpr int kMaxRed = 4;          // most values any kernel reduces (partials per unit and bucket)
 
 // ------------------------------------------------------------ transpose ----
 __global__ void k_fill_int(int *p, int v, long long n) {
@@ -255,6 +262,7 @@ __device__ __forceinline__ void wave_sum_range(const double *src, int first, int
 // others retire at once instead of waiting on the publish round trips.
 template <int NRED, class Op>
 __device__ __forceinline__ void reduce_system(const Ctx &c, int unit, int s, double (&v)[NRED], Op &op, double *sh) {
+    static_assert(NRED >= 1 && NRED <= kMaxRed, "partials hold at most kMaxRed values per unit and bucket");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int W = kThreads / 64;
     wave_sum<NRED>(v);
@@ -402,8 +410,17 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
 // are requested before the system's state is read, so that the state load
 // overlaps the matrix stream.  No LDS and no barrier except the reduction's.
 constexpr int kSellUnroll = 8;
+// optional occupancy floor (waves per SIMD) for the register allocator
+#ifndef SPAI_SELL_MIN_WAVES
+#define SPAI_SELL_MIN_WAVES 0
+#endif
+#if SPAI_SELL_MIN_WAVES > 0
+#define SPAI_SELL_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SPAI_SELL_MIN_WAVES)))
+#else
+#define SPAI_SELL_OCCUPANCY
+#endif
 template <bool HAS_MATRIX, class MT, class Op>
-__global__ __launch_bounds__(kThreads) void k_sell(const int *__restrict__ col, const MT *__restrict__ val,
+__global__ __launch_bounds__(kThreads) SPAI_SELL_OCCUPANCY void k_sell(const int *__restrict__ col, const MT *__restrict__ val,
                                                    UnitView uv, Ctx c, Op op) {
     constexpr int NRED = Op::NRED;
     constexpr int SPW = kSellSlicesPerWave;
@@ -494,13 +511,41 @@ __global__ __launch_bounds__(kThreads) void k_sell(const int *__restrict__ col, 
 }
 
 // ------------------------------------------------------------- the ops ----
-struct OpInit {  // r = b - A x0, r* = r, p = a = 0; rho = rr = (r, r); tol
+// Stop test after an iteration's update (the reference's loop tail,
+// bicgstab_solver.h:224-231): converged, then non-finite, then MAX_ITER.
+__device__ __forceinline__ void finish_iteration(const Ctx &c, int s, SysState &S, double rr) {
+    S.rr = rr;
+    S.iters += 1;
+    const double err = sqrt(rr);
+    if (c.hist && S.iters <= c.hist_stride) c.hist[(size_t)s * c.hist_stride + S.iters - 1] = err;
+    bool stop = false;
+    if (err < S.tol) {
+        S.status = kConverged;
+        stop = true;
+    } else if (!(err == err) || !isfinite(S.beta)) {
+        S.status = kBreakdown;
+        stop = true;
+    } else if (S.iters >= S.max_iter - 1) {
+        // the reference returns MAX_ITER after MAX_ITER - 1 iterations
+        S.status = kMaxIter;
+        S.iters = S.max_iter;
+        stop = true;
+    }
+    if (stop) {
+        S.active = 0;
+        atomicSub(&c.counter[c.nsys], 1);
+    }
+}
+
+// r = b - A x0, r* = r, s = r, p = a = t = y = z = 0; rho = rr = (r, r); tol.
+// With alpha = omega = beta = 0 the first K1 reproduces r and forms p' = r.
+struct OpInit {
     static constexpr int NRED = 2;
     struct Pre {
         double b;
     };
     const double *x, *b;
-    double *r, *rs, *p0, *p1, *a;
+    double *r, *rs, *p0, *p1, *a, *s, *t, *y, *z;
     double rel_tol, abs_tol;
     int restart;  // 1: restart of an active system from its current x (iterations and tol kept)
     __device__ bool enabled(const SysState &S) const { return !restart || S.active; }
@@ -511,20 +556,25 @@ struct OpInit {  // r = b - A x0, r* = r, p = a = 0; rho = rr = (r, r); tol
         double ri = -ax + pr.b;
         r[i] = ri;
         rs[i] = ri;
+        s[i] = ri;
         p0[i] = 0.0;
         p1[i] = 0.0;
         a[i] = 0.0;
+        t[i] = 0.0;
+        y[i] = 0.0;
+        z[i] = 0.0;
         acc[0] += ri * ri;
         acc[1] += pr.b * pr.b;
     }
-    __device__ void finish(const Ctx &c, int s, const double (&tot)[2]) const {
-        SysState &S = c.st[s];
+    __device__ void finish(const Ctx &c, int s_, const double (&tot)[2]) const {
+        SysState &S = c.st[s_];
         S.rho = tot[0];
         S.rr = tot[0];
         S.alpha = 0.0;
         S.omega = 0.0;
         S.beta = 0.0;
         S.true_rr = 0.0;
+        S.pending = 0;
         if (restart) {
             S.restarts += 1;
             if (sqrt(tot[0]) < S.tol) {
@@ -546,28 +596,50 @@ struct OpInit {  // r = b - A x0, r* = r, p = a = 0; rho = rr = (r, r); tol
     }
 };
 
-template <class MT>
-struct OpMapplyP {  // K1: y = M p', p' = beta p + r - beta omega a
-    static constexpr int NRED = 0;
+// K1.  Per row: the previous iteration's x += alpha y + omega z and
+// r = s - omega t, (r, r) reduced; p' = beta p + r - beta omega a.  With a
+// preconditioner (PRECOND) y = M p' is gathered from p'(j) formed on the fly
+// (bit-identical to the stored p'), else y = p'.
+template <class MT, bool PRECOND>
+struct OpMapplyP {
+    static constexpr int NRED = 1;
     struct Pre {
-        double p;
+        double x, y, z, r, p;
     };
-    const double *r, *pin, *a;
-    double *pout, *y;
-    double beta, nbw;
+    double *x, *r, *pout, *y;
+    const double *s, *t, *pin, *a, *z;
+    double alpha, omega, nomega, beta, nbw;
     __device__ bool enabled(const SysState &S) const { return S.active; }
     __device__ void load(const SysState &S) {
+        alpha = S.alpha;
+        omega = S.omega;
+        nomega = -S.omega;
         beta = S.beta;
         nbw = -S.beta * S.omega;
     }
-    __device__ double pval(int j) const { return (beta * pin[j] + r[j]) + nbw * a[j]; }
-    __device__ Pre pre(int i) const { return {pval(i)}; }
-    __device__ double gather(int j) const { return pval(j); }
-    __device__ void row(int i, double mx, const Pre &pr, double (&)[1]) const {
-        y[i] = mx;
-        pout[i] = pr.p;
+    __device__ double rval(int j) const { return s[j] + nomega * t[j]; }
+    __device__ double pval(int j, double rj) const { return (beta * pin[j] + rj) + nbw * a[j]; }
+    __device__ Pre pre(int i) const {
+        const double ri = rval(i);
+        return {x[i], y[i], z[i], ri, pval(i, ri)};
     }
-    __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
+    __device__ double gather(int j) const { return pval(j, rval(j)); }
+    __device__ void row(int i, double mx, const Pre &pr, double (&acc)[1]) const {
+        x[i] = (pr.x + alpha * pr.y) + omega * pr.z;
+        r[i] = pr.r;
+        pout[i] = pr.p;
+        y[i] = PRECOND ? mx : pr.p;
+        acc[0] += pr.r * pr.r;
+    }
+    __device__ void finish(const Ctx &c, int s_, const double (&tot)[1]) const {
+        SysState &S = c.st[s_];
+        if (!S.pending) {  // first K1 after OpInit: nothing to finish
+            S.pending = 1;
+            S.rr = tot[0];
+            return;
+        }
+        finish_iteration(c, s_, S, tot[0]);
+    }
 };
 
 struct OpSpmvAlpha {  // K2: a = A y, alpha = rho / (r*, a)
@@ -612,76 +684,30 @@ struct OpMapplyS {  // K3: z = M s, s = r - alpha a
     __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
 
-struct OpSpmvOmega {  // K4: t = A z, omega = (t, s) / (t, t)
-    static constexpr int NRED = 2;
+struct OpSpmvOmega {  // K4: t = A z; omega, rho' = (r*, s) - omega (r*, t), beta
+    static constexpr int NRED = 4;
     struct Pre {
-        double s;
+        double s, rs;
     };
-    const double *z, *s;
+    const double *z, *s, *rs;
     double *t;
     __device__ bool enabled(const SysState &S) const { return S.active; }
     __device__ void load(const SysState &) {}
-    __device__ Pre pre(int i) const { return {s[i]}; }
+    __device__ Pre pre(int i) const { return {s[i], rs[i]}; }
     __device__ double gather(int j) const { return z[j]; }
-    __device__ void row(int i, double ax, const Pre &pr, double (&acc)[2]) const {
+    __device__ void row(int i, double ax, const Pre &pr, double (&acc)[4]) const {
         t[i] = ax;
         acc[0] += ax * pr.s;
         acc[1] += ax * ax;
+        acc[2] += pr.rs * pr.s;
+        acc[3] += pr.rs * ax;
     }
-    __device__ void finish(const Ctx &c, int s_, const double (&tot)[2]) const {
+    __device__ void finish(const Ctx &c, int s_, const double (&tot)[4]) const {
         SysState &S = c.st[s_];
         S.omega = tot[0] / tot[1];
-    }
-};
-
-struct OpUpdate {  // K5: x, r update; rho', beta, ||r||, stop test
-    static constexpr int NRED = 2;
-    struct Pre {
-        double x, y, z, s, t, rs;
-    };
-    double *x, *r;
-    const double *y, *z, *s, *t, *rs;
-    double alpha, omega;
-    __device__ bool enabled(const SysState &S) const { return S.active; }
-    __device__ void load(const SysState &S) {
-        alpha = S.alpha;
-        omega = S.omega;
-    }
-    __device__ Pre pre(int i) const { return {x[i], y[i], z[i], s[i], t[i], rs[i]}; }
-    __device__ double gather(int) const { return 0.0; }
-    __device__ void row(int i, double, const Pre &pr, double (&acc)[2]) const {
-        x[i] = (pr.x + alpha * pr.y) + omega * pr.z;
-        double ri = pr.s + (-omega) * pr.t;
-        r[i] = ri;
-        acc[0] += pr.rs * ri;
-        acc[1] += ri * ri;
-    }
-    __device__ void finish(const Ctx &c, int s, const double (&tot)[2]) const {
-        SysState &S = c.st[s];
-        double rho_new = tot[0];
+        const double rho_new = tot[2] - S.omega * tot[3];
         S.beta = (rho_new / S.rho) * (S.alpha / S.omega);
         S.rho = rho_new;
-        S.rr = tot[1];
-        S.iters += 1;
-        double err = sqrt(tot[1]);
-        if (c.hist && S.iters <= c.hist_stride) c.hist[(size_t)s * c.hist_stride + S.iters - 1] = err;
-        bool stop = false;
-        if (err < S.tol) {
-            S.status = kConverged;
-            stop = true;
-        } else if (!(err == err) || !isfinite(S.beta)) {
-            S.status = kBreakdown;
-            stop = true;
-        } else if (S.iters >= S.max_iter - 1) {
-            // the reference returns MAX_ITER after MAX_ITER - 1 iterations
-            S.status = kMaxIter;
-            S.iters = S.max_iter;
-            stop = true;
-        }
-        if (stop) {
-            S.active = 0;
-            atomicSub(&c.counter[c.nsys], 1);
-        }
     }
 };
 
@@ -712,28 +738,6 @@ struct OpSpmvPlain {  // y = A x (timing of the bare product)
     __device__ Pre pre(int) const { return {}; }
     __device__ double gather(int j) const { return x[j]; }
     __device__ void row(int i, double ax, const Pre &, double (&)[1]) const { y[i] = ax; }
-    __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
-};
-
-struct OpCopyP {  // K1 without a preconditioner: y = p'
-    static constexpr int NRED = 0;
-    struct Pre {
-        double p;
-    };
-    const double *r, *pin, *a;
-    double *pout, *y;
-    double beta, nbw;
-    __device__ bool enabled(const SysState &S) const { return S.active; }
-    __device__ void load(const SysState &S) {
-        beta = S.beta;
-        nbw = -S.beta * S.omega;
-    }
-    __device__ Pre pre(int i) const { return {(beta * pin[i] + r[i]) + nbw * a[i]}; }
-    __device__ double gather(int) const { return 0.0; }
-    __device__ void row(int i, double, const Pre &pr, double (&)[1]) const {
-        pout[i] = pr.p;
-        y[i] = pr.p;
-    }
     __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
 
@@ -1162,13 +1166,13 @@ void BiCGStab::ensure(int n, int nsys, int nunits, int nbuckets) {
         drop_graph();
         state_.resize((size_t)nsys);
         counter_.resize((size_t)nsys + 2 + nbuckets);
-        bpartials_.resize((size_t)2 * std::max(nbuckets, 1));
+        bpartials_.resize((size_t)kMaxRed * std::max(nbuckets, 1));
         nsys_ = nsys;
         nbuckets_ = nbuckets;
     }
     if (nunits != nunits_) {
         drop_graph();
-        partials_.resize((size_t)2 * std::max(nunits, 1));
+        partials_.resize((size_t)kMaxRed * std::max(nunits, 1));
         nunits_ = nunits;
     }
 }
@@ -1182,7 +1186,7 @@ void BiCGStab::compact(const TileMap &tm, int nsys, hipStream_t st) {
     SPAI_CHECK_LAUNCH();
 }
 
-// One iteration, five launches.  grid > 0: walk the first `grid` entries of
+// One iteration, four launches.  grid > 0: walk the first `grid` entries of
 // the active-unit list; grid == 0: every unit.
 void BiCGStab::enqueue_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tm, int parity,
                                  int grid, hipStream_t st) {
@@ -1191,15 +1195,15 @@ void BiCGStab::enqueue_iteration(const OperatorView &A, const OperatorView &M, c
     double *pin = p_[parity], *pout = p_[parity ^ 1];
     if (M.val) {
         if (M.fp32) {
-            OpMapplyP<float> op{r_, pin, a_, pout, y_, 0.0, 0.0};
+            OpMapplyP<float, true> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
             launch_units<true, float>(M, tm, c, op, list, grid, st);
         } else {
-            OpMapplyP<double> op{r_, pin, a_, pout, y_, 0.0, 0.0};
+            OpMapplyP<double, true> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
             launch_units<true, double>(M, tm, c, op, list, grid, st);
         }
     } else {
         // no preconditioner: y = p' through the streaming path
-        OpCopyP op{r_, pin, a_, pout, y_, 0.0, 0.0};
+        OpMapplyP<double, false> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
         launch_units<false, double>(A, tm, c, op, list, grid, st);
     }
     {
@@ -1219,12 +1223,8 @@ void BiCGStab::enqueue_iteration(const OperatorView &A, const OperatorView &M, c
         launch_units<false, double>(A, tm, c, op, list, grid, st);
     }
     {
-        OpSpmvOmega op{z_, s_, t_};
+        OpSpmvOmega op{z_, s_, rs_, t_};
         launch_units<true, double>(A, tm, c, op, list, grid, st);
-    }
-    {
-        OpUpdate op{x_, r_, y_, z_, s_, t_, rs_, 0.0, 0.0};
-        launch_units<false, double>(A, tm, c, op, list, grid, st);
     }
     SPAI_CHECK_LAUNCH();
 }
@@ -1254,7 +1254,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     hipLaunchKernelGGL(k_state_init, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys, opt.max_iter,
                        counter_.get(), tm.nbuckets);
     {
-        OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, opt.rel_tol, opt.abs_tol, 0};
+        OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, s_, t_, y_, z_, opt.rel_tol, opt.abs_tol, 0};
         launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
     }
     SPAI_CHECK_LAUNCH();
@@ -1273,7 +1273,8 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
         active = h_counts[0];
         units = use_list ? h_counts[1] : tm.nunits;
     };
-    const int max_launch_iters = opt.max_iter;  // every system stops by then
+    // every system stops by then: the K1 of iteration k finishes iteration k - 1
+    const int max_launch_iters = opt.max_iter + 1;
     int done_iters = 0;
     try {
         auto iterate = [&]() {
@@ -1333,7 +1334,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
             SPAI_CHECK_LAUNCH();
             read_counts();
             if (active == 0) break;
-            OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, opt.rel_tol, opt.abs_tol, 1};
+            OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, s_, t_, y_, z_, opt.rel_tol, opt.abs_tol, 1};
             launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
             SPAI_CHECK_LAUNCH();
             done_iters = 0;
@@ -1411,10 +1412,10 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
         } break;
         case 1: {
             if (M.fp32) {
-                OpMapplyP<float> op{r_, p_[0], a_, p_[1], y_, 0.0, 0.0};
+                OpMapplyP<float, true> op{x_, r_, p_[1], y_, s_, t_, p_[0], a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
                 launch_units<true, float>(M, tm, c, op, nullptr, 0, st);
             } else {
-                OpMapplyP<double> op{r_, p_[0], a_, p_[1], y_, 0.0, 0.0};
+                OpMapplyP<double, true> op{x_, r_, p_[1], y_, s_, t_, p_[0], a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
                 launch_units<true, double>(M, tm, c, op, nullptr, 0, st);
             }
         } break;
@@ -1432,7 +1433,7 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
             }
         } break;
         case 4: {
-            OpSpmvOmega op{z_, s_, t_};
+            OpSpmvOmega op{z_, s_, rs_, t_};
             launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
         } break;
         }
@@ -1451,7 +1452,7 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
 }
 
 void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tm, int n, int nsys,
-                              int reps, hipStream_t st, float ms[5]) {
+                              int reps, hipStream_t st, float ms[4]) {
     require(A.val && M.val, "time_iteration: needs A and M");
     require(reps >= 1, "time_iteration: reps must be >= 1");
     ensure(n, nsys, tm.nunits, tm.nbuckets);
@@ -1462,11 +1463,11 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
     hipLaunchKernelGGL(k_fill_double, dim3(ceil_div(n, 256)), dim3(256), 0, st, tb, 1.0, (long long)n);
     hipLaunchKernelGGL(k_fill_double, dim3(ceil_div(n, 256)), dim3(256), 0, st, tx, 0.0, (long long)n);
     // no threshold is reached: every system stays active for the timed iterations
-    hipLaunchKernelGGL(k_state_init, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys, reps + 2,
+    hipLaunchKernelGGL(k_state_init, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys, reps + 3,
                        counter_.get(), tm.nbuckets);
     Ctx c = make_ctx(tm, nsys);
     {
-        OpInit op{tx, tb, r_, rs_, p_[0], p_[1], a_, 0.0, 1e-300, 0};
+        OpInit op{tx, tb, r_, rs_, p_[0], p_[1], a_, s_, t_, y_, z_, 0.0, 1e-300, 0};
         launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
     }
     SPAI_CHECK_LAUNCH();
@@ -1474,26 +1475,26 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
     double *x_keep = x_;
     b_ = tb;
     x_ = tx;
-    std::vector<hipEvent_t> ev((size_t)reps * 5 + 1);
+    std::vector<hipEvent_t> ev((size_t)reps * 4 + 1);
     for (auto &e : ev) SPAI_HIP(hipEventCreate(&e));
     try {
-        // the five kernels of enqueue_iteration, each bracketed by events
+        // the four kernels of enqueue_iteration, each bracketed by events
         for (int it = 0; it < reps; ++it) {
             double *pin = p_[it & 1], *pout = p_[(it & 1) ^ 1];
-            SPAI_HIP(hipEventRecord(ev[(size_t)it * 5], st));
+            SPAI_HIP(hipEventRecord(ev[(size_t)it * 4], st));
             if (M.fp32) {
-                OpMapplyP<float> op{r_, pin, a_, pout, y_, 0.0, 0.0};
+                OpMapplyP<float, true> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
                 launch_units<true, float>(M, tm, c, op, nullptr, 0, st);
             } else {
-                OpMapplyP<double> op{r_, pin, a_, pout, y_, 0.0, 0.0};
+                OpMapplyP<double, true> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
                 launch_units<true, double>(M, tm, c, op, nullptr, 0, st);
             }
-            SPAI_HIP(hipEventRecord(ev[(size_t)it * 5 + 1], st));
+            SPAI_HIP(hipEventRecord(ev[(size_t)it * 4 + 1], st));
             {
                 OpSpmvAlpha op{y_, rs_, a_};
                 launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
             }
-            SPAI_HIP(hipEventRecord(ev[(size_t)it * 5 + 2], st));
+            SPAI_HIP(hipEventRecord(ev[(size_t)it * 4 + 2], st));
             if (M.fp32) {
                 OpMapplyS<float> op{r_, a_, s_, z_, 0.0};
                 launch_units<true, float>(M, tm, c, op, nullptr, 0, st);
@@ -1501,25 +1502,20 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
                 OpMapplyS<double> op{r_, a_, s_, z_, 0.0};
                 launch_units<true, double>(M, tm, c, op, nullptr, 0, st);
             }
-            SPAI_HIP(hipEventRecord(ev[(size_t)it * 5 + 3], st));
+            SPAI_HIP(hipEventRecord(ev[(size_t)it * 4 + 3], st));
             {
-                OpSpmvOmega op{z_, s_, t_};
+                OpSpmvOmega op{z_, s_, rs_, t_};
                 launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
-            }
-            SPAI_HIP(hipEventRecord(ev[(size_t)it * 5 + 4], st));
-            {
-                OpUpdate op{x_, r_, y_, z_, s_, t_, rs_, 0.0, 0.0};
-                launch_units<false, double>(A, tm, c, op, nullptr, 0, st);
             }
         }
         SPAI_HIP(hipEventRecord(ev.back(), st));
         SPAI_HIP(hipEventSynchronize(ev.back()));
         SPAI_CHECK_LAUNCH();
-        for (int k = 0; k < 5; ++k) ms[k] = 0.0f;
+        for (int k = 0; k < 4; ++k) ms[k] = 0.0f;
         for (int it = 0; it < reps; ++it)
-            for (int k = 0; k < 5; ++k) {
+            for (int k = 0; k < 4; ++k) {
                 float t = 0.0f;
-                SPAI_HIP(hipEventElapsedTime(&t, ev[(size_t)it * 5 + k], ev[(size_t)it * 5 + k + 1]));
+                SPAI_HIP(hipEventElapsedTime(&t, ev[(size_t)it * 4 + k], ev[(size_t)it * 4 + k + 1]));
                 ms[k] += t / reps;
             }
     } catch (...) {

@@ -25,6 +25,7 @@ struct alignas(64) SysState {
     int status;      // SolveStatus
     int max_iter;
     int restarts;    // restarts from the true residual (true_residual_restart)
+    int pending;     // 1 once an iteration's x/r update is left for the next K1
 };
 
 enum SolveStatus { kRunning = 0, kConverged = 1, kMaxIter = 2, kBreakdown = 3, kConvergedAtStart = 4 };
@@ -158,11 +159,11 @@ public:
     // with the vectors and scalars of the last solve, every system active).
     float time_kernel(int which, const OperatorView &A, const OperatorView &M, const TileMap &tiles, int n, int nsys,
                       int reps, hipStream_t st);
-    // Average device time of each of K1..K5 over `reps` consecutive
+    // Average device time of each of K1..K4 over `reps` consecutive
     // iterations from x0 = 0, b = 1 with every system active (no threshold is
     // reached), HIP events between the kernels on the solver's stream.
     void time_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tiles, int n, int nsys,
-                        int reps, hipStream_t st, float ms[5]);
+                        int reps, hipStream_t st, float ms[4]);
 
 private:
     void ensure(int n, int nsys, int nunits, int nbuckets);

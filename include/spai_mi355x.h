@@ -175,17 +175,17 @@ int spai_system_residual_history(spai_system *sys, int s, double *out, int maxle
 int spai_system_adjoint(spai_system *sys, const double *x, const double *g, double *dA, double *db, int on_device,
                         const spai_solve_options *opt, spai_solve_report *report);
 
-/* Average device time (ms) of one launch of a solver kernel over `reps`
- * launches on the context's stream: which = 0 bare y = A x, 5 bare y = M x,
- * 1..4 the fused iteration kernels (1: y = M p', 2: a = A y + (r*, a),
- * 3: z = M s, 4: t = A z + (t, s), (t, t)) on the vectors and scalars of the
- * last solve. */
+/* Average device time (ms) of one launch of a solver kernel repeated `reps`
+ * times on the context's stream: which = 0 bare y = A x, 5 bare y = M x,
+ * 1..4 the iteration kernels K1..K4 (see spai_system_time_iteration) on the
+ * vectors and scalars of the last solve. */
 int spai_system_time_spmv(spai_system *sys, int which, int reps, float *ms);
 
-/* Average device time (ms) of each iteration kernel K1..K5 (ms[0..4]) over
- * `reps` consecutive iterations from x0 = 0, b = 1, every system active (no
- * threshold is reached), with HIP events between the kernels on the
- * context's stream: the per-kernel times of the real iteration sequence.
+/* Average device time (ms) of each iteration kernel K1..K4 (ms[0..3]: K1
+ * x/r update + y = M p', K2 a = A y + alpha, K3 z = M s, K4 t = A z + omega,
+ * beta) over `reps` consecutive iterations from x0 = 0, b = 1, every system
+ * active (no threshold is reached), with HIP events between the kernels on
+ * the context's stream: the per-kernel times of the real iteration sequence.
  * Overwrites the solver's work vectors; M must be built. */
 int spai_system_time_iteration(spai_system *sys, int reps, float *ms);
 
