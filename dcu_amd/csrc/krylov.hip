@@ -320,9 +320,10 @@ constexpr int kRowsPerThread = kRowsPerTile / kThreads;
 
 // Row tiles.  For every row i of the unit's tile: y_i = sum_p val[p] *
 // op.gather(idx[p]) (stored as products in LDS, then summed per row in
-// storage order), then op.row(i, y_i, pre_i, acc) where pre_i = op.pre(i) are
-// the row's own vector operands, loaded before the product so that they
-// overlap with it.  One tile needs two dependent global round trips: the
+// storage order), then op.row(i, y_i, pre_i, acc) where pre_i = op.pre(i, acc)
+// is the row's own work that does not need the product (operand loads, and
+// outputs that can be stored at once), done before the product so that it
+// overlaps with it and holds few registers across the gathers.  One tile needs two dependent global round trips: the
 // index/value/row-pointer/operand loads, then the gathers.  HAS_MATRIX =
 // false skips the product (streaming ops).
 template <bool HAS_MATRIX, class MT, class Op>
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
     const int r0 = uv.row[unit], r1 = uv.row[unit + 1];
     const int nrows = r1 - r0;
     if (!HAS_MATRIX) {
-        for (int i = r0 + tid; i < r1; i += kThreads) op.row(i, 0.0, op.pre(i), acc);
+        for (int i = r0 + tid; i < r1; i += kThreads) op.row(i, 0.0, op.pre(i, acc), acc);
     } else {
         const int p0 = ptr[r0];
         const int cnt = ptr[r1] - p0;
@@ -353,7 +354,7 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
 #pragma unroll
             for (int k = 0; k < kRowsPerThread; ++k) {
                 const int i = r0 + tid + k * kThreads;
-                if (i < r1) pre[k] = op.pre(i);
+                if (i < r1) pre[k] = op.pre(i, acc);
             }
             for (int k = tid; k <= nrows; k += kThreads) rp[k] = ptr[r0 + k] - p0;
             constexpr int U = kLdsProducts / kThreads;
@@ -395,7 +396,7 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
                 double part[1] = {0.0};
                 for (int q = b + tid; q < e; q += kThreads) part[0] += (double)val[q] * op.gather(idx[q]);
                 block_reduce<1>(part, sh);
-                if (tid == 0) op.row(i, part[0], op.pre(i), acc);
+                if (tid == 0) op.row(i, part[0], op.pre(i, acc), acc);
             }
         }
     }
@@ -409,7 +410,10 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
 // and each row is summed in storage order in a register.  The first columns
 // are requested before the system's state is read, so that the state load
 // overlaps the matrix stream.  No LDS and no barrier except the reduction's.
-constexpr int kSellUnroll = 8;
+#ifndef SPAI_SELL_UNROLL
+#define SPAI_SELL_UNROLL 8
+#endif
+constexpr int kSellUnroll = SPAI_SELL_UNROLL;  // columns in flight per slice and lane
 // optional occupancy floor (waves per SIMD) for the register allocator
 #ifndef SPAI_SELL_MIN_WAVES
 #define SPAI_SELL_MIN_WAVES 0
@@ -467,11 +471,11 @@ __global__ __launch_bounds__(kThreads) SPAI_SELL_OCCUPANCY void k_sell(const int
     if (!HAS_MATRIX) {
 #pragma unroll
         for (int j = 0; j < SPW; ++j)
-            if (live[j]) op.row(row[j], 0.0, op.pre(row[j]), acc);
+            if (live[j]) op.row(row[j], 0.0, op.pre(row[j], acc), acc);
     } else {
         typename Op::Pre pre[SPW];
 #pragma unroll
-        for (int j = 0; j < SPW; ++j) pre[j] = live[j] ? op.pre(row[j]) : typename Op::Pre{};
+        for (int j = 0; j < SPW; ++j) pre[j] = live[j] ? op.pre(row[j], acc) : typename Op::Pre{};
         double sum[SPW];
         int wmax = 0;
 #pragma unroll
@@ -550,7 +554,7 @@ struct OpInit {
     int restart;  // 1: restart of an active system from its current x (iterations and tol kept)
     __device__ bool enabled(const SysState &S) const { return !restart || S.active; }
     __device__ void load(const SysState &) {}
-    __device__ Pre pre(int i) const { return {b[i]}; }
+    __device__ Pre pre(int i, double (&)[2]) const { return {b[i]}; }
     __device__ double gather(int j) const { return x[j]; }
     __device__ void row(int i, double ax, const Pre &pr, double (&acc)[2]) const {
         double ri = -ax + pr.b;
@@ -604,7 +608,7 @@ template <class MT, bool PRECOND>
 struct OpMapplyP {
     static constexpr int NRED = 1;
     struct Pre {
-        double x, y, z, r, p;
+        double p;
     };
     double *x, *r, *pout, *y;
     const double *s, *t, *pin, *a, *z;
@@ -619,18 +623,19 @@ struct OpMapplyP {
     }
     __device__ double rval(int j) const { return s[j] + nomega * t[j]; }
     __device__ double pval(int j, double rj) const { return (beta * pin[j] + rj) + nbw * a[j]; }
-    __device__ Pre pre(int i) const {
+    // the row's x, r and p' are stored before the product (no other row's
+    // gather reads x, r or p')
+    __device__ Pre pre(int i, double (&acc)[1]) const {
         const double ri = rval(i);
-        return {x[i], y[i], z[i], ri, pval(i, ri)};
+        const double pi = pval(i, ri);
+        x[i] = (x[i] + alpha * y[i]) + omega * z[i];
+        r[i] = ri;
+        pout[i] = pi;
+        acc[0] += ri * ri;
+        return {pi};
     }
     __device__ double gather(int j) const { return pval(j, rval(j)); }
-    __device__ void row(int i, double mx, const Pre &pr, double (&acc)[1]) const {
-        x[i] = (pr.x + alpha * pr.y) + omega * pr.z;
-        r[i] = pr.r;
-        pout[i] = pr.p;
-        y[i] = PRECOND ? mx : pr.p;
-        acc[0] += pr.r * pr.r;
-    }
+    __device__ void row(int i, double mx, const Pre &pr, double (&)[1]) const { y[i] = PRECOND ? mx : pr.p; }
     __device__ void finish(const Ctx &c, int s_, const double (&tot)[1]) const {
         SysState &S = c.st[s_];
         if (!S.pending) {  // first K1 after OpInit: nothing to finish
@@ -651,7 +656,7 @@ struct OpSpmvAlpha {  // K2: a = A y, alpha = rho / (r*, a)
     double *a;
     __device__ bool enabled(const SysState &S) const { return S.active; }
     __device__ void load(const SysState &) {}
-    __device__ Pre pre(int i) const { return {rs[i]}; }
+    __device__ Pre pre(int i, double (&)[1]) const { return {rs[i]}; }
     __device__ double gather(int j) const { return y[j]; }
     __device__ void row(int i, double ax, const Pre &pr, double (&acc)[1]) const {
         a[i] = ax;
@@ -664,23 +669,21 @@ struct OpSpmvAlpha {  // K2: a = A y, alpha = rho / (r*, a)
 };
 
 template <class MT>
-struct OpMapplyS {  // K3: z = M s, s = r - alpha a
+struct OpMapplyS {  // K3: z = M s, s = r - alpha a (stored before the product; gathers read r and a)
     static constexpr int NRED = 0;
-    struct Pre {
-        double s;
-    };
+    struct Pre {};
     const double *r, *a;
     double *s, *z;
     double nalpha;
     __device__ bool enabled(const SysState &S) const { return S.active; }
     __device__ void load(const SysState &S) { nalpha = -S.alpha; }
     __device__ double sval(int j) const { return r[j] + nalpha * a[j]; }
-    __device__ Pre pre(int i) const { return {sval(i)}; }
-    __device__ double gather(int j) const { return sval(j); }
-    __device__ void row(int i, double mx, const Pre &pr, double (&)[1]) const {
-        z[i] = mx;
-        s[i] = pr.s;
+    __device__ Pre pre(int i, double (&)[1]) const {
+        s[i] = sval(i);
+        return {};
     }
+    __device__ double gather(int j) const { return sval(j); }
+    __device__ void row(int i, double mx, const Pre &, double (&)[1]) const { z[i] = mx; }
     __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
 
@@ -693,7 +696,7 @@ struct OpSpmvOmega {  // K4: t = A z; omega, rho' = (r*, s) - omega (r*, t), bet
     double *t;
     __device__ bool enabled(const SysState &S) const { return S.active; }
     __device__ void load(const SysState &) {}
-    __device__ Pre pre(int i) const { return {s[i], rs[i]}; }
+    __device__ Pre pre(int i, double (&)[4]) const { return {s[i], rs[i]}; }
     __device__ double gather(int j) const { return z[j]; }
     __device__ void row(int i, double ax, const Pre &pr, double (&acc)[4]) const {
         t[i] = ax;
@@ -719,7 +722,7 @@ struct OpTrueResidual {  // ||b - A x||^2 per system
     const double *x, *b;
     __device__ bool enabled(const SysState &) const { return true; }
     __device__ void load(const SysState &) {}
-    __device__ Pre pre(int i) const { return {b[i]}; }
+    __device__ Pre pre(int i, double (&)[1]) const { return {b[i]}; }
     __device__ double gather(int j) const { return x[j]; }
     __device__ void row(int, double ax, const Pre &pr, double (&acc)[1]) const {
         double d = pr.b - ax;
@@ -735,7 +738,7 @@ struct OpSpmvPlain {  // y = A x (timing of the bare product)
     double *y;
     __device__ bool enabled(const SysState &) const { return true; }
     __device__ void load(const SysState &) {}
-    __device__ Pre pre(int) const { return {}; }
+    __device__ Pre pre(int, double (&)[1]) const { return {}; }
     __device__ double gather(int j) const { return x[j]; }
     __device__ void row(int i, double ax, const Pre &, double (&)[1]) const { y[i] = ax; }
     __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
@@ -751,12 +754,13 @@ struct OpCopyS {  // K3 without a preconditioner: z = s
     double nalpha;
     __device__ bool enabled(const SysState &S) const { return S.active; }
     __device__ void load(const SysState &S) { nalpha = -S.alpha; }
-    __device__ Pre pre(int i) const { return {r[i] + nalpha * a[i]}; }
-    __device__ double gather(int) const { return 0.0; }
-    __device__ void row(int i, double, const Pre &pr, double (&)[1]) const {
-        s[i] = pr.s;
-        z[i] = pr.s;
+    __device__ Pre pre(int i, double (&)[1]) const {
+        const double si = r[i] + nalpha * a[i];
+        s[i] = si;
+        return {si};
     }
+    __device__ double gather(int) const { return 0.0; }
+    __device__ void row(int i, double, const Pre &pr, double (&)[1]) const { z[i] = pr.s; }
     __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
 

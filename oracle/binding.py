@@ -43,6 +43,8 @@ def lib() -> ctypes.CDLL:
         L.oracle_num_threads.restype = i
         L.oracle_set_threads.argtypes = [i]
         L.oracle_set_threads.restype = None
+        L.oracle_set_rho_mode.argtypes = [i]
+        L.oracle_set_rho_mode.restype = None
         _lib = L
     return _lib
 
@@ -80,6 +82,11 @@ def num_threads() -> int:
 
 def set_threads(t: int) -> None:
     lib().oracle_set_threads(int(t))
+
+
+def set_rho_mode(mode: int) -> None:
+    """0: rho' = (r*, r) as the reference; 1: (r*, s) - omega (r*, t)."""
+    lib().oracle_set_rho_mode(int(mode))
 
 
 def iteration_spread(a_csr, m_csr, b, tol: float, max_iter: int, threads=(1, 2, 3, 4, 5, 6, 7, 8, 12, 16)):

@@ -316,6 +316,14 @@ static double dot(int n, const double *a, const double *b) {
     return s;
 }
 
+/* rho' of the BiCGSTAB loop: 0 (default) (r*, r) after the r update, as
+ * bicgstab_solver.h:196-199; 1 by linearity, (r*, s) - omega (r*, A z), the
+ * rearrangement the GPU solver uses so that beta is known before its next
+ * M-apply.  Mode 1 only serves to compare the two roundings' iteration-count
+ * distributions in the tests. */
+static int g_rho_mode = 0;
+void oracle_set_rho_mode(int mode) { g_rho_mode = mode; }
+
 /* Right-preconditioned BiCGSTAB, restating bicgstab_solver.h:106-232.
  *   x      : in = x0, out = solution
  *   tol    : absolute threshold on the recursive residual ||r||_2 (strict <)
@@ -369,12 +377,17 @@ int oracle_bicgstab_history(int n, const int *arp, const int *aci, const double 
         spmv(n, arp, aci, av, z, az);
         double ts = dot(n, az, s), tt = dot(n, az, az);
         double w = ts / tt;
+        double rs_s = 0.0, rs_t = 0.0;
+        if (g_rho_mode == 1) {
+            rs_s = dot(n, rs, s);
+            rs_t = dot(n, rs, az);
+        }
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < n; ++i) {
             x[i] += alpha * y[i] + w * z[i];
             r[i] = s[i] + (-w) * az[i];
         }
-        rho = dot(n, rs, r);
+        rho = g_rho_mode == 1 ? rs_s - w * rs_t : dot(n, rs, r);
         double beta = (rho / rho_old) * (alpha / w);
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < n; ++i) p[i] = (beta * p[i] + r[i]) + (-beta * w) * ay[i]; /* scal, axpy, axpy */

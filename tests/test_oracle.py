@@ -162,6 +162,48 @@ def test_bicgstab_iterates_match_scipy_golden():
             assert np.linalg.norm(x - xs_np[k - 1]) <= 1e-10 * np.linalg.norm(xs_np[k - 1]), (name, k)
 
 
+def test_bicgstab_rho_by_linearity_matches_scipy_golden():
+    # rho' = (r*, s) - omega (r*, t) (the GPU's form) is (r*, r) by linearity:
+    # the same scipy iterates, to rounding
+    try:
+        O.set_rho_mode(1)
+        for name, n, cp, ri, va, m, b, xs, _ in _golden_systems():
+            A = O.csc2csr(n, cp, ri, va)[:3]
+            M = O.csc2csr(n, cp, ri, m)[:3]
+            for k in range(1, 7):
+                x, it, info = O.bicgstab(A, M, b, np.zeros(n), 0.0, k + 1)
+                assert np.linalg.norm(x - xs[k - 1]) <= 1e-12 * np.linalg.norm(xs[k - 1]), (name, k)
+        # a planted slip in that form (+ omega instead of - omega) is caught
+        name, n, cp, ri, va, m, b, xs, _ = next(_golden_systems())
+        A = sp.csc_matrix((va, ri, cp), shape=(n, n)).tocsr()
+        M = sp.csc_matrix((m, ri, cp), shape=(n, n)).tocsr()
+        r = b.copy()
+        rs, p = r.copy(), r.copy()
+        y = M @ p
+        ay = A @ y
+        alpha = (rs @ r) / (rs @ ay)
+        s = r - alpha * ay
+        z = M @ s
+        t = A @ z
+        omega = (t @ s) / (t @ t)
+        x = alpha * y + omega * z
+        r = s - omega * t
+        rho_wrong = rs @ s + omega * (rs @ t)
+        beta = (rho_wrong / (rs @ rs)) * (alpha / omega)
+        p = r + beta * (p - omega * ay)
+        y = M @ p
+        ay = A @ y
+        alpha = rho_wrong / (rs @ ay)
+        s = r - alpha * ay
+        z = M @ s
+        t = A @ z
+        omega = (t @ s) / (t @ t)
+        x = x + alpha * y + omega * z
+        assert np.linalg.norm(x - xs[1]) > 1e-6 * np.linalg.norm(xs[1])
+    finally:
+        O.set_rho_mode(0)
+
+
 def test_bicgstab_exact_inverse_one_iteration():
     rng = np.random.default_rng(2)
     Ad = rng.standard_normal((20, 20)) + 20 * np.eye(20)
