@@ -40,21 +40,20 @@
 
 namespace spai {
 
-// Reductions are counted in two levels so that no counter sees more than
-// kBucket atomic increments per launch (same-address atomics serialise at
-// the memory side): units -> buckets of up to kBucket consecutive units of
-// one system -> the system.
+// Reductions run up a tree per system: level 0 holds one partial per unit,
+// each further level one partial per group of up to kFanIn consecutive nodes
+// of the level below, and the root's value goes to the op's finish step.
+// Same-address atomics serialise at the memory side, so a counter that sees
+// few arrivals (kFanIn) finishes sooner than one shared by many units.
 struct Ctx {
     SysState *st;
-    const int *unit_bucket;  // bucket of each unit
-    const int *bucket_unit;  // nbuckets + 1: first unit of each bucket
-    const int *sys_buckets;  // nsys + 1: first bucket of each system
-    int *counter;            // [0, nsys) systems, [nsys] active systems, [nsys + 1] active units,
-                             // [nsys + 2 + b] buckets
-    double *partials;        // nunits * kMaxRed unit partials
-    double *bpartials;       // nbuckets * kMaxRed bucket partials
+    const int *sys_units;  // nsys + 1: first unit of each system
+    const int *sys_nodes;  // nsys + 1: first tree node of each system
+    int *counter;          // [0, nsys) unused, [nsys] active systems, [nsys + 1] active units,
+                           // [nsys + 2 + node] node arrivals
+    double *partials;      // kMaxRed per tree node
     int nsys;
-    double *hist;            // optional ||r_k|| history, hist_stride entries per system
+    double *hist;          // optional ||r_k|| history, hist_stride entries per system
     int hist_stride;
 };
 
@@ -62,8 +61,11 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kLdsProducts = 2048;  // products staged per tile (16 KB)
-constexpr int kBucket = 32;         // units per first-level reduction counter
-constexpr int kMaxRed = 4;          // most values any kernel reduces (partials per unit and bucket)
+#ifndef SPAI_REDUCE_FANIN
+#define SPAI_REDUCE_FANIN 8
+#endif
+constexpr int kFanIn = SPAI_REDUCE_FANIN;  // children per reduction tree node
+constexpr int kMaxRed = 4;          // most values any kernel reduces (partials per tree node)
 
 // ------------------------------------------------------------ transpose ----
 __global__ void k_fill_int(int *p, int v, long long n) {
@@ -208,9 +210,8 @@ __device__ __forceinline__ void block_reduce(double (&v)[NRED], double *sh) {
     __syncthreads();
 }
 
-// Publishes this group's partials; the last group of a bucket sums the
-// bucket, the last bucket of the system sums the buckets (both in a fixed
-// order) and calls op.finish on its thread 0.
+// Cross-workgroup reductions (reduce_system below) publish partials and
+// count arrivals per tree node; the last arrival combines them.
 //
 // No __threadfence(): on gfx950 an agent-scope fence writes back and
 // invalidates the whole L2 (buffer_wbl2 / buffer_inv), which, issued by
@@ -244,25 +245,28 @@ __device__ __forceinline__ bool publish_and_count(double *dst, const double (&v)
     return __shfl(last, 0, 64) != 0;
 }
 
-// Sum of src[t * NRED + q] over t in [first, last), by one wavefront, in a
-// fixed order (lane-strided partial sums, then a butterfly).
+// Sum of src[t * kMaxRed + q] over t in [first, last), by one wavefront, in
+// a fixed order (lane-strided partial sums, then a butterfly).
 template <int NRED>
 __device__ __forceinline__ void wave_sum_range(const double *src, int first, int last, double (&acc)[NRED]) {
 #pragma unroll
     for (int q = 0; q < NRED; ++q) acc[q] = 0.0;
     for (int t = first + (int)(threadIdx.x & 63); t < last; t += 64) {
 #pragma unroll
-        for (int q = 0; q < NRED; ++q) acc[q] += load_coherent(&src[(size_t)t * NRED + q]);
+        for (int q = 0; q < NRED; ++q) acc[q] += load_coherent(&src[(size_t)t * kMaxRed + q]);
     }
     wave_sum<NRED>(acc);
 }
 
-// Per-unit partials -> bucket -> system, then op.finish.  After the
-// workgroup's partials are combined in LDS, only wavefront 0 stays: the
-// others retire at once instead of waiting on the publish round trips.
+// Per-unit partials up the system's reduction tree, then op.finish.  After
+// the workgroup's partials are combined in LDS only wavefront 0 stays: the
+// others retire at once instead of waiting on the publish round trips.  The
+// last arrival at each node sums that node's children in a fixed order, so
+// the result does not depend on arrival order.  Node numbering follows from
+// the system's unit count alone: level l starts after the levels below it.
 template <int NRED, class Op>
 __device__ __forceinline__ void reduce_system(const Ctx &c, int unit, int s, double (&v)[NRED], Op &op, double *sh) {
-    static_assert(NRED >= 1 && NRED <= kMaxRed, "partials hold at most kMaxRed values per unit and bucket");
+    static_assert(NRED >= 1 && NRED <= kMaxRed, "partials hold at most kMaxRed values per tree node");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int W = kThreads / 64;
     wave_sum<NRED>(v);
@@ -272,23 +276,30 @@ __device__ __forceinline__ void reduce_system(const Ctx &c, int unit, int s, dou
     }
     __syncthreads();
     if (wave != 0) return;
-    double u[NRED];
+    double val[NRED];
 #pragma unroll
     for (int q = 0; q < NRED; ++q) {
-        u[q] = 0.0;
+        val[q] = 0.0;
 #pragma unroll
-        for (int w = 0; w < W; ++w) u[q] += sh[q * W + w];
+        for (int w = 0; w < W; ++w) val[q] += sh[q * W + w];
     }
-    const int b = c.unit_bucket[unit];
-    const int u0 = c.bucket_unit[b], u1 = c.bucket_unit[b + 1];
-    if (!publish_and_count<NRED>(&c.partials[(size_t)unit * NRED], u, &c.counter[c.nsys + 2 + b], u1 - u0)) return;
-    double bsum[NRED];
-    wave_sum_range<NRED>(c.partials, u0, u1, bsum);
-    const int b0 = c.sys_buckets[s], b1 = c.sys_buckets[s + 1];
-    if (!publish_and_count<NRED>(&c.bpartials[(size_t)b * NRED], bsum, &c.counter[s], b1 - b0)) return;
-    double tot[NRED];
-    wave_sum_range<NRED>(c.bpartials, b0, b1, tot);
-    if (lane == 0) op.finish(c, s, tot);
+    const int u0 = c.sys_units[s];
+    int n = c.sys_units[s + 1] - u0;  // nodes in the current level
+    int idx = unit - u0;              // this node's index in its level
+    int node = c.sys_nodes[s];        // first node of the current level
+    while (n > 1) {
+        const int parent = idx / kFanIn;
+        const int c0 = parent * kFanIn, c1 = min(c0 + kFanIn, n);
+        const int next = node + n;  // first node of the parent level
+        if (!publish_and_count<NRED>(&c.partials[(size_t)(node + idx) * kMaxRed], val,
+                                     &c.counter[c.nsys + 2 + next + parent], c1 - c0))
+            return;
+        wave_sum_range<NRED>(c.partials + (size_t)node * kMaxRed, c0, c1, val);
+        node = next;
+        idx = parent;
+        n = (n + kFanIn - 1) / kFanIn;
+    }
+    if (lane == 0) op.finish(c, s, val);
 }
 
 // ------------------------------------------------------- unit kernels ----
@@ -764,7 +775,7 @@ struct OpCopyS {  // K3 without a preconditioner: z = s
     __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
 
-__global__ void k_state_init(SysState *st, int nsys, int max_iter, int *counter, int nbuckets) {
+__global__ void k_state_init(SysState *st, int nsys, int max_iter, int *counter, int nnodes) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < nsys) {
         SysState z{};
@@ -778,7 +789,7 @@ __global__ void k_state_init(SysState *st, int nsys, int max_iter, int *counter,
         counter[nsys] = nsys;
         counter[nsys + 1] = 0;
     }
-    for (int b = s; b < nbuckets; b += gridDim.x * blockDim.x) counter[nsys + 2 + b] = 0;
+    for (int b = s; b < nnodes; b += gridDim.x * blockDim.x) counter[nsys + 2 + b] = 0;
 }
 
 // systems whose recursive residual converged but whose true residual did not
@@ -796,13 +807,13 @@ __global__ void k_check_true(SysState *st, int nsys, int *counter) {
 }
 
 // re-enables every system (scalars kept) so single kernels can be timed
-__global__ void k_state_activate(SysState *st, int nsys, int *counter, int nbuckets) {
+__global__ void k_state_activate(SysState *st, int nsys, int *counter, int nnodes) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < nsys) {
         st[s].active = 1;
         counter[s] = 0;
     }
-    for (int b = s; b < nbuckets; b += gridDim.x * blockDim.x) counter[nsys + 2 + b] = 0;
+    for (int b = s; b < nnodes; b += gridDim.x * blockDim.x) counter[nsys + 2 + b] = 0;
 }
 
 // Active-unit list, two launches.  k_compact_scan (one workgroup): base[s] =
@@ -1045,19 +1056,18 @@ void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector
     }
     urow.push_back(n);
     tm.nunits = (int)usys.size();
-    // buckets of up to kBucket consecutive units of one system
-    std::vector<int> ubucket((size_t)tm.nunits), bunit, sbuckets((size_t)nsys + 1, 0);
+    // reduction tree per system: its units, then ceil(n / kFanIn) nodes per
+    // level until one remains
+    std::vector<int> snodes((size_t)nsys + 1, 0);
     for (int s = 0; s < nsys; ++s) {
-        const int u0 = tm.h_sys_units[(size_t)s], u1 = tm.h_sys_units[(size_t)s + 1];
-        for (int u = u0; u < u1; u += kBucket) {
-            const int bidx = (int)bunit.size();
-            bunit.push_back(u);
-            for (int h = u; h < std::min(u + kBucket, u1); ++h) ubucket[(size_t)h] = bidx;
+        int n = tm.h_sys_units[(size_t)s + 1] - tm.h_sys_units[(size_t)s], total = n;
+        while (n > 1) {
+            n = (n + kFanIn - 1) / kFanIn;
+            total += n;
         }
-        sbuckets[(size_t)s + 1] = (int)bunit.size();
+        snodes[(size_t)s + 1] = snodes[(size_t)s] + total;
     }
-    bunit.push_back(tm.nunits);
-    tm.nbuckets = (int)bunit.size() - 1;
+    tm.nnodes = snodes[(size_t)nsys];
     auto up = [&](DeviceArray<int> &d, const std::vector<int> &h) {
         d.resize(std::max<size_t>(h.size(), 1));
         if (!h.empty()) SPAI_HIP(hipMemcpyAsync(d.get(), h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, st));
@@ -1066,9 +1076,7 @@ void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector
     up(tm.unit_sys, usys);
     up(tm.sys_units, tm.h_sys_units);
     up(tm.unit_slice, uslice);
-    up(tm.unit_bucket, ubucket);
-    up(tm.bucket_unit, bunit);
-    up(tm.sys_buckets, sbuckets);
+    up(tm.sys_nodes, snodes);
     tm.active.resize((size_t)std::max(tm.nunits, 1));
     tm.sys_base.resize((size_t)nsys);
     if (sell) {
@@ -1144,12 +1152,11 @@ void BiCGStab::drop_graph() {
 }
 
 Ctx BiCGStab::make_ctx(const TileMap &tm, int nsys) {
-    return Ctx{state_.get(),     tm.unit_bucket.get(), tm.bucket_unit.get(), tm.sys_buckets.get(),
-               counter_.get(),   partials_.get(),      bpartials_.get(),      nsys,
-               hist_ptr_,        hist_stride_};
+    return Ctx{state_.get(), tm.sys_units.get(), tm.sys_nodes.get(), counter_.get(), partials_.get(),
+               nsys,         hist_ptr_,          hist_stride_};
 }
 
-void BiCGStab::ensure(int n, int nsys, int nunits, int nbuckets) {
+void BiCGStab::ensure(int n, int nsys, int nunits, int nnodes) {
     if (n != n_) {
         drop_graph();
         vec_.resize((size_t)9 * std::max(n, 1));
@@ -1166,17 +1173,16 @@ void BiCGStab::ensure(int n, int nsys, int nunits, int nbuckets) {
         t_ = v + 8 * N;
         n_ = n;
     }
-    if (nsys != nsys_ || nbuckets != nbuckets_) {
+    if (nsys != nsys_ || nnodes != nnodes_) {
         drop_graph();
         state_.resize((size_t)nsys);
-        counter_.resize((size_t)nsys + 2 + nbuckets);
-        bpartials_.resize((size_t)kMaxRed * std::max(nbuckets, 1));
+        counter_.resize((size_t)nsys + 2 + nnodes);
+        partials_.resize((size_t)kMaxRed * std::max(nnodes, 1));
         nsys_ = nsys;
-        nbuckets_ = nbuckets;
+        nnodes_ = nnodes;
     }
     if (nunits != nunits_) {
         drop_graph();
-        partials_.resize((size_t)kMaxRed * std::max(nunits, 1));
         nunits_ = nunits;
     }
 }
@@ -1239,7 +1245,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     require(tm.kind == kLayoutSell || A.ptr, "bicgstab: row tiles need a pointer array");
     require(tm.nunits > 0 && (int)tm.h_sys_units.size() == nsys + 1, "bicgstab: layout does not match the systems");
     require(opt.max_iter >= 1, "bicgstab: max_iter must be >= 1");
-    ensure(n, nsys, tm.nunits, tm.nbuckets);
+    ensure(n, nsys, tm.nunits, tm.nnodes);
     b_ = b;
     x_ = x;
     if (opt.record_history) {
@@ -1256,7 +1262,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     SPAI_HIP(hipEventRecord(e0, st));
     Ctx c = make_ctx(tm, nsys);
     hipLaunchKernelGGL(k_state_init, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys, opt.max_iter,
-                       counter_.get(), tm.nbuckets);
+                       counter_.get(), tm.nnodes);
     {
         OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, s_, t_, y_, z_, opt.rel_tol, opt.abs_tol, 0};
         launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
@@ -1396,10 +1402,10 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
                             int nsys, int reps, hipStream_t st) {
     require(which >= 0 && which <= 5, "time_kernel: which must be 0..5");
     require((which != 1 && which != 3 && which != 5) || M.val, "time_kernel: no preconditioner");
-    ensure(n, nsys, tm.nunits, tm.nbuckets);
+    ensure(n, nsys, tm.nunits, tm.nnodes);
     Ctx c = make_ctx(tm, nsys);
     hipLaunchKernelGGL(k_state_activate, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys,
-                       counter_.get(), tm.nbuckets);
+                       counter_.get(), tm.nnodes);
     hipEvent_t e0, e1;
     SPAI_HIP(hipEventCreate(&e0));
     SPAI_HIP(hipEventCreate(&e1));
@@ -1459,7 +1465,7 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
                               int reps, hipStream_t st, float ms[4]) {
     require(A.val && M.val, "time_iteration: needs A and M");
     require(reps >= 1, "time_iteration: reps must be >= 1");
-    ensure(n, nsys, tm.nunits, tm.nbuckets);
+    ensure(n, nsys, tm.nunits, tm.nnodes);
     hist_ptr_ = nullptr;
     hist_stride_ = 0;
     tvec_.reserve((size_t)2 * n);
@@ -1468,7 +1474,7 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
     hipLaunchKernelGGL(k_fill_double, dim3(ceil_div(n, 256)), dim3(256), 0, st, tx, 0.0, (long long)n);
     // no threshold is reached: every system stays active for the timed iterations
     hipLaunchKernelGGL(k_state_init, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys, reps + 3,
-                       counter_.get(), tm.nbuckets);
+                       counter_.get(), tm.nnodes);
     Ctx c = make_ctx(tm, nsys);
     {
         OpInit op{tx, tb, r_, rs_, p_[0], p_[1], a_, s_, t_, y_, z_, 0.0, 1e-300, 0};

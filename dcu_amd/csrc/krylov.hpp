@@ -44,8 +44,8 @@ enum SolveStatus { kRunning = 0, kConverged = 1, kMaxIter = 2, kBreakdown = 3, k
 //    a tile of its own), over a CSR (or, for a transposed operator, CSC)
 //    pointer array.  Used for irregular rows and for operators given as CSR.
 //
-// Reducing kernels publish one partial per unit; units are counted in
-// buckets of up to kBucket consecutive units of one system, then per system.
+// Reducing kernels publish one partial per unit and combine them up a tree
+// per system (each node counts at most kFanIn arrivals).
 // `active` lists the units of systems still iterating (rebuilt on the device
 // from the system states); launches that use it exit at once past its count.
 enum LayoutKind { kLayoutTiles = 0, kLayoutSell = 1 };
@@ -72,11 +72,10 @@ struct TileMap {
     DeviceArray<int> slice_blk;   // nslices + 1 offsets in blocks of 64 entries
     DeviceArray<int> sell_col;    // sell_len column indices
     DeviceArray<int> sell_pos;    // nnz: CSR position -> SELL position
-    // two-level reduction count
-    int nbuckets = 0;
-    DeviceArray<int> unit_bucket;  // bucket of each unit
-    DeviceArray<int> bucket_unit;  // nbuckets + 1: first unit of each bucket
-    DeviceArray<int> sys_buckets;  // nsys + 1: first bucket of each system
+    // reduction tree per system (level 0 = its units, then groups of up to
+    // kFanIn nodes per level): nodes of system s start at sys_nodes[s]
+    int nnodes = 0;
+    DeviceArray<int> sys_nodes;  // nsys + 1
     // active-unit list: [0, nunits) unit ids; compaction scratch per system
     DeviceArray<int> active;
     DeviceArray<int> sys_base;
@@ -166,16 +165,16 @@ public:
                         int reps, hipStream_t st, float ms[4]);
 
 private:
-    void ensure(int n, int nsys, int nunits, int nbuckets);
+    void ensure(int n, int nsys, int nunits, int nnodes);
     void compact(const TileMap &tm, int nsys, hipStream_t st);
     Ctx make_ctx(const TileMap &tm, int nsys);
     void enqueue_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tiles, int parity, int grid,
                            hipStream_t st);
-    int n_ = 0, nsys_ = 0, nunits_ = 0, nbuckets_ = -1;
+    int n_ = 0, nsys_ = 0, nunits_ = 0, nnodes_ = -1;
     DeviceArray<double> vec_;       // x-independent work vectors
-    DeviceArray<double> partials_, bpartials_;
+    DeviceArray<double> partials_;
     DeviceArray<SysState> state_;
-    DeviceArray<int> counter_;      // per system, [nsys] active systems, [nsys + 1] active units, buckets
+    DeviceArray<int> counter_;      // [nsys] active systems, [nsys + 1] active units, [nsys + 2 + node] tree nodes
     DeviceArray<double> tvec_;      // b = 1 and x = 0 of time_iteration
     std::vector<SysState> h_states_;
     double *r_ = nullptr, *rs_ = nullptr, *p_[2] = {nullptr, nullptr}, *y_ = nullptr, *a_ = nullptr, *s_ = nullptr,

@@ -8,8 +8,13 @@ changes.  The GPU computes the reference recurrences with rho' = (r*, r)
 expanded by linearity as (r*, s) - omega (r*, t), another rounding of the same
 quantity.  `oracle_counts` samples the oracle's count over reduction orders
 and over both forms of rho'; `count_ok` accepts a GPU count inside that
-sample's range widened by half its width (+-1 where every sample agrees).
-The deterministic check beside it is that early iterates agree to 1e-8.
+sample's range widened by its width on each side (+-1 where every sample
+agrees).  The GPU's rounding differs in kind as well (fused multiply-adds in
+every product, tree-ordered reductions), and a few dozen CPU samples do not
+bound the tails: on batch system 0 the oracle alone spans 96-113 over 16
+orders and two forms, and GPU runs have given 94-104.  A wrong recurrence
+shows up as non-convergence or a count off by a factor, and above all in the
+deterministic check beside this one: early iterates agree to 1e-8.
 """
 from oracle import binding as O
 
@@ -29,5 +34,5 @@ def oracle_counts(a_csr, m_csr, b, tol, max_iter, threads=ORDERS):
 
 def count_ok(it, counts):
     lo, hi = min(counts), max(counts)
-    w = max(1, (hi - lo) // 2)
+    w = max(1, hi - lo)
     return lo - w <= it <= hi + w
