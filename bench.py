@@ -52,9 +52,6 @@ def cpu_model():
     return platform.processor()
 
 
-KERNELS = ("mapply_p", "spmv_alpha", "mapply_s", "spmv_omega")
-
-
 def kernel_bytes(kind, n, nnz, m_bytes, row_ptr):
     """Algorithmic HBM bytes of one launch of an iteration kernel with every
     system active: a 4 B column index and the value (A: 8 B, M: m_bytes) per
@@ -62,8 +59,12 @@ def kernel_bytes(kind, n, nnz, m_bytes, row_ptr):
     row pointers when the layout is row tiles (row_ptr).  The padding entries
     of the SELL-64 layout are not counted (they are overhead)."""
     rp = 4 * n if row_ptr else 0
-    if kind == "mapply_p":        # x, r update + y = M p'; reads M, x, y, z, s, t, p, a; writes x, r, p', y
+    if kind == "update_mapply_p":  # fused K1: x, r update + y = M p'; reads M, x, y, z, s, t, p, a; writes x, r, p', y
         return (m_bytes + 4) * nnz + rp + 8 * n * 11
+    if kind == "update":          # split K1, streaming half: reads x, y, z, s, t, p, a; writes x, r, p'
+        return 8 * n * 10
+    if kind == "mapply_p":        # split K1, y = M p'; reads M, p' (gather, once); writes y
+        return (m_bytes + 4) * nnz + rp + 8 * n * 2
     if kind == "spmv_alpha":      # a = A y; reads A, y (gather, once), r*; writes a
         return 12 * nnz + rp + 8 * n * 3
     if kind == "mapply_s":        # z = M s; reads M, r, a; writes s, z
@@ -71,6 +72,14 @@ def kernel_bytes(kind, n, nnz, m_bytes, row_ptr):
     if kind == "spmv_omega":      # t = A z; reads A, z, s, r*; writes t
         return 12 * nnz + rp + 8 * n * 4
     raise ValueError(kind)
+
+
+def iteration_kernels(ms5):
+    """Names the five slots of System.time_iteration for the K1 form used."""
+    if ms5[1] > 0.0:  # split K1
+        names = ("update", "mapply_p", "spmv_alpha", "mapply_s", "spmv_omega")
+        return dict(zip(names, ms5))
+    return dict(zip(("update_mapply_p", "spmv_alpha", "mapply_s", "spmv_omega"), [ms5[0]] + ms5[2:]))
 
 
 def main():
@@ -134,14 +143,13 @@ def main():
     row_ptr = layout != "sell64"
     if args.profile_kernels:
         step()
-        ms5 = S.time_iteration(args.profile_reps)
+        kms = iteration_kernels(S.time_iteration(args.profile_reps))
         copy_bytes = 256 * 1024 * 1024
         ctx.calibration_copy(copy_bytes, args.profile_reps)
         m_bytes = 4 if precision == D.PRECISION_MIXED else 8
         print(json.dumps({"profile_kernels": True, "n": n_tot, "nnz": nnz_tot, "layout": layout,
-                          "iteration_kernel_ms": dict(zip(KERNELS, ms5)),
-                          "algorithmic_bytes": {k: kernel_bytes(k, n_tot, nnz_tot, m_bytes, row_ptr)
-                                                for k in KERNELS},
+                          "iteration_kernel_ms": kms,
+                          "algorithmic_bytes": {k: kernel_bytes(k, n_tot, nnz_tot, m_bytes, row_ptr) for k in kms},
                           "copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes}))
         S.close()
         ctx.close()
@@ -169,10 +177,10 @@ def main():
     log("per-system iterations (n): " + ", ".join(f"{o['iterations']}({A.n})" for o, A in zip(oc, mats)))
 
     # ------------------------------------------------------- roofline ----
-    # per-kernel times of the real iteration sequence (K1..K4 in order, every
-    # system active, HIP events between the kernels on the library's stream)
+    # per-kernel times of the real iteration sequence (in order, every system
+    # active, HIP events between the kernels on the library's stream)
     m_bytes = 4 if precision == D.PRECISION_MIXED else 8
-    kernel_ms = dict(zip(KERNELS, S.time_iteration(20)))
+    kernel_ms = iteration_kernels(S.time_iteration(20))
     dominant = max(kernel_ms, key=kernel_ms.get)
     kb = kernel_bytes(dominant, n_tot, nnz_tot, m_bytes, row_ptr)
     achieved = kb / (kernel_ms[dominant] * 1e-3) / 1e9
@@ -196,11 +204,12 @@ def main():
     copy_ms = ctx.calibration_copy(256 * 1024 * 1024, 10)
     roofline["copy_kernel_GBs"] = round(2 * 256 * 1024 * 1024 / (copy_ms * 1e-3) / 1e9, 1)
     # whole solve: every system's iterations x its per-iteration algorithmic
-    # bytes (K1..K4 as above)
+    # bytes (the fused K1's count: the split form re-reads p')
     per_iter = 0.0
     for A, o in zip(mats, oc):
         it = o["iterations"] if o["status"] != D.SOLVE_MAX_ITER else o["iterations"] - 1
-        per_iter += it * sum(kernel_bytes(k, A.n, A.nnz, m_bytes, row_ptr) for k in KERNELS)
+        per_iter += it * sum(kernel_bytes(k, A.n, A.nnz, m_bytes, row_ptr)
+                             for k in ("update_mapply_p", "spmv_alpha", "mapply_s", "spmv_omega"))
     solve_gbs = per_iter / (float(np.median(solve_ms)) * 1e-3) / 1e9
     roofline["solve_achieved"] = round(solve_gbs, 1)
     roofline["solve_frac"] = round(solve_gbs / HBM_PEAK_GBS, 4)

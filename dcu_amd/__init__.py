@@ -166,7 +166,8 @@ def load_library() -> ctypes.CDLL:
         "spai_system_adjoint": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_int, ctypes.POINTER(SolveOptions),
                                                ctypes.POINTER(SolveReport)]),
         "spai_system_time_spmv": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
-        "spai_system_time_iteration": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
+        "spai_system_time_iteration": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int,
+                                                      ctypes.POINTER(ctypes.c_float)]),
         "spai_system_layout": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)]),
         "spai_calibration_copy": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_int,
                                                  ctypes.POINTER(ctypes.c_float)]),
@@ -510,10 +511,12 @@ class System:
         _check(self._lib.spai_system_time_spmv(self._h, int(which), int(reps), ctypes.byref(ms)))
         return ms.value
 
-    def time_iteration(self, reps: int = 20) -> List[float]:
-        """Average ms of K1..K4 over `reps` consecutive all-active iterations."""
-        ms = (ctypes.c_float * 4)()
-        _check(self._lib.spai_system_time_iteration(self._h, int(reps), ms))
+    def time_iteration(self, reps: int = 20, variant: int = -1) -> List[float]:
+        """Average ms per kernel over `reps` consecutive all-active iterations:
+        [K1 or its streaming half, its M-apply half (0 if fused), K2, K3, K4];
+        variant 0 fused K1, 1 split K1, -1 the solver's choice."""
+        ms = (ctypes.c_float * 5)()
+        _check(self._lib.spai_system_time_iteration(self._h, int(reps), int(variant), ms))
         return [float(v) for v in ms]
 
     def layout(self) -> Tuple[str, int]:

@@ -668,13 +668,13 @@ int spai_system_time_spmv(spai_system *sys, int which, int reps, float *ms) {
     });
 }
 
-int spai_system_time_iteration(spai_system *sys, int reps, float *ms) {
-    if (!sys || !ms || reps < 1) return fail(SPAI_ERR_INVALID, "bad arguments");
+int spai_system_time_iteration(spai_system *sys, int reps, int variant, float *ms) {
+    if (!sys || !ms || reps < 1 || variant < -1 || variant > 1) return fail(SPAI_ERR_INVALID, "bad arguments");
     if (sys->m_precision < 0) return fail(SPAI_ERR_NOT_BUILT, "M has not been built");
     return guarded([&] {
         SPAI_HIP(hipSetDevice(sys->ctx->device));
         sys->ctx->solver.time_iteration(a_view(sys), m_view(sys, false), sys->tiles, sys->n, sys->nsys, reps,
-                                        sys->ctx->stream, ms);
+                                        variant, sys->ctx->stream, ms);
     });
 }
 

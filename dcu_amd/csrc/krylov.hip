@@ -66,6 +66,13 @@ constexpr int kLdsProducts = 2048;  // products staged per tile (16 KB)
 #define SPAI_REDUCE_FANIN 8
 #endif
 constexpr int kFanIn = SPAI_REDUCE_FANIN;  // children per reduction tree node
+// K1 splits into a streaming update and a one-vector M-apply while at least
+// this many units are active (bandwidth-bound); below it the fused K1 saves a
+// launch (latency-bound).
+#ifndef SPAI_SPLIT_UNITS
+#define SPAI_SPLIT_UNITS 1024
+#endif
+constexpr int kSplitUnits = SPAI_SPLIT_UNITS;
 constexpr int kMaxRed = 4;          // most values any kernel reduces (partials per tree node)
 
 // ------------------------------------------------------------ transpose ----
@@ -648,13 +655,16 @@ struct OpInit {
 };
 
 // K1.  Per row: the previous iteration's x += alpha y + omega z and
-// r = s - omega t, (r, r) reduced; p' = beta p + r - beta omega a.  With a
-// preconditioner (PRECOND) y = M p' is gathered from p'(j) formed on the fly
-// (bit-identical to the stored p'), else y = p'.
-template <class MT, bool PRECOND>
+// r = s - omega t, (r, r) reduced; p' = beta p + r - beta omega a.  YMODE:
+//   kYProduct  y = M p' gathered from p'(j) formed on the fly (bit-identical
+//              to the stored p'): the fused K1, four gathered vectors
+//   kYCopy     y = p' (no preconditioner)
+//   kYNone     no y: the streaming half of the split K1, OpApplyM follows
+enum { kYProduct = 0, kYCopy = 1, kYNone = 2 };
+template <class MT, int YMODE>
 struct OpMapplyP {
     static constexpr int NRED = 1;
-    static constexpr bool kXcdBlocked = true;
+    static constexpr bool kXcdBlocked = YMODE == kYProduct;
     struct Pre {
         double p;
     };
@@ -683,7 +693,10 @@ struct OpMapplyP {
         return {pi};
     }
     __device__ double gather(int j) const { return pval(j, rval(j)); }
-    __device__ void row(int i, double mx, const Pre &pr, double (&)[1]) const { y[i] = PRECOND ? mx : pr.p; }
+    __device__ void row(int i, double mx, const Pre &pr, double (&)[1]) const {
+        if (YMODE == kYProduct) y[i] = mx;
+        else if (YMODE == kYCopy) y[i] = pr.p;
+    }
     __device__ void finish(const Ctx &c, int s_, const double (&tot)[1]) const {
         SysState &S = c.st[s_];
         if (!S.pending) {  // first K1 after OpInit: nothing to finish
@@ -693,6 +706,21 @@ struct OpMapplyP {
         }
         finish_iteration(c, s_, S, tot[0]);
     }
+};
+
+// Second half of the split K1: y = M p' with p' stored by the first half
+// (one gathered vector instead of four).
+struct OpApplyM {
+    static constexpr int NRED = 0;
+    struct Pre {};
+    const double *p;
+    double *y;
+    __device__ bool enabled(const SysState &S) const { return S.active; }
+    __device__ void load(const SysState &) {}
+    __device__ Pre pre(int, double (&)[1]) const { return {}; }
+    __device__ double gather(int j) const { return p[j]; }
+    __device__ void row(int i, double mx, const Pre &, double (&)[1]) const { y[i] = mx; }
+    __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
 
 struct OpSpmvAlpha {  // K2: a = A y, alpha = rho / (r*, a)
@@ -1196,6 +1224,7 @@ void BiCGStab::drop_graph() {
     graph_iters_ = 0;
     graph_grid_ = 0;
     graph_compact_ = 0;
+    graph_split_ = 0;
 }
 
 Ctx BiCGStab::make_ctx(const TileMap &tm, int nsys) {
@@ -1244,26 +1273,38 @@ void BiCGStab::compact(const TileMap &tm, int nsys, hipStream_t st) {
     SPAI_CHECK_LAUNCH();
 }
 
-// One iteration, four launches.  grid > 0: walk the first `grid` entries of
-// the active-unit list; grid == 0: every unit.
+// K1 of an iteration: fused (one kernel gathering four vectors) or split
+// (a streaming update, then y = M p' gathering p' alone).
+void BiCGStab::enqueue_k1(const OperatorView &A, const OperatorView &M, const TileMap &tm, int parity,
+                          const int *list, int grid, bool split, hipStream_t st) {
+    Ctx c = make_ctx(tm, nsys_);
+    double *pin = p_[parity], *pout = p_[parity ^ 1];
+    if (!M.val) {
+        // no preconditioner: y = p' through the streaming path
+        OpMapplyP<double, kYCopy> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
+        launch_units<false, double>(A, tm, c, op, list, grid, st);
+    } else if (split) {
+        OpMapplyP<double, kYNone> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
+        launch_units<false, double>(A, tm, c, op, list, grid, st);
+        OpApplyM om{pout, y_};
+        if (M.fp32) launch_units<true, float>(M, tm, c, om, list, grid, st);
+        else launch_units<true, double>(M, tm, c, om, list, grid, st);
+    } else if (M.fp32) {
+        OpMapplyP<float, kYProduct> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
+        launch_units<true, float>(M, tm, c, op, list, grid, st);
+    } else {
+        OpMapplyP<double, kYProduct> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
+        launch_units<true, double>(M, tm, c, op, list, grid, st);
+    }
+}
+
+// One iteration: K1 (one or two launches), K2, K3, K4.  grid > 0: walk the
+// first `grid` entries of the active-unit list; grid == 0: every unit.
 void BiCGStab::enqueue_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tm, int parity,
-                                 int grid, hipStream_t st) {
+                                 int grid, bool split, hipStream_t st) {
     Ctx c = make_ctx(tm, nsys_);
     const int *list = grid > 0 ? tm.active.get() : nullptr;
-    double *pin = p_[parity], *pout = p_[parity ^ 1];
-    if (M.val) {
-        if (M.fp32) {
-            OpMapplyP<float, true> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
-            launch_units<true, float>(M, tm, c, op, list, grid, st);
-        } else {
-            OpMapplyP<double, true> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
-            launch_units<true, double>(M, tm, c, op, list, grid, st);
-        }
-    } else {
-        // no preconditioner: y = p' through the streaming path
-        OpMapplyP<double, false> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
-        launch_units<false, double>(A, tm, c, op, list, grid, st);
-    }
+    enqueue_k1(A, M, tm, parity, list, grid, split, st);
     {
         OpSpmvAlpha op{y_, rs_, a_};
         launch_units<true, double>(A, tm, c, op, list, grid, st);
@@ -1346,7 +1387,9 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                     // the graph's grid follows the active-unit count: captured
                     // again when it has fallen below 5/8 of the captured grid
                     const int grid = use_list ? units : 0;
-                    bool same = graph_exec_ && gi == graph_iters_ && graph_compact_ == (int)use_list;
+                    const bool split = M.val && units >= kSplitUnits;
+                    bool same = graph_exec_ && gi == graph_iters_ && graph_compact_ == (int)use_list &&
+                                graph_split_ == (int)split;
                     for (int q = 0; q < 7 && same; ++q) same = key[q] == graph_key_[q];
                     if (same && use_list) same = grid <= graph_grid_ && grid * 8 >= graph_grid_ * 5;
                     if (!same) {
@@ -1354,7 +1397,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                         hipStream_t cap;
                         SPAI_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
                         SPAI_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-                        for (int k = 0; k < gi; ++k) enqueue_iteration(A, M, tm, k & 1, grid, cap);
+                        for (int k = 0; k < gi; ++k) enqueue_iteration(A, M, tm, k & 1, grid, split, cap);
                         if (use_list) compact(tm, nsys, cap);
                         SPAI_HIP(hipStreamEndCapture(cap, &graph_));
                         SPAI_HIP(hipStreamDestroy(cap));
@@ -1362,6 +1405,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                         graph_iters_ = gi;
                         graph_grid_ = grid;
                         graph_compact_ = (int)use_list;
+                        graph_split_ = (int)split;
                         for (int q = 0; q < 7; ++q) graph_key_[q] = key[q];
                     }
                     SPAI_HIP(hipGraphLaunch(graph_exec_, st));
@@ -1373,8 +1417,9 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                 int parity = 0;
                 while (active > 0 && done_iters < max_launch_iters) {
                     const int grid = use_list ? units : 0;
+                    const bool split = M.val && units >= kSplitUnits;
                     for (int k = 0; k < every; ++k) {
-                        enqueue_iteration(A, M, tm, parity, grid, st);
+                        enqueue_iteration(A, M, tm, parity, grid, split, st);
                         parity ^= 1;
                     }
                     if (use_list) compact(tm, nsys, st);
@@ -1470,10 +1515,10 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
         } break;
         case 1: {
             if (M.fp32) {
-                OpMapplyP<float, true> op{x_, r_, p_[1], y_, s_, t_, p_[0], a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
+                OpMapplyP<float, kYProduct> op{x_, r_, p_[1], y_, s_, t_, p_[0], a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
                 launch_units<true, float>(M, tm, c, op, nullptr, 0, st);
             } else {
-                OpMapplyP<double, true> op{x_, r_, p_[1], y_, s_, t_, p_[0], a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
+                OpMapplyP<double, kYProduct> op{x_, r_, p_[1], y_, s_, t_, p_[0], a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
                 launch_units<true, double>(M, tm, c, op, nullptr, 0, st);
             }
         } break;
@@ -1510,9 +1555,11 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
 }
 
 void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tm, int n, int nsys,
-                              int reps, hipStream_t st, float ms[4]) {
+                              int reps, int variant, hipStream_t st, float ms[5]) {
     require(A.val && M.val, "time_iteration: needs A and M");
     require(reps >= 1, "time_iteration: reps must be >= 1");
+    require(variant >= -1 && variant <= 1, "time_iteration: variant must be -1, 0 or 1");
+    const bool split = variant < 0 ? tm.nunits >= kSplitUnits : variant == 1;
     ensure(n, nsys, tm.nunits, tm.nnodes);
     hist_ptr_ = nullptr;
     hist_stride_ = 0;
@@ -1533,26 +1580,37 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
     double *x_keep = x_;
     b_ = tb;
     x_ = tx;
-    std::vector<hipEvent_t> ev((size_t)reps * 4 + 1);
+    constexpr int K = 5;  // event slots per iteration: K1 (or its halves), K2, K3, K4
+    std::vector<hipEvent_t> ev((size_t)reps * K + 1);
     for (auto &e : ev) SPAI_HIP(hipEventCreate(&e));
     try {
-        // the four kernels of enqueue_iteration, each bracketed by events
         for (int it = 0; it < reps; ++it) {
             double *pin = p_[it & 1], *pout = p_[(it & 1) ^ 1];
-            SPAI_HIP(hipEventRecord(ev[(size_t)it * 4], st));
-            if (M.fp32) {
-                OpMapplyP<float, true> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
-                launch_units<true, float>(M, tm, c, op, nullptr, 0, st);
+            hipEvent_t *e = &ev[(size_t)it * K];
+            SPAI_HIP(hipEventRecord(e[0], st));
+            if (split) {
+                OpMapplyP<double, kYNone> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
+                launch_units<false, double>(A, tm, c, op, nullptr, 0, st);
+                SPAI_HIP(hipEventRecord(e[1], st));
+                OpApplyM om{pout, y_};
+                if (M.fp32) launch_units<true, float>(M, tm, c, om, nullptr, 0, st);
+                else launch_units<true, double>(M, tm, c, om, nullptr, 0, st);
             } else {
-                OpMapplyP<double, true> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
-                launch_units<true, double>(M, tm, c, op, nullptr, 0, st);
+                if (M.fp32) {
+                    OpMapplyP<float, kYProduct> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
+                    launch_units<true, float>(M, tm, c, op, nullptr, 0, st);
+                } else {
+                    OpMapplyP<double, kYProduct> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
+                    launch_units<true, double>(M, tm, c, op, nullptr, 0, st);
+                }
+                SPAI_HIP(hipEventRecord(e[1], st));
             }
-            SPAI_HIP(hipEventRecord(ev[(size_t)it * 4 + 1], st));
+            SPAI_HIP(hipEventRecord(e[2], st));
             {
                 OpSpmvAlpha op{y_, rs_, a_};
                 launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
             }
-            SPAI_HIP(hipEventRecord(ev[(size_t)it * 4 + 2], st));
+            SPAI_HIP(hipEventRecord(e[3], st));
             if (M.fp32) {
                 OpMapplyS<float> op{r_, a_, s_, z_, 0.0};
                 launch_units<true, float>(M, tm, c, op, nullptr, 0, st);
@@ -1560,7 +1618,7 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
                 OpMapplyS<double> op{r_, a_, s_, z_, 0.0};
                 launch_units<true, double>(M, tm, c, op, nullptr, 0, st);
             }
-            SPAI_HIP(hipEventRecord(ev[(size_t)it * 4 + 3], st));
+            SPAI_HIP(hipEventRecord(e[4], st));
             {
                 OpSpmvOmega op{z_, s_, rs_, t_};
                 launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
@@ -1569,11 +1627,13 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
         SPAI_HIP(hipEventRecord(ev.back(), st));
         SPAI_HIP(hipEventSynchronize(ev.back()));
         SPAI_CHECK_LAUNCH();
-        for (int k = 0; k < 4; ++k) ms[k] = 0.0f;
+        // ms: [K1 or its streaming half, M-apply half (0 when fused), K2, K3, K4]
+        for (int k = 0; k < K; ++k) ms[k] = 0.0f;
         for (int it = 0; it < reps; ++it)
-            for (int k = 0; k < 4; ++k) {
+            for (int k = 0; k < K; ++k) {
+                if (k == 1 && !split) continue;
                 float t = 0.0f;
-                SPAI_HIP(hipEventElapsedTime(&t, ev[(size_t)it * 4 + k], ev[(size_t)it * 4 + k + 1]));
+                SPAI_HIP(hipEventElapsedTime(&t, ev[(size_t)it * K + k], ev[(size_t)it * K + k + 1]));
                 ms[k] += t / reps;
             }
     } catch (...) {

@@ -159,18 +159,23 @@ public:
     // with the vectors and scalars of the last solve, every system active).
     float time_kernel(int which, const OperatorView &A, const OperatorView &M, const TileMap &tiles, int n, int nsys,
                       int reps, hipStream_t st);
-    // Average device time of each of K1..K4 over `reps` consecutive
+    // Average device time of each iteration kernel over `reps` consecutive
     // iterations from x0 = 0, b = 1 with every system active (no threshold is
     // reached), HIP events between the kernels on the solver's stream.
+    // variant: 0 fused K1, 1 split K1, -1 the solver's choice for every unit
+    // active.  ms: K1 (or its streaming half), its M-apply half (0 when
+    // fused), K2, K3, K4.
     void time_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tiles, int n, int nsys,
-                        int reps, hipStream_t st, float ms[4]);
+                        int reps, int variant, hipStream_t st, float ms[5]);
 
 private:
     void ensure(int n, int nsys, int nunits, int nnodes);
     void compact(const TileMap &tm, int nsys, hipStream_t st);
     Ctx make_ctx(const TileMap &tm, int nsys);
     void enqueue_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tiles, int parity, int grid,
-                           hipStream_t st);
+                           bool split, hipStream_t st);
+    void enqueue_k1(const OperatorView &A, const OperatorView &M, const TileMap &tiles, int parity, const int *list,
+                    int grid, bool split, hipStream_t st);
     int n_ = 0, nsys_ = 0, nunits_ = 0, nnodes_ = -1;
     DeviceArray<double> vec_;       // x-independent work vectors
     DeviceArray<double> partials_;
@@ -189,6 +194,7 @@ private:
     int graph_iters_ = 0;
     int graph_grid_ = 0;
     int graph_compact_ = 0;
+    int graph_split_ = 0;
     void drop_graph();
     DeviceArray<double> hist_;
     double *hist_ptr_ = nullptr;

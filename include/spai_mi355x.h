@@ -181,13 +181,17 @@ int spai_system_adjoint(spai_system *sys, const double *x, const double *g, doub
  * vectors and scalars of the last solve. */
 int spai_system_time_spmv(spai_system *sys, int which, int reps, float *ms);
 
-/* Average device time (ms) of each iteration kernel K1..K4 (ms[0..3]: K1
- * x/r update + y = M p', K2 a = A y + alpha, K3 z = M s, K4 t = A z + omega,
- * beta) over `reps` consecutive iterations from x0 = 0, b = 1, every system
- * active (no threshold is reached), with HIP events between the kernels on
- * the context's stream: the per-kernel times of the real iteration sequence.
- * Overwrites the solver's work vectors; M must be built. */
-int spai_system_time_iteration(spai_system *sys, int reps, float *ms);
+/* Average device time (ms) of each iteration kernel over `reps` consecutive
+ * iterations from x0 = 0, b = 1, every system active (no threshold is
+ * reached), with HIP events between the kernels on the context's stream: the
+ * per-kernel times of the real iteration sequence.  K1 (x/r update and
+ * y = M p') runs fused or split in two (a streaming update, then y = M p');
+ * variant 0 times the fused form, 1 the split form, -1 the form the solver
+ * uses when every unit is active.  ms[5]: K1 or its streaming half, the
+ * M-apply half (0 when fused), K2 (a = A y, alpha), K3 (z = M s), K4
+ * (t = A z, omega, beta).  Overwrites the solver's work vectors; M must be
+ * built. */
+int spai_system_time_iteration(spai_system *sys, int reps, int variant, float *ms);
 
 /* Row layout chosen for the solver's operators: SPAI_LAYOUT_SELL64 (slices
  * of 64 rows, column-major, when padding adds at most 20%) or
