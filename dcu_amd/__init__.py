@@ -31,6 +31,12 @@ PRECISION_MIXED = 1
 SCHEDULE_BALANCED_COLUMNS = 0
 SCHEDULE_BALANCED_NNZ = 1
 
+# M's sparsity pattern (the reference's GetSparsityPattern flags, plus A's own)
+PATTERN_A = -1
+PATTERN_AT = 0
+PATTERN_I_PLUS_A = 1
+PATTERN_I_PLUS_A_PLUS_AT = 2
+
 SOLVE_RUNNING = 0
 SOLVE_CONVERGED = 1
 SOLVE_MAX_ITER = 2
@@ -46,7 +52,7 @@ C_SYMBOLS = [
     "spai_system_set_values", "spai_system_build", "spai_system_get_m",
     "spai_system_get_row_set_sizes", "spai_solve_options_default", "spai_system_solve",
     "spai_system_outcomes", "spai_system_residual_history", "spai_system_adjoint", "spai_system_time_spmv",
-    "spai_system_time_iteration", "spai_system_layout",
+    "spai_system_time_iteration", "spai_system_layout", "spai_system_set_pattern", "spai_system_m_pattern",
     "spai_calibration_copy", "spai_csc_to_csr_device",
     "spai_problem_generate", "spai_problem_generate_power_law", "spai_problem_generate_batched",
     "spai_problem_generate_grf", "spai_problem_convection_diffusion", "spai_problem_read_mtx", "spai_problem_free",
@@ -169,6 +175,8 @@ def load_library() -> ctypes.CDLL:
         "spai_system_time_iteration": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int,
                                                       ctypes.POINTER(ctypes.c_float)]),
         "spai_system_layout": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)]),
+        "spai_system_set_pattern": (ctypes.c_int, [vp, ctypes.c_int]),
+        "spai_system_m_pattern": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), vp, vp]),
         "spai_calibration_copy": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_int,
                                                  ctypes.POINTER(ctypes.c_float)]),
         "spai_csc_to_csr_device": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp,
@@ -447,10 +455,27 @@ class System:
                 "max_n2": rep.max_n2, "empty_columns": rep.empty_columns, "sum_gathered": rep.sum_gathered,
                 "static_tier": rep.static_tier}
 
+    def set_pattern(self, pattern: int) -> None:
+        """M's pattern for the next builds: PATTERN_A (default), PATTERN_AT,
+        PATTERN_I_PLUS_A or PATTERN_I_PLUS_A_PLUS_AT."""
+        _check(self._lib.spai_system_set_pattern(self._h, int(pattern)))
+
+    def m_pattern(self) -> Tuple[np.ndarray, np.ndarray]:
+        """M's pattern as CSC (colptr, rowidx)."""
+        nnz = ctypes.c_int()
+        _check(self._lib.spai_system_m_pattern(self._h, ctypes.byref(nnz), None, None))
+        cp = np.empty(self.n + 1, dtype=np.int32)
+        ri = np.empty(max(nnz.value, 1), dtype=np.int32)
+        _check(self._lib.spai_system_m_pattern(self._h, None, cp.ctypes.data, ri.ctypes.data))
+        return cp, ri[: nnz.value]
+
     def m_values(self) -> np.ndarray:
-        out = np.empty(max(self.nnz, 1), dtype=np.float64)
+        """M's values in its pattern's CSC order (A's CSC order by default)."""
+        nnz = ctypes.c_int()
+        _check(self._lib.spai_system_m_pattern(self._h, ctypes.byref(nnz), None, None))
+        out = np.empty(max(nnz.value, 1), dtype=np.float64)
         _check(self._lib.spai_system_get_m(self._h, out.ctypes.data, 0))
-        return out[: self.nnz]
+        return out[: nnz.value]
 
     def row_set_sizes(self) -> np.ndarray:
         out = np.empty(self.n, dtype=np.int32)

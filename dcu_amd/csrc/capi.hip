@@ -2,6 +2,7 @@
 // (spai_multi_dcu_api.h) on top of the SPAI builder and the BiCGSTAB solver.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -140,8 +141,16 @@ struct spai_system {
     DeviceArray<int> colptr, rowidx, rowptr, colidx, perm, n1;
     DeviceArray<double> cscval, csrval;
     // the solver's layout (tiles: CSR order; SELL-64: padded slice order)
-    DeviceArray<int> perm_lay;   // CSC position -> position in the layout's value order
+    DeviceArray<int> perm_lay;   // CSC position of M's pattern -> position in M's layout value order
     DeviceArray<double> sellval;  // A's values in SELL order (SELL only)
+    bool sell_layout = false;     // the layout chosen for A at creation
+    // M's sparsity pattern: A's own (SPAI_PATTERN_A) or one of the reference's
+    // alternatives (GetSparsityPattern flags 0..2)
+    int pattern = SPAI_PATTERN_A;
+    int m_nnz = 0;
+    DeviceArray<int> pcolptr, prowidx;        // pattern CSC (pattern != A)
+    DeviceArray<int> mrowptr, mcolidx, mperm;  // its CSR and CSC -> CSR positions
+    SellOperator msell;                        // its SELL-64 arrays (SELL layout)
     DeviceArray<double> m64_csc, m64_lay;
     DeviceArray<float> m32_csc, m32_lay;
     int m_precision = -1;
@@ -167,6 +176,8 @@ void finish_setup(spai_system *S, hipStream_t st) {
     csc_to_csr_device(S->n, S->nnz, S->colptr.get(), S->rowidx.get(), S->cscval.get(), S->rowptr.get(),
                       S->colidx.get(), S->csrval.get(), S->perm.get(), st);
     build_layout(S->tiles, S->rowptr.get(), S->colidx.get(), S->h_row_off, S->h_nnz_off, true, st);
+    S->sell_layout = S->tiles.kind == kLayoutSell;
+    S->m_nnz = S->nnz;
     S->perm_lay.resize((size_t)std::max(S->nnz, 1));
     layout_perm(S->tiles, S->nnz, S->perm.get(), S->perm_lay.get(), st);
     if (S->tiles.kind == kLayoutSell) {
@@ -174,6 +185,33 @@ void finish_setup(spai_system *S, hipStream_t st) {
         layout_values<double>(S->tiles, S->nnz, S->csrval.get(), S->sellval.get(), st);
     }
     SPAI_HIP(hipStreamSynchronize(st));
+}
+
+// Pattern of M for the reference's GetSparsityPattern flags
+// (cuFormatConversion.h:598-974), column by column, rows sorted ascending:
+//   0  A^T          column k = row k of A
+//   1  E + A        column k of A plus the diagonal entry
+//   2  E + A + A^T  column k of A, row k of A and the diagonal entry
+// Built on the host from A's CSC and CSR patterns (a set-up step).
+void pattern_host(int n, const std::vector<int> &cp, const std::vector<int> &ri, const std::vector<int> &rp,
+                  const std::vector<int> &ci, int flag, std::vector<int> &pcp, std::vector<int> &pri) {
+    pcp.assign((size_t)n + 1, 0);
+    pri.clear();
+    std::vector<int> col;
+    for (int k = 0; k < n; ++k) {
+        col.clear();
+        if (flag == SPAI_PATTERN_AT || flag == SPAI_PATTERN_I_PLUS_A_PLUS_AT)
+            col.insert(col.end(), ci.begin() + rp[(size_t)k], ci.begin() + rp[(size_t)k + 1]);
+        if (flag != SPAI_PATTERN_AT) {
+            col.insert(col.end(), ri.begin() + cp[(size_t)k], ri.begin() + cp[(size_t)k + 1]);
+            col.push_back(k);
+        }
+        std::sort(col.begin(), col.end());
+        col.erase(std::unique(col.begin(), col.end()), col.end());
+        pri.insert(pri.end(), col.begin(), col.end());
+        require(pri.size() < (size_t)INT32_MAX, "pattern: more than 2^31-1 entries");
+        pcp[(size_t)k + 1] = (int)pri.size();
+    }
 }
 
 // A in the solver's layout
@@ -226,7 +264,9 @@ spai_system *create_from_host(spai_context *ctx, const std::vector<const CSC_Mat
 
 void ensure_transposed_tiles(spai_system *S) {
     if (S->tiles_t_ready) return;
-    build_layout(S->tiles_t, S->colptr.get(), S->rowidx.get(), S->h_row_off, S->h_nnz_off, false, S->ctx->stream);
+    const bool own = S->pattern != SPAI_PATTERN_A;
+    build_layout(S->tiles_t, S->colptr.get(), S->rowidx.get(), S->h_row_off, S->h_nnz_off, false, S->ctx->stream,
+                 own ? S->pcolptr.get() : nullptr);
     S->tiles_t_ready = true;
 }
 
@@ -258,8 +298,17 @@ void fill_report(spai_solve_report *r, const SolveResult &res) {
 OperatorView m_view(spai_system *S, bool transposed) {
     OperatorView M;
     const bool sell = !transposed && S->tiles.kind == kLayoutSell;
-    M.ptr = transposed ? S->colptr.get() : (sell ? nullptr : S->rowptr.get());
-    M.idx = transposed ? S->rowidx.get() : (sell ? S->tiles.sell_col.get() : S->colidx.get());
+    const bool own = S->pattern != SPAI_PATTERN_A;
+    if (transposed) {
+        M.ptr = own ? S->pcolptr.get() : S->colptr.get();
+        M.idx = own ? S->prowidx.get() : S->rowidx.get();
+    } else if (sell) {
+        M.idx = own ? S->msell.col.get() : S->tiles.sell_col.get();
+        M.blk = own ? S->msell.slice_blk.get() : nullptr;
+    } else {
+        M.ptr = own ? S->mrowptr.get() : S->rowptr.get();
+        M.idx = own ? S->mcolidx.get() : S->colidx.get();
+    }
     if (S->m_precision == SPAI_PRECISION_MIXED) {
         M.val = transposed ? (const void *)S->m32_csc.get() : (const void *)S->m32_lay.get();
         M.fp32 = 1;
@@ -458,9 +507,11 @@ int spai_system_build(spai_system *sys, int precision, int schedule, int col_beg
     return guarded([&] {
         hipStream_t st = sys->ctx->stream;
         SPAI_HIP(hipSetDevice(sys->ctx->device));
-        size_t nz = (size_t)std::max(sys->nnz, 1);
-        const long long lay = std::max<long long>(sys->tiles.value_len(sys->nnz), 1);
+        const bool own = sys->pattern != SPAI_PATTERN_A;
+        size_t nz = (size_t)std::max(sys->m_nnz, 1);
         const bool sell = sys->tiles.kind == kLayoutSell;
+        const long long lay =
+            std::max<long long>(sell ? (own ? sys->msell.len : sys->tiles.sell_len) : (long long)sys->m_nnz, 1);
         if (precision == SPAI_PRECISION_MIXED) {
             sys->m32_csc.resize(nz);
             sys->m32_lay.resize((size_t)lay);
@@ -478,11 +529,11 @@ int spai_system_build(spai_system *sys, int precision, int schedule, int col_beg
         bool partial = cb > 0 || ce < sys->n;
         if (partial) {
             if (precision == SPAI_PRECISION_MIXED) {
-                fill(sys->m32_csc.get(), sys->nnz, st);
-                if (!sell) fill(sys->m32_lay.get(), sys->nnz, st);
+                fill(sys->m32_csc.get(), sys->m_nnz, st);
+                if (!sell) fill(sys->m32_lay.get(), sys->m_nnz, st);
             } else {
-                fill(sys->m64_csc.get(), sys->nnz, st);
-                if (!sell) fill(sys->m64_lay.get(), sys->nnz, st);
+                fill(sys->m64_csc.get(), sys->m_nnz, st);
+                if (!sell) fill(sys->m64_lay.get(), sys->m_nnz, st);
             }
         }
         SpaiBuildParams p;
@@ -491,6 +542,10 @@ int spai_system_build(spai_system *sys, int precision, int schedule, int col_beg
         p.rowidx = sys->rowidx.get();
         p.val = sys->cscval.get();
         p.perm = sys->perm_lay.get();
+        if (own) {
+            p.pcolptr = sys->pcolptr.get();
+            p.prowidx = sys->prowidx.get();
+        }
         p.precision = precision;
         p.schedule = schedule;
         p.col_begin = cb;
@@ -514,18 +569,90 @@ int spai_system_build(spai_system *sys, int precision, int schedule, int col_beg
     });
 }
 
+int spai_system_set_pattern(spai_system *sys, int pattern) {
+    if (!sys) return fail(SPAI_ERR_INVALID, "sys is NULL");
+    if (pattern < SPAI_PATTERN_A || pattern > SPAI_PATTERN_I_PLUS_A_PLUS_AT) return fail(SPAI_ERR_INVALID, "unknown pattern");
+    return guarded([&] {
+        hipStream_t st = sys->ctx->stream;
+        SPAI_HIP(hipSetDevice(sys->ctx->device));
+        const int n = sys->n, nnz = sys->nnz;
+        sys->m_precision = -1;  // M must be built again on the new pattern
+        sys->tiles_t_ready = false;
+        if (pattern == SPAI_PATTERN_A) {
+            sys->pattern = pattern;
+            sys->m_nnz = nnz;
+            if (!sys->sell_layout)
+                build_layout(sys->tiles, sys->rowptr.get(), sys->colidx.get(), sys->h_row_off, sys->h_nnz_off, false,
+                             st);
+            layout_perm(sys->tiles, nnz, sys->perm.get(), sys->perm_lay.get(), st);
+            SPAI_HIP(hipStreamSynchronize(st));
+            return;
+        }
+        std::vector<int> cp((size_t)n + 1), ri((size_t)std::max(nnz, 1)), rp((size_t)n + 1),
+            ci((size_t)std::max(nnz, 1));
+        SPAI_HIP(hipMemcpyAsync(cp.data(), sys->colptr.get(), sizeof(int) * cp.size(), hipMemcpyDeviceToHost, st));
+        SPAI_HIP(hipMemcpyAsync(ri.data(), sys->rowidx.get(), sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost, st));
+        SPAI_HIP(hipMemcpyAsync(rp.data(), sys->rowptr.get(), sizeof(int) * rp.size(), hipMemcpyDeviceToHost, st));
+        SPAI_HIP(hipMemcpyAsync(ci.data(), sys->colidx.get(), sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost, st));
+        SPAI_HIP(hipStreamSynchronize(st));
+        std::vector<int> pcp, pri;
+        pattern_host(n, cp, ri, rp, ci, pattern, pcp, pri);
+        const int mn = pcp[(size_t)n];
+        sys->pcolptr.resize(pcp.size());
+        sys->prowidx.resize((size_t)std::max(mn, 1));
+        SPAI_HIP(hipMemcpyAsync(sys->pcolptr.get(), pcp.data(), sizeof(int) * pcp.size(), hipMemcpyHostToDevice, st));
+        SPAI_HIP(hipMemcpyAsync(sys->prowidx.get(), pri.data(), sizeof(int) * (size_t)mn, hipMemcpyHostToDevice, st));
+        sys->mrowptr.resize((size_t)n + 1);
+        sys->mcolidx.resize((size_t)std::max(mn, 1));
+        sys->mperm.resize((size_t)std::max(mn, 1));
+        csc_to_csr_device(n, mn, sys->pcolptr.get(), sys->prowidx.get(), nullptr, sys->mrowptr.get(),
+                          sys->mcolidx.get(), nullptr, sys->mperm.get(), st);
+        sys->perm_lay.resize((size_t)std::max(mn, 1));
+        if (sys->sell_layout) {
+            build_sell_operator(sys->tiles, sys->mrowptr.get(), sys->mcolidx.get(), mn, sys->msell, st);
+            compose_positions(mn, sys->mperm.get(), sys->msell.pos.get(), sys->perm_lay.get(), st);
+        } else {
+            // tiles cut so that neither A's nor M's rows overflow one
+            build_layout(sys->tiles, sys->rowptr.get(), sys->colidx.get(), sys->h_row_off, sys->h_nnz_off, false, st,
+                         sys->mrowptr.get());
+            SPAI_HIP(hipMemcpyAsync(sys->perm_lay.get(), sys->mperm.get(), sizeof(int) * (size_t)mn,
+                                    hipMemcpyDeviceToDevice, st));
+        }
+        SPAI_HIP(hipStreamSynchronize(st));
+        sys->m_nnz = mn;
+        sys->pattern = pattern;
+    });
+}
+
+int spai_system_m_pattern(const spai_system *sys, int *nnz, int *colptr, int *rowidx) {
+    if (!sys) return fail(SPAI_ERR_INVALID, "sys is NULL");
+    return guarded([&] {
+        if (nnz) *nnz = sys->m_nnz;
+        const bool own = sys->pattern != SPAI_PATTERN_A;
+        hipStream_t st = sys->ctx->stream;
+        SPAI_HIP(hipSetDevice(sys->ctx->device));
+        if (colptr)
+            SPAI_HIP(hipMemcpyAsync(colptr, own ? sys->pcolptr.get() : sys->colptr.get(), sizeof(int) * ((size_t)sys->n + 1),
+                                    hipMemcpyDeviceToHost, st));
+        if (rowidx)
+            SPAI_HIP(hipMemcpyAsync(rowidx, own ? sys->prowidx.get() : sys->rowidx.get(), sizeof(int) * (size_t)sys->m_nnz,
+                                    hipMemcpyDeviceToHost, st));
+        SPAI_HIP(hipStreamSynchronize(st));
+    });
+}
+
 int spai_system_get_m(spai_system *sys, double *values, int values_on_device) {
     if (!sys || !values) return fail(SPAI_ERR_INVALID, "sys/values is NULL");
     if (sys->m_precision < 0) return fail(SPAI_ERR_NOT_BUILT, "M has not been built");
     return guarded([&] {
         hipStream_t st = sys->ctx->stream;
         SPAI_HIP(hipSetDevice(sys->ctx->device));
-        size_t bytes = sizeof(double) * (size_t)sys->nnz;
+        size_t bytes = sizeof(double) * (size_t)sys->m_nnz;
         const double *src = sys->m64_csc.get();
         DeviceArray<double> tmp;
         if (sys->m_precision == SPAI_PRECISION_MIXED) {
-            tmp.resize((size_t)std::max(sys->nnz, 1));
-            convert_f32_to_f64(sys->m32_csc.get(), tmp.get(), sys->nnz, st);
+            tmp.resize((size_t)std::max(sys->m_nnz, 1));
+            convert_f32_to_f64(sys->m32_csc.get(), tmp.get(), sys->m_nnz, st);
             src = tmp.get();
         }
         SPAI_HIP(hipMemcpyAsync(values, src, bytes, values_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,

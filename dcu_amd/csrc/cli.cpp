@@ -32,7 +32,7 @@ static void usage(const char *prog) {
     std::printf("usage: %s [dcu_count] [--algo static|dynamic] [--partition balanced_columns|balanced_nnz]\n", prog);
     std::printf("           [--matrix <path> | --gen <1..5> [--seed S]] [--precondition-only]\n");
     std::printf("           [--precision fp64|mixed] [--tol T | --rel-tol R] [--max-iter K]\n");
-    std::printf("           [--x0 ones|zero] [--rhs ones|problem] [--graph] [--help]\n");
+    std::printf("           [--x0 ones|zero] [--rhs ones|problem] [--graph] [--pattern a|at|i+a|i+a+at] [--help]\n");
 }
 
 static void check(int st, const char *what) {
@@ -46,7 +46,7 @@ int main(int argc, char **argv) {
     std::string matrix = "matrices/circuit_2.mtx";
     std::string algo = "static";
     bool nnz_balance = false, partition_given = false, precondition_only = false, use_graph = false;
-    int gen = 0, max_iter = -1, precision = SPAI_PRECISION_FP64;
+    int gen = 0, max_iter = -1, precision = SPAI_PRECISION_FP64, pattern = SPAI_PATTERN_A;
     unsigned long long seed = 0;
     double tol = -1.0, rel_tol = -1.0;
     std::string x0_mode, rhs_mode;
@@ -78,6 +78,17 @@ int main(int argc, char **argv) {
         else if (a == "--x0") x0_mode = next();
         else if (a == "--rhs") rhs_mode = next();
         else if (a == "--graph") use_graph = true;
+        else if (a == "--pattern") {
+            const std::string v = next();
+            if (v == "a") pattern = SPAI_PATTERN_A;
+            else if (v == "at") pattern = SPAI_PATTERN_AT;
+            else if (v == "i+a") pattern = SPAI_PATTERN_I_PLUS_A;
+            else if (v == "i+a+at") pattern = SPAI_PATTERN_I_PLUS_A_PLUS_AT;
+            else {
+                std::printf("unknown pattern %s (a, at, i+a, i+a+at)\n", v.c_str());
+                return EXIT_FAILURE;
+            }
+        }
         else if (a[0] != '-') requested = std::atoi(a.c_str());
     }
     if (!partition_given) nnz_balance = (algo == "dynamic");
@@ -116,6 +127,7 @@ int main(int argc, char **argv) {
     spai_system_info info;
     spai_system_get_info(S, &info);
 
+    if (pattern != SPAI_PATTERN_A) check(spai_system_set_pattern(S, pattern), "pattern");
     spai_build_report br{};
     check(spai_system_build(S, precision, nnz_balance ? SPAI_SCHEDULE_BALANCED_NNZ : SPAI_SCHEDULE_BALANCED_COLUMNS,
                             0, -1, &br),

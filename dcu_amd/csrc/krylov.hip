@@ -1004,7 +1004,8 @@ UnitView unit_view(const TileMap &tm, bool use_list, const int *count) {
 template <bool HM, class MT, class Op>
 void launch_units(const OperatorView &O, const TileMap &tm, const Ctx &c, const Op &op, const int *list, int grid,
                   hipStream_t st) {
-    const UnitView uv = unit_view(tm, list != nullptr, c.counter + c.nsys + 1);
+    UnitView uv = unit_view(tm, list != nullptr, c.counter + c.nsys + 1);
+    if (O.blk) uv.slice_blk = O.blk;  // an operator with its own pattern
     const int g = list ? grid : tm.nunits;
     if (g <= 0) return;
     if (tm.kind == kLayoutSell)
@@ -1040,13 +1041,24 @@ void csc_to_csr_device(int n, int nnz, const int *colptr, const int *rowidx, con
 }
 
 void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector<int> &h_row_off,
-                  const std::vector<long long> &h_nnz_off, bool allow_sell, hipStream_t st) {
+                  const std::vector<long long> &h_nnz_off, bool allow_sell, hipStream_t st, const int *ptr2) {
     const int nsys = (int)h_row_off.size() - 1;
     require(nsys >= 1, "build_layout: no systems");
     const int n = h_row_off[(size_t)nsys];
     std::vector<int> hp((size_t)n + 1);
     SPAI_HIP(hipMemcpyAsync(hp.data(), ptr, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost, st));
     SPAI_HIP(hipStreamSynchronize(st));
+    if (ptr2) {
+        // tiles are cut on the longer of the two operators' rows: hp becomes
+        // the prefix of max(len_1, len_2)
+        std::vector<int> h2((size_t)n + 1);
+        SPAI_HIP(hipMemcpyAsync(h2.data(), ptr2, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost, st));
+        SPAI_HIP(hipStreamSynchronize(st));
+        std::vector<int> m((size_t)n + 1, 0);
+        for (int i = 0; i < n; ++i)
+            m[(size_t)i + 1] = m[(size_t)i] + std::max(hp[(size_t)i + 1] - hp[(size_t)i], h2[(size_t)i + 1] - h2[(size_t)i]);
+        hp.swap(m);
+    }
     const long long total = h_nnz_off[(size_t)nsys];
     // SELL-64 candidate: slices of 64 rows inside each system
     std::vector<int> srow, sblk(1, 0), sys_slices((size_t)nsys + 1, 0);
@@ -1186,6 +1198,46 @@ void layout_values(const TileMap &tm, long long nnz, const double *in, MT_OUT *o
 }
 template void layout_values<double>(const TileMap &, long long, const double *, double *, hipStream_t);
 template void layout_values<float>(const TileMap &, long long, const double *, float *, hipStream_t);
+
+void build_sell_operator(const TileMap &tm, const int *rowptr, const int *colidx, int nnz, SellOperator &op,
+                         hipStream_t st) {
+    require(tm.kind == kLayoutSell, "build_sell_operator: the layout is not SELL-64");
+    const int n = [&] {
+        int last = 0;
+        SPAI_HIP(hipMemcpyAsync(&last, tm.slice_row.get() + tm.nslices, sizeof(int), hipMemcpyDeviceToHost, st));
+        SPAI_HIP(hipStreamSynchronize(st));
+        return last;
+    }();
+    std::vector<int> hp((size_t)n + 1), srow((size_t)tm.nslices + 1), sblk((size_t)tm.nslices + 1, 0);
+    SPAI_HIP(hipMemcpyAsync(hp.data(), rowptr, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost, st));
+    SPAI_HIP(hipMemcpyAsync(srow.data(), tm.slice_row.get(), sizeof(int) * srow.size(), hipMemcpyDeviceToHost, st));
+    SPAI_HIP(hipStreamSynchronize(st));
+    long long blocks = 0;
+    for (int k = 0; k < tm.nslices; ++k) {
+        int w = 0;
+        for (int i = srow[(size_t)k]; i < srow[(size_t)k + 1]; ++i) w = std::max(w, hp[(size_t)i + 1] - hp[(size_t)i]);
+        blocks += w;
+        require(blocks * kSliceRows < (long long)INT32_MAX, "build_sell_operator: more than 2^31 padded entries");
+        sblk[(size_t)k + 1] = (int)blocks;
+    }
+    op.len = blocks * kSliceRows;
+    op.slice_blk.resize(sblk.size());
+    SPAI_HIP(hipMemcpyAsync(op.slice_blk.get(), sblk.data(), sizeof(int) * sblk.size(), hipMemcpyHostToDevice, st));
+    op.col.resize((size_t)std::max<long long>(op.len, 1));
+    op.pos.resize((size_t)std::max(nnz, 1));
+    const long long threads = (long long)tm.nslices * kSliceRows;
+    if (threads > 0)
+        hipLaunchKernelGGL(k_sell_fill, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, tm.nslices,
+                           tm.slice_row.get(), op.slice_blk.get(), rowptr, colidx, op.col.get(), op.pos.get());
+    SPAI_CHECK_LAUNCH();
+    SPAI_HIP(hipStreamSynchronize(st));
+}
+
+void compose_positions(int nnz, const int *perm_csr, const int *sell_pos, int *perm_out, hipStream_t st) {
+    if (nnz <= 0) return;
+    hipLaunchKernelGGL(k_compose_perm, dim3(ceil_div(nnz, 256)), dim3(256), 0, st, nnz, perm_csr, sell_pos, perm_out);
+    SPAI_CHECK_LAUNCH();
+}
 
 void layout_perm(const TileMap &tm, int nnz, const int *perm_csr, int *perm_out, hipStream_t st) {
     if (nnz <= 0) return;

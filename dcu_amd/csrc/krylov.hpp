@@ -93,9 +93,12 @@ void csc_to_csr_device(int n, int nnz, const int *colptr, const int *rowidx, con
 // `idx` entries, on the device) for systems with row offsets h_row_off
 // (nsys+1, host) and nonzero offsets h_nnz_off (nsys+1, host).  SELL-64 is
 // considered only when allow_sell is set (the operators then share `idx`'s
-// pattern); otherwise, or when its padding is too large, row tiles.
+// pattern); otherwise, or when its padding is too large, row tiles.  ptr2
+// (optional) is a second operator's row pointers over the same rows (M with
+// its own pattern): tiles are cut so that neither operator overflows one.
 void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector<int> &h_row_off,
-                  const std::vector<long long> &h_nnz_off, bool allow_sell, hipStream_t st);
+                  const std::vector<long long> &h_nnz_off, bool allow_sell, hipStream_t st,
+                  const int *ptr2 = nullptr);
 
 // Values of a CSR-ordered array in the layout's order: out[sell_pos[p]] =
 // in[p] (SELL, padding 0) or a copy (tiles).  MT_OUT is double or float.
@@ -137,7 +140,25 @@ struct OperatorView {
     const int *idx = nullptr;
     const void *val = nullptr;
     int fp32 = 0;
+    const int *blk = nullptr;  // SELL-64: the operator's own slice offsets when its pattern is not A's
 };
+
+// SELL-64 arrays of an operator whose pattern differs from A's (an
+// alternative SPAI pattern for M), on the layout's slices.
+struct SellOperator {
+    long long len = 0;            // padded entries
+    DeviceArray<int> slice_blk;   // nslices + 1 offsets in blocks of 64 entries
+    DeviceArray<int> col;         // len column indices
+    DeviceArray<int> pos;         // nnz: CSR position -> SELL position
+};
+
+// Builds an operator's SELL-64 arrays on the layout's slices from its CSR
+// pattern (rowptr n+1, colidx nnz, on the device).
+void build_sell_operator(const TileMap &tm, const int *rowptr, const int *colidx, int nnz, SellOperator &op,
+                         hipStream_t st);
+
+// perm_out[p] = SELL position of the entry whose CSR position is perm_csr[p].
+void compose_positions(int nnz, const int *perm_csr, const int *sell_pos, int *perm_out, hipStream_t st);
 
 struct Ctx;  // device-side view of the solver state (krylov.hip)
 

@@ -69,10 +69,12 @@ constexpr int kLdsTierNJ = 64;         // largest |J_k|
 constexpr int kLdsTierDoubles = 7168;  // tile + R + y (56 KB)
 
 struct ColumnArgs {
-    const int *colptr;
+    const int *colptr;   // A (CSC): the columns gathered for A(I_k, J_k)
     const int *rowidx;
     const double *val;
-    const int *perm;  // CSC position -> CSR position
+    const int *pcolptr;  // pattern of M (CSC): J_k = prowidx[pcolptr[k] .. pcolptr[k+1])
+    const int *prowidx;
+    const int *perm;     // pattern position -> position of M's solver layout
     void *m_csc;      // MT[nnz]
     void *m_csr;      // MT[nnz]
     int *n1_out;      // optional per-column |I_k|
@@ -166,14 +168,14 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
     double *Rs = lds + TR::ROWS * TS;         // NJ x NJ
     double *ys = Rs + NJ * NJ;                // NJ
 
-    const int c0 = a.colptr[k];
-    const int n2 = a.colptr[k + 1] - c0;
+    const int c0 = a.pcolptr[k];
+    const int n2 = a.pcolptr[k + 1] - c0;
     if (n2 <= 0) return;
 
     // J_k and the extent of each column A(:, J_k[j])
     int cs = 0, cl = 0;
     if (lane < n2) {
-        int jr = a.rowidx[c0 + lane];
+        int jr = a.prowidx[c0 + lane];
         cs = a.colptr[jr];
         cl = a.colptr[jr + 1] - cs;
     }
@@ -461,14 +463,14 @@ __global__ __launch_bounds__(kBlock) void spai_reg_tier_kernel(ColumnArgs a) {
 template <class MT>
 __device__ void solve_column_lds(const ColumnArgs &a, int k, int lane, double *lds, int *ilds) {
     constexpr int W = 64, EPL = kLdsTierEPL, NJ = kLdsTierNJ;
-    const int c0 = a.colptr[k];
-    const int n2 = a.colptr[k + 1] - c0;
+    const int c0 = a.pcolptr[k];
+    const int n2 = a.pcolptr[k + 1] - c0;
     if (n2 <= 0) return;
     int *offs = ilds;        // NJ
     int *starts = ilds + NJ; // NJ
     int cs = 0, cl = 0;
     if (lane < n2) {
-        int jr = a.rowidx[c0 + lane];
+        int jr = a.prowidx[c0 + lane];
         cs = a.colptr[jr];
         cl = a.colptr[jr + 1] - cs;
     }
@@ -712,8 +714,8 @@ template <class MT>
 __device__ void solve_column_global(const ColumnArgs &a, int k, const GlobalScratch &g, int slot, double *red,
                                     int *ired, int *sh) {
     const int tid = threadIdx.x;
-    const int c0 = a.colptr[k];
-    const int n2 = a.colptr[k + 1] - c0;
+    const int c0 = a.pcolptr[k];
+    const int n2 = a.pcolptr[k + 1] - c0;
     if (n2 <= 0) return;
     double *tile = g.tile + (size_t)slot * g.tile_cap;
     double *yv = g.vec + (size_t)slot * g.vec_cap;
@@ -726,14 +728,14 @@ __device__ void solve_column_global(const ColumnArgs &a, int k, const GlobalScra
 
     // J offsets: exclusive prefix of |A(:, J_k[j])|
     const int T = glob_block_scan(
-        n2, [&](int j) { int jr = a.rowidx[c0 + j]; return a.colptr[jr + 1] - a.colptr[jr]; }, offs, ired);
+        n2, [&](int j) { int jr = a.prowidx[c0 + j]; return a.colptr[jr + 1] - a.colptr[jr]; }, offs, ired);
     int Tp = 1;
     while (Tp < T) Tp <<= 1;
     for (int e = tid; e < Tp; e += kGlobThreads) {
         uint64_t key = ~0ull;
         if (e < T) {
             const int j = glob_find_j(offs, n2, e);
-            const int jr = a.rowidx[c0 + j];
+            const int jr = a.prowidx[c0 + j];
             key = ((uint64_t)(unsigned)a.rowidx[a.colptr[jr] + e - offs[j]] << 32) | (unsigned)e;
         }
         keys[e] = key;
@@ -780,7 +782,7 @@ __device__ void solve_column_global(const ColumnArgs &a, int k, const GlobalScra
     for (int e = tid; e < T; e += kGlobThreads) {
         const int ee = (int)(unsigned)keys[e];
         const int j = glob_find_j(offs, n2, ee);
-        const int jr = a.rowidx[c0 + j];
+        const int jr = a.prowidx[c0 + j];
         tile[(size_t)pos[e] * n2 + j] = a.val[a.colptr[jr] + ee - offs[j]];
     }
     __syncthreads();
@@ -919,7 +921,8 @@ __global__ void spai_zero_counters(int *c, int n, unsigned long long *s64, int n
 // stats64: [0] sum T_k, [1] max over global-tier columns of max(min(T_k, n), n2_k) * n2_k
 // (a bound on that column's tile), [2] the same over all columns.
 template <bool DYNAMIC>
-__global__ __launch_bounds__(256) void spai_plan_kernel(const int *colptr, const int *rowidx, int n, int col_begin,
+__global__ __launch_bounds__(256) void spai_plan_kernel(const int *colptr, const int *pcolptr, const int *prowidx,
+                                                        int n, int col_begin,
                                                         int col_end, int *counters,
                                                         unsigned long long *stats64, int *lists,
                                                         int list_stride) {
@@ -939,10 +942,10 @@ __global__ __launch_bounds__(256) void spai_plan_kernel(const int *colptr, const
     const int k = col_begin + blockIdx.x * blockDim.x + tid;
     int T = 0, n2 = 0, tier = -1;
     if (k < col_end) {
-        int c0 = colptr[k];
-        n2 = colptr[k + 1] - c0;
+        int c0 = pcolptr[k];
+        n2 = pcolptr[k + 1] - c0;
         for (int j = 0; j < n2; ++j) {
-            int jr = rowidx[c0 + j];
+            int jr = prowidx[c0 + j];
             T += colptr[jr + 1] - colptr[jr];
         }
         tier = n2 > 0 ? classify(T, n2, lds_tier_fits(T, n2)) : -1;
@@ -1053,6 +1056,9 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
     SPAI_HIP(hipGetDevice(&dev));
     if (cus_ == 0) SPAI_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
     const bool dynamic = p.schedule == 1;
+    // J_k from M's pattern (A's own unless another pattern is given)
+    const int *pc = p.pcolptr ? p.pcolptr : p.colptr;
+    const int *pr = p.pcolptr ? p.prowidx : p.rowidx;
     const int nc = 2 * kTiers + 3;
     counters_.reserve(nc);
     stats64_.reserve(3);
@@ -1063,10 +1069,10 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
     if (ce > cb) {
         int grid = ceil_div(ce - cb, 256);
         if (dynamic)
-            hipLaunchKernelGGL(spai_plan_kernel<true>, dim3(grid), dim3(256), 0, st, p.colptr, p.rowidx, p.n, cb, ce,
+            hipLaunchKernelGGL(spai_plan_kernel<true>, dim3(grid), dim3(256), 0, st, p.colptr, pc, pr, p.n, cb, ce,
                                counters_.get(), stats64_.get(), lists_.get(), list_stride);
         else
-            hipLaunchKernelGGL(spai_plan_kernel<false>, dim3(grid), dim3(256), 0, st, p.colptr, p.rowidx, p.n, cb,
+            hipLaunchKernelGGL(spai_plan_kernel<false>, dim3(grid), dim3(256), 0, st, p.colptr, pc, pr, p.n, cb,
                                ce, counters_.get(), stats64_.get(), (int *)nullptr, list_stride);
         SPAI_CHECK_LAUNCH();
     }
@@ -1080,6 +1086,8 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
     ColumnArgs a{};
     a.colptr = p.colptr;
     a.rowidx = p.rowidx;
+    a.pcolptr = pc;
+    a.prowidx = pr;
     a.val = p.val;
     a.perm = p.perm;
     a.m_csc = p.m_csc;

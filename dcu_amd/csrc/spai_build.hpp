@@ -17,12 +17,14 @@ struct SpaiBuildParams {
     const int *colptr = nullptr;  // device CSC of A
     const int *rowidx = nullptr;
     const double *val = nullptr;
-    const int *perm = nullptr;    // CSC position -> CSR position
+    const int *pcolptr = nullptr;  // M's pattern (CSC) when it is not A's: J_k =
+    const int *prowidx = nullptr;  // prowidx[pcolptr[k] .. pcolptr[k+1])
+    const int *perm = nullptr;    // pattern position -> position in M's solver layout
     int precision = 0;            // 0: FP64 QR, FP64 M; 1: FP32 QR + FP64 sweep, FP32 M
     int schedule = 0;             // 0: balanced_columns (static), 1: balanced_nnz (queues)
     int col_begin = 0, col_end = -1;
-    void *m_csc = nullptr;        // double[nnz] or float[nnz] (precision 1)
-    void *m_csr = nullptr;        // same type, CSR order
+    void *m_csc = nullptr;        // double[nnz(M)] or float[nnz(M)] (precision 1), pattern order
+    void *m_csr = nullptr;        // same type, M's solver layout order
     int *n1_out = nullptr;        // optional per-column |I_k|
 };
 

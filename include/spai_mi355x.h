@@ -112,12 +112,32 @@ typedef struct {
     int static_tier;         /* tier used by balanced_columns, -1 for balanced_nnz */
 } spai_build_report;
 
-/* Builds M with pattern(M) = pattern(A) for columns [col_begin, col_end)
- * (col_end < 0 means n); other columns of M are zero. */
+/* Sparsity pattern of M.  SPAI_PATTERN_A (the default) is pattern(A), as
+ * the reference's SPAI uses; the others are the reference's
+ * GetSparsityPattern flags (include/common/cuFormatConversion.h:598), which
+ * its drivers never call: 0 = A^T, 1 = E + A, 2 = E + A + A^T (E the
+ * identity).  E + A and E + A + A^T give every column a row set that holds
+ * its own index whenever A's row k is non-empty, so M has no zero column
+ * where A lacks a diagonal entry. */
+#define SPAI_PATTERN_A (-1)
+#define SPAI_PATTERN_AT 0
+#define SPAI_PATTERN_I_PLUS_A 1
+#define SPAI_PATTERN_I_PLUS_A_PLUS_AT 2
+/* Sets M's pattern for the next builds (M must be built again); no
+ * counterpart in the reference's API. */
+int spai_system_set_pattern(spai_system *sys, int pattern);
+/* M's pattern as CSC: *nnz entries, colptr (n + 1) and rowidx (nnz) host
+ * arrays (any of the three may be NULL). */
+int spai_system_m_pattern(const spai_system *sys, int *nnz, int *colptr, int *rowidx);
+
+/* Builds M on its pattern (pattern(A) unless spai_system_set_pattern chose
+ * another) for columns [col_begin, col_end) (col_end < 0 means n); other
+ * columns of M are zero. */
 int spai_system_build(spai_system *sys, int precision, int schedule, int col_begin, int col_end,
                       spai_build_report *report);
 
-/* M's values in A's CSC storage order, widened to double. */
+/* M's values in the CSC storage order of its pattern (A's, by default),
+ * widened to double. */
 int spai_system_get_m(spai_system *sys, double *values, int values_on_device);
 /* |I_k| per column from the last build (n ints), for diagnostics. */
 int spai_system_get_row_set_sizes(spai_system *sys, int *n1);

@@ -32,6 +32,7 @@ def lib() -> ctypes.CDLL:
         vp, i = ctypes.c_void_p, ctypes.c_int
         L.oracle_column_sets.argtypes = [i, vp, vp, vp, vp, vp, vp]
         L.oracle_spai.argtypes = [i, vp, vp, vp, i, i, i, vp]
+        L.oracle_spai_pattern.argtypes = [i, vp, vp, vp, vp, vp, i, i, i, vp]
         L.oracle_csc2csr.argtypes = [i, i, vp, vp, vp, vp, vp, vp, vp]
         L.oracle_spmv.argtypes = [i, vp, vp, vp, vp, vp]
         L.oracle_spmv.restype = None
@@ -121,6 +122,17 @@ def spai(n, colptr, rowidx, val, qr_mode: int = 0, col_begin: int = 0, col_end: 
                            n if col_end < 0 else col_end, m.ctypes.data)
     assert st == 0
     return m[: len(val)]
+
+
+def spai_pattern(n, colptr, rowidx, val, pcolptr, prowidx, qr_mode: int = 0) -> np.ndarray:
+    """M's values on the pattern (pcolptr, prowidx), in its CSC order."""
+    colptr, rowidx, val = _i32(colptr), _i32(rowidx), _f64(val)
+    pcolptr, prowidx = _i32(pcolptr), _i32(prowidx)
+    m = np.zeros(max(len(prowidx), 1))
+    st = lib().oracle_spai_pattern(n, colptr.ctypes.data, rowidx.ctypes.data, val.ctypes.data, pcolptr.ctypes.data,
+                                   prowidx.ctypes.data, qr_mode, 0, n, m.ctypes.data)
+    assert st == 0
+    return m[: len(prowidx)]
 
 
 def csc2csr(n, colptr, rowidx, val) -> Tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:

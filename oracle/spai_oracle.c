@@ -200,9 +200,22 @@ int oracle_column_sets(int n, const int *colptr, const int *rowidx, int *n1, int
  * mval receives M's values in A's CSC storage order (M.mPtr = A.mPtr,
  * M.mIndex = A.mIndex, src/spai_single_dcu.cpp:2617-2624).  Columns outside
  * [col_begin, col_end) are left untouched. */
+int oracle_spai_pattern(int n, const int *colptr, const int *rowidx, const double *val, const int *pcolptr,
+                        const int *prowidx, int qr_mode, int col_begin, int col_end, double *mval);
+
 int oracle_spai(int n, const int *colptr, const int *rowidx, const double *val, int qr_mode,
                 int col_begin, int col_end, double *mval) {
-    if (n < 0 || !colptr || !rowidx || !val || !mval) return ORACLE_ERR_ARG;
+    return oracle_spai_pattern(n, colptr, rowidx, val, colptr, rowidx, qr_mode, col_begin, col_end, mval);
+}
+
+/* As oracle_spai with M's pattern given separately (CSC pcolptr/prowidx):
+ * J_k = prowidx[pcolptr[k] .. pcolptr[k+1]), the least squares still over
+ * A's columns, mval in the pattern's order.  The reference's alternative
+ * patterns (GetSparsityPattern, cuFormatConversion.h:598-974: A^T, E + A,
+ * E + A + A^T) are built by the caller. */
+int oracle_spai_pattern(int n, const int *colptr, const int *rowidx, const double *val, const int *pcolptr,
+                        const int *prowidx, int qr_mode, int col_begin, int col_end, double *mval) {
+    if (n < 0 || !colptr || !rowidx || !val || !pcolptr || !prowidx || !mval) return ORACLE_ERR_ARG;
     if (col_begin < 0) col_begin = 0;
     if (col_end > n) col_end = n;
     int fail = 0;
@@ -213,9 +226,9 @@ int oracle_spai(int n, const int *colptr, const int *rowidx, const double *val, 
         double *tile = NULL, *R = NULL, *Rs = NULL, *x = NULL, *y = NULL, *tau = NULL;
 #pragma omp for schedule(dynamic, 32)
         for (int k = col_begin; k < col_end; ++k) {
-            int c0 = colptr[k], n2 = colptr[k + 1] - c0;
+            int c0 = pcolptr[k], n2 = pcolptr[k + 1] - c0;
             if (n2 == 0) continue;
-            const int *J = rowidx + c0;
+            const int *J = prowidx + c0;
             int tot = 0;
             for (int j = 0; j < n2; ++j) tot += colptr[J[j] + 1] - colptr[J[j]];
             if (tot + 1 > capI) {

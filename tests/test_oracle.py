@@ -44,6 +44,34 @@ def test_spai_matches_lstsq_golden(qr_mode):
         assert np.max(np.abs(m - m_gold)) <= 1e-12 * scale, name
 
 
+@pytest.mark.parametrize("flag", [0, 1, 2])
+def test_spai_pattern_matches_lstsq(flag):
+    # M on an alternative pattern P: column k minimises ||A[:, J] m - e_k||
+    # over all n rows, J = P's column k; numpy lstsq on the dense A pins it
+    from patterns import pattern_csc
+    for name, n, cp, ri, va, _, _ in gold_mats():
+        if n > 200:
+            continue
+        pcp, pri = pattern_csc(n, cp, ri, flag)
+        m = O.spai_pattern(n, cp, ri, va, pcp, pri, 0)
+        Ad = np.zeros((n, n))
+        for c in range(n):
+            Ad[ri[cp[c]:cp[c + 1]], c] = va[cp[c]:cp[c + 1]]
+        ref = np.zeros(len(pri))
+        for k in range(n):
+            J = pri[pcp[k]:pcp[k + 1]]
+            if len(J) == 0:
+                continue
+            e = np.zeros(n)
+            e[k] = 1.0
+            ref[pcp[k]:pcp[k + 1]] = np.linalg.lstsq(Ad[:, J], e, rcond=None)[0]
+        scale = np.max(np.abs(ref))
+        assert np.max(np.abs(m - ref)) <= 1e-11 * scale, (name, flag)
+        # Householder agrees too
+        mh = O.spai_pattern(n, cp, ri, va, pcp, pri, 1)
+        assert np.max(np.abs(mh - ref)) <= 1e-11 * scale, (name, flag)
+
+
 def test_row_sets_match_brute_force_golden():
     for name, n, cp, ri, va, _, n1_gold in gold_mats():
         n1, n2, t, e = O.column_sets(n, cp, ri)
