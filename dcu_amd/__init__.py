@@ -53,6 +53,7 @@ C_SYMBOLS = [
     "spai_system_get_row_set_sizes", "spai_solve_options_default", "spai_system_solve",
     "spai_system_outcomes", "spai_system_residual_history", "spai_system_adjoint", "spai_system_time_spmv",
     "spai_system_time_iteration", "spai_system_layout", "spai_system_set_pattern", "spai_system_m_pattern",
+    "spai_system_quality", "spai_system_product",
     "spai_calibration_copy", "spai_csc_to_csr_device",
     "spai_problem_generate", "spai_problem_generate_power_law", "spai_problem_generate_batched",
     "spai_problem_generate_grf", "spai_problem_convection_diffusion", "spai_problem_read_mtx", "spai_problem_free",
@@ -176,6 +177,8 @@ def load_library() -> ctypes.CDLL:
                                                       ctypes.POINTER(ctypes.c_float)]),
         "spai_system_layout": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)]),
         "spai_system_set_pattern": (ctypes.c_int, [vp, ctypes.c_int]),
+        "spai_system_quality": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double), vp]),
+        "spai_system_product": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_longlong), vp, vp, vp]),
         "spai_system_m_pattern": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), vp, vp]),
         "spai_calibration_copy": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_int,
                                                  ctypes.POINTER(ctypes.c_float)]),
@@ -468,6 +471,24 @@ class System:
         ri = np.empty(max(nnz.value, 1), dtype=np.int32)
         _check(self._lib.spai_system_m_pattern(self._h, None, cp.ctypes.data, ri.ctypes.data))
         return cp, ri[: nnz.value]
+
+    def quality(self) -> Tuple[float, np.ndarray]:
+        """(||A M - I||_F, ||A m_k - e_k||_2 per column) of the built M."""
+        f = ctypes.c_double()
+        cols = np.empty(self.n)
+        _check(self._lib.spai_system_quality(self._h, ctypes.byref(f), cols.ctypes.data))
+        return float(f.value), cols
+
+    def product(self) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """C = A M as CSC (colptr, rowidx, values)."""
+        nnz = ctypes.c_longlong()
+        _check(self._lib.spai_system_product(self._h, ctypes.byref(nnz), None, None, None))
+        cp = np.empty(self.n + 1, dtype=np.int32)
+        ri = np.empty(max(nnz.value, 1), dtype=np.int32)
+        va = np.empty(max(nnz.value, 1))
+        _check(self._lib.spai_system_product(self._h, ctypes.byref(nnz), cp.ctypes.data, ri.ctypes.data,
+                                             va.ctypes.data))
+        return cp, ri[: nnz.value], va[: nnz.value]
 
     def m_values(self) -> np.ndarray:
         """M's values in its pattern's CSC order (A's CSC order by default)."""

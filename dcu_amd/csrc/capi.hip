@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -638,6 +639,105 @@ int spai_system_m_pattern(const spai_system *sys, int *nnz, int *colptr, int *ro
             SPAI_HIP(hipMemcpyAsync(rowidx, own ? sys->prowidx.get() : sys->rowidx.get(), sizeof(int) * (size_t)sys->m_nnz,
                                     hipMemcpyDeviceToHost, st));
         SPAI_HIP(hipStreamSynchronize(st));
+    });
+}
+
+namespace {
+// Runs the product kernel over every column; with ccolptr set it also
+// writes C.  Caps come from the current pattern over all columns.
+void run_product(spai_system *sys, DeviceArray<double> &colres2, DeviceArray<int> &n1, const int *ccolptr,
+                 int *crow, double *cval) {
+    hipStream_t st = sys->ctx->stream;
+    const int n = sys->n;
+    const bool own = sys->pattern != SPAI_PATTERN_A;
+    const int *pc = own ? sys->pcolptr.get() : sys->colptr.get();
+    const int *pr = own ? sys->prowidx.get() : sys->rowidx.get();
+    std::vector<int> acp((size_t)n + 1), pcp((size_t)n + 1), pri((size_t)std::max(sys->m_nnz, 1));
+    SPAI_HIP(hipMemcpyAsync(acp.data(), sys->colptr.get(), sizeof(int) * acp.size(), hipMemcpyDeviceToHost, st));
+    SPAI_HIP(hipMemcpyAsync(pcp.data(), pc, sizeof(int) * pcp.size(), hipMemcpyDeviceToHost, st));
+    SPAI_HIP(hipMemcpyAsync(pri.data(), pr, sizeof(int) * (size_t)sys->m_nnz, hipMemcpyDeviceToHost, st));
+    SPAI_HIP(hipStreamSynchronize(st));
+    long long max_t = 0;
+    int max_n2 = 0;
+    for (int k = 0; k < n; ++k) {
+        long long t = 0;
+        for (int q = pcp[(size_t)k]; q < pcp[(size_t)k + 1]; ++q) t += acp[(size_t)pri[(size_t)q] + 1] - acp[(size_t)pri[(size_t)q]];
+        max_t = std::max(max_t, t);
+        max_n2 = std::max(max_n2, pcp[(size_t)k + 1] - pcp[(size_t)k]);
+    }
+    require(max_t < (1LL << 30), "product: a column gathers more than 2^30 entries");
+    SpaiProductParams p;
+    p.n = n;
+    p.colptr = sys->colptr.get();
+    p.rowidx = sys->rowidx.get();
+    p.val = sys->cscval.get();
+    p.pcolptr = pc;
+    p.prowidx = pr;
+    p.m_fp32 = sys->m_precision == SPAI_PRECISION_MIXED;
+    p.mval = p.m_fp32 ? (const void *)sys->m32_csc.get() : (const void *)sys->m64_csc.get();
+    p.max_gathered = (int)max_t;
+    p.max_n2 = max_n2;
+    colres2.resize((size_t)std::max(n, 1));
+    n1.resize((size_t)std::max(n, 1));
+    p.colres2 = colres2.get();
+    p.n1_out = n1.get();
+    p.ccolptr = ccolptr;
+    p.crow = crow;
+    p.cval = cval;
+    DeviceArray<uint64_t> keys;
+    DeviceArray<int> ints;
+    spai_product(p, st, sys->ctx->cus, keys, ints);
+}
+}  // namespace
+
+int spai_system_quality(spai_system *sys, double *frobenius, double *column_residuals) {
+    if (!sys || !frobenius) return fail(SPAI_ERR_INVALID, "sys/frobenius is NULL");
+    if (sys->m_precision < 0) return fail(SPAI_ERR_NOT_BUILT, "M has not been built");
+    return guarded([&] {
+        SPAI_HIP(hipSetDevice(sys->ctx->device));
+        DeviceArray<double> colres2;
+        DeviceArray<int> n1;
+        run_product(sys, colres2, n1, nullptr, nullptr, nullptr);
+        std::vector<double> h((size_t)sys->n);
+        SPAI_HIP(hipMemcpy(h.data(), colres2.get(), sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+        double tot = 0.0;
+        for (int k = 0; k < sys->n; ++k) {
+            tot += h[(size_t)k];
+            if (column_residuals) column_residuals[k] = std::sqrt(h[(size_t)k]);
+        }
+        *frobenius = std::sqrt(tot);
+    });
+}
+
+int spai_system_product(spai_system *sys, long long *nnz, int *colptr, int *rowidx, double *values) {
+    if (!sys || !nnz) return fail(SPAI_ERR_INVALID, "sys/nnz is NULL");
+    if (sys->m_precision < 0) return fail(SPAI_ERR_NOT_BUILT, "M has not been built");
+    return guarded([&] {
+        SPAI_HIP(hipSetDevice(sys->ctx->device));
+        hipStream_t st = sys->ctx->stream;
+        DeviceArray<double> colres2;
+        DeviceArray<int> n1;
+        run_product(sys, colres2, n1, nullptr, nullptr, nullptr);
+        std::vector<int> hn1((size_t)sys->n);
+        SPAI_HIP(hipMemcpy(hn1.data(), n1.get(), sizeof(int) * hn1.size(), hipMemcpyDeviceToHost));
+        std::vector<int> cp((size_t)sys->n + 1, 0);
+        long long total = 0;
+        for (int k = 0; k < sys->n; ++k) {
+            total += hn1[(size_t)k];
+            require(total < (long long)INT32_MAX, "product: more than 2^31-1 entries");
+            cp[(size_t)k + 1] = (int)total;
+        }
+        *nnz = total;
+        if (!colptr && !rowidx && !values) return;
+        DeviceArray<int> dcp(cp.size()), drow((size_t)std::max<long long>(total, 1));
+        DeviceArray<double> dval((size_t)std::max<long long>(total, 1));
+        SPAI_HIP(hipMemcpyAsync(dcp.get(), cp.data(), sizeof(int) * cp.size(), hipMemcpyHostToDevice, st));
+        run_product(sys, colres2, n1, dcp.get(), drow.get(), dval.get());
+        if (colptr) std::memcpy(colptr, cp.data(), sizeof(int) * cp.size());
+        if (rowidx)
+            SPAI_HIP(hipMemcpy(rowidx, drow.get(), sizeof(int) * (size_t)total, hipMemcpyDeviceToHost));
+        if (values)
+            SPAI_HIP(hipMemcpy(values, dval.get(), sizeof(double) * (size_t)total, hipMemcpyDeviceToHost));
     });
 }
 

@@ -1043,6 +1043,112 @@ void launch_tiers(const ColumnArgs &base, bool dynamic, int static_tier, int *co
     }
 }
 
+// ---------------------------------------------------------------------------
+// Product C = A M and the SPAI quality ||A M - I||_F.  Column k of C is
+// A(:, J_k) m_k, supported on the row set I_k of the SPAI build.  One
+// 512-thread workgroup per column (persistent over columns), scratch slot in
+// global memory: the gathered (row, entry) keys are bitonic-sorted, runs of
+// equal rows are summed in storage order, C(:, k) is written at ccolptr[k]
+// and ||C(:, k) - e_k||^2 is reduced.
+// ---------------------------------------------------------------------------
+struct ProductArgs {
+    const int *colptr, *rowidx;  // A (CSC)
+    const double *val;
+    const int *pcolptr, *prowidx;  // M's pattern (CSC)
+    const void *mval;              // M's values in pattern order (double or float)
+    int n;
+    const int *ccolptr;  // C's column offsets (prefix of |I_k|), nullptr: quality only
+    int *crow;
+    double *cval;
+    double *colres2;  // ||A m_k - e_k||^2 per column
+    int *n1_out;      // |I_k| per column (nullable)
+    uint64_t *keys;   // slots * key_cap
+    int *ints;        // slots * int_cap: positions (key_cap + 1), J offsets (n2_cap + 1)
+    long long key_cap, int_cap;
+};
+
+template <class MT>
+__global__ __launch_bounds__(kGlobThreads) void spai_product_kernel(ProductArgs a) {
+    __shared__ double red[64 + kGlobThreads];
+    __shared__ int ired[kGlobThreads / 64];
+    __shared__ int sh[1];
+    const int tid = threadIdx.x;
+    const MT *m = (const MT *)a.mval;
+    uint64_t *keys = a.keys + (size_t)blockIdx.x * a.key_cap;
+    int *pos = a.ints + (size_t)blockIdx.x * a.int_cap;
+    int *offs = pos + a.key_cap + 1;
+    for (int k = blockIdx.x; k < a.n; k += gridDim.x) {
+        const int c0 = a.pcolptr[k];
+        const int n2 = a.pcolptr[k + 1] - c0;
+        if (n2 <= 0) {  // M(:, k) = 0: the column's residual is ||e_k||^2 = 1
+            if (tid == 0) {
+                a.colres2[k] = 1.0;
+                if (a.n1_out) a.n1_out[k] = 0;
+            }
+            continue;
+        }
+        const int T = glob_block_scan(
+            n2, [&](int j) { int jr = a.prowidx[c0 + j]; return a.colptr[jr + 1] - a.colptr[jr]; }, offs, ired);
+        int Tp = 1;
+        while (Tp < T) Tp <<= 1;
+        for (int e = tid; e < Tp; e += kGlobThreads) {
+            uint64_t key = ~0ull;
+            if (e < T) {
+                const int j = glob_find_j(offs, n2, e);
+                const int jr = a.prowidx[c0 + j];
+                key = ((uint64_t)(unsigned)a.rowidx[a.colptr[jr] + e - offs[j]] << 32) | (unsigned)e;
+            }
+            keys[e] = key;
+        }
+        __syncthreads();
+        for (int size = 2; size <= Tp; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int q = tid; q < Tp / 2; q += kGlobThreads) {
+                    const int i = 2 * stride * (q / stride) + (q % stride);
+                    const int l = i + stride;
+                    const bool up = (i & size) == 0;
+                    const uint64_t x = keys[i], y = keys[l];
+                    if ((x > y) == up) {
+                        keys[i] = y;
+                        keys[l] = x;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // output slot of each run of equal rows: exclusive count of run starts
+        const int n1 = glob_block_scan(
+            T, [&](int e) { return (e == 0 || (keys[e] >> 32) != (keys[e - 1] >> 32)) ? 1 : 0; }, pos, ired);
+        if (tid == 0) sh[0] = 0;
+        __syncthreads();
+        double part = 0.0;
+        for (int e = tid; e < T; e += kGlobThreads) {
+            const int row = (int)(keys[e] >> 32);
+            if (e > 0 && row == (int)(keys[e - 1] >> 32)) continue;  // not a run start
+            double sum = 0.0;
+            for (int f = e; f < T && (int)(keys[f] >> 32) == row; ++f) {
+                const int ee = (int)(unsigned)keys[f];
+                const int j = glob_find_j(offs, n2, ee);
+                const int jr = a.prowidx[c0 + j];
+                sum += a.val[a.colptr[jr] + ee - offs[j]] * (double)m[c0 + j];
+            }
+            if (a.ccolptr) {
+                a.crow[a.ccolptr[k] + pos[e]] = row;
+                a.cval[a.ccolptr[k] + pos[e]] = sum;
+            }
+            const double d = row == k ? sum - 1.0 : sum;
+            part += d * d;
+            if (row == k) sh[0] = 1;
+        }
+        const double tot = glob_block_sum(part, red);
+        if (tid == 0) {
+            a.colres2[k] = tot + (sh[0] ? 0.0 : 1.0);  // e_k outside I_k contributes 1
+            if (a.n1_out) a.n1_out[k] = n1;
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 SpaiBuilder::SpaiBuilder() = default;
@@ -1175,6 +1281,41 @@ int classify_host(int T, int n2) {
     long long need = (long long)T * n2 + (long long)n2 * n2 + n2;
     if (n2 <= kLdsTierNJ && T <= kLdsTierT && need <= kLdsTierDoubles) return 3;
     return 4;
+}
+
+void spai_product(const SpaiProductParams &p, hipStream_t st, int cus, DeviceArray<uint64_t> &keys,
+                  DeviceArray<int> &ints) {
+    require(p.n >= 0 && p.colptr && p.rowidx && p.val && p.pcolptr && p.prowidx && p.mval && p.colres2,
+            "spai product: bad arguments");
+    if (p.n == 0) return;
+    ProductArgs a{};
+    a.colptr = p.colptr;
+    a.rowidx = p.rowidx;
+    a.val = p.val;
+    a.pcolptr = p.pcolptr;
+    a.prowidx = p.prowidx;
+    a.mval = p.mval;
+    a.n = p.n;
+    a.ccolptr = p.ccolptr;
+    a.crow = p.crow;
+    a.cval = p.cval;
+    a.colres2 = p.colres2;
+    a.n1_out = p.n1_out;
+    long long kp = 1;
+    while (kp < std::max(1, p.max_gathered)) kp <<= 1;
+    a.key_cap = kp;
+    a.int_cap = kp + 1 + (long long)std::max(1, p.max_n2) + 1;
+    const int slots = std::max(1, std::min(p.n, 2 * std::max(cus, 1)));
+    keys.reserve((size_t)slots * (size_t)a.key_cap);
+    ints.reserve((size_t)slots * (size_t)a.int_cap);
+    a.keys = keys.get();
+    a.ints = ints.get();
+    if (p.m_fp32)
+        hipLaunchKernelGGL(spai_product_kernel<float>, dim3(slots), dim3(kGlobThreads), 0, st, a);
+    else
+        hipLaunchKernelGGL(spai_product_kernel<double>, dim3(slots), dim3(kGlobThreads), 0, st, a);
+    SPAI_CHECK_LAUNCH();
+    SPAI_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace spai

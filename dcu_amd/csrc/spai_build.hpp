@@ -39,6 +39,27 @@ struct SpaiBuildStats {
 
 int classify_host(int T, int n2);
 
+// C = A M and ||A m_k - e_k||^2 per column (the SPAI quality ||A M - I||_F).
+// Column k of C is A(:, J_k) m_k on the row set I_k; ccolptr (prefix of
+// |I_k|, n + 1) may be null to compute the residuals only.
+struct SpaiProductParams {
+    int n = 0;
+    const int *colptr = nullptr, *rowidx = nullptr;  // A (CSC, device)
+    const double *val = nullptr;
+    const int *pcolptr = nullptr, *prowidx = nullptr;  // M's pattern (CSC, device)
+    const void *mval = nullptr;                        // M's values, pattern order
+    int m_fp32 = 0;
+    int max_gathered = 0;  // max_k T_k and max_k |J_k| (planner statistics)
+    int max_n2 = 0;
+    const int *ccolptr = nullptr;
+    int *crow = nullptr;
+    double *cval = nullptr;
+    double *colres2 = nullptr;  // n doubles
+    int *n1_out = nullptr;      // |I_k| per column (n ints, nullable)
+};
+void spai_product(const SpaiProductParams &p, hipStream_t st, int cus, DeviceArray<uint64_t> &keys,
+                  DeviceArray<int> &ints);
+
 class SpaiBuilder {
 public:
     SpaiBuilder();

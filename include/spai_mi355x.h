@@ -139,6 +139,20 @@ int spai_system_build(spai_system *sys, int precision, int schedule, int col_beg
 /* M's values in the CSC storage order of its pattern (A's, by default),
  * widened to double. */
 int spai_system_get_m(spai_system *sys, double *values, int values_on_device);
+/* SPAI quality: ||A M - I||_F of the built M (as stored: FP32 in mixed
+ * mode), and optionally ||A m_k - e_k||_2 per column (n doubles, may be
+ * NULL).  The SPAI least squares minimise each column's term.  Computed
+ * from the product below (a column of A M is A(:, J_k) m_k on I_k).  No
+ * counterpart in the reference's drivers. */
+int spai_system_quality(spai_system *sys, double *frobenius, double *column_residuals);
+
+/* C = A M in CSC (the reference's SpMMWithoutSparsity,
+ * include/common/SpMM_lib.h:15, which its drivers never call): column k is
+ * A(:, J_k) m_k on the row set I_k, rows ascending, duplicates summed in
+ * storage order.  Call with colptr = rowidx = values = NULL to get *nnz;
+ * then pass host arrays of n + 1, nnz and nnz entries. */
+int spai_system_product(spai_system *sys, long long *nnz, int *colptr, int *rowidx, double *values);
+
 /* |I_k| per column from the last build (n ints), for diagnostics. */
 int spai_system_get_row_set_sizes(spai_system *sys, int *n1);
 
