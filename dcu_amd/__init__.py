@@ -566,10 +566,11 @@ class System:
         return [float(v) for v in ms]
 
     def layout(self) -> Tuple[str, int]:
-        """("sell64" | "row_tiles", stored entries per operator)."""
+        """("sell64" | "sell64_off16" | "row_tiles", stored entries per
+        operator).  sell64_off16 stores 16-bit column - row offsets."""
         kind, entries = ctypes.c_int(), ctypes.c_longlong()
         _check(self._lib.spai_system_layout(self._h, ctypes.byref(kind), ctypes.byref(entries)))
-        return ("sell64" if kind.value == 1 else "row_tiles"), int(entries.value)
+        return {0: "row_tiles", 1: "sell64", 2: "sell64_off16"}[kind.value], int(entries.value)
 
 
 def _report(rep: SolveReport) -> dict:

@@ -217,7 +217,11 @@ void pattern_host(int n, const std::vector<int> &cp, const std::vector<int> &ri,
 
 // A in the solver's layout
 OperatorView a_view(spai_system *S) {
-    if (S->tiles.kind == kLayoutSell) return OperatorView{nullptr, S->tiles.sell_col.get(), S->sellval.get(), 0};
+    if (S->tiles.kind == kLayoutSell) {
+        OperatorView A{nullptr, S->tiles.sell_col.get(), S->sellval.get(), 0};
+        A.off = S->tiles.sell_off.get();
+        return A;
+    }
     return OperatorView{S->rowptr.get(), S->colidx.get(), S->csrval.get(), 0};
 }
 
@@ -305,6 +309,7 @@ OperatorView m_view(spai_system *S, bool transposed) {
         M.idx = own ? S->prowidx.get() : S->rowidx.get();
     } else if (sell) {
         M.idx = own ? S->msell.col.get() : S->tiles.sell_col.get();
+        M.off = own ? S->msell.off.get() : S->tiles.sell_off.get();
         M.blk = own ? S->msell.slice_blk.get() : nullptr;
     } else {
         M.ptr = own ? S->mrowptr.get() : S->rowptr.get();
@@ -907,7 +912,10 @@ int spai_system_time_iteration(spai_system *sys, int reps, int variant, float *m
 
 int spai_system_layout(const spai_system *sys, int *kind, long long *entries) {
     if (!sys) return fail(SPAI_ERR_INVALID, "sys is NULL");
-    if (kind) *kind = sys->tiles.kind == kLayoutSell ? SPAI_LAYOUT_SELL64 : SPAI_LAYOUT_ROW_TILES;
+    if (kind)
+        *kind = sys->tiles.kind != kLayoutSell ? SPAI_LAYOUT_ROW_TILES
+                : sys->tiles.sell_off.get()    ? SPAI_LAYOUT_SELL64_OFF16
+                                               : SPAI_LAYOUT_SELL64;
     if (entries) *entries = sys->tiles.value_len(sys->nnz);
     return SPAI_OK;
 }

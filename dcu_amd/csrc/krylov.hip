@@ -36,6 +36,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <type_traits>
 
@@ -477,9 +478,17 @@ constexpr int kSellUnroll = SPAI_SELL_UNROLL;  // columns in flight per slice an
 #else
 #define SPAI_SELL_OCCUPANCY
 #endif
-template <bool HAS_MATRIX, class MT, class Op>
-__global__ __launch_bounds__(kThreads) SPAI_SELL_OCCUPANCY void k_sell(const int *__restrict__ col, const MT *__restrict__ val,
-                                                   UnitView uv, Ctx c, Op op) {
+// IT = int: absolute column indices; IT = short: offsets from the lane's row.
+template <class IT>
+__device__ __forceinline__ int sell_column(int row, IT j) {
+    if constexpr (sizeof(IT) == sizeof(int)) return (int)j;
+    else return row + (int)j;
+}
+
+template <bool HAS_MATRIX, class MT, class Op, class IT>
+__global__ __launch_bounds__(kThreads) SPAI_SELL_OCCUPANCY void k_sell(const IT *__restrict__ col,
+                                                                       const MT *__restrict__ val, UnitView uv, Ctx c,
+                                                                       Op op) {
     constexpr int NRED = Op::NRED;
     constexpr int SPW = kSellSlicesPerWave;
     constexpr int U = kSellUnroll;
@@ -504,7 +513,7 @@ __global__ __launch_bounds__(kThreads) SPAI_SELL_OCCUPANCY void k_sell(const int
         w[j] = b1 - b0;
         base[j] = (long long)b0 * kSliceRows + lane;
     }
-    int jj[SPW][U];
+    IT jj[SPW][U];
     MT vv[SPW][U];
     if (HAS_MATRIX) {
 #pragma unroll
@@ -513,7 +522,7 @@ __global__ __launch_bounds__(kThreads) SPAI_SELL_OCCUPANCY void k_sell(const int
             for (int k = 0; k < U; ++k) {
                 const bool ok = k < w[j];
                 const long long q = base[j] + (long long)k * kSliceRows;
-                jj[j][k] = ok ? stream_load(&col[q]) : 0;
+                jj[j][k] = ok ? stream_load(&col[q]) : (IT)0;
                 vv[j][k] = ok ? stream_load(&val[q]) : (MT)0;
             }
     }
@@ -542,7 +551,8 @@ __global__ __launch_bounds__(kThreads) SPAI_SELL_OCCUPANCY void k_sell(const int
 #pragma unroll
             for (int j = 0; j < SPW; ++j)
 #pragma unroll
-                for (int k = 0; k < U; ++k) gv[j][k] = (k0 + k < w[j]) ? op.gather(jj[j][k]) : 0.0;
+                for (int k = 0; k < U; ++k)
+                    gv[j][k] = (k0 + k < w[j]) ? op.gather(sell_column(row[j], jj[j][k])) : 0.0;
 #pragma unroll
             for (int j = 0; j < SPW; ++j)
 #pragma unroll
@@ -557,7 +567,7 @@ __global__ __launch_bounds__(kThreads) SPAI_SELL_OCCUPANCY void k_sell(const int
                 for (int k = 0; k < U; ++k) {
                     const bool ok = k0 + k < w[j];
                     const long long q = base[j] + (long long)(k0 + k) * kSliceRows;
-                    jj[j][k] = ok ? stream_load(&col[q]) : 0;
+                    jj[j][k] = ok ? stream_load(&col[q]) : (IT)0;
                     vv[j][k] = ok ? stream_load(&val[q]) : (MT)0;
                 }
         }
@@ -926,8 +936,11 @@ __global__ void k_compact_fill(const SysState *st, const int *unit_sys, const in
 // slice sl holds row slice_row[sl] + l; its entries go to column k of the
 // slice, padding entries are (row, 0) - or (first row of the slice, 0) for a
 // lane past the system's last row - so every padded gather stays in bounds.
+// With sell_off set, each entry's column is also stored as a 16-bit offset
+// from the lane's row (slice_row[sl] + l, past the system's end too); *wide
+// is set when an offset does not fit.
 __global__ void k_sell_fill(int nslices, const int *slice_row, const int *slice_blk, const int *rowptr,
-                            const int *colidx, int *sell_col, int *sell_pos) {
+                            const int *colidx, int *sell_col, short *sell_off, int *wide, int *sell_pos) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)nslices * kSliceRows) return;
     const int sl = (int)(t / kSliceRows), lane = (int)(t % kSliceRows);
@@ -936,15 +949,25 @@ __global__ void k_sell_fill(int nslices, const int *slice_row, const int *slice_
     const int w = slice_blk[sl + 1] - slice_blk[sl];
     const long long base = (long long)slice_blk[sl] * kSliceRows + lane;
     const int p0 = live ? rowptr[row] : 0, len = live ? rowptr[row + 1] - p0 : 0;
+    bool fits = true;
     for (int k = 0; k < w; ++k) {
         const long long q = base + (long long)k * kSliceRows;
+        int c;
         if (k < len) {
-            sell_col[q] = colidx[p0 + k];
+            c = colidx[p0 + k];
             sell_pos[p0 + k] = (int)q;
         } else {
-            sell_col[q] = live ? row : r0;
+            c = live ? row : r0;
+        }
+        sell_col[q] = c;
+        if (sell_off) {
+            const int d = c - row;
+            const bool ok = d >= -32768 && d <= 32767;
+            sell_off[q] = ok ? (short)d : (short)0;
+            fits = fits && ok;
         }
     }
+    if (!fits) atomicOr(wide, 1);
 }
 
 template <class MT_OUT>
@@ -1008,9 +1031,12 @@ void launch_units(const OperatorView &O, const TileMap &tm, const Ctx &c, const 
     if (O.blk) uv.slice_blk = O.blk;  // an operator with its own pattern
     const int g = list ? grid : tm.nunits;
     if (g <= 0) return;
-    if (tm.kind == kLayoutSell)
-        hipLaunchKernelGGL((k_sell<HM, MT, Op>), dim3(g), dim3(kThreads), 0, st, O.idx, (const MT *)O.val, uv, c,
-                           op);
+    if (tm.kind == kLayoutSell && O.off)
+        hipLaunchKernelGGL((k_sell<HM, MT, Op, short>), dim3(g), dim3(kThreads), 0, st, O.off, (const MT *)O.val, uv,
+                           c, op);
+    else if (tm.kind == kLayoutSell)
+        hipLaunchKernelGGL((k_sell<HM, MT, Op, int>), dim3(g), dim3(kThreads), 0, st, O.idx, (const MT *)O.val, uv,
+                           c, op);
     else
         hipLaunchKernelGGL((k_tiles<HM, MT, Op>), dim3(g), dim3(kThreads), 0, st, O.ptr, O.idx, (const MT *)O.val,
                            uv, c, op);
@@ -1038,6 +1064,33 @@ void csc_to_csr_device(int n, int nnz, const int *colptr, const int *rowidx, con
     }
     SPAI_CHECK_LAUNCH();
     SPAI_HIP(hipStreamSynchronize(st));  // temporaries are freed on return
+}
+
+// SELL-64 column indices (and positions) of an operator on the layout's
+// slices; the 16-bit offsets are kept only when every entry's fits.  The
+// environment variable SPAI_MI355X_SELL_OFFSETS=0 keeps 32-bit indices
+// (both forms give bit-identical results).
+#ifndef SPAI_SELL_OFFSETS
+#define SPAI_SELL_OFFSETS 1
+#endif
+static void fill_sell(int nslices, const int *slice_row, const int *slice_blk, const int *rowptr, const int *colidx,
+                      long long len, DeviceArray<int> &col, DeviceArray<short> &off, int *pos, hipStream_t st) {
+    col.resize((size_t)std::max<long long>(len, 1));
+    const char *env = std::getenv("SPAI_MI355X_SELL_OFFSETS");
+    const bool want = SPAI_SELL_OFFSETS != 0 && len > 0 && !(env && env[0] == '0');
+    if (want) off.resize((size_t)len);
+    else off.release();
+    DeviceArray<int> wide(1);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, wide.get(), 0, 1LL);
+    const long long threads = (long long)nslices * kSliceRows;
+    if (threads > 0)
+        hipLaunchKernelGGL(k_sell_fill, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, nslices, slice_row,
+                           slice_blk, rowptr, colidx, col.get(), want ? off.get() : nullptr, wide.get(), pos);
+    SPAI_CHECK_LAUNCH();
+    int h_wide = 0;
+    SPAI_HIP(hipMemcpyAsync(&h_wide, wide.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+    SPAI_HIP(hipStreamSynchronize(st));
+    if (h_wide) off.release();
 }
 
 void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector<int> &h_row_off,
@@ -1169,12 +1222,9 @@ void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector
     if (sell) {
         up(tm.slice_row, srow);
         up(tm.slice_blk, sblk);
-        tm.sell_col.resize((size_t)tm.sell_len);
         tm.sell_pos.resize((size_t)std::max<long long>(total, 1));
-        const long long threads = (long long)tm.nslices * kSliceRows;
-        hipLaunchKernelGGL(k_sell_fill, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, tm.nslices,
-                           tm.slice_row.get(), tm.slice_blk.get(), ptr, idx, tm.sell_col.get(), tm.sell_pos.get());
-        SPAI_CHECK_LAUNCH();
+        fill_sell(tm.nslices, tm.slice_row.get(), tm.slice_blk.get(), ptr, idx, tm.sell_len, tm.sell_col, tm.sell_off,
+                  tm.sell_pos.get(), st);
     } else {
         tm.slice_row.resize(1);
         tm.slice_blk.resize(1);
@@ -1223,14 +1273,9 @@ void build_sell_operator(const TileMap &tm, const int *rowptr, const int *colidx
     op.len = blocks * kSliceRows;
     op.slice_blk.resize(sblk.size());
     SPAI_HIP(hipMemcpyAsync(op.slice_blk.get(), sblk.data(), sizeof(int) * sblk.size(), hipMemcpyHostToDevice, st));
-    op.col.resize((size_t)std::max<long long>(op.len, 1));
     op.pos.resize((size_t)std::max(nnz, 1));
-    const long long threads = (long long)tm.nslices * kSliceRows;
-    if (threads > 0)
-        hipLaunchKernelGGL(k_sell_fill, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, tm.nslices,
-                           tm.slice_row.get(), op.slice_blk.get(), rowptr, colidx, op.col.get(), op.pos.get());
-    SPAI_CHECK_LAUNCH();
-    SPAI_HIP(hipStreamSynchronize(st));
+    fill_sell(tm.nslices, tm.slice_row.get(), op.slice_blk.get(), rowptr, colidx, op.len, op.col, op.off,
+              op.pos.get(), st);
 }
 
 void compose_positions(int nnz, const int *perm_csr, const int *sell_pos, int *perm_out, hipStream_t st) {

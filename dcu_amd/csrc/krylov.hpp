@@ -71,6 +71,7 @@ struct TileMap {
     DeviceArray<int> slice_row;   // nslices + 1 row boundaries
     DeviceArray<int> slice_blk;   // nslices + 1 offsets in blocks of 64 entries
     DeviceArray<int> sell_col;    // sell_len column indices
+    DeviceArray<short> sell_off;  // sell_len column - row offsets; empty when one does not fit 16 bits
     DeviceArray<int> sell_pos;    // nnz: CSR position -> SELL position
     // reduction tree per system (level 0 = its units, then groups of up to
     // kFanIn nodes per level): nodes of system s start at sys_nodes[s]
@@ -141,6 +142,7 @@ struct OperatorView {
     const void *val = nullptr;
     int fp32 = 0;
     const int *blk = nullptr;  // SELL-64: the operator's own slice offsets when its pattern is not A's
+    const short *off = nullptr;  // SELL-64: 16-bit column - row offsets (used instead of idx when set)
 };
 
 // SELL-64 arrays of an operator whose pattern differs from A's (an
@@ -149,6 +151,7 @@ struct SellOperator {
     long long len = 0;            // padded entries
     DeviceArray<int> slice_blk;   // nslices + 1 offsets in blocks of 64 entries
     DeviceArray<int> col;         // len column indices
+    DeviceArray<short> off;       // len column - row offsets; empty when one does not fit 16 bits
     DeviceArray<int> pos;         // nnz: CSR position -> SELL position
 };
 

@@ -228,11 +228,14 @@ int spai_system_time_spmv(spai_system *sys, int which, int reps, float *ms);
 int spai_system_time_iteration(spai_system *sys, int reps, int variant, float *ms);
 
 /* Row layout chosen for the solver's operators: SPAI_LAYOUT_SELL64 (slices
- * of 64 rows, column-major, when padding adds at most 20%) or
+ * of 64 rows, column-major, when padding adds at most 20%; 32-bit column
+ * indices), SPAI_LAYOUT_SELL64_OFF16 (the same with 16-bit column - row
+ * offsets, when every entry of A's pattern has |column - row| < 2^15) or
  * SPAI_LAYOUT_ROW_TILES; `entries` = stored entries per operator (nnz, or
  * nnz plus padding). */
 #define SPAI_LAYOUT_ROW_TILES 0
 #define SPAI_LAYOUT_SELL64 1
+#define SPAI_LAYOUT_SELL64_OFF16 2
 int spai_system_layout(const spai_system *sys, int *kind, long long *entries);
 
 /* Average device time (ms) of a plain copy kernel that reads and writes
