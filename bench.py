@@ -52,27 +52,27 @@ def cpu_model():
     return platform.processor()
 
 
-def kernel_bytes(kind, n, nnz, m_bytes, layout):
+def kernel_bytes(kind, n, nnz, m_bytes):
     """Algorithmic HBM bytes of one launch of an iteration kernel with every
-    system active: the column index (4 B, or 2 B in the sell64_off16 layout
-    that stores column - row offsets) and the value (A: 8 B, M: m_bytes) per
-    nonzero, each FP64 vector read or written once, plus 4 B per row for the
-    row pointers when the layout is row tiles.  The padding entries of the
-    SELL-64 layout are not counted (they are overhead)."""
-    rp = 4 * n if layout == "row_tiles" else 0
-    ib = 2 if layout == "sell64_off16" else 4
+    system active, in the CSR terms of the reference's operators: per
+    nonzero a 4 B column index and the value (A: 8 B, M: m_bytes), 4 B per
+    row pointer, and each FP64 vector read or written once.  The solver's
+    SELL-64 layouts store less index data (16-bit offsets, or one offset per
+    column block of a slice) and some padding; the measured traffic is the
+    `traffic` figure."""
+    rp = 4 * (n + 1)
     if kind == "update_mapply_p":  # fused K1: x, r update + y = M p'; reads M, x, y, z, s, t, p, a; writes x, r, p', y
-        return (m_bytes + ib) * nnz + rp + 8 * n * 11
+        return (m_bytes + 4) * nnz + rp + 8 * n * 11
     if kind == "update":          # split K1, streaming half: reads x, y, z, s, t, p, a; writes x, r, p'
         return 8 * n * 10
     if kind == "mapply_p":        # split K1, y = M p'; reads M, p' (gather, once); writes y
-        return (m_bytes + ib) * nnz + rp + 8 * n * 2
+        return (m_bytes + 4) * nnz + rp + 8 * n * 2
     if kind == "spmv_alpha":      # a = A y; reads A, y (gather, once), r*; writes a
-        return (8 + ib) * nnz + rp + 8 * n * 3
+        return 12 * nnz + rp + 8 * n * 3
     if kind == "mapply_s":        # z = M s; reads M, r, a; writes s, z
-        return (m_bytes + ib) * nnz + rp + 8 * n * 4
+        return (m_bytes + 4) * nnz + rp + 8 * n * 4
     if kind == "spmv_omega":      # t = A z; reads A, z, s, r*; writes t
-        return (8 + ib) * nnz + rp + 8 * n * 4
+        return 12 * nnz + rp + 8 * n * 4
     raise ValueError(kind)
 
 
@@ -150,7 +150,7 @@ def main():
         m_bytes = 4 if precision == D.PRECISION_MIXED else 8
         print(json.dumps({"profile_kernels": True, "n": n_tot, "nnz": nnz_tot, "layout": layout,
                           "iteration_kernel_ms": kms,
-                          "algorithmic_bytes": {k: kernel_bytes(k, n_tot, nnz_tot, m_bytes, layout) for k in kms},
+                          "algorithmic_bytes": {k: kernel_bytes(k, n_tot, nnz_tot, m_bytes) for k in kms},
                           "copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes}))
         S.close()
         ctx.close()
@@ -183,14 +183,14 @@ def main():
     m_bytes = 4 if precision == D.PRECISION_MIXED else 8
     kernel_ms = iteration_kernels(S.time_iteration(20))
     dominant = max(kernel_ms, key=kernel_ms.get)
-    kb = kernel_bytes(dominant, n_tot, nnz_tot, m_bytes, layout)
+    kb = kernel_bytes(dominant, n_tot, nnz_tot, m_bytes)
     achieved = kb / (kernel_ms[dominant] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_launch": int(kb), "launch_ms": round(kernel_ms[dominant], 5),
                 "layout": layout, "stored_entries": layout_entries,
                 "all_kernels_ms": {k: round(v, 5) for k, v in kernel_ms.items()},
-                "all_kernels_frac": {k: round(kernel_bytes(k, n_tot, nnz_tot, m_bytes, layout) / (v * 1e-3) /
+                "all_kernels_frac": {k: round(kernel_bytes(k, n_tot, nnz_tot, m_bytes) / (v * 1e-3) /
                                               1e9 / HBM_PEAK_GBS, 4) for k, v in kernel_ms.items()}}
     # HBM traffic per launch from the committed counter summary of this
     # workload (two rocprofv3 --pmc passes, calibrated against a copy kernel)
@@ -209,7 +209,7 @@ def main():
     per_iter = 0.0
     for A, o in zip(mats, oc):
         it = o["iterations"] if o["status"] != D.SOLVE_MAX_ITER else o["iterations"] - 1
-        per_iter += it * sum(kernel_bytes(k, A.n, A.nnz, m_bytes, layout)
+        per_iter += it * sum(kernel_bytes(k, A.n, A.nnz, m_bytes)
                              for k in ("update_mapply_p", "spmv_alpha", "mapply_s", "spmv_omega"))
     solve_gbs = per_iter / (float(np.median(solve_ms)) * 1e-3) / 1e9
     roofline["solve_achieved"] = round(solve_gbs, 1)

@@ -566,11 +566,12 @@ class System:
         return [float(v) for v in ms]
 
     def layout(self) -> Tuple[str, int]:
-        """("sell64" | "sell64_off16" | "row_tiles", stored entries per
-        operator).  sell64_off16 stores 16-bit column - row offsets."""
+        """("sell64" | "sell64_off16" | "sell64_diag" | "row_tiles", stored
+        entries per operator).  sell64_off16 stores 16-bit column - row
+        offsets, sell64_diag one offset per column block of a slice."""
         kind, entries = ctypes.c_int(), ctypes.c_longlong()
         _check(self._lib.spai_system_layout(self._h, ctypes.byref(kind), ctypes.byref(entries)))
-        return {0: "row_tiles", 1: "sell64", 2: "sell64_off16"}[kind.value], int(entries.value)
+        return {0: "row_tiles", 1: "sell64", 2: "sell64_off16", 3: "sell64_diag"}[kind.value], int(entries.value)
 
 
 def _report(rep: SolveReport) -> dict:

@@ -220,6 +220,7 @@ OperatorView a_view(spai_system *S) {
     if (S->tiles.kind == kLayoutSell) {
         OperatorView A{nullptr, S->tiles.sell_col.get(), S->sellval.get(), 0};
         A.off = S->tiles.sell_off.get();
+        A.diag = S->tiles.sell_diag.get();
         return A;
     }
     return OperatorView{S->rowptr.get(), S->colidx.get(), S->csrval.get(), 0};
@@ -310,6 +311,7 @@ OperatorView m_view(spai_system *S, bool transposed) {
     } else if (sell) {
         M.idx = own ? S->msell.col.get() : S->tiles.sell_col.get();
         M.off = own ? S->msell.off.get() : S->tiles.sell_off.get();
+        M.diag = own ? nullptr : S->tiles.sell_diag.get();
         M.blk = own ? S->msell.slice_blk.get() : nullptr;
     } else {
         M.ptr = own ? S->mrowptr.get() : S->rowptr.get();
@@ -914,6 +916,7 @@ int spai_system_layout(const spai_system *sys, int *kind, long long *entries) {
     if (!sys) return fail(SPAI_ERR_INVALID, "sys is NULL");
     if (kind)
         *kind = sys->tiles.kind != kLayoutSell ? SPAI_LAYOUT_ROW_TILES
+                : sys->tiles.sell_diag.get()   ? SPAI_LAYOUT_SELL64_DIAG
                 : sys->tiles.sell_off.get()    ? SPAI_LAYOUT_SELL64_OFF16
                                                : SPAI_LAYOUT_SELL64;
     if (entries) *entries = sys->tiles.value_len(sys->nnz);

@@ -321,6 +321,7 @@ struct UnitView {
     const int *list;          // active units, nullptr: every unit (grid = nunits)
     const int *list_blocked;  // the same units in XCD-blocked order
     const int *count;         // number of entries of the lists
+    const int *sys_row;       // nsys + 1 row boundaries of the systems
 };
 
 // The workgroup's unit, or -1 past the end of the active list.
@@ -478,17 +479,30 @@ constexpr int kSellUnroll = SPAI_SELL_UNROLL;  // columns in flight per slice an
 #else
 #define SPAI_SELL_OCCUPANCY
 #endif
-// IT = int: absolute column indices; IT = short: offsets from the lane's row.
-template <class IT>
-__device__ __forceinline__ int sell_column(int row, IT j) {
-    if constexpr (sizeof(IT) == sizeof(int)) return (int)j;
-    else return row + (int)j;
-}
+// Column storage of a SELL-64 operator (one instance of k_sell each):
+//   kIdx32    32-bit column per entry;
+//   kIdx16    16-bit column - row offset per entry;
+//   kIdxDiag  one column - row offset per column block of a slice, shared by
+//             its 64 rows (a slice's distinct offsets in ascending order,
+//             0.0 where a row has no entry at that offset).  The gathered
+//             index row + offset is clamped into the system's rows, so an
+//             entry of 0.0 reads a row of its own system; the index does not
+//             wait for the value.
+enum { kIdx32 = 0, kIdx16 = 1, kIdxDiag = 2 };
+template <int IX>
+struct SellIndex {
+    using T = int;
+};
+template <>
+struct SellIndex<kIdx16> {
+    using T = short;
+};
 
-template <bool HAS_MATRIX, class MT, class Op, class IT>
-__global__ __launch_bounds__(kThreads) SPAI_SELL_OCCUPANCY void k_sell(const IT *__restrict__ col,
+template <bool HAS_MATRIX, class MT, class Op, int IX>
+__global__ __launch_bounds__(kThreads) SPAI_SELL_OCCUPANCY void k_sell(const typename SellIndex<IX>::T *__restrict__ col,
                                                                        const MT *__restrict__ val, UnitView uv, Ctx c,
                                                                        Op op) {
+    using IT = typename SellIndex<IX>::T;
     constexpr int NRED = Op::NRED;
     constexpr int SPW = kSellSlicesPerWave;
     constexpr int U = kSellUnroll;
@@ -498,34 +512,39 @@ __global__ __launch_bounds__(kThreads) SPAI_SELL_OCCUPANCY void k_sell(const IT 
     const int s = uv.sys[unit];
     const int lane = threadIdx.x & (kSliceRows - 1);
     const int sl_end = uv.slice[unit + 1];
-    const int sl0 = uv.slice[unit] + (int)(threadIdx.x / kSliceRows) * SPW;
-    int row[SPW], w[SPW];
+    const int sl0 = uv.slice[unit] + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kSliceRows)) * SPW;
+    int row[SPW], w[SPW], b0[SPW];
     bool live[SPW];
-    long long base[SPW];
 #pragma unroll
     for (int j = 0; j < SPW; ++j) {
         const int sl = sl0 + j;
         const bool have = sl < sl_end;
         const int r0 = have ? uv.slice_row[sl] : 0, r1 = have ? uv.slice_row[sl + 1] : 0;
-        const int b0 = have ? uv.slice_blk[sl] : 0, b1 = have ? uv.slice_blk[sl + 1] : 0;
+        b0[j] = have ? uv.slice_blk[sl] : 0;
+        w[j] = have ? uv.slice_blk[sl + 1] - b0[j] : 0;
         row[j] = r0 + lane;
         live[j] = row[j] < r1;
-        w[j] = b1 - b0;
-        base[j] = (long long)b0 * kSliceRows + lane;
+    }
+    int lo = 0, hi = 0;  // the system's rows (diagonal slices)
+    if constexpr (IX == kIdxDiag) {
+        lo = uv.sys_row[s];
+        hi = uv.sys_row[s + 1] - 1;
     }
     IT jj[SPW][U];
     MT vv[SPW][U];
-    if (HAS_MATRIX) {
+    auto load_cols = [&](int k0) {
 #pragma unroll
         for (int j = 0; j < SPW; ++j)
 #pragma unroll
             for (int k = 0; k < U; ++k) {
-                const bool ok = k < w[j];
-                const long long q = base[j] + (long long)k * kSliceRows;
-                jj[j][k] = ok ? stream_load(&col[q]) : (IT)0;
+                const bool ok = k0 + k < w[j];
+                const long long q = (long long)(b0[j] + k0 + k) * kSliceRows + lane;
+                if constexpr (IX == kIdxDiag) jj[j][k] = ok ? col[b0[j] + k0 + k] : 0;
+                else jj[j][k] = ok ? stream_load(&col[q]) : (IT)0;
                 vv[j][k] = ok ? stream_load(&val[q]) : (MT)0;
             }
-    }
+    };
+    if (HAS_MATRIX) load_cols(0);
     if (!op.enabled(c.st[s])) return;
     op.load(c.st[s]);
     double acc[NRED > 0 ? NRED : 1];
@@ -551,8 +570,13 @@ __global__ __launch_bounds__(kThreads) SPAI_SELL_OCCUPANCY void k_sell(const IT 
 #pragma unroll
             for (int j = 0; j < SPW; ++j)
 #pragma unroll
-                for (int k = 0; k < U; ++k)
-                    gv[j][k] = (k0 + k < w[j]) ? op.gather(sell_column(row[j], jj[j][k])) : 0.0;
+                for (int k = 0; k < U; ++k) {
+                    int cj;
+                    if constexpr (IX == kIdx32) cj = jj[j][k];
+                    else if constexpr (IX == kIdx16) cj = row[j] + (int)jj[j][k];
+                    else cj = min(max(row[j] + jj[j][k], lo), hi);
+                    gv[j][k] = (k0 + k < w[j]) ? op.gather(cj) : 0.0;
+                }
 #pragma unroll
             for (int j = 0; j < SPW; ++j)
 #pragma unroll
@@ -560,16 +584,7 @@ __global__ __launch_bounds__(kThreads) SPAI_SELL_OCCUPANCY void k_sell(const IT 
                     if (k0 + k < w[j]) sum[j] += (double)vv[j][k] * gv[j][k];
             k0 += U;
             if (k0 >= wmax) break;
-            // rows longer than U: the next U columns
-#pragma unroll
-            for (int j = 0; j < SPW; ++j)
-#pragma unroll
-                for (int k = 0; k < U; ++k) {
-                    const bool ok = k0 + k < w[j];
-                    const long long q = base[j] + (long long)(k0 + k) * kSliceRows;
-                    jj[j][k] = ok ? stream_load(&col[q]) : (IT)0;
-                    vv[j][k] = ok ? stream_load(&val[q]) : (MT)0;
-                }
+            load_cols(k0);  // rows longer than U: the next U columns
         }
 #pragma unroll
         for (int j = 0; j < SPW; ++j)
@@ -891,13 +906,26 @@ __global__ void k_state_activate(SysState *st, int nsys, int *counter, int nnode
     for (int b = s; b < nnodes; b += gridDim.x * blockDim.x) counter[nsys + 2 + b] = 0;
 }
 
+// The graph loop's host check: active systems and active units into
+// host-coherent memory (system-scope stores), read by the host once an event
+// after the graph has completed.
+__device__ __forceinline__ void publish_counts(int *pub, int systems, int units) {
+    __hip_atomic_store(&pub[0], systems, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&pub[1], units, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__global__ void k_publish_counts(const int *counter2, int *pub) {
+    if (threadIdx.x == 0) publish_counts(pub, counter2[0], counter2[1]);
+}
+
 // Active-unit list, two launches.  k_compact_scan (one workgroup): base[s] =
-// number of units of active systems before s, and the total into *count.
+// number of units of active systems before s, the total into counter2[1]
+// (counter2[0] holds the active systems) and, with pub, both into host memory.
 // k_compact_fill: every unit of an active system writes its id at base[s] +
 // its rank inside the system, so the list keeps the layout's unit order.
 constexpr int kCompactBlock = 1024;
 __global__ __launch_bounds__(kCompactBlock) void k_compact_scan(const SysState *st, const int *sys_units, int nsys,
-                                                                int *base, int *count) {
+                                                                int *base, int *counter2, int *pub) {
     __shared__ int sh[kCompactBlock];
     __shared__ int carry;
     if (threadIdx.x == 0) carry = 0;
@@ -918,7 +946,10 @@ __global__ __launch_bounds__(kCompactBlock) void k_compact_scan(const SysState *
         if (threadIdx.x == 0) carry += sh[kCompactBlock - 1];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *count = carry;
+    if (threadIdx.x == 0) {
+        counter2[1] = carry;
+        if (pub) publish_counts(pub, counter2[0], carry);
+    }
 }
 
 __global__ void k_compact_fill(const SysState *st, const int *unit_sys, const int *sys_units, const int *base,
@@ -930,6 +961,42 @@ __global__ void k_compact_fill(const SysState *st, const int *unit_sys, const in
     const int rank = base[s] + u - sys_units[s];
     list[rank] = u;
     list_blocked[xcd_workgroup(rank, *count)] = u;
+}
+
+// Both steps in one workgroup when the batch has at most kCompactBlock
+// systems (one launch per host check instead of two): the scan in LDS, then
+// the workgroup walks every unit.
+__global__ __launch_bounds__(kCompactBlock) void k_compact_small(const SysState *st, const int *unit_sys,
+                                                                 const int *sys_units, int nsys, int nunits,
+                                                                 int *counter2, int *list, int *list_blocked,
+                                                                 int *pub) {
+    __shared__ int sh[kCompactBlock];
+    __shared__ int base[kCompactBlock];
+    const int s = (int)threadIdx.x;
+    const int v = (s < nsys && st[s].active) ? sys_units[s + 1] - sys_units[s] : 0;
+    sh[s] = v;
+    __syncthreads();
+    for (int o = 1; o < kCompactBlock; o <<= 1) {
+        const int t = s >= o ? sh[s - o] : 0;
+        __syncthreads();
+        sh[s] += t;
+        __syncthreads();
+    }
+    base[s] = v > 0 ? sh[s] - v : -1;  // -1: inactive (or no units)
+    const int total = sh[kCompactBlock - 1];
+    __syncthreads();
+    for (int u = s; u < nunits; u += kCompactBlock) {
+        const int q = unit_sys[u];
+        const int b = base[q];
+        if (b < 0) continue;
+        const int rank = b + u - sys_units[q];
+        list[rank] = u;
+        list_blocked[xcd_workgroup(rank, total)] = u;
+    }
+    if (s == 0) {
+        counter2[1] = total;
+        if (pub) publish_counts(pub, counter2[0], total);
+    }
 }
 
 // SELL-64 construction from CSR: one thread per (slice, lane).  Lane l of
@@ -1019,7 +1086,8 @@ UnitView unit_view(const TileMap &tm, bool use_list, const int *count) {
                     tm.slice_blk.get(),
                     use_list ? tm.active.get() : nullptr,
                     use_list ? tm.active_blocked.get() : nullptr,
-                    count};
+                    count,
+                    tm.sys_row.get()};
 }
 
 // One launch of op over the layout: every unit (grid = nunits, list unused)
@@ -1031,11 +1099,14 @@ void launch_units(const OperatorView &O, const TileMap &tm, const Ctx &c, const 
     if (O.blk) uv.slice_blk = O.blk;  // an operator with its own pattern
     const int g = list ? grid : tm.nunits;
     if (g <= 0) return;
-    if (tm.kind == kLayoutSell && O.off)
-        hipLaunchKernelGGL((k_sell<HM, MT, Op, short>), dim3(g), dim3(kThreads), 0, st, O.off, (const MT *)O.val, uv,
+    if (tm.kind == kLayoutSell && O.diag)
+        hipLaunchKernelGGL((k_sell<HM, MT, Op, kIdxDiag>), dim3(g), dim3(kThreads), 0, st, O.diag, (const MT *)O.val,
+                           uv, c, op);
+    else if (tm.kind == kLayoutSell && O.off)
+        hipLaunchKernelGGL((k_sell<HM, MT, Op, kIdx16>), dim3(g), dim3(kThreads), 0, st, O.off, (const MT *)O.val, uv,
                            c, op);
     else if (tm.kind == kLayoutSell)
-        hipLaunchKernelGGL((k_sell<HM, MT, Op, int>), dim3(g), dim3(kThreads), 0, st, O.idx, (const MT *)O.val, uv,
+        hipLaunchKernelGGL((k_sell<HM, MT, Op, kIdx32>), dim3(g), dim3(kThreads), 0, st, O.idx, (const MT *)O.val, uv,
                            c, op);
     else
         hipLaunchKernelGGL((k_tiles<HM, MT, Op>), dim3(g), dim3(kThreads), 0, st, O.ptr, O.idx, (const MT *)O.val,
@@ -1066,18 +1137,66 @@ void csc_to_csr_device(int n, int nnz, const int *colptr, const int *rowidx, con
     SPAI_HIP(hipStreamSynchronize(st));  // temporaries are freed on return
 }
 
+// Most compact SELL-64 column storage allowed: the environment variable
+// SPAI_MI355X_SELL_INDEX = 32 or 16 caps it (32-bit columns, or 16-bit
+// offsets at most); unset allows diagonal slices.  Every form gives
+// bit-identical results.
+static int sell_index_cap() {
+    const char *env = std::getenv("SPAI_MI355X_SELL_INDEX");
+    if (env && std::strcmp(env, "32") == 0) return kIdx32;
+    if (env && std::strcmp(env, "16") == 0) return kIdx16;
+    return kIdxDiag;
+}
+
+// Diagonal slices: the distinct column - row offsets of each slice's rows,
+// ascending, become its column blocks, so a row's entries keep their CSR
+// (ascending column) order and the gaps hold 0.0.  Taken when they add at
+// most 10% to the blocks of the row-length padding (stencil matrices add
+// none).  On success dblk (nslices + 1 block offsets), doff (one offset per
+// block) and pos (CSR position -> layout position) are filled.
+static bool build_diagonal_slices(const std::vector<int> &srow, const std::vector<int> &hp,
+                                  const std::vector<int> &hidx, long long sell_blocks, std::vector<int> &dblk,
+                                  std::vector<int> &doff, std::vector<int> &pos) {
+    const int nslices = (int)srow.size() - 1;  // srow: nslices + 1 row boundaries
+    dblk.assign((size_t)nslices + 1, 0);
+    doff.clear();
+    std::vector<int> d;
+    long long blocks = 0;
+    for (int k = 0; k < nslices; ++k) {
+        const int r0 = srow[(size_t)k], r1 = srow[(size_t)k + 1];
+        d.clear();
+        for (int i = r0; i < r1; ++i)
+            for (int p = hp[(size_t)i]; p < hp[(size_t)i + 1]; ++p) d.push_back(hidx[(size_t)p] - i);
+        std::sort(d.begin(), d.end());
+        d.erase(std::unique(d.begin(), d.end()), d.end());
+        blocks += (long long)d.size();
+        if (blocks * 10 > sell_blocks * 11) return false;
+        dblk[(size_t)k + 1] = (int)blocks;
+        doff.insert(doff.end(), d.begin(), d.end());
+    }
+    if (blocks * kSliceRows >= (long long)INT32_MAX) return false;
+    pos.assign(hidx.size(), 0);
+    for (int k = 0; k < nslices; ++k) {
+        const int r0 = srow[(size_t)k], r1 = srow[(size_t)k + 1];
+        const int *o0 = doff.data() + dblk[(size_t)k], *o1 = doff.data() + dblk[(size_t)k + 1];
+        for (int i = r0; i < r1; ++i)
+            for (int p = hp[(size_t)i]; p < hp[(size_t)i + 1]; ++p) {
+                const int slot = (int)(std::lower_bound(o0, o1, hidx[(size_t)p] - i) - o0);
+                pos[(size_t)p] = (dblk[(size_t)k] + slot) * kSliceRows + (i - r0);
+            }
+    }
+    return true;
+}
+
 // SELL-64 column indices (and positions) of an operator on the layout's
-// slices; the 16-bit offsets are kept only when every entry's fits.  The
-// environment variable SPAI_MI355X_SELL_OFFSETS=0 keeps 32-bit indices
-// (both forms give bit-identical results).
+// slices; the 16-bit offsets are kept only when every entry's fits.
 #ifndef SPAI_SELL_OFFSETS
 #define SPAI_SELL_OFFSETS 1
 #endif
 static void fill_sell(int nslices, const int *slice_row, const int *slice_blk, const int *rowptr, const int *colidx,
                       long long len, DeviceArray<int> &col, DeviceArray<short> &off, int *pos, hipStream_t st) {
     col.resize((size_t)std::max<long long>(len, 1));
-    const char *env = std::getenv("SPAI_MI355X_SELL_OFFSETS");
-    const bool want = SPAI_SELL_OFFSETS != 0 && len > 0 && !(env && env[0] == '0');
+    const bool want = SPAI_SELL_OFFSETS != 0 && len > 0 && sell_index_cap() != kIdx32;
     if (want) off.resize((size_t)len);
     else off.release();
     DeviceArray<int> wide(1);
@@ -1212,6 +1331,7 @@ void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector
         if (!h.empty()) SPAI_HIP(hipMemcpyAsync(d.get(), h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, st));
     };
     up(tm.unit_row, urow);
+    up(tm.sys_row, h_row_off);
     up(tm.unit_sys, usys);
     up(tm.sys_units, tm.h_sys_units);
     up(tm.unit_slice, uslice);
@@ -1221,10 +1341,29 @@ void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector
     tm.sys_base.resize((size_t)nsys);
     if (sell) {
         up(tm.slice_row, srow);
-        up(tm.slice_blk, sblk);
+        std::vector<int> dblk, doff, hpos;
+        bool diag = false;
+        if (sell_index_cap() == kIdxDiag && !ptr2) {
+            std::vector<int> hidx((size_t)total);
+            SPAI_HIP(hipMemcpyAsync(hidx.data(), idx, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost, st));
+            SPAI_HIP(hipStreamSynchronize(st));
+            diag = build_diagonal_slices(srow, hp, hidx, (long long)sblk.back(), dblk, doff, hpos);
+        }
         tm.sell_pos.resize((size_t)std::max<long long>(total, 1));
-        fill_sell(tm.nslices, tm.slice_row.get(), tm.slice_blk.get(), ptr, idx, tm.sell_len, tm.sell_col, tm.sell_off,
-                  tm.sell_pos.get(), st);
+        if (diag) {
+            tm.sell_len = (long long)dblk.back() * kSliceRows;
+            up(tm.slice_blk, dblk);
+            up(tm.sell_diag, doff);
+            up(tm.sell_pos, hpos);
+            SPAI_HIP(hipStreamSynchronize(st));  // the host vectors end with this block
+            tm.sell_col.release();
+            tm.sell_off.release();
+        } else {
+            tm.sell_diag.release();
+            up(tm.slice_blk, sblk);
+            fill_sell(tm.nslices, tm.slice_row.get(), tm.slice_blk.get(), ptr, idx, tm.sell_len, tm.sell_col,
+                      tm.sell_off, tm.sell_pos.get(), st);
+        }
     } else {
         tm.slice_row.resize(1);
         tm.slice_blk.resize(1);
@@ -1310,7 +1449,12 @@ void convert_f32_to_f64(const float *in, double *out, long long n, hipStream_t s
 }
 
 BiCGStab::BiCGStab() = default;
-BiCGStab::~BiCGStab() { drop_graph(); }
+BiCGStab::~BiCGStab() {
+    drop_graph();
+    for (auto &e : check_ev_)
+        if (e) (void)hipEventDestroy(e);
+    if (h_pub_) (void)hipHostFree(h_pub_);
+}
 
 void BiCGStab::drop_graph() {
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
@@ -1361,12 +1505,18 @@ void BiCGStab::ensure(int n, int nsys, int nunits, int nnodes) {
 }
 
 // Rebuilds the active-unit list from the system states (stream-ordered).
-void BiCGStab::compact(const TileMap &tm, int nsys, hipStream_t st) {
-    hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(kCompactBlock), 0, st, state_.get(), tm.sys_units.get(), nsys,
-                       tm.sys_base.get(), counter_.get() + nsys + 1);
-    hipLaunchKernelGGL(k_compact_fill, dim3(ceil_div(tm.nunits, 256)), dim3(256), 0, st, state_.get(),
-                       tm.unit_sys.get(), tm.sys_units.get(), tm.sys_base.get(), counter_.get() + nsys + 1,
-                       tm.nunits, tm.active.get(), tm.active_blocked.get());
+void BiCGStab::compact(const TileMap &tm, int nsys, hipStream_t st, int *pub) {
+    if (nsys <= kCompactBlock) {
+        hipLaunchKernelGGL(k_compact_small, dim3(1), dim3(kCompactBlock), 0, st, state_.get(), tm.unit_sys.get(),
+                           tm.sys_units.get(), nsys, tm.nunits, counter_.get() + nsys, tm.active.get(),
+                           tm.active_blocked.get(), pub);
+    } else {
+        hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(kCompactBlock), 0, st, state_.get(), tm.sys_units.get(),
+                           nsys, tm.sys_base.get(), counter_.get() + nsys, pub);
+        hipLaunchKernelGGL(k_compact_fill, dim3(ceil_div(tm.nunits, 256)), dim3(256), 0, st, state_.get(),
+                           tm.unit_sys.get(), tm.sys_units.get(), tm.sys_base.get(), counter_.get() + nsys + 1,
+                           tm.nunits, tm.active.get(), tm.active_blocked.get());
+    }
     SPAI_CHECK_LAUNCH();
 }
 
@@ -1427,7 +1577,7 @@ void BiCGStab::enqueue_iteration(const OperatorView &A, const OperatorView &M, c
 
 void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap &tm, int n, int nsys,
                      const double *b, double *x, const SolveOptions &opt, hipStream_t st, SolveResult *res) {
-    require(n > 0 && nsys >= 1 && A.idx && A.val && b && x, "bicgstab: bad arguments");
+    require(n > 0 && nsys >= 1 && (A.idx || A.diag) && A.val && b && x, "bicgstab: bad arguments");
     require(tm.kind == kLayoutSell || A.ptr, "bicgstab: row tiles need a pointer array");
     require(tm.nunits > 0 && (int)tm.h_sys_units.size() == nsys + 1, "bicgstab: layout does not match the systems");
     require(opt.max_iter >= 1, "bicgstab: max_iter must be >= 1");
@@ -1457,6 +1607,12 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     // A batch walks the active-unit list, rebuilt after every host check; a
     // single system walks every unit.
     const bool use_list = nsys > 1;
+    for (auto &e : check_ev_)
+        if (!e) SPAI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (!h_pub_) {
+        SPAI_HIP(hipHostMalloc((void **)&h_pub_, 64, hipHostMallocMapped | hipHostMallocCoherent));
+        SPAI_HIP(hipHostGetDevicePointer((void **)&d_pub_, h_pub_, 0));
+    }
     int active = nsys;
     int units = tm.nunits;
     int checks = 0;
@@ -1480,35 +1636,76 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                 const int gi = std::max(2, opt.graph_iters + (opt.graph_iters & 1));
                 const void *key[7] = {A.val, M.val, tm.unit_row.get(), b, x, (const void *)(size_t)M.fp32,
                                       hist_ptr_};
-                while (active > 0 && done_iters < max_launch_iters) {
-                    // the graph's grid follows the active-unit count: captured
-                    // again when it has fallen below 5/8 of the captured grid
-                    const int grid = use_list ? units : 0;
-                    const bool split = M.val && units >= kSplitUnits;
-                    bool same = graph_exec_ && gi == graph_iters_ && graph_compact_ == (int)use_list &&
-                                graph_split_ == (int)split;
-                    for (int q = 0; q < 7 && same; ++q) same = key[q] == graph_key_[q];
-                    if (same && use_list) same = grid <= graph_grid_ && grid * 8 >= graph_grid_ * 5;
-                    if (!same) {
-                        drop_graph();
-                        hipStream_t cap;
-                        SPAI_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-                        SPAI_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-                        for (int k = 0; k < gi; ++k) enqueue_iteration(A, M, tm, k & 1, grid, split, cap);
-                        if (use_list) compact(tm, nsys, cap);
-                        SPAI_HIP(hipStreamEndCapture(cap, &graph_));
-                        SPAI_HIP(hipStreamDestroy(cap));
-                        SPAI_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
-                        graph_iters_ = gi;
-                        graph_grid_ = grid;
-                        graph_compact_ = (int)use_list;
-                        graph_split_ = (int)split;
-                        for (int q = 0; q < 7; ++q) graph_key_[q] = key[q];
+                // One graph runs ahead of the host check: graph k + 1 is
+                // queued before the counts copied after graph k are read, so
+                // the device does not wait for the host between graphs, and a
+                // new capture overlaps the graph in flight.  The counts only
+                // fall while iterating, so a grid sized from counts one graph
+                // old covers every active unit; a graph launched after the
+                // last system stopped finds every unit inactive.
+                std::vector<hipGraphExec_t> retired;  // replaced executables that may still be in flight
+                int slot = 0, pending = 0;
+                auto read_published = [&]() {
+                    ++checks;
+                    active = __atomic_load_n(&h_pub_[0], __ATOMIC_ACQUIRE);
+                    units = use_list ? __atomic_load_n(&h_pub_[1], __ATOMIC_ACQUIRE) : tm.nunits;
+                };
+                auto finish_checks = [&]() {
+                    SPAI_HIP(hipStreamSynchronize(st));
+                    for (auto g : retired) (void)hipGraphExecDestroy(g);
+                    retired.clear();
+                };
+                try {
+                    while (active > 0 && done_iters < max_launch_iters) {
+                        // the graph's grid follows the active-unit count: captured
+                        // again when it has fallen below 5/8 of the captured grid
+                        const int grid = use_list ? units : 0;
+                        const bool split = M.val && units >= kSplitUnits;
+                        bool same = graph_exec_ && gi == graph_iters_ && graph_compact_ == (int)use_list &&
+                                    graph_split_ == (int)split;
+                        for (int q = 0; q < 7 && same; ++q) same = key[q] == graph_key_[q];
+                        if (same && use_list) same = grid <= graph_grid_ && grid * 8 >= graph_grid_ * 5;
+                        if (!same) {
+                            if (graph_exec_) retired.push_back(graph_exec_);
+                            graph_exec_ = nullptr;
+                            drop_graph();
+                            hipStream_t cap;
+                            SPAI_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+                            SPAI_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+                            for (int k = 0; k < gi; ++k) enqueue_iteration(A, M, tm, k & 1, grid, split, cap);
+                            if (use_list) compact(tm, nsys, cap, d_pub_);
+                            else
+                                hipLaunchKernelGGL(k_publish_counts, dim3(1), dim3(64), 0, cap, counter_.get() + nsys,
+                                                   d_pub_);
+                            SPAI_HIP(hipStreamEndCapture(cap, &graph_));
+                            SPAI_HIP(hipStreamDestroy(cap));
+                            SPAI_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+                            graph_iters_ = gi;
+                            graph_grid_ = grid;
+                            graph_compact_ = (int)use_list;
+                            graph_split_ = (int)split;
+                            for (int q = 0; q < 7; ++q) graph_key_[q] = key[q];
+                        }
+                        SPAI_HIP(hipGraphLaunch(graph_exec_, st));
+                        done_iters += gi;
+                        SPAI_HIP(hipEventRecord(check_ev_[slot], st));
+                        slot ^= 1;
+                        if (++pending == 2) {
+                            // graph k has published its counts (graph k + 1,
+                            // running, may already have published newer ones:
+                            // either is valid for sizing graph k + 2)
+                            SPAI_HIP(hipEventSynchronize(check_ev_[slot]));
+                            --pending;
+                            read_published();
+                        }
                     }
-                    SPAI_HIP(hipGraphLaunch(graph_exec_, st));
-                    done_iters += gi;
-                    read_counts();
+                } catch (...) {
+                    (void)hipStreamSynchronize(st);
+                    for (auto g : retired) (void)hipGraphExecDestroy(g);
+                    throw;
                 }
+                finish_checks();
+                if (pending) read_published();  // the newest graph's counts
             } else {
                 const int every = std::max(1, opt.check_every);
                 int parity = 0;

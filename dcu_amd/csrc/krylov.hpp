@@ -63,6 +63,7 @@ struct TileMap {
     DeviceArray<int> unit_row;   // nunits + 1 row boundaries
     DeviceArray<int> unit_sys;   // system of each unit
     DeviceArray<int> sys_units;  // nsys + 1 unit offsets per system
+    DeviceArray<int> sys_row;    // nsys + 1 row offsets per system
     std::vector<int> h_sys_units;
     // SELL-64 only
     int nslices = 0;
@@ -72,6 +73,7 @@ struct TileMap {
     DeviceArray<int> slice_blk;   // nslices + 1 offsets in blocks of 64 entries
     DeviceArray<int> sell_col;    // sell_len column indices
     DeviceArray<short> sell_off;  // sell_len column - row offsets; empty when one does not fit 16 bits
+    DeviceArray<int> sell_diag;   // diagonal slices: one column - row offset per block (sell_col, sell_off empty)
     DeviceArray<int> sell_pos;    // nnz: CSR position -> SELL position
     // reduction tree per system (level 0 = its units, then groups of up to
     // kFanIn nodes per level): nodes of system s start at sys_nodes[s]
@@ -143,6 +145,7 @@ struct OperatorView {
     int fp32 = 0;
     const int *blk = nullptr;  // SELL-64: the operator's own slice offsets when its pattern is not A's
     const short *off = nullptr;  // SELL-64: 16-bit column - row offsets (used instead of idx when set)
+    const int *diag = nullptr;   // SELL-64 diagonal slices: one offset per block (used instead of idx/off)
 };
 
 // SELL-64 arrays of an operator whose pattern differs from A's (an
@@ -194,7 +197,7 @@ public:
 
 private:
     void ensure(int n, int nsys, int nunits, int nnodes);
-    void compact(const TileMap &tm, int nsys, hipStream_t st);
+    void compact(const TileMap &tm, int nsys, hipStream_t st, int *pub = nullptr);
     Ctx make_ctx(const TileMap &tm, int nsys);
     void enqueue_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tiles, int parity, int grid,
                            bool split, hipStream_t st);
@@ -219,6 +222,8 @@ private:
     int graph_grid_ = 0;
     int graph_compact_ = 0;
     int graph_split_ = 0;
+    hipEvent_t check_ev_[2] = {nullptr, nullptr};  // host checks of the graph loop
+    int *h_pub_ = nullptr, *d_pub_ = nullptr;      // counts published by the graphs (host-coherent)
     void drop_graph();
     DeviceArray<double> hist_;
     double *hist_ptr_ = nullptr;

@@ -230,12 +230,15 @@ int spai_system_time_iteration(spai_system *sys, int reps, int variant, float *m
 /* Row layout chosen for the solver's operators: SPAI_LAYOUT_SELL64 (slices
  * of 64 rows, column-major, when padding adds at most 20%; 32-bit column
  * indices), SPAI_LAYOUT_SELL64_OFF16 (the same with 16-bit column - row
- * offsets, when every entry of A's pattern has |column - row| < 2^15) or
- * SPAI_LAYOUT_ROW_TILES; `entries` = stored entries per operator (nnz, or
- * nnz plus padding). */
+ * offsets, when every entry of A's pattern has |column - row| < 2^15),
+ * SPAI_LAYOUT_SELL64_DIAG (one column - row offset per column block of a
+ * slice, shared by its 64 rows, when a slice's distinct offsets add at most
+ * 10% to the row-length padding: stencil matrices) or SPAI_LAYOUT_ROW_TILES;
+ * `entries` = stored entries per operator (nnz plus padding, or nnz). */
 #define SPAI_LAYOUT_ROW_TILES 0
 #define SPAI_LAYOUT_SELL64 1
 #define SPAI_LAYOUT_SELL64_OFF16 2
+#define SPAI_LAYOUT_SELL64_DIAG 3
 int spai_system_layout(const spai_system *sys, int *kind, long long *entries);
 
 /* Average device time (ms) of a plain copy kernel that reads and writes

@@ -1,13 +1,15 @@
-"""16-bit column offsets in the SELL-64 layout.
+"""Column storage of the SELL-64 layout.
 
-The layout stores each entry's column as column - row in 16 bits when every
-entry of the pattern fits (|column - row| <= 2^15 - 1, or -2^15), otherwise
-32-bit columns.  Both forms gather the same values in the same order, so a
-solve must give bit-identical x, iteration counts and M with either
-(SPAI_MI355X_SELL_OFFSETS=0 forces 32-bit columns).  The edge of the 16-bit
-range is pinned with a matrix whose longest reach is 2^15 - 1 (fits) or 2^15
-(does not), each checked against the oracle's iteration count and the true
-residual.
+Three forms, the most compact that applies: diagonal slices (one column -
+row offset per column block of a slice, shared by its 64 rows, when a
+slice's distinct offsets add at most 10% to the row-length padding), 16-bit
+column - row offsets per entry (when every entry has |column - row| < 2^15)
+or 32-bit columns.  All gather the same values in the same order (diagonal
+slices add exact zeros), so a solve must give bit-identical x, iteration
+counts and M with each; SPAI_MI355X_SELL_INDEX = 16 or 32 caps the form.
+The edge of the 16-bit range is pinned with a matrix whose longest reach is
+2^15 - 1 (fits) or 2^15 (does not), and diagonal slices with a reach far past
+it, each checked against the oracle's iteration count and the true residual.
 """
 import os
 
@@ -22,39 +24,51 @@ from parity import count_ok, oracle_counts
 pytestmark = pytest.mark.gpu
 
 
-def build_system(ctx, mats, offsets):
-    old = os.environ.get("SPAI_MI355X_SELL_OFFSETS")
+def build_system(ctx, mats, cap=None):
+    old = os.environ.get("SPAI_MI355X_SELL_INDEX")
     try:
-        if offsets:
-            os.environ.pop("SPAI_MI355X_SELL_OFFSETS", None)
+        if cap is None:
+            os.environ.pop("SPAI_MI355X_SELL_INDEX", None)
         else:
-            os.environ["SPAI_MI355X_SELL_OFFSETS"] = "0"
+            os.environ["SPAI_MI355X_SELL_INDEX"] = str(cap)
         return D.System(ctx, mats)
     finally:
         if old is None:
-            os.environ.pop("SPAI_MI355X_SELL_OFFSETS", None)
+            os.environ.pop("SPAI_MI355X_SELL_INDEX", None)
         else:
-            os.environ["SPAI_MI355X_SELL_OFFSETS"] = old
+            os.environ["SPAI_MI355X_SELL_INDEX"] = old
 
 
 @pytest.mark.parametrize("precision", [D.PRECISION_FP64, D.PRECISION_MIXED])
-def test_offsets_bit_identical_batched(ctx, precision):
+def test_forms_bit_identical_batched(ctx, precision):
     P = D.Problem.batched(6, 11)
     mats = P.systems()
     b = np.concatenate([P.rhs(s) for s in range(len(mats))])
     out = {}
-    for offsets in (True, False):
-        S = build_system(ctx, mats, offsets)
+    for cap in (None, 16, 32):
+        S = build_system(ctx, mats, cap)
         S.build(precision, D.SCHEDULE_BALANCED_NNZ)
         x, rep, oc = S.solve(b, rel_tol=1e-8, use_graph=True, true_residual_restart=True)
-        out[offsets] = (S.layout()[0], S.m_values(), x, [o["iterations"] for o in oc], [o["status"] for o in oc])
+        out[cap] = (S.layout(), S.m_values(), x, [o["iterations"] for o in oc], [o["status"] for o in oc])
         S.close()
-    print(f"layouts {out[True][0]} / {out[False][0]}, iterations {out[True][3]}", flush=True)
-    assert out[True][0] == "sell64_off16" and out[False][0] == "sell64"
-    assert np.array_equal(out[True][1], out[False][1])
-    assert np.array_equal(out[True][2], out[False][2])
-    assert out[True][3] == out[False][3]
-    assert all(st == D.SOLVE_CONVERGED for st in out[True][4])
+    print(f"layouts {[out[c][0] for c in out]}, iterations {out[None][3]}", flush=True)
+    assert [out[c][0][0] for c in (None, 16, 32)] == ["sell64_diag", "sell64_off16", "sell64"]
+    # stencil slices: the distinct offsets barely exceed the row-length
+    # padding (a slice whose rows miss different neighbours adds a block)
+    assert out[16][0][1] <= out[None][0][1] <= 1.1 * out[16][0][1]
+    for cap in (16, 32):
+        assert np.array_equal(out[None][1], out[cap][1])
+        assert np.array_equal(out[None][2], out[cap][2])
+        assert out[None][3] == out[cap][3]
+    assert all(st == D.SOLVE_CONVERGED for st in out[None][4])
+
+
+def test_diagonal_slices_rejected_for_scattered_rows(ctx):
+    # random columns: a slice's distinct offsets far exceed its row length
+    A = D.Problem.generate(1, 1).system(0)
+    S = build_system(ctx, A)
+    assert S.layout()[0] == "sell64_off16"
+    S.close()
 
 
 def reach_matrix(n, reach):
@@ -70,12 +84,12 @@ def reach_matrix(n, reach):
     return D.CSC(n, A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64))
 
 
-@pytest.mark.parametrize("reach,layout", [(32767, "sell64_off16"), (32768, "sell64")])
-def test_offsets_range_edge(ctx, reach, layout):
-    n = 40000
+@pytest.mark.parametrize("n,reach,cap,layout", [(40000, 32767, 16, "sell64_off16"), (40000, 32768, 16, "sell64"),
+                                                 (150000, 100000, None, "sell64_diag")])
+def test_offsets_range_edge(ctx, n, reach, cap, layout):
     A = reach_matrix(n, reach)
     b = np.random.default_rng(reach).uniform(-1, 1, n)
-    S = build_system(ctx, A, True)
+    S = build_system(ctx, A, cap)
     S.build(D.PRECISION_FP64, D.SCHEDULE_BALANCED_NNZ)
     assert S.layout()[0] == layout
     x, rep, oc = S.solve(b, rel_tol=1e-10, max_iter=1000)
