@@ -79,9 +79,9 @@ struct TileMap {
     // kFanIn nodes per level): nodes of system s start at sys_nodes[s]
     int nnodes = 0;
     DeviceArray<int> sys_nodes;  // nsys + 1
-    // active-unit list: [0, nunits) unit ids, in layout order and in
-    // XCD-blocked order; compaction scratch per system
-    DeviceArray<int> active, active_blocked;
+    // active-unit list: [0, nunits) unit ids in XCD-blocked order;
+    // compaction scratch per system
+    DeviceArray<int> active;
     DeviceArray<int> sys_base;
     // entries of an operator's value array in this layout
     long long value_len(long long nnz) const { return kind == kLayoutSell ? sell_len : nnz; }
