@@ -44,23 +44,32 @@ namespace {
 
 constexpr int kBlock = 256;  // threads per workgroup for the column kernels
 
-// Per-tier group shapes: W lanes per column, EPL gathered entries (and tile
-// rows) per lane, NJ = largest |J_k| the tier holds in registers.
-template <int W_, int EPL_, int NJ_>
+// Per-tier group shapes: W lanes per column, EPL gathered entries per lane,
+// NJ = largest |J_k| the tier holds in registers, RPL tile rows per lane.
+// A column is routed by T_k and |J_k|; its |I_k| is known only once the
+// gathered rows are merged.  When |I_k| exceeds the tier's ROWS (< TMAX)
+// the column goes to the next tier's overflow list, solved after the
+// tiers' own lists.  The row tiles are sized for the common case instead of
+// the largest T_k: the 3D 7-point stencil gathers T = 49 entries into
+// |I_k| = 25 rows, the 2D 5-point 25 into 13.
+template <int W_, int EPL_, int NJ_, int RPL_>
 struct RegTier {
     static constexpr int W = W_;
     static constexpr int EPL = EPL_;
     static constexpr int NJ = NJ_;
-    static constexpr int ROWS = W * EPL;     // largest |I_k|
+    static constexpr int RPL = RPL_;
+    static constexpr int ROWS = W * RPL;     // largest |I_k|
     static constexpr int TMAX = W * EPL;     // largest T_k
     static constexpr int TS = NJ + 1;        // LDS tile row stride (doubles)
     static constexpr int GROUPS = kBlock / W;
     static constexpr int LDS_DOUBLES = ROWS * TS + NJ * NJ + NJ;
 };
 
-using Tier0 = RegTier<16, 2, 8>;
-using Tier1 = RegTier<32, 2, 8>;
-using Tier2 = RegTier<64, 2, 16>;
+using Tier0 = RegTier<16, 2, 8, 1>;
+using Tier1 = RegTier<32, 2, 8, 1>;
+using Tier2 = RegTier<64, 2, 16, 2>;
+static_assert(Tier0::TMAX <= Tier1::ROWS && Tier1::TMAX <= Tier2::ROWS && Tier2::TMAX <= Tier2::ROWS,
+              "an overflowed column must fit the next tier's rows");
 
 // The LDS tier: one wavefront per column, tile in LDS, FP64 arithmetic.
 constexpr int kLdsTierT = 1024;        // largest T_k
@@ -83,6 +92,8 @@ struct ColumnArgs {
     const int *count; // dynamic: number of columns in list
     int *head;        // dynamic: queue head
     int col_begin, col_end;  // static range
+    int *ovf_list;    // columns with |I_k| above the tier's rows go to the next tier's list
+    int *ovf_count;
 };
 
 // 64-bit shuffle helpers on top of the 32-bit cross-lane permute.
@@ -163,7 +174,7 @@ __device__ __forceinline__ void store_m(const ColumnArgs &a, int pos, double x) 
 template <class TR, class QT, bool REFINE, class MT>
 __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *lds) {
     constexpr int W = TR::W, EPL = TR::EPL, NJ = TR::NJ, TS = TR::TS;
-    constexpr int RPL = EPL;
+    constexpr int RPL = TR::RPL;
     double *tile = lds;                       // ROWS x TS
     double *Rs = lds + TR::ROWS * TS;         // NJ x NJ
     double *ys = Rs + NJ * NJ;                // NJ
@@ -221,6 +232,10 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
         if (key[i] != ~0ull && rowv[i] == k) e_pos = pos[i];
     }
     const int E = group_max<W>(e_pos);
+    if (TR::ROWS < TR::TMAX && n1 > TR::ROWS) {  // the next tier takes it
+        if (lane == 0) a.ovf_list[atomicAdd(a.ovf_count, 1)] = k;
+        return;
+    }
 
     // zero the rows this lane owns, then scatter A(I_k, J_k) into LDS
 #pragma unroll
@@ -986,23 +1001,40 @@ int persistent_grid(KernelT kernel, int block, int device_cus) {
     return per_cu * device_cus;
 }
 
+// counters beyond the planner's: overflow count and queue head for the
+// overflow lists of tiers 1 and 2; lists: kTiers planner lists, then the two
+// overflow lists
+constexpr int kOvfCount = 2 * kTiers + 3;
+constexpr int kOvfHead = kOvfCount + 2;
+constexpr int kCounters = kOvfHead + 2;
+constexpr int kLists = kTiers + 2;
+
 template <class MT, bool REFINE>
 void launch_tiers(const ColumnArgs &base, bool dynamic, int static_tier, int *counters, int *lists,
                   int list_stride, int cus, hipStream_t st) {
     using QT = typename std::conditional<REFINE, float, double>::type;
     ColumnArgs a = base;
-    auto setup = [&](int t) {
+    // t: tier; ovf: run over the tier's overflow list instead of its own
+    auto setup = [&](int t, bool ovf) {
         a = base;
-        if (dynamic) {
+        if (ovf) {
+            a.list = lists + (size_t)(kTiers + t - 1) * list_stride;
+            a.count = counters + kOvfCount + t - 1;
+            a.head = counters + kOvfHead + t - 1;
+        } else if (dynamic) {
             a.list = lists + (size_t)t * list_stride;
             a.count = counters + t;
             a.head = counters + kTiers + t;
         }
+        if (t < 2) {  // tiers 0 and 1 hand wider row sets on
+            a.ovf_list = lists + (size_t)(kTiers + t) * list_stride;
+            a.ovf_count = counters + kOvfCount + t;
+        }
     };
-    auto run_reg = [&](auto tier_tag, int t) {
+    auto run_reg = [&](auto tier_tag, int t, bool ovf = false) {
         using TR = decltype(tier_tag);
-        setup(t);
-        if (dynamic) {
+        setup(t, ovf);
+        if (dynamic || ovf) {
             auto k = spai_reg_tier_kernel<TR, QT, REFINE, MT, true>;
             hipLaunchKernelGGL(k, dim3(persistent_grid(k, kBlock, cus)), dim3(kBlock), 0, st, a);
         } else {
@@ -1015,7 +1047,7 @@ void launch_tiers(const ColumnArgs &base, bool dynamic, int static_tier, int *co
         SPAI_CHECK_LAUNCH();
     };
     auto run_lds = [&](int t) {
-        setup(t);
+        setup(t, false);
         if (dynamic) {
             auto k = spai_lds_tier_kernel<MT, true>;
             hipLaunchKernelGGL(k, dim3(persistent_grid(k, 64, cus)), dim3(64), 0, st, a);
@@ -1041,6 +1073,9 @@ void launch_tiers(const ColumnArgs &base, bool dynamic, int static_tier, int *co
         default: run_lds(3); break;
         }
     }
+    // the overflow lists (each run exits at once when its list is empty)
+    if (dynamic || static_tier == 0) run_reg(Tier1{}, 1, true);
+    if (dynamic || static_tier <= 1) run_reg(Tier2{}, 2, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -1165,11 +1200,11 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
     // J_k from M's pattern (A's own unless another pattern is given)
     const int *pc = p.pcolptr ? p.pcolptr : p.colptr;
     const int *pr = p.pcolptr ? p.prowidx : p.rowidx;
-    const int nc = 2 * kTiers + 3;
+    const int nc = kCounters;
     counters_.reserve(nc);
     stats64_.reserve(3);
     const int list_stride = std::max(1, ce - cb);
-    if (dynamic) lists_.reserve((size_t)kTiers * list_stride);
+    lists_.reserve((size_t)kLists * list_stride);
     hipLaunchKernelGGL(spai_zero_counters, dim3(1), dim3(64), 0, st, counters_.get(), nc, stats64_.get(), 3);
     SPAI_CHECK_LAUNCH();
     if (ce > cb) {
@@ -1182,7 +1217,7 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
                                ce, counters_.get(), stats64_.get(), (int *)nullptr, list_stride);
         SPAI_CHECK_LAUNCH();
     }
-    int hc[2 * kTiers + 3] = {0};
+    int hc[kCounters] = {0};
     unsigned long long h64[3] = {0, 0, 0};
     auto fetch = [&]() {
         SPAI_HIP(hipMemcpyAsync(hc, counters_.get(), sizeof(int) * nc, hipMemcpyDeviceToHost, st));
@@ -1266,6 +1301,11 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
     if (stats) {
         for (int t = 0; t < kTiers; ++t) stats->tier_columns[t] = dynamic ? hc[t] : 0;
         if (!dynamic && ce > cb) stats->tier_columns[static_tier] = ce - cb - hc[2 * kTiers + 2];
+        // columns handed on to the next tier are counted where they were solved
+        for (int t = 0; t < 2; ++t) {
+            stats->tier_columns[t] -= hc[kOvfCount + t];
+            stats->tier_columns[t + 1] += hc[kOvfCount + t];
+        }
         stats->max_gathered = hc[2 * kTiers];
         stats->max_n2 = hc[2 * kTiers + 1];
         stats->empty_columns = hc[2 * kTiers + 2];
