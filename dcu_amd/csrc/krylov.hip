@@ -461,6 +461,23 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
 #define SPAI_SELL_UNROLL 8
 #endif
 constexpr int kSellUnroll = SPAI_SELL_UNROLL;  // columns in flight per slice and lane
+// Ops whose gather reads several vectors (K3 forms s = r - alpha a from two,
+// the fused K1 p' from four) stage what the gather of each of the
+// workgroup's own rows returns in LDS, and gathers inside the unit read that
+// copy.  An op opts in with kStage = true; for single-vector gathers the
+// barrier costs more than the loads it saves.  SPAI_SELL_STAGE: 0 none, 1
+// opted-in ops, 2 every op.
+#ifndef SPAI_SELL_STAGE
+#define SPAI_SELL_STAGE 1
+#endif
+template <class Op, class = void>
+struct StageTrait : std::false_type {};
+template <class Op>
+struct StageTrait<Op, std::void_t<decltype(Op::kStage)>> : std::integral_constant<bool, Op::kStage> {};
+template <class Op>
+constexpr bool sell_stage() {
+    return SPAI_SELL_STAGE == 2 || (SPAI_SELL_STAGE == 1 && StageTrait<Op>::value);
+}
 // optional occupancy floor (waves per SIMD) for the register allocator
 #ifndef SPAI_SELL_MIN_WAVES
 #define SPAI_SELL_MIN_WAVES 0
@@ -497,13 +514,17 @@ __global__ __launch_bounds__(kThreads) SPAI_SELL_OCCUPANCY void k_sell(const typ
     constexpr int NRED = Op::NRED;
     constexpr int SPW = kSellSlicesPerWave;
     constexpr int U = kSellUnroll;
+    constexpr bool STAGE = HAS_MATRIX && sell_stage<Op>();
     __shared__ double sh[(NRED > 0 ? NRED : 1) * 4];
+    __shared__ double sv[STAGE ? kSlicesPerUnit * kSliceRows : 1];  // the unit's gathered values
     const int unit = unit_of(uv);
     if (unit < 0) return;
     const int s = uv.sys[unit];
     const int lane = threadIdx.x & (kSliceRows - 1);
     const int sl_end = uv.slice[unit + 1];
     const int sl0 = uv.slice[unit] + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kSliceRows)) * SPW;
+    // the unit's rows [u_lo, u_hi) (its slices are consecutive)
+    const int u_lo = STAGE ? uv.slice_row[uv.slice[unit]] : 0, u_hi = STAGE ? uv.slice_row[sl_end] : 0;
     int row[SPW], w[SPW], b0[SPW];
     bool live[SPW];
 #pragma unroll
@@ -549,6 +570,14 @@ __global__ __launch_bounds__(kThreads) SPAI_SELL_OCCUPANCY void k_sell(const typ
         typename Op::Pre pre[SPW];
 #pragma unroll
         for (int j = 0; j < SPW; ++j) pre[j] = live[j] ? op.pre(row[j], acc) : typename Op::Pre{};
+        if constexpr (STAGE) {
+            // what a gather of each own row returns, for the gathers that
+            // stay inside the unit (every wavefront reaches the barrier)
+#pragma unroll
+            for (int j = 0; j < SPW; ++j)
+                if (live[j]) sv[row[j] - u_lo] = op.stage(row[j], pre[j]);
+            __syncthreads();
+        }
         double sum[SPW];
         int wmax = 0;
 #pragma unroll
@@ -566,7 +595,20 @@ __global__ __launch_bounds__(kThreads) SPAI_SELL_OCCUPANCY void k_sell(const typ
                     if constexpr (IX == kIdx32) cj = jj[j][k];
                     else if constexpr (IX == kIdx16) cj = row[j] + (int)jj[j][k];
                     else cj = min(max(row[j] + jj[j][k], lo), hi);
-                    gv[j][k] = (k0 + k < w[j]) ? op.gather(cj) : 0.0;
+                    double g = 0.0;
+                    if (k0 + k < w[j]) {
+                        if constexpr (!STAGE) {
+                            g = op.gather(cj);
+                        } else if constexpr (IX == kIdxDiag) {
+                            // the wavefront's 64 rows share the offset
+                            const int first = row[j] - lane + jj[j][k];
+                            g = (first >= u_lo && first + kSliceRows <= u_hi) ? sv[cj - u_lo] : op.gather(cj);
+                        } else {
+                            const unsigned rel = (unsigned)(cj - u_lo);
+                            g = rel < (unsigned)(u_hi - u_lo) ? sv[rel] : op.gather(cj);
+                        }
+                    }
+                    gv[j][k] = g;
                 }
 #pragma unroll
             for (int j = 0; j < SPW; ++j)
@@ -626,6 +668,7 @@ struct OpInit {
     __device__ void load(const SysState &) {}
     __device__ Pre pre(int i, double (&)[2]) const { return {b[i]}; }
     __device__ double gather(int j) const { return x[j]; }
+    __device__ double stage(int i, const Pre &) const { return x[i]; }
     __device__ void row(int i, double ax, const Pre &pr, double (&acc)[2]) const {
         double ri = -ax + pr.b;
         r[i] = ri;
@@ -677,9 +720,13 @@ struct OpInit {
 //   kYCopy     y = p' (no preconditioner)
 //   kYNone     no y: the streaming half of the split K1, OpApplyM follows
 enum { kYProduct = 0, kYCopy = 1, kYNone = 2 };
+#ifndef SPAI_STAGE_K1
+#define SPAI_STAGE_K1 1
+#endif
 template <class MT, int YMODE>
 struct OpMapplyP {
     static constexpr int NRED = 1;
+    static constexpr bool kStage = SPAI_STAGE_K1 != 0;
     struct Pre {
         double p;
     };
@@ -708,6 +755,7 @@ struct OpMapplyP {
         return {pi};
     }
     __device__ double gather(int j) const { return pval(j, rval(j)); }
+    __device__ double stage(int, const Pre &pr) const { return pr.p; }
     __device__ void row(int i, double mx, const Pre &pr, double (&)[1]) const {
         if (YMODE == kYProduct) y[i] = mx;
         else if (YMODE == kYCopy) y[i] = pr.p;
@@ -734,6 +782,7 @@ struct OpApplyM {
     __device__ void load(const SysState &) {}
     __device__ Pre pre(int, double (&)[1]) const { return {}; }
     __device__ double gather(int j) const { return p[j]; }
+    __device__ double stage(int i, const Pre &) const { return p[i]; }
     __device__ void row(int i, double mx, const Pre &, double (&)[1]) const { y[i] = mx; }
     __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
@@ -749,6 +798,7 @@ struct OpSpmvAlpha {  // K2: a = A y, alpha = rho / (r*, a)
     __device__ void load(const SysState &) {}
     __device__ Pre pre(int i, double (&)[1]) const { return {rs[i]}; }
     __device__ double gather(int j) const { return y[j]; }
+    __device__ double stage(int i, const Pre &) const { return y[i]; }
     __device__ void row(int i, double ax, const Pre &pr, double (&acc)[1]) const {
         a[i] = ax;
         acc[0] += pr.rs * ax;
@@ -762,7 +812,10 @@ struct OpSpmvAlpha {  // K2: a = A y, alpha = rho / (r*, a)
 template <class MT>
 struct OpMapplyS {  // K3: z = M s, s = r - alpha a (stored before the product; gathers read r and a)
     static constexpr int NRED = 0;
-    struct Pre {};
+    static constexpr bool kStage = true;
+    struct Pre {
+        double s;
+    };
     const double *r, *a;
     double *s, *z;
     double nalpha;
@@ -770,10 +823,12 @@ struct OpMapplyS {  // K3: z = M s, s = r - alpha a (stored before the product; 
     __device__ void load(const SysState &S) { nalpha = -S.alpha; }
     __device__ double sval(int j) const { return r[j] + nalpha * a[j]; }
     __device__ Pre pre(int i, double (&)[1]) const {
-        s[i] = sval(i);
-        return {};
+        const double si = sval(i);
+        s[i] = si;
+        return {si};
     }
     __device__ double gather(int j) const { return sval(j); }
+    __device__ double stage(int, const Pre &pr) const { return pr.s; }
     __device__ void row(int i, double mx, const Pre &, double (&)[1]) const { z[i] = mx; }
     __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
@@ -789,6 +844,7 @@ struct OpSpmvOmega {  // K4: t = A z; omega, rho' = (r*, s) - omega (r*, t), bet
     __device__ void load(const SysState &) {}
     __device__ Pre pre(int i, double (&)[4]) const { return {s[i], rs[i]}; }
     __device__ double gather(int j) const { return z[j]; }
+    __device__ double stage(int i, const Pre &) const { return z[i]; }
     __device__ void row(int i, double ax, const Pre &pr, double (&acc)[4]) const {
         t[i] = ax;
         acc[0] += ax * pr.s;
@@ -815,6 +871,7 @@ struct OpTrueResidual {  // ||b - A x||^2 per system
     __device__ void load(const SysState &) {}
     __device__ Pre pre(int i, double (&)[1]) const { return {b[i]}; }
     __device__ double gather(int j) const { return x[j]; }
+    __device__ double stage(int i, const Pre &) const { return x[i]; }
     __device__ void row(int, double ax, const Pre &pr, double (&acc)[1]) const {
         double d = pr.b - ax;
         acc[0] += d * d;
@@ -831,6 +888,7 @@ struct OpSpmvPlain {  // y = A x (timing of the bare product)
     __device__ void load(const SysState &) {}
     __device__ Pre pre(int, double (&)[1]) const { return {}; }
     __device__ double gather(int j) const { return x[j]; }
+    __device__ double stage(int i, const Pre &) const { return x[i]; }
     __device__ void row(int i, double ax, const Pre &, double (&)[1]) const { y[i] = ax; }
     __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
@@ -851,6 +909,7 @@ struct OpCopyS {  // K3 without a preconditioner: z = s
         return {si};
     }
     __device__ double gather(int) const { return 0.0; }
+    __device__ double stage(int, const Pre &) const { return 0.0; }
     __device__ void row(int i, double, const Pre &pr, double (&)[1]) const { z[i] = pr.s; }
     __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };

@@ -434,8 +434,17 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
     wave_lds_fence();  // LDS reused by this group's next column
 }
 
+// optional occupancy floor (waves per SIMD) for the register allocator
+#ifndef SPAI_TIER_MIN_WAVES
+#define SPAI_TIER_MIN_WAVES 0
+#endif
+#if SPAI_TIER_MIN_WAVES > 0
+#define SPAI_TIER_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SPAI_TIER_MIN_WAVES)))
+#else
+#define SPAI_TIER_OCCUPANCY
+#endif
 template <class TR, class QT, bool REFINE, class MT, bool DYNAMIC>
-__global__ __launch_bounds__(kBlock) void spai_reg_tier_kernel(ColumnArgs a) {
+__global__ __launch_bounds__(kBlock) SPAI_TIER_OCCUPANCY void spai_reg_tier_kernel(ColumnArgs a) {
     constexpr int W = TR::W;
     constexpr int G = 64 / W;    // groups per wavefront
     constexpr int CH = 4;        // columns per group per queue grab
