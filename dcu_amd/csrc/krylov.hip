@@ -74,6 +74,7 @@ constexpr int kFanIn = SPAI_REDUCE_FANIN;  // children per reduction tree node
 #define SPAI_SPLIT_UNITS 1024
 #endif
 constexpr int kSplitUnits = SPAI_SPLIT_UNITS;
+constexpr int kCompactEvery = 8;  // iterations between active-list rebuilds inside a graph
 constexpr int kMaxRed = 4;          // most values any kernel reduces (partials per tree node)
 
 // ------------------------------------------------------------ transpose ----
@@ -1502,15 +1503,9 @@ BiCGStab::~BiCGStab() {
 }
 
 void BiCGStab::drop_graph() {
-    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-    if (graph_) (void)hipGraphDestroy(graph_);
-    graph_exec_ = nullptr;
-    graph_ = nullptr;
+    for (auto &g : graphs_) (void)hipGraphExecDestroy(g.exec);
+    graphs_.clear();
     for (auto &k : graph_key_) k = nullptr;
-    graph_iters_ = 0;
-    graph_grid_ = 0;
-    graph_compact_ = 0;
-    graph_split_ = 0;
 }
 
 Ctx BiCGStab::make_ctx(const TileMap &tm, int nsys) {
@@ -1678,59 +1673,70 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
             read_counts();
             if (opt.use_graph) {
                 const int gi = std::max(2, opt.graph_iters + (opt.graph_iters & 1));
-                const void *key[7] = {A.val, M.val, tm.unit_row.get(), b, x, (const void *)(size_t)M.fp32,
-                                      hist_ptr_};
+                const void *key[kGraphKey] = {A.val, A.idx, A.off, A.diag, M.val, M.idx, M.off, M.diag, M.blk,
+                                              (const void *)(size_t)M.fp32, tm.unit_row.get(), b, x, hist_ptr_};
                 // One graph runs ahead of the host check: graph k + 1 is
-                // queued before the counts copied after graph k are read, so
-                // the device does not wait for the host between graphs, and a
-                // new capture overlaps the graph in flight.  The counts only
-                // fall while iterating, so a grid sized from counts one graph
-                // old covers every active unit; a graph launched after the
-                // last system stopped finds every unit inactive.
-                std::vector<hipGraphExec_t> retired;  // replaced executables that may still be in flight
+                // queued before the counts graph k published are read, so the
+                // device does not wait for the host between graphs.  The
+                // counts only fall while iterating, so a grid sized from
+                // counts one graph old covers every active unit; a graph
+                // launched after the last system stopped finds every unit
+                // inactive.  Graphs of other operands are dropped.
+                bool same_key = true;
+                for (int q = 0; q < kGraphKey; ++q) same_key = same_key && key[q] == graph_key_[q];
+                if (!same_key) {
+                    drop_graph();
+                    for (int q = 0; q < kGraphKey; ++q) graph_key_[q] = key[q];
+                }
+                // Grid rungs: nunits, then ceil(5/8) of the previous rung; a
+                // graph walks the smallest rung holding every active unit, so
+                // at most 3/8 of its workgroups find no unit.
+                auto rung = [&](int u) {
+                    int g = tm.nunits;
+                    while (g > 1) {
+                        const int next = (g * 5 + 7) / 8;
+                        if (next < u || next == g) break;
+                        g = next;
+                    }
+                    return g;
+                };
+                auto graph_for = [&](int grid, bool split) {
+                    for (auto &g : graphs_)
+                        if (g.iters == gi && g.grid == grid && g.compact == (int)use_list && g.split == (int)split)
+                            return g.exec;
+                    hipStream_t cap;
+                    hipGraph_t graph = nullptr;
+                    hipGraphExec_t exec = nullptr;
+                    SPAI_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+                    SPAI_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+                    for (int k = 0; k < gi; ++k) {
+                        enqueue_iteration(A, M, tm, k & 1, grid, split, cap);
+                        // long graphs also rebuild the list every kCompactEvery iterations
+                        if (use_list && (k + 1) % kCompactEvery == 0 && k + 1 < gi) compact(tm, nsys, cap);
+                    }
+                    if (use_list) compact(tm, nsys, cap, d_pub_);
+                    else
+                        hipLaunchKernelGGL(k_publish_counts, dim3(1), dim3(64), 0, cap, counter_.get() + nsys,
+                                           d_pub_);
+                    SPAI_HIP(hipStreamEndCapture(cap, &graph));
+                    SPAI_HIP(hipStreamDestroy(cap));
+                    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(graph);
+                    SPAI_HIP(e);
+                    graphs_.push_back(GraphEntry{gi, grid, (int)use_list, (int)split, exec});
+                    return exec;
+                };
                 int slot = 0, pending = 0;
                 auto read_published = [&]() {
                     ++checks;
                     active = __atomic_load_n(&h_pub_[0], __ATOMIC_ACQUIRE);
                     units = use_list ? __atomic_load_n(&h_pub_[1], __ATOMIC_ACQUIRE) : tm.nunits;
                 };
-                auto finish_checks = [&]() {
-                    SPAI_HIP(hipStreamSynchronize(st));
-                    for (auto g : retired) (void)hipGraphExecDestroy(g);
-                    retired.clear();
-                };
                 try {
                     while (active > 0 && done_iters < max_launch_iters) {
-                        // the graph's grid follows the active-unit count: captured
-                        // again when it has fallen below 5/8 of the captured grid
-                        const int grid = use_list ? units : 0;
+                        const int grid = use_list ? rung(units) : 0;
                         const bool split = M.val && units >= kSplitUnits;
-                        bool same = graph_exec_ && gi == graph_iters_ && graph_compact_ == (int)use_list &&
-                                    graph_split_ == (int)split;
-                        for (int q = 0; q < 7 && same; ++q) same = key[q] == graph_key_[q];
-                        if (same && use_list) same = grid <= graph_grid_ && grid * 8 >= graph_grid_ * 5;
-                        if (!same) {
-                            if (graph_exec_) retired.push_back(graph_exec_);
-                            graph_exec_ = nullptr;
-                            drop_graph();
-                            hipStream_t cap;
-                            SPAI_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-                            SPAI_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-                            for (int k = 0; k < gi; ++k) enqueue_iteration(A, M, tm, k & 1, grid, split, cap);
-                            if (use_list) compact(tm, nsys, cap, d_pub_);
-                            else
-                                hipLaunchKernelGGL(k_publish_counts, dim3(1), dim3(64), 0, cap, counter_.get() + nsys,
-                                                   d_pub_);
-                            SPAI_HIP(hipStreamEndCapture(cap, &graph_));
-                            SPAI_HIP(hipStreamDestroy(cap));
-                            SPAI_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
-                            graph_iters_ = gi;
-                            graph_grid_ = grid;
-                            graph_compact_ = (int)use_list;
-                            graph_split_ = (int)split;
-                            for (int q = 0; q < 7; ++q) graph_key_[q] = key[q];
-                        }
-                        SPAI_HIP(hipGraphLaunch(graph_exec_, st));
+                        SPAI_HIP(hipGraphLaunch(graph_for(grid, split), st));
                         done_iters += gi;
                         SPAI_HIP(hipEventRecord(check_ev_[slot], st));
                         slot ^= 1;
@@ -1745,10 +1751,9 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                     }
                 } catch (...) {
                     (void)hipStreamSynchronize(st);
-                    for (auto g : retired) (void)hipGraphExecDestroy(g);
                     throw;
                 }
-                finish_checks();
+                SPAI_HIP(hipStreamSynchronize(st));
                 if (pending) read_published();  // the newest graph's counts
             } else {
                 const int every = std::max(1, opt.check_every);

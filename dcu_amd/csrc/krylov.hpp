@@ -215,13 +215,15 @@ private:
     const double *b_ = nullptr;
     double *x_ = nullptr;
     // graph cache
-    hipGraphExec_t graph_exec_ = nullptr;
-    hipGraph_t graph_ = nullptr;
-    const void *graph_key_[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    int graph_iters_ = 0;
-    int graph_grid_ = 0;
-    int graph_compact_ = 0;
-    int graph_split_ = 0;
+    // replayable iteration graphs of the current operands, one per grid rung
+    // and K1 form, kept across solves
+    struct GraphEntry {
+        int iters, grid, compact, split;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs_;
+    static constexpr int kGraphKey = 14;  // operand pointers the graphs were captured with
+    const void *graph_key_[kGraphKey] = {};
     hipEvent_t check_ev_[2] = {nullptr, nullptr};  // host checks of the graph loop
     int *h_pub_ = nullptr, *d_pub_ = nullptr;      // counts published by the graphs (host-coherent)
     void drop_graph();
