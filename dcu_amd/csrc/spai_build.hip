@@ -30,7 +30,8 @@
 // n2_k = |J_k|) into size tiers.  balanced_columns runs every column through
 // the one tier that fits the largest column, in contiguous column blocks per
 // wavefront; balanced_nnz runs each tier's column list through its own
-// kernel with an atomic work queue (largest tier first).
+// kernel with an atomic work queue (smallest tier first, each register
+// tier's queue ending with the columns the tier before it handed on).
 #include "device_common.hpp"
 #include "spai_build.hpp"
 
@@ -94,6 +95,8 @@ struct ColumnArgs {
     int col_begin, col_end;  // static range
     int *ovf_list;    // columns with |I_k| above the tier's rows go to the next tier's list
     int *ovf_count;
+    const int *ovf_in_list;   // dynamic: columns handed on by the previous tier, after list
+    const int *ovf_in_count;
 };
 
 // 64-bit shuffle helpers on top of the 32-bit cross-lane permute.
@@ -456,15 +459,19 @@ __global__ __launch_bounds__(kBlock) SPAI_TIER_OCCUPANCY void spai_reg_tier_kern
     double *lds = lds_all + grp * TR::LDS_DOUBLES;
 
     if (DYNAMIC) {
+        // the tier's list, then the columns the previous tier handed on
         const int count = *a.count;
+        const int total = count + (a.ovf_in_count ? *a.ovf_in_count : 0);
         for (;;) {
             int base = 0;
             if ((tid & 63) == 0) base = atomicAdd(a.head, G * CH);
             base = __shfl(base, 0, 64);
-            if (base >= count) break;
+            if (base >= total) break;
             for (int it = 0; it < CH; ++it) {
                 int idx = base + it * G + wgrp;
-                if (idx < count) solve_column_reg<TR, QT, REFINE, MT>(a, a.list[idx], lane, lds);
+                if (idx < total)
+                    solve_column_reg<TR, QT, REFINE, MT>(a, idx < count ? a.list[idx] : a.ovf_in_list[idx - count],
+                                                         lane, lds);
             }
         }
     } else {
@@ -1023,7 +1030,8 @@ void launch_tiers(const ColumnArgs &base, bool dynamic, int static_tier, int *co
                   int list_stride, int cus, hipStream_t st) {
     using QT = typename std::conditional<REFINE, float, double>::type;
     ColumnArgs a = base;
-    // t: tier; ovf: run over the tier's overflow list instead of its own
+    // t: tier; ovf: a static run's follow-up over the tier's overflow list
+    // alone (in a dynamic run a tier's queue takes that list after its own)
     auto setup = [&](int t, bool ovf) {
         a = base;
         if (ovf) {
@@ -1034,6 +1042,10 @@ void launch_tiers(const ColumnArgs &base, bool dynamic, int static_tier, int *co
             a.list = lists + (size_t)t * list_stride;
             a.count = counters + t;
             a.head = counters + kTiers + t;
+            if (t == 1 || t == 2) {
+                a.ovf_in_list = lists + (size_t)(kTiers + t - 1) * list_stride;
+                a.ovf_in_count = counters + kOvfCount + t - 1;
+            }
         }
         if (t < 2) {  // tiers 0 and 1 hand wider row sets on
             a.ovf_list = lists + (size_t)(kTiers + t) * list_stride;
@@ -1069,11 +1081,13 @@ void launch_tiers(const ColumnArgs &base, bool dynamic, int static_tier, int *co
         SPAI_CHECK_LAUNCH();
     };
     if (dynamic) {
-        // largest tier first so the long columns start early
-        run_lds(3);
-        run_reg(Tier2{}, 2);
-        run_reg(Tier1{}, 1);
+        // smallest tier first: each register tier's queue ends with the
+        // columns the tier before it handed on (the launches run one after
+        // another on the stream either way)
         run_reg(Tier0{}, 0);
+        run_reg(Tier1{}, 1);
+        run_reg(Tier2{}, 2);
+        run_lds(3);
     } else {
         switch (static_tier) {
         case 0: run_reg(Tier0{}, 0); break;
@@ -1081,10 +1095,10 @@ void launch_tiers(const ColumnArgs &base, bool dynamic, int static_tier, int *co
         case 2: run_reg(Tier2{}, 2); break;
         default: run_lds(3); break;
         }
+        // the overflow lists of the tiers after the static one
+        if (static_tier == 0) run_reg(Tier1{}, 1, true);
+        if (static_tier <= 1) run_reg(Tier2{}, 2, true);
     }
-    // the overflow lists (each run exits at once when its list is empty)
-    if (dynamic || static_tier == 0) run_reg(Tier1{}, 1, true);
-    if (dynamic || static_tier <= 1) run_reg(Tier2{}, 2, true);
 }
 
 // ---------------------------------------------------------------------------

@@ -137,3 +137,38 @@ def test_missing_diagonal_gives_zero_column(ctx):
     m_ref = O.spai(n, A.colptr, A.rowidx, A.val, 0)
     assert np.all(m[A.colptr[0]:A.colptr[1]] == 0)
     assert np.allclose(m, m_ref, rtol=1e-12, atol=1e-14)
+
+
+def scattered_columns(n=3000, seed=7):
+    """Every column: the diagonal and 2-4 random rows, values U(-1,1) with a
+    dominant diagonal.  T_k <= 32 puts every column in the 16-lane tier,
+    whose tile holds 16 rows, while |I_k| (the union of 3-5 random columns)
+    is 17 or more for part of them: those overflow to the 32-lane tier."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for k in range(n):
+        rs = set(rng.choice(n, int(rng.integers(2, 5)), replace=False).tolist()) | {k}
+        rows += sorted(rs)
+        cols += [k] * len(rs)
+    vals = rng.uniform(-1, 1, len(rows))
+    A = sp.csc_matrix((vals, (rows, cols)), shape=(n, n))
+    A.setdiag(8.0)
+    A.sort_indices()
+    return D.CSC(n, A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64))
+
+
+@pytest.mark.parametrize("schedule", [D.SCHEDULE_BALANCED_COLUMNS, D.SCHEDULE_BALANCED_NNZ])
+@pytest.mark.parametrize("precision,tol", [(D.PRECISION_FP64, 1e-10), (D.PRECISION_MIXED, 4 * 2.0 ** -23)])
+def test_tier_overflow(ctx, schedule, precision, tol):
+    A = scattered_columns()
+    n1_ref, n2_ref, t_ref, _ = O.column_sets(A.n, A.colptr, A.rowidx)
+    assert t_ref.max() <= 32 and n2_ref.max() <= 8
+    assert 100 < (n1_ref > 16).sum() < A.n - 100  # part of the columns overflow the 16-row tile
+    S, rep, _, _ = build_and_compare(ctx, A, precision, schedule, tol)
+    assert np.array_equal(S.row_set_sizes(), n1_ref)
+    if schedule == D.SCHEDULE_BALANCED_COLUMNS:
+        assert rep["static_tier"] == 0
+    # the overflowed columns are counted where they were solved
+    assert rep["tier_columns"][1] == int((n1_ref > 16).sum())
+    assert rep["tier_columns"][0] == int((n1_ref <= 16).sum())
