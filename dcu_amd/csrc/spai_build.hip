@@ -49,10 +49,12 @@ constexpr int kBlock = 256;  // threads per workgroup for the column kernels
 // NJ = largest |J_k| the tier holds in registers, RPL tile rows per lane.
 // A column is routed by T_k and |J_k|; its |I_k| is known only once the
 // gathered rows are merged.  When |I_k| exceeds the tier's ROWS (< TMAX)
-// the column goes to the next tier's overflow list, solved after the
-// tiers' own lists.  The row tiles are sized for the common case instead of
-// the largest T_k: the 3D 7-point stencil gathers T = 49 entries into
-// |I_k| = 25 rows, the 2D 5-point 25 into 13.
+// the column goes to the next tier's overflow list, which that tier's
+// queue takes after its own list.  The row tiles are sized for the common
+// case instead of the largest T_k: the 3D 7-point stencil gathers T = 49
+// entries into |I_k| = 25 rows, the 2D 5-point 25 into 13.  Narrow groups
+// (8 or 16 lanes, 4 entries and 2 rows per lane) put 8 or 4 columns in a
+// wavefront, so every cross-lane step serves that many columns.
 template <int W_, int EPL_, int NJ_, int RPL_>
 struct RegTier {
     static constexpr int W = W_;
@@ -66,8 +68,8 @@ struct RegTier {
     static constexpr int LDS_DOUBLES = ROWS * TS + NJ * NJ + NJ;
 };
 
-using Tier0 = RegTier<16, 2, 8, 1>;
-using Tier1 = RegTier<32, 2, 8, 1>;
+using Tier0 = RegTier<8, 4, 8, 2>;
+using Tier1 = RegTier<16, 4, 8, 2>;
 using Tier2 = RegTier<64, 2, 16, 2>;
 static_assert(Tier0::TMAX <= Tier1::ROWS && Tier1::TMAX <= Tier2::ROWS && Tier2::TMAX <= Tier2::ROWS,
               "an overflowed column must fit the next tier's rows");
