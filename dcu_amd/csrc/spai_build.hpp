@@ -7,7 +7,7 @@
 
 namespace spai {
 
-// Size tiers: 0..2 register tiers (16, 32, 64 lanes per column), 3 the LDS
+// Size tiers: 0..2 register tiers (8, 16, 64 lanes per column), 3 the LDS
 // wavefront tier, 4 the global-memory tier (one workgroup per very wide
 // column, FP64, tile in a scratch slot).
 constexpr int kTiers = 5;

@@ -63,7 +63,7 @@ def test_random4096_mixed(ctx):
 
 @pytest.mark.parametrize("precision,tol", [(D.PRECISION_FP64, 1e-10), (D.PRECISION_MIXED, 2.0 ** -23 * 4)])
 def test_power_law_all_tiers(ctx, precision, tol):
-    # 60k rows of the power-law generator reach the 16/32/64-lane tiers and
+    # 60k rows of the power-law generator reach the 8/16/64-lane tiers and
     # the LDS tier
     P = D.Problem.power_law(60000, 7)
     A = P.system(0)
@@ -141,9 +141,9 @@ def test_missing_diagonal_gives_zero_column(ctx):
 
 def scattered_columns(n=3000, seed=7):
     """Every column: the diagonal and 2-4 random rows, values U(-1,1) with a
-    dominant diagonal.  T_k <= 32 puts every column in the 16-lane tier,
+    dominant diagonal.  T_k <= 32 puts every column in the 8-lane tier,
     whose tile holds 16 rows, while |I_k| (the union of 3-5 random columns)
-    is 17 or more for part of them: those overflow to the 32-lane tier."""
+    is 17 or more for part of them: those overflow to the 16-lane tier."""
     import scipy.sparse as sp
     rng = np.random.default_rng(seed)
     rows, cols = [], []
