@@ -274,11 +274,11 @@ __device__ __forceinline__ void wave_sum_range(const double *src, int first, int
 // last arrival at each node sums that node's children in a fixed order, so
 // the result does not depend on arrival order.  Node numbering follows from
 // the system's unit count alone: level l starts after the levels below it.
-template <int NRED, class Op>
+// W = wavefronts of the workgroup (sh holds NRED * W doubles)
+template <int NRED, int W, class Op>
 __device__ __forceinline__ void reduce_system(const Ctx &c, int unit, int s, double (&v)[NRED], Op &op, double *sh) {
     static_assert(NRED >= 1 && NRED <= kMaxRed, "partials hold at most kMaxRed values per tree node");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    constexpr int W = kThreads / 64;
     wave_sum<NRED>(v);
     if (lane == 0) {
 #pragma unroll
@@ -448,7 +448,7 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
             }
         }
     }
-    if constexpr (NRED > 0) reduce_system<NRED>(c, unit, s, acc, op, sh);
+    if constexpr (NRED > 0) reduce_system<NRED, kThreads / 64>(c, unit, s, acc, op, sh);
 }
 
 // SELL-64.  Wavefront w of the workgroup takes kSellSlicesPerWave
@@ -507,8 +507,9 @@ struct SellIndex<kIdx16> {
     using T = short;
 };
 
+constexpr int kSellThreads = 64 * kSellWaves;
 template <bool HAS_MATRIX, class MT, class Op, int IX>
-__global__ __launch_bounds__(kThreads) SPAI_SELL_OCCUPANCY void k_sell(const typename SellIndex<IX>::T *__restrict__ col,
+__global__ __launch_bounds__(kSellThreads) SPAI_SELL_OCCUPANCY void k_sell(const typename SellIndex<IX>::T *__restrict__ col,
                                                                        const MT *__restrict__ val, UnitView uv, Ctx c,
                                                                        Op op) {
     using IT = typename SellIndex<IX>::T;
@@ -516,7 +517,7 @@ __global__ __launch_bounds__(kThreads) SPAI_SELL_OCCUPANCY void k_sell(const typ
     constexpr int SPW = kSellSlicesPerWave;
     constexpr int U = kSellUnroll;
     constexpr bool STAGE = HAS_MATRIX && sell_stage<Op>();
-    __shared__ double sh[(NRED > 0 ? NRED : 1) * 4];
+    __shared__ double sh[(NRED > 0 ? NRED : 1) * kSellWaves];
     __shared__ double sv[STAGE ? kSlicesPerUnit * kSliceRows : 1];  // the unit's gathered values
     const int unit = unit_of(uv);
     if (unit < 0) return;
@@ -624,7 +625,7 @@ __global__ __launch_bounds__(kThreads) SPAI_SELL_OCCUPANCY void k_sell(const typ
         for (int j = 0; j < SPW; ++j)
             if (live[j]) op.row(row[j], sum[j], pre[j], acc);
     }
-    if constexpr (NRED > 0) reduce_system<NRED>(c, unit, s, acc, op, sh);
+    if constexpr (NRED > 0) reduce_system<NRED, kSellWaves>(c, unit, s, acc, op, sh);
 }
 
 // ------------------------------------------------------------- the ops ----
@@ -1147,13 +1148,13 @@ void launch_units(const OperatorView &O, const TileMap &tm, const Ctx &c, const 
     const int g = list ? grid : tm.nunits;
     if (g <= 0) return;
     if (tm.kind == kLayoutSell && O.diag)
-        hipLaunchKernelGGL((k_sell<HM, MT, Op, kIdxDiag>), dim3(g), dim3(kThreads), 0, st, O.diag, (const MT *)O.val,
+        hipLaunchKernelGGL((k_sell<HM, MT, Op, kIdxDiag>), dim3(g), dim3(kSellThreads), 0, st, O.diag, (const MT *)O.val,
                            uv, c, op);
     else if (tm.kind == kLayoutSell && O.off)
-        hipLaunchKernelGGL((k_sell<HM, MT, Op, kIdx16>), dim3(g), dim3(kThreads), 0, st, O.off, (const MT *)O.val, uv,
+        hipLaunchKernelGGL((k_sell<HM, MT, Op, kIdx16>), dim3(g), dim3(kSellThreads), 0, st, O.off, (const MT *)O.val, uv,
                            c, op);
     else if (tm.kind == kLayoutSell)
-        hipLaunchKernelGGL((k_sell<HM, MT, Op, kIdx32>), dim3(g), dim3(kThreads), 0, st, O.idx, (const MT *)O.val, uv,
+        hipLaunchKernelGGL((k_sell<HM, MT, Op, kIdx32>), dim3(g), dim3(kSellThreads), 0, st, O.idx, (const MT *)O.val, uv,
                            c, op);
     else
         hipLaunchKernelGGL((k_tiles<HM, MT, Op>), dim3(g), dim3(kThreads), 0, st, O.ptr, O.idx, (const MT *)O.val,

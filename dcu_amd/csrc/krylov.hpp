@@ -54,7 +54,11 @@ enum LayoutKind { kLayoutTiles = 0, kLayoutSell = 1 };
 #endif
 constexpr int kSliceRows = 64;
 constexpr int kSellSlicesPerWave = SPAI_SELL_SLICES_PER_WAVE;
-constexpr int kSlicesPerUnit = 4 * kSellSlicesPerWave;  // 4 wavefronts per workgroup
+#ifndef SPAI_SELL_WAVES
+#define SPAI_SELL_WAVES 4
+#endif
+constexpr int kSellWaves = SPAI_SELL_WAVES;  // wavefronts per SELL workgroup
+constexpr int kSlicesPerUnit = kSellWaves * kSellSlicesPerWave;
 
 struct TileMap {
     int kind = kLayoutTiles;
