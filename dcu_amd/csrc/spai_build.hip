@@ -65,7 +65,6 @@ struct RegTier {
     static constexpr int TMAX = W * EPL;     // largest T_k
     static constexpr int TS = NJ + 1;        // LDS tile row stride (doubles)
     static constexpr int GROUPS = kBlock / W;
-    static constexpr int LDS_DOUBLES = ROWS * TS + NJ * NJ + NJ;
 };
 
 using Tier0 = RegTier<8, 4, 8, 2>;
@@ -73,6 +72,19 @@ using Tier1 = RegTier<16, 4, 8, 2>;
 using Tier2 = RegTier<64, 2, 16, 2>;
 static_assert(Tier0::TMAX <= Tier1::ROWS && Tier1::TMAX <= Tier2::ROWS && Tier2::TMAX <= Tier2::ROWS,
               "an overflowed column must fit the next tier's rows");
+
+// LDS of one column group, in bytes.  The tile and R are held in the QR's
+// type (FP32 in mixed mode); mixed mode's FP64 refinement reads the column's
+// gathered entries (values, columns, row starts) over the tile, which is dead
+// once its rows are in registers.
+template <class TR, class QT, bool REFINE>
+struct TierLds {
+    static constexpr int TILE = TR::ROWS * TR::TS * (int)sizeof(QT);
+    static constexpr int REF = REFINE ? TR::TMAX * 8 + TR::TMAX * 4 + (TR::ROWS + 1) * 4 : 0;
+    static constexpr int RS_OFF = ((TILE > REF ? TILE : REF) + 7) / 8 * 8;
+    static constexpr int YS_OFF = RS_OFF + (TR::NJ * TR::NJ * (int)sizeof(QT) + 7) / 8 * 8;
+    static constexpr int DOUBLES = YS_OFF / 8 + TR::NJ;
+};
 
 // The LDS tier: one wavefront per column, tile in LDS, FP64 arithmetic.
 constexpr int kLdsTierT = 1024;        // largest T_k
@@ -180,9 +192,13 @@ template <class TR, class QT, bool REFINE, class MT>
 __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *lds) {
     constexpr int W = TR::W, EPL = TR::EPL, NJ = TR::NJ, TS = TR::TS;
     constexpr int RPL = TR::RPL;
-    double *tile = lds;                       // ROWS x TS
-    double *Rs = lds + TR::ROWS * TS;         // NJ x NJ
-    double *ys = Rs + NJ * NJ;                // NJ
+    using L = TierLds<TR, QT, REFINE>;
+    QT *tile = (QT *)lds;                                     // ROWS x TS
+    QT *Rs = (QT *)((char *)lds + L::RS_OFF);                 // NJ x NJ
+    double *ys = (double *)((char *)lds + L::YS_OFF);         // NJ (then x in the refinement)
+    double *vslot = lds;                                      // refinement: entry values (sorted order)
+    int *jslot = (int *)(vslot + TR::TMAX);                   //   their columns
+    int *rstart = jslot + TR::TMAX;                           //   first entry of each row, n1 + 1
 
     const int c0 = a.pcolptr[k];
     const int n2 = a.pcolptr[k + 1] - c0;
@@ -248,10 +264,12 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
         int rho = lane + W * r;
         if (rho < n1) {
 #pragma unroll
-            for (int c = 0; c < NJ; ++c) tile[rho * TS + c] = 0.0;
+            for (int c = 0; c < NJ; ++c) tile[rho * TS + c] = (QT)0;
         }
     }
     wave_lds_fence();
+    double vv[EPL];  // the entries' FP64 values and columns, for the refinement
+    int jv[EPL];
 #pragma unroll
     for (int i = 0; i < EPL; ++i) {
         // shuffles run on every lane of the group; only the store is masked
@@ -259,7 +277,9 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
         const int e = valid ? (int)(unsigned)key[i] : 0;
         const int j = locate_j<W, NJ>(e, off, n2);
         const int src = __shfl(cs, j, W) + (e - __shfl(off, j, W));
-        if (valid) tile[pos[i] * TS + j] = a.val[src];
+        vv[i] = valid ? a.val[src] : 0.0;
+        jv[i] = j;
+        if (valid) tile[pos[i] * TS + j] = (QT)vv[i];
     }
     wave_lds_fence();
 
@@ -269,7 +289,21 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
     for (int r = 0; r < RPL; ++r) {
         int rho = lane + W * r;
 #pragma unroll
-        for (int c = 0; c < NJ; ++c) A[r][c] = (rho < n1 && c < n2) ? (QT)tile[rho * TS + c] : (QT)0;
+        for (int c = 0; c < NJ; ++c) A[r][c] = (rho < n1 && c < n2) ? tile[rho * TS + c] : (QT)0;
+    }
+    if constexpr (REFINE) {
+        // the entries in sorted (row, then column) order over the dead tile
+        wave_lds_fence();
+#pragma unroll
+        for (int i = 0; i < EPL; ++i) {
+            const int slot = lane * EPL + i;
+            if (key[i] != ~0ull) {
+                vslot[slot] = vv[i];
+                jslot[slot] = jv[i];
+                if (first[i]) rstart[pos[i]] = slot;
+            }
+        }
+        if (lane == 0) rstart[n1] = T;
     }
 
     // Householder QR: H_j = I - tau_j v_j v_j^T, v_j[j] = 1 (v below the
@@ -354,7 +388,7 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
         int rho = lane + W * r;
         if (rho < n2) {
 #pragma unroll
-            for (int c = 0; c < NJ; ++c) Rs[rho * NJ + c] = (double)A[r][c];
+            for (int c = 0; c < NJ; ++c) Rs[rho * NJ + c] = A[r][c];
             ys[rho] = (double)y[r];
         }
     }
@@ -367,8 +401,8 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
             double t = ys[c];
 #pragma unroll
             for (int c2 = c + 1; c2 < NJ; ++c2)
-                if (c2 < n2) t -= Rs[c * NJ + c2] * x[c2];
-            x[c] = t / Rs[c * NJ + c];
+                if (c2 < n2) t -= (double)Rs[c * NJ + c2] * x[c2];
+            x[c] = t / (double)Rs[c * NJ + c];
         }
     }
     if (E < 0) {
@@ -376,55 +410,74 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
         for (int c = 0; c < NJ; ++c) x[c] = 0.0;
     }
 
-    if (REFINE && E >= 0) {
-        // one FP64 corrected-semi-normal-equations sweep with the FP64 tile
-        double res[RPL];
-#pragma unroll
-        for (int r = 0; r < RPL; ++r) {
-            int rho = lane + W * r;
-            double t = (rho == E) ? 1.0 : 0.0;
-            if (rho < n1) {
+    if constexpr (REFINE) {
+        if (E >= 0) {
+            // one FP64 corrected-semi-normal-equations sweep from the column's
+            // gathered FP64 entries: a row's entries are visited in column
+            // order, as a dense row of the tile would be
+            double *xs = ys;  // y is dead after the back substitution
+            static_assert(TR::W >= TR::NJ, "lane c holds x[c]");
+            wave_lds_fence();
+            {
+                double xv = 0.0;
 #pragma unroll
                 for (int c = 0; c < NJ; ++c)
-                    if (c < n2) t -= tile[rho * TS + c] * x[c];
+                    if (lane == c) xv = x[c];
+                if (lane < n2) xs[lane] = xv;
             }
-            res[r] = (rho < n1) ? t : 0.0;
-        }
-        double g[NJ];
-#pragma unroll
-        for (int c = 0; c < NJ; ++c) {
-            double s = 0.0;
+            wave_lds_fence();
+            double res[RPL];
 #pragma unroll
             for (int r = 0; r < RPL; ++r) {
-                int rho = lane + W * r;
-                if (rho < n1 && c < n2) s += tile[rho * TS + c] * res[r];
+                const int rho = lane + W * r;
+                double t = (rho == E) ? 1.0 : 0.0;
+                if (rho < n1) {
+                    const int q1 = rstart[rho + 1];
+                    for (int q = rstart[rho]; q < q1; ++q) t = fma(-vslot[q], xs[jslot[q]], t);
+                }
+                res[r] = (rho < n1) ? t : 0.0;
             }
-            g[c] = s;
-        }
+            double g[NJ];
 #pragma unroll
-        for (int o = W / 2; o > 0; o >>= 1) {
+            for (int c = 0; c < NJ; ++c) g[c] = 0.0;
 #pragma unroll
-            for (int c = 0; c < NJ; ++c) g[c] += __shfl_xor(g[c], o, W);
-        }
-        // R^T u = g (forward), R d = u (backward), x += d
+            for (int r = 0; r < RPL; ++r) {
+                const int rho = lane + W * r;
+                if (rho < n1) {
+                    const int q1 = rstart[rho + 1];
+                    for (int q = rstart[rho]; q < q1; ++q) {
+                        const int cq = jslot[q];
+                        const double v = vslot[q];
 #pragma unroll
-        for (int c = 0; c < NJ; ++c) {
-            if (c < n2) {
-                double t = g[c];
-#pragma unroll
-                for (int c2 = 0; c2 < c; ++c2) t -= Rs[c2 * NJ + c] * g[c2];
-                g[c] = t / Rs[c * NJ + c];
+                        for (int c = 0; c < NJ; ++c) g[c] = fma(c == cq ? v : 0.0, res[r], g[c]);
+                    }
+                }
             }
-        }
 #pragma unroll
-        for (int c = NJ - 1; c >= 0; --c) {
-            if (c < n2) {
-                double t = g[c];
+            for (int o = W / 2; o > 0; o >>= 1) {
 #pragma unroll
-                for (int c2 = c + 1; c2 < NJ; ++c2)
-                    if (c2 < n2) t -= Rs[c * NJ + c2] * g[c2];
-                g[c] = t / Rs[c * NJ + c];
-                x[c] += g[c];
+                for (int c = 0; c < NJ; ++c) g[c] += __shfl_xor(g[c], o, W);
+            }
+            // R^T u = g (forward), R d = u (backward), x += d
+#pragma unroll
+            for (int c = 0; c < NJ; ++c) {
+                if (c < n2) {
+                    double t = g[c];
+#pragma unroll
+                    for (int c2 = 0; c2 < c; ++c2) t -= (double)Rs[c2 * NJ + c] * g[c2];
+                    g[c] = t / (double)Rs[c * NJ + c];
+                }
+            }
+#pragma unroll
+            for (int c = NJ - 1; c >= 0; --c) {
+                if (c < n2) {
+                    double t = g[c];
+#pragma unroll
+                    for (int c2 = c + 1; c2 < NJ; ++c2)
+                        if (c2 < n2) t -= (double)Rs[c * NJ + c2] * g[c2];
+                    g[c] = t / (double)Rs[c * NJ + c];
+                    x[c] += g[c];
+                }
             }
         }
     }
@@ -439,6 +492,9 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
     wave_lds_fence();  // LDS reused by this group's next column
 }
 
+#ifndef SPAI_TIER_GRAB
+#define SPAI_TIER_GRAB 4
+#endif
 // optional occupancy floor (waves per SIMD) for the register allocator
 #ifndef SPAI_TIER_MIN_WAVES
 #define SPAI_TIER_MIN_WAVES 0
@@ -452,13 +508,13 @@ template <class TR, class QT, bool REFINE, class MT, bool DYNAMIC>
 __global__ __launch_bounds__(kBlock) SPAI_TIER_OCCUPANCY void spai_reg_tier_kernel(ColumnArgs a) {
     constexpr int W = TR::W;
     constexpr int G = 64 / W;    // groups per wavefront
-    constexpr int CH = 4;        // columns per group per queue grab
-    __shared__ double lds_all[TR::GROUPS * TR::LDS_DOUBLES];
+    constexpr int CH = SPAI_TIER_GRAB;  // columns per group per queue grab
+    __shared__ double lds_all[TR::GROUPS * TierLds<TR, QT, REFINE>::DOUBLES];
     const int tid = threadIdx.x;
     const int lane = tid % W;
     const int grp = tid / W;
     const int wgrp = (tid % 64) / W;  // group index inside the wavefront
-    double *lds = lds_all + grp * TR::LDS_DOUBLES;
+    double *lds = lds_all + grp * TierLds<TR, QT, REFINE>::DOUBLES;
 
     if (DYNAMIC) {
         // the tier's list, then the columns the previous tier handed on
