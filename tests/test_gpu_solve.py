@@ -185,3 +185,50 @@ def test_converged_at_start(ctx):
     x, rep, oc = S.solve(b, x0=np.ones(A.n), rel_tol=1e-8)
     assert oc[0]["status"] == D.SOLVE_CONVERGED_AT_START and oc[0]["iterations"] == 0
     assert np.array_equal(x, np.ones(A.n))
+
+
+def small_convdiff(m, pe, rng):
+    """m x m upwind convection-diffusion (5-point), Dirichlet boundaries."""
+    import scipy.sparse as sp
+    n = m * m
+    h = 1.0 / (m + 1)
+    vx, vy = pe * rng.uniform(0.5, 1.0), pe * rng.uniform(0.0, 0.5)
+    rows, cols, vals = [], [], []
+    for j in range(m):
+        for i in range(m):
+            r = j * m + i
+            diag = 4.0 + h * (vx + vy)
+            rows.append(r); cols.append(r); vals.append(diag)
+            for di, dj, w in ((-1, 0, -1.0 - h * vx), (1, 0, -1.0), (0, -1, -1.0 - h * vy), (0, 1, -1.0)):
+                ii, jj = i + di, j + dj
+                if 0 <= ii < m and 0 <= jj < m:
+                    rows.append(r); cols.append(jj * m + ii); vals.append(w)
+    A = sp.csc_matrix((vals, (rows, cols)), shape=(n, n))
+    A.sort_indices()
+    return D.CSC(n, A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64))
+
+
+def test_batch_of_more_than_1024_systems(ctx):
+    # 1100 systems: the active list is rebuilt by the two-kernel compaction
+    # (scan, then fill) that batches of more than 1024 systems use
+    rng = np.random.default_rng(2)
+    mats = [small_convdiff(int(rng.integers(6, 15)), float(rng.uniform(1, 30)), rng) for _ in range(1100)]
+    S = D.System(ctx, mats)  # systems of 36-196 rows: row tiles (a 64-row slice would be mostly padding)
+    S.build(D.PRECISION_MIXED, D.SCHEDULE_BALANCED_NNZ)
+    b = rng.uniform(0, 1, sum(A.n for A in mats))
+    x, rep, oc = S.solve(b, rel_tol=1e-8, use_graph=True, true_residual_restart=True)
+    off = S.row_offsets
+    m_gpu = S.m_values()
+    moff = np.concatenate([[0], np.cumsum([A.nnz for A in mats])])
+    iters = [o["iterations"] for o in oc]
+    print(f"{len(mats)} systems: iterations {min(iters)}..{max(iters)}, graph replays {rep['host_checks']}", flush=True)
+    for s, A in enumerate(mats):
+        bs = b[off[s]:off[s + 1]]
+        Acsr = O.csc2csr(A.n, A.colptr, A.rowidx, A.val)[:3]
+        true_rel = O.true_residual(Acsr, bs, x[off[s]:off[s + 1]]) / np.linalg.norm(bs)
+        assert oc[s]["status"] == D.SOLVE_CONVERGED, (s, oc[s])
+        assert true_rel < 1e-8, (s, true_rel)
+        if s % 100 == 0:  # counts on a sample, against the oracle with the GPU's M
+            Mcsr = O.csc2csr(A.n, A.colptr, A.rowidx, m_gpu[moff[s]:moff[s + 1]])[:3]
+            spread = oracle_counts(Acsr, Mcsr, bs, 1e-8 * np.linalg.norm(bs), 1000, threads=(1, 2))
+            assert count_ok(oc[s]["iterations"], spread), (s, oc[s]["iterations"], sorted(spread))
