@@ -92,6 +92,41 @@ SPAI_DEV T group_sum(T v) {
     return v;
 }
 
+// A value from another lane of the same row of 16 lanes (a DPP move; CTRL
+// is the DPP control: 0xB1 and 0x4E swap neighbours and pairs inside a quad,
+// 0x141 mirrors each half-row of 8 lanes, 0x140 each row of 16).
+template <int CTRL>
+SPAI_DEV int dpp_move(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+SPAI_DEV float dpp_move(float v) {
+    return __int_as_float(dpp_move<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+SPAI_DEV double dpp_move(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_move<CTRL>((int)b), hi = dpp_move<CTRL>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over aligned groups of W lanes, every lane receiving the total.  Inside
+// a row of 16 lanes the partners come from DPP moves (quad swaps, then the
+// half-row and row mirrors, each pairing the lanes symmetrically, so every
+// lane adds the same two values); wider groups finish with cross-lane
+// permutes.
+template <int W, class T>
+SPAI_DEV T group_allsum(T v) {
+    static_assert(W == 1 || W == 2 || W == 4 || W == 8 || W == 16 || W == 32 || W == 64, "W: power of 2 up to 64");
+    if constexpr (W >= 2) v += dpp_move<0xB1>(v);
+    if constexpr (W >= 4) v += dpp_move<0x4E>(v);
+    if constexpr (W >= 8) v += dpp_move<0x141>(v);
+    if constexpr (W >= 16) v += dpp_move<0x140>(v);
+    if constexpr (W >= 32) v += __shfl_xor(v, 16, W);
+    if constexpr (W >= 64) v += __shfl_xor(v, 32, W);
+    return v;
+}
+
 template <int W>
 SPAI_DEV int group_max(int v) {
 #pragma unroll

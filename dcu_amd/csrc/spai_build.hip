@@ -320,7 +320,7 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
                 int rho = lane + W * r;
                 sl += (rho > j) ? A[r][j] * A[r][j] : (QT)0;
             }
-            QT sigma = group_sum<W>(sl);
+            QT sigma = group_allsum<W>(sl);
             QT beta = alpha, t = 0, scale = 0;
             if (sigma != (QT)0) {
                 QT nrm = sqrt(alpha * alpha + sigma);
@@ -346,10 +346,7 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
                 w[c] = s;
             }
 #pragma unroll
-            for (int o = W / 2; o > 0; o >>= 1) {
-#pragma unroll
-                for (int c = j + 1; c < NJ; ++c) w[c] += __shfl_xor(w[c], o, W);
-            }
+            for (int c = j + 1; c < NJ; ++c) w[c] = group_allsum<W>(w[c]);
 #pragma unroll
             for (int c = j + 1; c < NJ; ++c) {
                 if (c < n2) {
@@ -376,7 +373,7 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
                 v[r] = (rho > j) ? A[r][j] : (rho == j ? (QT)1 : (QT)0);
                 s += v[r] * y[r];
             }
-            s = group_sum<W>(s) * tau[j];
+            s = group_allsum<W>(s) * tau[j];
 #pragma unroll
             for (int r = 0; r < RPL; ++r) y[r] -= s * v[r];
         }
@@ -454,10 +451,7 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
                 }
             }
 #pragma unroll
-            for (int o = W / 2; o > 0; o >>= 1) {
-#pragma unroll
-                for (int c = 0; c < NJ; ++c) g[c] += __shfl_xor(g[c], o, W);
-            }
+            for (int c = 0; c < NJ; ++c) g[c] = group_allsum<W>(g[c]);
             // R^T u = g (forward), R d = u (backward), x += d
 #pragma unroll
             for (int c = 0; c < NJ; ++c) {
