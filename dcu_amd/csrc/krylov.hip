@@ -230,12 +230,20 @@ __device__ __forceinline__ void block_reduce(double (&v)[NRED], double *sh) {
 // L2s), the thread waits until they are acknowledged (vmcnt(0)) before the
 // relaxed counter increment, and the last workgroup reads them back with
 // agent-scope atomic loads.
+#ifndef SPAI_WAVE_SUM_DPP
+#define SPAI_WAVE_SUM_DPP 1
+#endif
 template <int NRED>
 __device__ __forceinline__ void wave_sum(double (&v)[NRED]) {
 #pragma unroll
-    for (int q = 0; q < NRED; ++q)
+    for (int q = 0; q < NRED; ++q) {
+        if constexpr (SPAI_WAVE_SUM_DPP != 0) {
+            v[q] = group_allsum<64>(v[q]);  // DPP inside rows of 16 lanes, then 2 permutes
+        } else {
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+            for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+        }
+    }
 }
 
 // Lane 0 publishes the partials, waits until they are acknowledged, then
