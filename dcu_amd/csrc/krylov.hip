@@ -1677,9 +1677,12 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     const int max_launch_iters = opt.max_iter + 1;
     int done_iters = 0;
     try {
+        // `active` holds nsys, or the count read after the last restart
+        // check; the unit count starts at its bound (the list is sized by
+        // the compaction itself), so no host round trip precedes the loop
         auto iterate = [&]() {
             if (use_list) compact(tm, nsys, st);
-            read_counts();
+            units = tm.nunits;
             if (opt.use_graph) {
                 const int gi = std::max(2, opt.graph_iters + (opt.graph_iters & 1));
                 const void *key[kGraphKey] = {A.val, A.idx, A.off, A.diag, M.val, M.idx, M.off, M.diag, M.blk,
@@ -1762,8 +1765,8 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                     (void)hipStreamSynchronize(st);
                     throw;
                 }
-                SPAI_HIP(hipStreamSynchronize(st));
-                if (pending) read_published();  // the newest graph's counts
+                // the graph still in flight (if any) runs before whatever
+                // is queued next; its counts are not needed
             } else {
                 const int every = std::max(1, opt.check_every);
                 int parity = 0;
@@ -1781,6 +1784,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
             }
         };
         iterate();
+        bool true_current = false;  // ||b - A x|| is in the states for the final x
         for (int round = 0; opt.true_residual_restart && round < 4; ++round) {
             OpTrueResidual tr{x, b};
             launch_units<true, double>(A, tm, c, tr, nullptr, 0, st);
@@ -1788,14 +1792,17 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                                counter_.get());
             SPAI_CHECK_LAUNCH();
             read_counts();
-            if (active == 0) break;
+            if (active == 0) {
+                true_current = true;
+                break;
+            }
             OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, s_, t_, y_, z_, opt.rel_tol, opt.abs_tol, 1};
             launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
             SPAI_CHECK_LAUNCH();
             done_iters = 0;
             iterate();
         }
-        {
+        if (!true_current) {
             OpTrueResidual op{x, b};
             launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
             SPAI_CHECK_LAUNCH();
