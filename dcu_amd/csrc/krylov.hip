@@ -1664,8 +1664,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     int active = nsys;
     int units = tm.nunits;
     int checks = 0;
-    int *h_counts = nullptr;
-    SPAI_HIP(hipHostMalloc((void **)&h_counts, 2 * sizeof(int), hipHostMallocDefault));
+    int *h_counts = h_pub_ + 4;  // copied counts (the graphs publish into h_pub_[0..1])
     auto read_counts = [&]() {
         SPAI_HIP(hipMemcpyAsync(h_counts, counter_.get() + nsys, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
         SPAI_HIP(hipStreamSynchronize(st));
@@ -1810,12 +1809,10 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
         SPAI_HIP(hipEventRecord(e1, st));
         SPAI_HIP(hipEventSynchronize(e1));
     } catch (...) {
-        (void)hipHostFree(h_counts);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         throw;
     }
-    (void)hipHostFree(h_counts);
     float ms = 0.0f;
     SPAI_HIP(hipEventElapsedTime(&ms, e0, e1));
     SPAI_HIP(hipEventDestroy(e0));
