@@ -232,3 +232,42 @@ def test_batch_of_more_than_1024_systems(ctx):
             Mcsr = O.csc2csr(A.n, A.colptr, A.rowidx, m_gpu[moff[s]:moff[s + 1]])[:3]
             spread = oracle_counts(Acsr, Mcsr, bs, 1e-8 * np.linalg.norm(bs), 1000, threads=(1, 2))
             assert count_ok(oc[s]["iterations"], spread), (s, oc[s]["iterations"], sorted(spread))
+
+
+@pytest.mark.parametrize("cap", [None, 32])
+def test_kernel_timers(ctx, cap):
+    # the per-kernel timers run every kernel of the iteration on every unit
+    # (both K1 forms), on diagonal slices and on 32-bit columns; afterwards a
+    # solve still converges (the timers leave no state behind)
+    import os
+    old = os.environ.get("SPAI_MI355X_SELL_INDEX")
+    try:
+        if cap is None:
+            os.environ.pop("SPAI_MI355X_SELL_INDEX", None)
+        else:
+            os.environ["SPAI_MI355X_SELL_INDEX"] = str(cap)
+        P = D.Problem.batched(4, 3)
+        mats = P.systems()
+        S = D.System(ctx, mats)
+    finally:
+        if old is None:
+            os.environ.pop("SPAI_MI355X_SELL_INDEX", None)
+        else:
+            os.environ["SPAI_MI355X_SELL_INDEX"] = old
+    S.build(D.PRECISION_MIXED, D.SCHEDULE_BALANCED_NNZ)
+    b = np.concatenate([P.rhs(s) for s in range(len(mats))])
+    S.solve(b, rel_tol=1e-8, use_graph=True)  # the single-kernel timers run on the last solve's vectors
+    for which in range(6):
+        assert S.time_spmv(which, 3) > 0.0
+    for variant in (0, 1, -1):
+        ms = S.time_iteration(3, variant)
+        assert ms[0] > 0 and ms[2] > 0 and ms[3] > 0 and ms[4] > 0
+        if variant >= 0:
+            assert (ms[1] > 0) == (variant == 1)
+    x, rep, oc = S.solve(b, rel_tol=1e-8, use_graph=True)
+    off = S.row_offsets
+    for s, A in enumerate(mats):
+        Acsr = O.csc2csr(A.n, A.colptr, A.rowidx, A.val)[:3]
+        bs = b[off[s]:off[s + 1]]
+        assert oc[s]["status"] == D.SOLVE_CONVERGED
+        assert O.true_residual(Acsr, bs, x[off[s]:off[s + 1]]) / np.linalg.norm(bs) < 1.5e-8
