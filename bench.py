@@ -97,6 +97,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")
     ap.add_argument("--graph-iters", type=int, default=8)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="build with the default tile shapes instead of tuning them once before the warm-up")
+    ap.add_argument("--tune-reps", type=int, default=5)
     ap.add_argument("--profile-kernels", action="store_true",
                     help="after one solve, launch each fused kernel --profile-reps times and a device copy of "
                          "known size (256 MiB read + 256 MiB written) as many times, for counter collection")
@@ -156,6 +159,12 @@ def main():
         ctx.close()
         return
 
+    tune = None
+    if not args.no_autotune:
+        # one-time set-up, like the graph capture: the tile shape of each
+        # size bucket is chosen on this batch's own columns, then kept
+        tune = S.autotune(precision, D.SCHEDULE_BALANCED_NNZ, reps=args.tune_reps)
+        log(f"autotune: shapes {tune['shape']} in {tune['tune_ms']:.0f} ms, tier ms per candidate {tune['ms']}")
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -309,6 +318,9 @@ def main():
         "speedup_vs_plain_fp64": (round(value / plain["value"], 3) if plain else None),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "autotune": ({"tile_shapes": [D.tile_shape(t, c) for t, c in enumerate(tune["shape"])],
+                      "tier_ms_per_candidate": [[None if v is None else round(v, 4) for v in r] for r in tune["ms"]],
+                      "tune_ms": round(tune["tune_ms"], 1)} if tune else None),
     }
     S.close()
     ctx.close()

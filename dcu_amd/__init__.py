@@ -54,6 +54,8 @@ C_SYMBOLS = [
     "spai_system_outcomes", "spai_system_residual_history", "spai_system_adjoint", "spai_system_time_spmv",
     "spai_system_time_iteration", "spai_system_layout", "spai_system_set_pattern", "spai_system_m_pattern",
     "spai_system_quality", "spai_system_product",
+    "spai_tile_shape_count", "spai_tile_shape_get", "spai_system_autotune", "spai_system_set_tile_shapes",
+    "spai_system_get_tile_shapes",
     "spai_calibration_copy", "spai_csc_to_csr_device",
     "spai_problem_generate", "spai_problem_generate_power_law", "spai_problem_generate_batched",
     "spai_problem_generate_grf", "spai_problem_convection_diffusion", "spai_problem_read_mtx", "spai_problem_free",
@@ -101,7 +103,21 @@ class SystemInfo(ctypes.Structure):
 class BuildReport(ctypes.Structure):
     _fields_ = [("ms", ctypes.c_float), ("tier_columns", ctypes.c_int * 5), ("max_gathered", ctypes.c_int),
                 ("max_n2", ctypes.c_int), ("empty_columns", ctypes.c_int), ("sum_gathered", ctypes.c_longlong),
-                ("static_tier", ctypes.c_int)]
+                ("static_tier", ctypes.c_int), ("tile_shape", ctypes.c_int * 3)]
+
+
+TUNE_TIERS = 3
+TUNE_MAX_CANDIDATES = 8
+
+
+class TileShape(ctypes.Structure):
+    _fields_ = [("lanes", ctypes.c_int), ("entries_per_lane", ctypes.c_int), ("max_cols", ctypes.c_int),
+                ("rows_per_lane", ctypes.c_int)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("shape", ctypes.c_int * TUNE_TIERS), ("columns", ctypes.c_int * TUNE_TIERS),
+                ("ms", (ctypes.c_float * TUNE_MAX_CANDIDATES) * TUNE_TIERS), ("tune_ms", ctypes.c_float)]
 
 
 class SolveOptions(ctypes.Structure):
@@ -177,6 +193,12 @@ def load_library() -> ctypes.CDLL:
                                                       ctypes.POINTER(ctypes.c_float)]),
         "spai_system_layout": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)]),
         "spai_system_set_pattern": (ctypes.c_int, [vp, ctypes.c_int]),
+        "spai_tile_shape_count": (ctypes.c_int, [ctypes.c_int]),
+        "spai_tile_shape_get": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(TileShape)]),
+        "spai_system_autotune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                ctypes.POINTER(Tuning)]),
+        "spai_system_set_tile_shapes": (ctypes.c_int, [vp, ip]),
+        "spai_system_get_tile_shapes": (ctypes.c_int, [vp, ip]),
         "spai_system_quality": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double), vp]),
         "spai_system_product": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_longlong), vp, vp, vp]),
         "spai_system_m_pattern": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), vp, vp]),
@@ -205,6 +227,20 @@ def load_library() -> ctypes.CDLL:
         f.argtypes = args
     _lib = lib
     return lib
+
+
+def tile_shape_count(tier: int) -> int:
+    """Candidate group shapes of register tier `tier` (0..2)."""
+    return int(load_library().spai_tile_shape_count(int(tier)))
+
+
+def tile_shape(tier: int, candidate: int) -> dict:
+    """Lanes per column, gathered entries per lane, largest |J_k| and tile
+    rows per lane of one candidate."""
+    t = TileShape()
+    _check(load_library().spai_tile_shape_get(int(tier), int(candidate), ctypes.byref(t)))
+    return {"lanes": t.lanes, "entries_per_lane": t.entries_per_lane, "max_cols": t.max_cols,
+            "rows_per_lane": t.rows_per_lane}
 
 
 def _check(status: int) -> None:
@@ -456,7 +492,28 @@ class System:
                                            ctypes.byref(rep)))
         return {"ms": rep.ms, "tier_columns": list(rep.tier_columns), "max_gathered": rep.max_gathered,
                 "max_n2": rep.max_n2, "empty_columns": rep.empty_columns, "sum_gathered": rep.sum_gathered,
-                "static_tier": rep.static_tier}
+                "static_tier": rep.static_tier, "tile_shape": list(rep.tile_shape)}
+
+    def autotune(self, precision: int = PRECISION_MIXED, schedule: int = SCHEDULE_BALANCED_NNZ,
+                 reps: int = 5) -> dict:
+        """Times every candidate group shape of each populated register tier
+        on this system's columns and keeps the fastest; M is left built with
+        them.  Returns the chosen shape per tier, the columns per tier and the
+        best tier-kernel ms per candidate (None where not timed)."""
+        t = Tuning()
+        _check(self._lib.spai_system_autotune(self._h, int(precision), int(schedule), int(reps), ctypes.byref(t)))
+        return {"shape": list(t.shape), "columns": list(t.columns), "tune_ms": t.tune_ms,
+                "ms": [[(v if v >= 0 else None) for v in t.ms[i][:tile_shape_count(i)]] for i in range(TUNE_TIERS)]}
+
+    def set_tile_shapes(self, shape) -> None:
+        a = np.ascontiguousarray(shape, dtype=np.int32)
+        assert a.size == TUNE_TIERS
+        _check(self._lib.spai_system_set_tile_shapes(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+
+    def tile_shapes(self) -> List[int]:
+        a = np.zeros(TUNE_TIERS, dtype=np.int32)
+        _check(self._lib.spai_system_get_tile_shapes(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return [int(v) for v in a]
 
     def set_pattern(self, pattern: int) -> None:
         """M's pattern for the next builds: PATTERN_A (default), PATTERN_AT,

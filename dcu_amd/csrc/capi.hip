@@ -159,6 +159,7 @@ struct spai_system {
     bool tiles_t_ready = false;
     DeviceArray<double> stage_b, stage_x, stage_g, stage_dA, stage_db;
     float setup_ms = 0.0f;
+    int tile_shape[kTunedTiers] = {0, 0, 0};  // register-tier group shapes of the next builds
     std::vector<SysState> last_states;
     const BiCGStab *last_solver = nullptr;
 };
@@ -505,75 +506,173 @@ int spai_system_set_values(spai_system *sys, const double *values, int values_on
     });
 }
 
-int spai_system_build(spai_system *sys, int precision, int schedule, int col_begin, int col_end,
-                      spai_build_report *report) {
+}  // extern "C"
+
+namespace {
+// One SPAI build of M with the register-tier shapes `shape`; tiers_ms
+// (nullable) receives the tier kernels' device time.  Throws on failure.
+void build_m(spai_system *sys, int precision, int schedule, int col_begin, int col_end, const int *shape,
+             float *tiers_ms, spai_build_report *report) {
+    hipStream_t st = sys->ctx->stream;
+    SPAI_HIP(hipSetDevice(sys->ctx->device));
+    const bool own = sys->pattern != SPAI_PATTERN_A;
+    size_t nz = (size_t)std::max(sys->m_nnz, 1);
+    const bool sell = sys->tiles.kind == kLayoutSell;
+    const long long lay =
+        std::max<long long>(sell ? (own ? sys->msell.len : sys->tiles.sell_len) : (long long)sys->m_nnz, 1);
+    if (precision == SPAI_PRECISION_MIXED) {
+        sys->m32_csc.resize(nz);
+        sys->m32_lay.resize((size_t)lay);
+        if (sell) fill(sys->m32_lay.get(), lay, st);  // SELL padding entries stay 0
+    } else {
+        sys->m64_csc.resize(nz);
+        sys->m64_lay.resize((size_t)lay);
+        if (sell) fill(sys->m64_lay.get(), lay, st);
+    }
+    sys->n1.resize((size_t)sys->n);
+    int cb = std::max(0, col_begin), ce = col_end < 0 ? sys->n : std::min(col_end, sys->n);
+    require(cb <= ce, "column range is inverted");
+    sys->m_precision = -1;
+    Timer tm(st);
+    bool partial = cb > 0 || ce < sys->n;
+    if (partial) {
+        if (precision == SPAI_PRECISION_MIXED) {
+            fill(sys->m32_csc.get(), sys->m_nnz, st);
+            if (!sell) fill(sys->m32_lay.get(), sys->m_nnz, st);
+        } else {
+            fill(sys->m64_csc.get(), sys->m_nnz, st);
+            if (!sell) fill(sys->m64_lay.get(), sys->m_nnz, st);
+        }
+    }
+    SpaiBuildParams p;
+    p.n = sys->n;
+    p.colptr = sys->colptr.get();
+    p.rowidx = sys->rowidx.get();
+    p.val = sys->cscval.get();
+    p.perm = sys->perm_lay.get();
+    if (own) {
+        p.pcolptr = sys->pcolptr.get();
+        p.prowidx = sys->prowidx.get();
+    }
+    p.precision = precision;
+    p.schedule = schedule;
+    p.col_begin = cb;
+    p.col_end = ce;
+    p.m_csc = precision == SPAI_PRECISION_MIXED ? (void *)sys->m32_csc.get() : (void *)sys->m64_csc.get();
+    p.m_csr = precision == SPAI_PRECISION_MIXED ? (void *)sys->m32_lay.get() : (void *)sys->m64_lay.get();
+    p.n1_out = sys->n1.get();
+    for (int t = 0; t < kTunedTiers; ++t) p.shape[t] = shape[t];
+    p.tiers_ms = tiers_ms;
+    SpaiBuildStats stats;
+    sys->ctx->builder.build(p, st, &stats);
+    float ms = tm.stop();
+    sys->m_precision = precision;
+    if (report) {
+        report->ms = ms;
+        for (int t = 0; t < 5; ++t) report->tier_columns[t] = stats.tier_columns[t];
+        report->max_gathered = stats.max_gathered;
+        report->max_n2 = stats.max_n2;
+        report->empty_columns = stats.empty_columns;
+        report->sum_gathered = stats.sum_gathered;
+        report->static_tier = stats.static_tier;
+        for (int t = 0; t < kTunedTiers; ++t) report->tile_shape[t] = shape[t];
+    }
+}
+
+int check_build_args(const spai_system *sys, int precision, int schedule) {
     if (!sys) return fail(SPAI_ERR_INVALID, "sys is NULL");
     if (precision != SPAI_PRECISION_FP64 && precision != SPAI_PRECISION_MIXED)
         return fail(SPAI_ERR_INVALID, "unknown precision");
     if (schedule != SPAI_SCHEDULE_BALANCED_COLUMNS && schedule != SPAI_SCHEDULE_BALANCED_NNZ)
         return fail(SPAI_ERR_INVALID, "unknown schedule");
+    return SPAI_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int spai_system_build(spai_system *sys, int precision, int schedule, int col_begin, int col_end,
+                      spai_build_report *report) {
+    if (int rc = check_build_args(sys, precision, schedule)) return rc;
+    return guarded([&] { build_m(sys, precision, schedule, col_begin, col_end, sys->tile_shape, nullptr, report); });
+}
+
+int spai_tile_shape_count(int tier) { return tile_shape_count(tier); }
+
+int spai_tile_shape_get(int tier, int candidate, spai_tile_shape *out) {
+    if (!out || candidate < 0 || candidate >= tile_shape_count(tier))
+        return fail(SPAI_ERR_INVALID, "no such tier or candidate");
+    const TileShape t = tile_shape(tier, candidate);
+    out->lanes = t.lanes;
+    out->entries_per_lane = t.entries_per_lane;
+    out->max_cols = t.max_cols;
+    out->rows_per_lane = t.rows_per_lane;
+    return SPAI_OK;
+}
+
+int spai_system_set_tile_shapes(spai_system *sys, const int *shape) {
+    if (!sys || !shape) return fail(SPAI_ERR_INVALID, "bad arguments");
+    for (int t = 0; t < kTunedTiers; ++t)
+        if (shape[t] < 0 || shape[t] >= tile_shape_count(t)) return fail(SPAI_ERR_INVALID, "no such tile shape");
+    for (int t = 0; t < kTunedTiers; ++t) sys->tile_shape[t] = shape[t];
+    return SPAI_OK;
+}
+
+int spai_system_get_tile_shapes(const spai_system *sys, int *shape) {
+    if (!sys || !shape) return fail(SPAI_ERR_INVALID, "bad arguments");
+    for (int t = 0; t < kTunedTiers; ++t) shape[t] = sys->tile_shape[t];
+    return SPAI_OK;
+}
+
+// Coordinate search, one register tier (size bucket) at a time from the
+// smallest: every candidate of the tier is timed (one untimed build, then
+// the best of `reps` timed ones) with the shapes chosen so far for the
+// others; ties keep the lower index.  The tiers' kernel time is the
+// measure, since the shapes change nothing else of the build.  A tier with
+// no columns under the default shapes (whose rows per group are the
+// smallest, so they hand on the most columns) keeps its current shape.
+int spai_system_autotune(spai_system *sys, int precision, int schedule, int reps, spai_tuning *out) {
+    if (int rc = check_build_args(sys, precision, schedule)) return rc;
+    if (reps < 1 || reps > 100) return fail(SPAI_ERR_INVALID, "reps must be in 1..100");
     return guarded([&] {
-        hipStream_t st = sys->ctx->stream;
-        SPAI_HIP(hipSetDevice(sys->ctx->device));
-        const bool own = sys->pattern != SPAI_PATTERN_A;
-        size_t nz = (size_t)std::max(sys->m_nnz, 1);
-        const bool sell = sys->tiles.kind == kLayoutSell;
-        const long long lay =
-            std::max<long long>(sell ? (own ? sys->msell.len : sys->tiles.sell_len) : (long long)sys->m_nnz, 1);
-        if (precision == SPAI_PRECISION_MIXED) {
-            sys->m32_csc.resize(nz);
-            sys->m32_lay.resize((size_t)lay);
-            if (sell) fill(sys->m32_lay.get(), lay, st);  // SELL padding entries stay 0
-        } else {
-            sys->m64_csc.resize(nz);
-            sys->m64_lay.resize((size_t)lay);
-            if (sell) fill(sys->m64_lay.get(), lay, st);
+        const auto t0 = std::chrono::steady_clock::now();
+        spai_tuning res{};
+        for (int t = 0; t < SPAI_TUNE_TIERS; ++t)
+            for (int c = 0; c < SPAI_TUNE_MAX_CANDIDATES; ++c) res.ms[t][c] = -1.0f;
+        static_assert(SPAI_TUNE_TIERS == kTunedTiers, "one entry per register tier");
+        int shape[kTunedTiers] = {0, 0, 0};
+        spai_build_report rep{};
+        build_m(sys, precision, schedule, 0, -1, shape, nullptr, &rep);
+        for (int t = 0; t < kTunedTiers; ++t) {
+            res.columns[t] = rep.tier_columns[t];
+            shape[t] = sys->tile_shape[t];
         }
-        sys->n1.resize((size_t)sys->n);
-        int cb = std::max(0, col_begin), ce = col_end < 0 ? sys->n : std::min(col_end, sys->n);
-        require(cb <= ce, "column range is inverted");
-        sys->m_precision = -1;
-        Timer tm(st);
-        bool partial = cb > 0 || ce < sys->n;
-        if (partial) {
-            if (precision == SPAI_PRECISION_MIXED) {
-                fill(sys->m32_csc.get(), sys->m_nnz, st);
-                if (!sell) fill(sys->m32_lay.get(), sys->m_nnz, st);
-            } else {
-                fill(sys->m64_csc.get(), sys->m_nnz, st);
-                if (!sell) fill(sys->m64_lay.get(), sys->m_nnz, st);
+        for (int t = 0; t < kTunedTiers; ++t) {
+            const int nc = tile_shape_count(t);
+            require(nc <= SPAI_TUNE_MAX_CANDIDATES, "too many tile-shape candidates");
+            if (res.columns[t] == 0) continue;
+            int best = -1;
+            for (int c = 0; c < nc; ++c) {
+                shape[t] = c;
+                build_m(sys, precision, schedule, 0, -1, shape, nullptr, nullptr);  // code load, caches
+                float m = 0.0f;
+                for (int r = 0; r < reps; ++r) {
+                    float ms = 0.0f;
+                    build_m(sys, precision, schedule, 0, -1, shape, &ms, nullptr);
+                    m = r == 0 ? ms : std::min(m, ms);
+                }
+                res.ms[t][c] = m;
+                if (best < 0 || m < res.ms[t][best]) best = c;
             }
+            shape[t] = best;
         }
-        SpaiBuildParams p;
-        p.n = sys->n;
-        p.colptr = sys->colptr.get();
-        p.rowidx = sys->rowidx.get();
-        p.val = sys->cscval.get();
-        p.perm = sys->perm_lay.get();
-        if (own) {
-            p.pcolptr = sys->pcolptr.get();
-            p.prowidx = sys->prowidx.get();
+        for (int t = 0; t < kTunedTiers; ++t) {
+            sys->tile_shape[t] = shape[t];
+            res.shape[t] = shape[t];
         }
-        p.precision = precision;
-        p.schedule = schedule;
-        p.col_begin = cb;
-        p.col_end = ce;
-        p.m_csc = precision == SPAI_PRECISION_MIXED ? (void *)sys->m32_csc.get() : (void *)sys->m64_csc.get();
-        p.m_csr = precision == SPAI_PRECISION_MIXED ? (void *)sys->m32_lay.get() : (void *)sys->m64_lay.get();
-        p.n1_out = sys->n1.get();
-        SpaiBuildStats stats;
-        sys->ctx->builder.build(p, st, &stats);
-        float ms = tm.stop();
-        sys->m_precision = precision;
-        if (report) {
-            report->ms = ms;
-            for (int t = 0; t < 5; ++t) report->tier_columns[t] = stats.tier_columns[t];
-            report->max_gathered = stats.max_gathered;
-            report->max_n2 = stats.max_n2;
-            report->empty_columns = stats.empty_columns;
-            report->sum_gathered = stats.sum_gathered;
-            report->static_tier = stats.static_tier;
-        }
+        build_m(sys, precision, schedule, 0, -1, shape, nullptr, nullptr);  // M from the chosen shapes
+        res.tune_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (out) *out = res;
     });
 }
 

@@ -14,6 +14,8 @@
 //               --max-iter K               (reference: 10000)
 //               --x0 ones|zero, --rhs ones|problem
 //               --graph                    replay iterations from a hipGraph
+//               --autotune                 time the tile-shape candidates of each
+//                                          size bucket first and build with the fastest
 //
 // With --matrix the solve defaults follow the reference driver
 // (src/spai_multi_dcu.cpp:1920-1934: b = 1, x0 = 1, TOL = 1e-7 absolute,
@@ -32,7 +34,8 @@ static void usage(const char *prog) {
     std::printf("usage: %s [dcu_count] [--algo static|dynamic] [--partition balanced_columns|balanced_nnz]\n", prog);
     std::printf("           [--matrix <path> | --gen <1..5> [--seed S]] [--precondition-only]\n");
     std::printf("           [--precision fp64|mixed] [--tol T | --rel-tol R] [--max-iter K]\n");
-    std::printf("           [--x0 ones|zero] [--rhs ones|problem] [--graph] [--pattern a|at|i+a|i+a+at] [--help]\n");
+    std::printf("           [--x0 ones|zero] [--rhs ones|problem] [--graph] [--pattern a|at|i+a|i+a+at]\n");
+    std::printf("           [--autotune] [--help]\n");
 }
 
 static void check(int st, const char *what) {
@@ -46,6 +49,7 @@ int main(int argc, char **argv) {
     std::string matrix = "matrices/circuit_2.mtx";
     std::string algo = "static";
     bool nnz_balance = false, partition_given = false, precondition_only = false, use_graph = false;
+    bool autotune = false;
     int gen = 0, max_iter = -1, precision = SPAI_PRECISION_FP64, pattern = SPAI_PATTERN_A;
     unsigned long long seed = 0;
     double tol = -1.0, rel_tol = -1.0;
@@ -78,6 +82,7 @@ int main(int argc, char **argv) {
         else if (a == "--x0") x0_mode = next();
         else if (a == "--rhs") rhs_mode = next();
         else if (a == "--graph") use_graph = true;
+        else if (a == "--autotune") autotune = true;
         else if (a == "--pattern") {
             const std::string v = next();
             if (v == "a") pattern = SPAI_PATTERN_A;
@@ -128,10 +133,21 @@ int main(int argc, char **argv) {
     spai_system_get_info(S, &info);
 
     if (pattern != SPAI_PATTERN_A) check(spai_system_set_pattern(S, pattern), "pattern");
+    const int schedule = nnz_balance ? SPAI_SCHEDULE_BALANCED_NNZ : SPAI_SCHEDULE_BALANCED_COLUMNS;
+    if (autotune) {
+        spai_tuning tu{};
+        check(spai_system_autotune(S, precision, schedule, 5, &tu), "autotune");
+        std::printf("autotune: %.1f ms", tu.tune_ms);
+        for (int t = 0; t < SPAI_TUNE_TIERS; ++t) {
+            spai_tile_shape sh{};
+            check(spai_tile_shape_get(t, tu.shape[t], &sh), "tile shape");
+            std::printf("  tier %d: %d lanes x %d entries, %d rows/lane (%d columns)", t, sh.lanes,
+                        sh.entries_per_lane, sh.rows_per_lane, tu.columns[t]);
+        }
+        std::printf("\n");
+    }
     spai_build_report br{};
-    check(spai_system_build(S, precision, nnz_balance ? SPAI_SCHEDULE_BALANCED_NNZ : SPAI_SCHEDULE_BALANCED_COLUMNS,
-                            0, -1, &br),
-          "SPAI build");
+    check(spai_system_build(S, precision, schedule, 0, -1, &br), "SPAI build");
     std::printf("SPAI build: %.4f ms  max|J|=%d  max gathered=%d  tiers=[%d %d %d %d %d]\n", br.ms, br.max_n2,
                 br.max_gathered, br.tier_columns[0], br.tier_columns[1], br.tier_columns[2], br.tier_columns[3],
                 br.tier_columns[4]);

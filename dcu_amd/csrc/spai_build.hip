@@ -37,6 +37,9 @@
 
 #include <algorithm>
 #include <cstring>
+#include <tuple>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace spai {
@@ -67,11 +70,62 @@ struct RegTier {
     static constexpr int GROUPS = kBlock / W;
 };
 
-using Tier0 = RegTier<8, 4, 8, 2>;
-using Tier1 = RegTier<16, 4, 8, 2>;
-using Tier2 = RegTier<64, 2, 16, 2>;
-static_assert(Tier0::TMAX <= Tier1::ROWS && Tier1::TMAX <= Tier2::ROWS && Tier2::TMAX <= Tier2::ROWS,
-              "an overflowed column must fit the next tier's rows");
+// Size buckets of the register tiers: a column goes to the first bucket
+// whose largest T_k and |J_k| hold it.  Each bucket has a list of candidate
+// group shapes (the first is the default); the autotuner times them on a
+// system's own columns and keeps the fastest per bucket.  Every candidate of
+// a bucket gathers exactly the bucket's T_k (TMAX), holds its |J_k| (NJ, one
+// lane per x entry in the refinement: W >= NJ) and, when its rows fall short
+// of T_k, hands a wider column to the next bucket, whose every candidate
+// holds the rows.  The shape changes the order of the group sums, so M
+// differs between shapes in the last bits (well inside the parity bars).
+constexpr int kRegTiers = kTunedTiers;
+constexpr int kBucketT[kRegTiers] = {32, 64, 128};
+constexpr int kBucketNJ[kRegTiers] = {8, 8, 16};
+using Bucket0 = std::tuple<RegTier<8, 4, 8, 2>, RegTier<16, 2, 8, 1>, RegTier<8, 4, 8, 4>, RegTier<16, 2, 8, 2>>;
+using Bucket1 = std::tuple<RegTier<16, 4, 8, 2>, RegTier<32, 2, 8, 1>, RegTier<16, 4, 8, 4>, RegTier<8, 8, 8, 4>>;
+using Bucket2 = std::tuple<RegTier<64, 2, 16, 2>, RegTier<32, 4, 16, 4>>;
+template <int B>
+using BucketOf = std::tuple_element_t<B, std::tuple<Bucket0, Bucket1, Bucket2>>;
+
+constexpr bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+template <int B, class TR>
+constexpr bool shape_ok() {
+    return TR::TMAX == kBucketT[B] && TR::NJ == kBucketNJ[B] && TR::W >= TR::NJ && pow2(TR::W) && TR::W <= 64 &&
+           pow2(TR::EPL) && kBlock % TR::W == 0 && (B == 0 || TR::ROWS >= kBucketT[B - 1]) &&
+           (B < kRegTiers - 1 || TR::ROWS >= TR::TMAX);
+}
+template <int B, size_t... I>
+constexpr bool bucket_ok(std::index_sequence<I...>) {
+    return (shape_ok<B, std::tuple_element_t<I, BucketOf<B>>>() && ...);
+}
+template <int B>
+constexpr bool bucket_ok() {
+    return bucket_ok<B>(std::make_index_sequence<std::tuple_size_v<BucketOf<B>>>{});
+}
+static_assert(bucket_ok<0>() && bucket_ok<1>() && bucket_ok<2>(),
+              "a candidate must gather its bucket's T_k, hold its |J_k|, and an overflowed column must fit the "
+              "next bucket's rows");
+
+// f(shape) for candidate c of bucket B (c is range-checked by the caller)
+template <int B, class F, size_t... I>
+void with_shape(int c, F &&f, std::index_sequence<I...>) {
+    ((c == (int)I ? (f(std::tuple_element_t<I, BucketOf<B>>{}), 0) : 0), ...);
+}
+template <int B, class F>
+void with_shape(int c, F &&f) {
+    with_shape<B>(c, f, std::make_index_sequence<std::tuple_size_v<BucketOf<B>>>{});
+}
+template <int B, size_t... I>
+TileShape shape_info(int c, std::index_sequence<I...>) {
+    TileShape r{};
+    ((c == (int)I ? (r = TileShape{std::tuple_element_t<I, BucketOf<B>>::W, std::tuple_element_t<I, BucketOf<B>>::EPL,
+                                   std::tuple_element_t<I, BucketOf<B>>::NJ, std::tuple_element_t<I, BucketOf<B>>::RPL},
+                     0)
+                  : 0),
+     ...);
+    return r;
+}
 
 // LDS of one column group, in bytes.  The tile and R are held in the QR's
 // type (FP32 in mixed mode); mixed mode's FP64 refinement reads the column's
@@ -979,9 +1033,9 @@ __global__ __launch_bounds__(kGlobThreads) void spai_global_tier_kernel(ColumnAr
 // [kTiers..2kTiers) queue heads, [2kTiers] max T, [2kTiers+1] max n2,
 // [2kTiers+2] empty-column count; stats64[0] = sum of T_k.
 __device__ __forceinline__ int classify(int T, int n2, int lds_fit) {
-    if (n2 <= Tier0::NJ && T <= Tier0::TMAX) return 0;
-    if (n2 <= Tier1::NJ && T <= Tier1::TMAX) return 1;
-    if (n2 <= Tier2::NJ && T <= Tier2::TMAX) return 2;
+    if (n2 <= kBucketNJ[0] && T <= kBucketT[0]) return 0;
+    if (n2 <= kBucketNJ[1] && T <= kBucketT[1]) return 1;
+    if (n2 <= kBucketNJ[2] && T <= kBucketT[2]) return 2;
     if (n2 <= kLdsTierNJ && T <= kLdsTierT && lds_fit) return 3;
     return 4;
 }
@@ -1144,7 +1198,7 @@ constexpr int kCounters = kOvfHead + 2;
 constexpr int kLists = kTiers + 2;
 
 template <class MT, bool REFINE>
-void launch_tiers(const ColumnArgs &base, bool dynamic, int static_tier, int *counters, int *lists,
+void launch_tiers(const ColumnArgs &base, bool dynamic, int static_tier, const int *shape, int *counters, int *lists,
                   int list_stride, int cus, hipStream_t st) {
     using QT = typename std::conditional<REFINE, float, double>::type;
     ColumnArgs a = base;
@@ -1198,24 +1252,24 @@ void launch_tiers(const ColumnArgs &base, bool dynamic, int static_tier, int *co
         }
         SPAI_CHECK_LAUNCH();
     };
+    // register tier t with the bucket's chosen shape
+    auto run_bucket = [&](int t, bool ovf = false) {
+        auto go = [&](auto tier_tag) { run_reg(tier_tag, t, ovf); };
+        if (t == 0) with_shape<0>(shape[0], go);
+        else if (t == 1) with_shape<1>(shape[1], go);
+        else with_shape<2>(shape[2], go);
+    };
     if (dynamic) {
         // smallest tier first: each register tier's queue ends with the
         // columns the tier before it handed on (the launches run one after
         // another on the stream either way)
-        run_reg(Tier0{}, 0);
-        run_reg(Tier1{}, 1);
-        run_reg(Tier2{}, 2);
+        for (int t = 0; t < kRegTiers; ++t) run_bucket(t);
         run_lds(3);
     } else {
-        switch (static_tier) {
-        case 0: run_reg(Tier0{}, 0); break;
-        case 1: run_reg(Tier1{}, 1); break;
-        case 2: run_reg(Tier2{}, 2); break;
-        default: run_lds(3); break;
-        }
+        if (static_tier < kRegTiers) run_bucket(static_tier);
+        else run_lds(3);
         // the overflow lists of the tiers after the static one
-        if (static_tier == 0) run_reg(Tier1{}, 1, true);
-        if (static_tier <= 1) run_reg(Tier2{}, 2, true);
+        for (int t = static_tier + 1; t < kRegTiers; ++t) run_bucket(t, true);
     }
 }
 
@@ -1385,14 +1439,29 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
         fetch();
         static_tier = classify_host(hc[2 * kTiers], hc[2 * kTiers + 1]);
     }
+    for (int t = 0; t < kTunedTiers; ++t)
+        require(p.shape[t] >= 0 && p.shape[t] < tile_shape_count(t), "spai build: no such tile shape");
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (p.tiers_ms) {
+        SPAI_HIP(hipEventCreate(&ev[0]));
+        SPAI_HIP(hipEventCreate(&ev[1]));
+        SPAI_HIP(hipEventRecord(ev[0], st));
+    }
     if (ce > cb && static_tier < kTiers - 1) {
         if (p.precision == 1)
-            launch_tiers<float, true>(a, dynamic, static_tier, counters_.get(), lists_.get(), list_stride, cus_, st);
+            launch_tiers<float, true>(a, dynamic, static_tier, p.shape, counters_.get(), lists_.get(), list_stride,
+                                      cus_, st);
         else
-            launch_tiers<double, false>(a, dynamic, static_tier, counters_.get(), lists_.get(), list_stride, cus_,
-                                        st);
+            launch_tiers<double, false>(a, dynamic, static_tier, p.shape, counters_.get(), lists_.get(), list_stride,
+                                        cus_, st);
     }
+    if (p.tiers_ms) SPAI_HIP(hipEventRecord(ev[1], st));
     fetch();
+    if (p.tiers_ms) {
+        SPAI_HIP(hipEventElapsedTime(p.tiers_ms, ev[0], ev[1]));
+        SPAI_HIP(hipEventDestroy(ev[0]));
+        SPAI_HIP(hipEventDestroy(ev[1]));
+    }
     const int global_cols = dynamic ? hc[kTiers - 1] : (static_tier == kTiers - 1 ? ce - cb : 0);
     if (global_cols > 0) {
         // very wide columns: one workgroup per column with a global scratch
@@ -1455,10 +1524,27 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
     }
 }
 
+int tile_shape_count(int tier) {
+    switch (tier) {
+    case 0: return (int)std::tuple_size_v<Bucket0>;
+    case 1: return (int)std::tuple_size_v<Bucket1>;
+    case 2: return (int)std::tuple_size_v<Bucket2>;
+    default: return 0;
+    }
+}
+
+TileShape tile_shape(int tier, int c) {
+    if (c < 0 || c >= tile_shape_count(tier)) return TileShape{};
+    switch (tier) {
+    case 0: return shape_info<0>(c, std::make_index_sequence<std::tuple_size_v<Bucket0>>{});
+    case 1: return shape_info<1>(c, std::make_index_sequence<std::tuple_size_v<Bucket1>>{});
+    default: return shape_info<2>(c, std::make_index_sequence<std::tuple_size_v<Bucket2>>{});
+    }
+}
+
 int classify_host(int T, int n2) {
-    if (n2 <= 8 && T <= 32) return 0;
-    if (n2 <= 8 && T <= 64) return 1;
-    if (n2 <= 16 && T <= 128) return 2;
+    for (int b = 0; b < kRegTiers; ++b)
+        if (n2 <= kBucketNJ[b] && T <= kBucketT[b]) return b;
     long long need = (long long)T * n2 + (long long)n2 * n2 + n2;
     if (n2 <= kLdsTierNJ && T <= kLdsTierT && need <= kLdsTierDoubles) return 3;
     return 4;

@@ -12,6 +12,17 @@ namespace spai {
 // column, FP64, tile in a scratch slot).
 constexpr int kTiers = 5;
 
+// Group shape of a register tier: W lanes per column, EPL gathered entries
+// per lane, NJ largest |J_k|, RPL tile rows per lane.  Each of the three
+// register tiers (size buckets) has a list of candidates; entry 0 is the
+// default and the autotuner picks one per bucket.
+struct TileShape {
+    int lanes = 0, entries_per_lane = 0, max_cols = 0, rows_per_lane = 0;
+};
+constexpr int kTunedTiers = 3;
+int tile_shape_count(int tier);          // candidates of register tier `tier`, 0 if none
+TileShape tile_shape(int tier, int c);   // candidate c (zeros when out of range)
+
 struct SpaiBuildParams {
     int n = 0;
     const int *colptr = nullptr;  // device CSC of A
@@ -26,6 +37,8 @@ struct SpaiBuildParams {
     void *m_csc = nullptr;        // double[nnz(M)] or float[nnz(M)] (precision 1), pattern order
     void *m_csr = nullptr;        // same type, M's solver layout order
     int *n1_out = nullptr;        // optional per-column |I_k|
+    int shape[kTunedTiers] = {0, 0, 0};  // candidate per register tier
+    float *tiers_ms = nullptr;    // optional: device time of the tier kernels (planner and global tier excluded)
 };
 
 struct SpaiBuildStats {

@@ -110,6 +110,7 @@ typedef struct {
     int empty_columns;
     long long sum_gathered;  /* entries of A gathered over all columns */
     int static_tier;         /* tier used by balanced_columns, -1 for balanced_nnz */
+    int tile_shape[3];       /* candidate group shape of each register tier (see spai_system_autotune) */
 } spai_build_report;
 
 /* Sparsity pattern of M.  SPAI_PATTERN_A (the default) is pattern(A), as
@@ -135,6 +136,42 @@ int spai_system_m_pattern(const spai_system *sys, int *nnz, int *colptr, int *ro
  * columns of M are zero. */
 int spai_system_build(spai_system *sys, int precision, int schedule, int col_begin, int col_end,
                       spai_build_report *report);
+
+/* ------------------------------------------- tile-shape autotuner ---- */
+/* The three register tiers are size buckets of the columns' least squares
+ * (largest T_k = sum_{j in J_k} |A(:,j)| of 32, 64, 128 and largest |J_k| of
+ * 8, 8, 16).  Each has candidate group shapes: lanes per column, gathered
+ * entries per lane, largest |J_k| and tile rows per lane (|I_k| above
+ * lanes x rows hands the column to the next bucket).  Candidate 0 is the
+ * default.  The shape orders the group sums differently, so M moves in its
+ * last bits between shapes.  No counterpart in the reference (whose kernel
+ * shapes are fixed, src/spai_single_dcu.cpp:1245-1252). */
+#define SPAI_TUNE_TIERS 3
+#define SPAI_TUNE_MAX_CANDIDATES 8
+typedef struct {
+    int lanes;
+    int entries_per_lane;
+    int max_cols;
+    int rows_per_lane;
+} spai_tile_shape;
+/* Number of candidates of register tier `tier` (0..2); 0 for other tiers. */
+int spai_tile_shape_count(int tier);
+int spai_tile_shape_get(int tier, int candidate, spai_tile_shape *out);
+typedef struct {
+    int shape[SPAI_TUNE_TIERS];      /* chosen candidate per tier */
+    int columns[SPAI_TUNE_TIERS];    /* columns solved in each tier under the default shapes */
+    float ms[SPAI_TUNE_TIERS][SPAI_TUNE_MAX_CANDIDATES]; /* tier kernels' device ms, best of reps; -1: not timed */
+    float tune_ms;                   /* host wall time of the whole search */
+} spai_tuning;
+/* Times every candidate of each populated tier on this system's own
+ * columns (a coordinate search from the smallest tier; best of `reps`
+ * builds after one untimed one) and keeps the fastest for the system's later
+ * builds.  M is left built with the chosen shapes.  `out` may be NULL. */
+int spai_system_autotune(spai_system *sys, int precision, int schedule, int reps, spai_tuning *out);
+/* Pins the shapes (one candidate index per tier) for the next builds, e.g.
+ * a table an earlier autotune returned, so that builds are reproducible. */
+int spai_system_set_tile_shapes(spai_system *sys, const int *shape);
+int spai_system_get_tile_shapes(const spai_system *sys, int *shape);
 
 /* M's values in the CSC storage order of its pattern (A's, by default),
  * widened to double. */

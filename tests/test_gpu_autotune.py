@@ -1,0 +1,138 @@
+"""Tile-shape candidates of the SPAI register tiers and the autotuner.
+
+Every candidate of every size bucket is pinned against the oracle on
+columns that reach it: M within the usual bars (FP64 1e-10, mixed 4 * 2^-23,
+relative per column), |I_k| bit-exact, and the columns counted in the tier
+that solved them, which follows from the oracle's (T_k, |J_k|, |I_k|) and
+the candidate's rows per group (a column whose |I_k| exceeds them is solved
+by the next bucket).  The autotuner must return candidates that exist, time
+every candidate of a populated bucket, leave M built with its choice (bit
+for bit what a pinned build with that choice gives) and keep the choice for
+later builds.
+"""
+import numpy as np
+import pytest
+
+import dcu_amd as D
+from oracle import binding as O
+from test_gpu_spai import column_rel_err, scattered_columns
+
+pytestmark = pytest.mark.gpu
+
+BUCKET_T = (32, 64, 128)
+BUCKET_NJ = (8, 8, 16)
+TOL = {D.PRECISION_FP64: 1e-10, D.PRECISION_MIXED: 4 * 2.0 ** -23}
+
+
+def expected_tiers(n1, n2, t, shape):
+    """Columns solved per tier under `shape` (balanced_nnz): the first bucket
+    that holds (T_k, |J_k|), then on to the next while |I_k| exceeds the
+    candidate's rows; 3 the LDS tier, 4 the global tier."""
+    rows = [D.tile_shape(b, shape[b])["lanes"] * D.tile_shape(b, shape[b])["rows_per_lane"] for b in range(3)]
+    out = np.zeros(5, dtype=np.int64)
+    for a, b_, c in zip(n1, n2, t):
+        if b_ == 0:
+            continue
+        tier = next((b for b in range(3) if b_ <= BUCKET_NJ[b] and c <= BUCKET_T[b]), None)
+        if tier is None:
+            need = c * b_ + b_ * b_ + b_
+            tier = 3 if (b_ <= 64 and c <= 1024 and need <= 7168) else 4
+        while tier < 2 and a > rows[tier]:
+            tier += 1
+        out[tier] += 1
+    return out.tolist()
+
+
+def shapes_to_try():
+    out = []
+    for b in range(3):
+        for c in range(D.tile_shape_count(b)):
+            s = [0, 0, 0]
+            s[b] = c
+            if s not in out:
+                out.append(s)
+    return out
+
+
+@pytest.fixture(scope="module")
+def power_law():
+    A = D.Problem.power_law(60000, 7).system(0)
+    m_ref = O.spai(A.n, A.colptr, A.rowidx, A.val, 0)
+    sets = O.column_sets(A.n, A.colptr, A.rowidx)
+    return A, m_ref, sets
+
+
+@pytest.mark.parametrize("precision", [D.PRECISION_FP64, D.PRECISION_MIXED])
+def test_every_shape_matches_oracle(ctx, power_law, precision):
+    A, m_ref, (n1_ref, n2_ref, t_ref, _) = power_law
+    S = D.System(ctx, A)
+    for shape in shapes_to_try():
+        S.set_tile_shapes(shape)
+        rep = S.build(precision, D.SCHEDULE_BALANCED_NNZ)
+        err = column_rel_err(A, S.m_values(), m_ref)
+        exp = expected_tiers(n1_ref, n2_ref, t_ref, shape)
+        print(f"shape {shape} prec {precision}: tiers {rep['tier_columns']} (expected {exp}) "
+              f"build {rep['ms']:.3f} ms err {err:.3e}", flush=True)
+        assert rep["tile_shape"] == shape
+        assert err <= TOL[precision], (shape, err)
+        assert np.array_equal(S.row_set_sizes(), n1_ref)
+        assert rep["tier_columns"] == exp
+    S.close()
+
+
+@pytest.mark.parametrize("schedule", [D.SCHEDULE_BALANCED_COLUMNS, D.SCHEDULE_BALANCED_NNZ])
+@pytest.mark.parametrize("c0", [0, 1, 2, 3])
+def test_overflow_follows_the_rows(ctx, schedule, c0):
+    # T_k <= 32 everywhere: the smallest bucket; 16-row candidates hand the
+    # columns with |I_k| > 16 to the next bucket, 32-row ones keep them all
+    A = scattered_columns()
+    n1_ref, n2_ref, t_ref, _ = O.column_sets(A.n, A.colptr, A.rowidx)
+    m_ref = O.spai(A.n, A.colptr, A.rowidx, A.val, 0)
+    S = D.System(ctx, A)
+    S.set_tile_shapes([c0, 0, 0])
+    rep = S.build(D.PRECISION_FP64, schedule)
+    rows = D.tile_shape(0, c0)["lanes"] * D.tile_shape(0, c0)["rows_per_lane"]
+    err = column_rel_err(A, S.m_values(), m_ref)
+    print(f"c0 {c0} rows {rows} sched {schedule}: tiers {rep['tier_columns']} err {err:.3e}", flush=True)
+    assert err <= 1e-10
+    assert rep["tier_columns"][1] == int((n1_ref > rows).sum())
+    assert rep["tier_columns"][0] == int((n1_ref <= rows).sum())
+    S.close()
+
+
+def test_autotune_batched(ctx):
+    P = D.Problem.batched(8, 3)
+    mats = P.systems()
+    S = D.System(ctx, mats)
+    tune = S.autotune(D.PRECISION_MIXED, D.SCHEDULE_BALANCED_NNZ, reps=2)
+    print(f"autotune: {tune}", flush=True)
+    for b in range(3):
+        assert 0 <= tune["shape"][b] < D.tile_shape_count(b)
+        if tune["columns"][b] > 0:
+            assert all(v is not None and v > 0 for v in tune["ms"][b])
+            assert tune["ms"][b][tune["shape"][b]] == min(tune["ms"][b])
+        else:
+            assert all(v is None for v in tune["ms"][b])
+            assert tune["shape"][b] == 0
+    assert tune["columns"][0] > 0 and tune["columns"][1] > 0  # 2D 5-point and 3D 7-point columns
+    assert S.tile_shapes() == tune["shape"]
+    m_tuned = S.m_values()
+    rep = S.build(D.PRECISION_MIXED, D.SCHEDULE_BALANCED_NNZ)
+    assert rep["tile_shape"] == tune["shape"]
+    assert np.array_equal(S.m_values(), m_tuned)
+    b = np.concatenate([P.rhs(s) for s in range(len(mats))])
+    x, rep, oc = S.solve(b, rel_tol=1e-8, use_graph=True)
+    assert all(o["status"] == D.SOLVE_CONVERGED for o in oc)
+    S.close()
+
+
+def test_shape_arguments_checked(ctx):
+    S = D.System(ctx, D.Problem.generate(1, 1).system(0))
+    before = S.tile_shapes()
+    for bad in ([D.tile_shape_count(0), 0, 0], [0, -1, 0], [0, 0, D.tile_shape_count(2)]):
+        with pytest.raises(D.SpaiError):
+            S.set_tile_shapes(bad)
+    assert S.tile_shapes() == before
+    with pytest.raises(D.SpaiError):
+        S.autotune(D.PRECISION_FP64, D.SCHEDULE_BALANCED_NNZ, reps=0)
+    S.close()

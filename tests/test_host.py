@@ -39,6 +39,37 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, mangled), mangled
 
 
+def test_tile_shape_candidates():
+    # each size bucket's candidates gather exactly its T_k, hold its |J_k|
+    # with a lane per x entry, and a column handed on by a bucket (|I_k| <=
+    # T_k of that bucket) fits every candidate of the next; the last bucket
+    # never hands on; candidate 0 is the default
+    bucket_t, bucket_nj = (32, 64, 128), (8, 8, 16)
+    assert [D.tile_shape_count(t) for t in range(4)][3] == 0
+    for t in range(3):
+        n = D.tile_shape_count(t)
+        assert 1 <= n <= D.TUNE_MAX_CANDIDATES
+        shapes = [D.tile_shape(t, c) for c in range(n)]
+        assert len({tuple(s.values()) for s in shapes}) == n
+        for s in shapes:
+            lanes, epl = s["lanes"], s["entries_per_lane"]
+            assert lanes * epl == bucket_t[t] and s["max_cols"] == bucket_nj[t]
+            assert lanes >= s["max_cols"] and 64 % lanes == 0 and epl & (epl - 1) == 0
+            rows = lanes * s["rows_per_lane"]
+            if t > 0:
+                assert rows >= bucket_t[t - 1]
+            if t == 2:
+                assert rows >= bucket_t[t]
+    assert [D.tile_shape(t, 0) for t in range(3)] == [
+        {"lanes": 8, "entries_per_lane": 4, "max_cols": 8, "rows_per_lane": 2},
+        {"lanes": 16, "entries_per_lane": 4, "max_cols": 8, "rows_per_lane": 2},
+        {"lanes": 64, "entries_per_lane": 2, "max_cols": 16, "rows_per_lane": 2}]
+    with pytest.raises(D.SpaiError):
+        D.tile_shape(0, D.tile_shape_count(0))
+    with pytest.raises(D.SpaiError):
+        D.tile_shape(3, 0)
+
+
 def test_struct_layout_matches_reference():
     # dataType.h:24-42: int*, double*, int*, 4 ints
     assert D.CSC_Matrix.mIndex.offset == 0
