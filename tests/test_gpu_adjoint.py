@@ -5,8 +5,9 @@ with BiCGSTAB preconditioned by M^T and returns dL/db = lambda,
 dL/dA_ij = -lambda_i x_j on A's pattern.  Checked against the FP64 CPU
 gradient of the oracle (<= 1e-8 relative), against central finite
 differences with h = 1e-6 on random entries (solves at 1e-13 so that the
-differencing noise tol/h stays near 1e-7), and through the
-torch.autograd.Function wrapper.
+differencing noise tol/h stays near 1e-7), on small grids and on the full
+64^3 workload (100 entries), and through the torch.autograd.Function
+wrapper.
 """
 import numpy as np
 import pytest
@@ -81,6 +82,63 @@ def test_adjoint_central_differences(ctx):
         worst = max(worst, err)
     print(f"worst relative FD mismatch {worst:.2e}", flush=True)
     assert worst <= 1e-5
+
+
+def test_adjoint_full_size_grf64(ctx):
+    # the differentiation workload at full size: 3D 7-point diffusion on a
+    # 64^3 grid (262,144 unknowns), kappa from a Gaussian random field, b and
+    # x_t N(0,1); central differences (h = 1e-6) on 100 random entries of A
+    # and the FP64 CPU gradient of the oracle.  M stays the one built from
+    # the unperturbed A (it only preconditions).
+    P = D.Problem.generate(5, 5)
+    A = P.system(0)
+    b, xt = P.rhs(0), P.target(0)
+    assert A.n == 64 ** 3
+    S = D.System(ctx, A)
+    S.build(D.PRECISION_FP64, D.SCHEDULE_BALANCED_NNZ)
+    L0, x = loss_of(S, b, xt)
+    dA, db, rep = S.adjoint(x, x - xt, rel_tol=TOL, max_iter=5000)
+    assert rep["all_converged"]
+    m_ref = O.spai(A.n, A.colptr, A.rowidx, A.val, 0)
+    dA_ref, db_ref, it, info = O.adjoint(A.n, A.colptr, A.rowidx, A.val, m_ref, x, xt,
+                                         TOL * np.linalg.norm(x - xt), 5000)
+    e_db = np.linalg.norm(db - db_ref) / np.linalg.norm(db_ref)
+    e_dA = np.linalg.norm(dA - dA_ref) / np.linalg.norm(dA_ref)
+    rng = np.random.default_rng(64)
+    h = 1e-6
+    vals = A.val.copy()
+    scale = np.max(np.abs(dA))
+    errs = []
+
+    def perturbed(v):
+        # warm start from x: the solve only forms the O(h) correction, to an
+        # absolute residual far below h (at 1e-13 of ||b|| from x0 = 0 the
+        # two solves' errors would differ by about 1e-3 of the derivative)
+        S.set_values(v)
+        xs, _, oc = S.solve(b, x0=x, abs_tol=1e-12, max_iter=5000)
+        assert oc[0]["status"] in (D.SOLVE_CONVERGED, D.SOLVE_CONVERGED_AT_START)
+        return xs
+
+    for p in rng.choice(A.nnz, 100, replace=False):
+        vp = vals.copy()
+        vp[p] += h
+        xp = perturbed(vp)
+        vp[p] = vals[p] - h
+        xm = perturbed(vp)
+        # L(+h) - L(-h) = 0.5 sum (x+ - x-)(x+ + x- - 2 x_t), summed without
+        # forming the two losses (about 1e5 each, 1e-8 apart)
+        fd = 0.5 * float(np.sum((xp - xm) * (xp + xm - 2.0 * xt))) / (2 * h)
+        errs.append(abs(fd - dA[p]) / max(abs(dA[p]), 1e-3 * scale))
+    S.set_values(vals)
+    errs = np.array(errs)
+    print(f"grf64: adjoint {rep['iterations_max']} its; vs oracle gradient: dA {e_dA:.2e} db {e_db:.2e}; "
+          f"central differences on 100 entries: median {np.median(errs):.2e} worst {errs.max():.2e}", flush=True)
+    assert e_db <= 1e-8 and e_dA <= 1e-8
+    # the differences' own floor: x+ - x- is of order h, and the perturbed
+    # solves reach an absolute residual of about 1e-12 (the FP64 floor of
+    # b - A x at this size), so the difference quotient carries errors of
+    # 1e-5 to 1e-4 of the derivative (the small grids above reach 1e-6)
+    assert np.median(errs) <= 1e-4 and errs.max() <= 1e-3
 
 
 def test_autograd_wrapper(ctx):
