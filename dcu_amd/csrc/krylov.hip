@@ -330,6 +330,7 @@ struct UnitView {
     const int *list;          // active units in XCD-blocked order, nullptr: every unit (grid = nunits)
     const int *count;         // number of entries of the list
     const int *sys_row;       // nsys + 1 row boundaries of the systems
+    int reverse;              // each XCD walks its range of units from the end
 };
 
 // The workgroup's unit, or -1 past the end of the active list.
@@ -360,10 +361,21 @@ __device__ __forceinline__ int xcd_workgroup(int L, int G) {
 // order).  The compaction stores the active list in that order; without a
 // list the workgroup index is mapped.
 __device__ __forceinline__ int unit_of(const UnitView &uv) {
-    if (!uv.list) return xcd_position((int)blockIdx.x, (int)gridDim.x);
+    const int b = (int)blockIdx.x;
+    if (!uv.list) {
+        if (!uv.reverse) return xcd_position(b, (int)gridDim.x);
+        const int G = (int)gridDim.x, x = b & 7, k = b >> 3, base = G >> 3, rem = G & 7;
+        const int nx = base + (x < rem ? 1 : 0);
+        return x * base + min(x, rem) + (nx - 1 - k);
+    }
     const int cnt = *uv.count;
-    const int u = uv.list[blockIdx.x];  // the grid never exceeds the list's capacity
-    return (int)blockIdx.x < cnt ? u : -1;
+    if (b >= cnt) return -1;
+    int bb = b;
+    if (uv.reverse) {
+        const int x = b & 7, nx = (cnt - x + 7) >> 3;
+        bb = 8 * (nx - 1 - (b >> 3)) + x;
+    }
+    return uv.list[bb];  // the grid never exceeds the list's capacity
 }
 
 template <class T>
@@ -1143,15 +1155,19 @@ UnitView unit_view(const TileMap &tm, bool use_list, const int *count) {
                     tm.slice_blk.get(),
                     use_list ? tm.active.get() : nullptr,
                     count,
-                    tm.sys_row.get()};
+                    tm.sys_row.get(),
+                    0};
 }
 
+
 // One launch of op over the layout: every unit (grid = nunits, list unused)
-// or the first `grid` entries of the active list.
+// or the first `grid` entries of the active list; `reverse` walks each XCD's
+// range of units from its end.
 template <bool HM, class MT, class Op>
 void launch_units(const OperatorView &O, const TileMap &tm, const Ctx &c, const Op &op, const int *list, int grid,
-                  hipStream_t st) {
+                  hipStream_t st, bool reverse) {
     UnitView uv = unit_view(tm, list != nullptr, c.counter + c.nsys + 1);
+    uv.reverse = reverse ? 1 : 0;
     if (O.blk) uv.slice_blk = O.blk;  // an operator with its own pattern
     const int g = list ? grid : tm.nunits;
     if (g <= 0) return;
@@ -1577,19 +1593,19 @@ void BiCGStab::enqueue_k1(const OperatorView &A, const OperatorView &M, const Ti
     if (!M.val) {
         // no preconditioner: y = p' through the streaming path
         OpMapplyP<double, kYCopy> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
-        launch_units<false, double>(A, tm, c, op, list, grid, st);
+        launch_units<false, double>(A, tm, c, op, list, grid, st, flip());
     } else if (split) {
         OpMapplyP<double, kYNone> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
-        launch_units<false, double>(A, tm, c, op, list, grid, st);
+        launch_units<false, double>(A, tm, c, op, list, grid, st, flip());
         OpApplyM om{pout, y_};
-        if (M.fp32) launch_units<true, float>(M, tm, c, om, list, grid, st);
-        else launch_units<true, double>(M, tm, c, om, list, grid, st);
+        if (M.fp32) launch_units<true, float>(M, tm, c, om, list, grid, st, flip());
+        else launch_units<true, double>(M, tm, c, om, list, grid, st, flip());
     } else if (M.fp32) {
         OpMapplyP<float, kYProduct> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
-        launch_units<true, float>(M, tm, c, op, list, grid, st);
+        launch_units<true, float>(M, tm, c, op, list, grid, st, flip());
     } else {
         OpMapplyP<double, kYProduct> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
-        launch_units<true, double>(M, tm, c, op, list, grid, st);
+        launch_units<true, double>(M, tm, c, op, list, grid, st, flip());
     }
 }
 
@@ -1602,23 +1618,23 @@ void BiCGStab::enqueue_iteration(const OperatorView &A, const OperatorView &M, c
     enqueue_k1(A, M, tm, parity, list, grid, split, st);
     {
         OpSpmvAlpha op{y_, rs_, a_};
-        launch_units<true, double>(A, tm, c, op, list, grid, st);
+        launch_units<true, double>(A, tm, c, op, list, grid, st, flip());
     }
     if (M.val) {
         if (M.fp32) {
             OpMapplyS<float> op{r_, a_, s_, z_, 0.0};
-            launch_units<true, float>(M, tm, c, op, list, grid, st);
+            launch_units<true, float>(M, tm, c, op, list, grid, st, flip());
         } else {
             OpMapplyS<double> op{r_, a_, s_, z_, 0.0};
-            launch_units<true, double>(M, tm, c, op, list, grid, st);
+            launch_units<true, double>(M, tm, c, op, list, grid, st, flip());
         }
     } else {
         OpCopyS op{r_, a_, s_, z_, 0.0};
-        launch_units<false, double>(A, tm, c, op, list, grid, st);
+        launch_units<false, double>(A, tm, c, op, list, grid, st, flip());
     }
     {
         OpSpmvOmega op{z_, s_, rs_, t_};
-        launch_units<true, double>(A, tm, c, op, list, grid, st);
+        launch_units<true, double>(A, tm, c, op, list, grid, st, flip());
     }
     SPAI_CHECK_LAUNCH();
 }
@@ -1649,7 +1665,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                        counter_.get(), tm.nnodes);
     {
         OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, s_, t_, y_, z_, opt.rel_tol, opt.abs_tol, 0};
-        launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
+        launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
     }
     SPAI_CHECK_LAUNCH();
     // A batch walks the active-unit list, rebuilt after every host check; a
@@ -1786,7 +1802,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
         bool true_current = false;  // ||b - A x|| is in the states for the final x
         for (int round = 0; opt.true_residual_restart && round < 4; ++round) {
             OpTrueResidual tr{x, b};
-            launch_units<true, double>(A, tm, c, tr, nullptr, 0, st);
+            launch_units<true, double>(A, tm, c, tr, nullptr, 0, st, flip());
             hipLaunchKernelGGL(k_check_true, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys,
                                counter_.get());
             SPAI_CHECK_LAUNCH();
@@ -1796,14 +1812,14 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                 break;
             }
             OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, s_, t_, y_, z_, opt.rel_tol, opt.abs_tol, 1};
-            launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
+            launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
             SPAI_CHECK_LAUNCH();
             done_iters = 0;
             iterate();
         }
         if (!true_current) {
             OpTrueResidual op{x, b};
-            launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
+            launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
             SPAI_CHECK_LAUNCH();
         }
         SPAI_HIP(hipEventRecord(e1, st));
@@ -1862,38 +1878,38 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
         switch (which) {
         case 0: {
             OpSpmvPlain op{p_[0], y_};
-            launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
+            launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
         } break;
         case 5: {
             OpSpmvPlain op{p_[0], y_};
-            if (M.fp32) launch_units<true, float>(M, tm, c, op, nullptr, 0, st);
-            else launch_units<true, double>(M, tm, c, op, nullptr, 0, st);
+            if (M.fp32) launch_units<true, float>(M, tm, c, op, nullptr, 0, st, flip());
+            else launch_units<true, double>(M, tm, c, op, nullptr, 0, st, flip());
         } break;
         case 1: {
             if (M.fp32) {
                 OpMapplyP<float, kYProduct> op{x_, r_, p_[1], y_, s_, t_, p_[0], a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
-                launch_units<true, float>(M, tm, c, op, nullptr, 0, st);
+                launch_units<true, float>(M, tm, c, op, nullptr, 0, st, flip());
             } else {
                 OpMapplyP<double, kYProduct> op{x_, r_, p_[1], y_, s_, t_, p_[0], a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
-                launch_units<true, double>(M, tm, c, op, nullptr, 0, st);
+                launch_units<true, double>(M, tm, c, op, nullptr, 0, st, flip());
             }
         } break;
         case 2: {
             OpSpmvAlpha op{y_, rs_, a_};
-            launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
+            launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
         } break;
         case 3: {
             if (M.fp32) {
                 OpMapplyS<float> op{r_, a_, s_, z_, 0.0};
-                launch_units<true, float>(M, tm, c, op, nullptr, 0, st);
+                launch_units<true, float>(M, tm, c, op, nullptr, 0, st, flip());
             } else {
                 OpMapplyS<double> op{r_, a_, s_, z_, 0.0};
-                launch_units<true, double>(M, tm, c, op, nullptr, 0, st);
+                launch_units<true, double>(M, tm, c, op, nullptr, 0, st, flip());
             }
         } break;
         case 4: {
             OpSpmvOmega op{z_, s_, rs_, t_};
-            launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
+            launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
         } break;
         }
     };
@@ -1929,7 +1945,7 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
     Ctx c = make_ctx(tm, nsys);
     {
         OpInit op{tx, tb, r_, rs_, p_[0], p_[1], a_, s_, t_, y_, z_, 0.0, 1e-300, 0};
-        launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
+        launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
     }
     SPAI_CHECK_LAUNCH();
     const double *b_keep = b_;
@@ -1946,38 +1962,38 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
             SPAI_HIP(hipEventRecord(e[0], st));
             if (split) {
                 OpMapplyP<double, kYNone> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
-                launch_units<false, double>(A, tm, c, op, nullptr, 0, st);
+                launch_units<false, double>(A, tm, c, op, nullptr, 0, st, flip());
                 SPAI_HIP(hipEventRecord(e[1], st));
                 OpApplyM om{pout, y_};
-                if (M.fp32) launch_units<true, float>(M, tm, c, om, nullptr, 0, st);
-                else launch_units<true, double>(M, tm, c, om, nullptr, 0, st);
+                if (M.fp32) launch_units<true, float>(M, tm, c, om, nullptr, 0, st, flip());
+                else launch_units<true, double>(M, tm, c, om, nullptr, 0, st, flip());
             } else {
                 if (M.fp32) {
                     OpMapplyP<float, kYProduct> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
-                    launch_units<true, float>(M, tm, c, op, nullptr, 0, st);
+                    launch_units<true, float>(M, tm, c, op, nullptr, 0, st, flip());
                 } else {
                     OpMapplyP<double, kYProduct> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
-                    launch_units<true, double>(M, tm, c, op, nullptr, 0, st);
+                    launch_units<true, double>(M, tm, c, op, nullptr, 0, st, flip());
                 }
                 SPAI_HIP(hipEventRecord(e[1], st));
             }
             SPAI_HIP(hipEventRecord(e[2], st));
             {
                 OpSpmvAlpha op{y_, rs_, a_};
-                launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
+                launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
             }
             SPAI_HIP(hipEventRecord(e[3], st));
             if (M.fp32) {
                 OpMapplyS<float> op{r_, a_, s_, z_, 0.0};
-                launch_units<true, float>(M, tm, c, op, nullptr, 0, st);
+                launch_units<true, float>(M, tm, c, op, nullptr, 0, st, flip());
             } else {
                 OpMapplyS<double> op{r_, a_, s_, z_, 0.0};
-                launch_units<true, double>(M, tm, c, op, nullptr, 0, st);
+                launch_units<true, double>(M, tm, c, op, nullptr, 0, st, flip());
             }
             SPAI_HIP(hipEventRecord(e[4], st));
             {
                 OpSpmvOmega op{z_, s_, rs_, t_};
-                launch_units<true, double>(A, tm, c, op, nullptr, 0, st);
+                launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
             }
         }
         SPAI_HIP(hipEventRecord(ev.back(), st));

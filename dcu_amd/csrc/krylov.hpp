@@ -231,6 +231,12 @@ private:
     hipEvent_t check_ev_[2] = {nullptr, nullptr};  // host checks of the graph loop
     int *h_pub_ = nullptr, *d_pub_ = nullptr;      // counts published by the graphs (host-coherent)
     void drop_graph();
+    // Consecutive unit launches walk the units in alternating directions, so
+    // that a kernel starts on the rows the one before it touched last (still
+    // in the memory-side cache).  Captured graphs keep the directions they
+    // were captured with.
+    int sweep_ = 0;
+    bool flip() { return (sweep_++ & 1) != 0; }
     DeviceArray<double> hist_;
     double *hist_ptr_ = nullptr;
     int hist_stride_ = 0;
