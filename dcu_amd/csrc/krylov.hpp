@@ -49,13 +49,16 @@ enum SolveStatus { kRunning = 0, kConverged = 1, kMaxIter = 2, kBreakdown = 3, k
 // `active` lists the units of systems still iterating (rebuilt on the device
 // from the system states); launches that use it exit at once past its count.
 enum LayoutKind { kLayoutTiles = 0, kLayoutSell = 1 };
+// A unit is 8 slices (512 rows), one per wavefront of its workgroup: with
+// few units active (the end of a batched solve) eight wavefronts per unit hide
+// more latency than four wavefronts of two slices each.
 #ifndef SPAI_SELL_SLICES_PER_WAVE
-#define SPAI_SELL_SLICES_PER_WAVE 2
+#define SPAI_SELL_SLICES_PER_WAVE 1
 #endif
 constexpr int kSliceRows = 64;
 constexpr int kSellSlicesPerWave = SPAI_SELL_SLICES_PER_WAVE;
 #ifndef SPAI_SELL_WAVES
-#define SPAI_SELL_WAVES 4
+#define SPAI_SELL_WAVES 8
 #endif
 constexpr int kSellWaves = SPAI_SELL_WAVES;  // wavefronts per SELL workgroup
 constexpr int kSlicesPerUnit = kSellWaves * kSellSlicesPerWave;
