@@ -71,7 +71,7 @@ constexpr int kFanIn = SPAI_REDUCE_FANIN;  // children per reduction tree node
 // this many units are active (bandwidth-bound); below it the fused K1 saves a
 // launch (latency-bound).
 #ifndef SPAI_SPLIT_UNITS
-#define SPAI_SPLIT_UNITS 1024
+#define SPAI_SPLIT_UNITS 4096
 #endif
 constexpr int kSplitUnits = SPAI_SPLIT_UNITS;
 constexpr int kCompactEvery = 8;  // iterations between active-list rebuilds inside a graph
