@@ -74,7 +74,10 @@ constexpr int kFanIn = SPAI_REDUCE_FANIN;  // children per reduction tree node
 #define SPAI_SPLIT_UNITS 4096
 #endif
 constexpr int kSplitUnits = SPAI_SPLIT_UNITS;
-constexpr int kCompactEvery = 8;  // iterations between active-list rebuilds inside a graph
+#ifndef SPAI_COMPACT_EVERY
+#define SPAI_COMPACT_EVERY 8
+#endif
+constexpr int kCompactEvery = SPAI_COMPACT_EVERY;  // iterations between active-list rebuilds inside a graph
 constexpr int kMaxRed = 4;          // most values any kernel reduces (partials per tree node)
 
 // ------------------------------------------------------------ transpose ----
