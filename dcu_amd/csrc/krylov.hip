@@ -69,11 +69,16 @@ constexpr int kLdsProducts = 2048;  // products staged per tile (16 KB)
 constexpr int kFanIn = SPAI_REDUCE_FANIN;  // children per reduction tree node
 // K1 splits into a streaming update and a one-vector M-apply while at least
 // this many units are active (bandwidth-bound); below it the fused K1 saves a
-// launch (latency-bound).
+// launch (latency-bound).  SELL-64 units (8 wavefronts of one slice) keep the
+// fused K1 fed up to a larger count than row tiles.
 #ifndef SPAI_SPLIT_UNITS
 #define SPAI_SPLIT_UNITS 4096
 #endif
+#ifndef SPAI_SPLIT_UNITS_TILES
+#define SPAI_SPLIT_UNITS_TILES 1024
+#endif
 constexpr int kSplitUnits = SPAI_SPLIT_UNITS;
+constexpr int kSplitUnitsTiles = SPAI_SPLIT_UNITS_TILES;
 #ifndef SPAI_COMPACT_EVERY
 #define SPAI_COMPACT_EVERY 8
 #endif
@@ -1150,6 +1155,8 @@ __global__ void k_f32_to_f64(const float *in, double *out, long long n) {
     if (i < n) out[i] = (double)in[i];
 }
 
+int split_units(const TileMap &tm) { return tm.kind == kLayoutSell ? kSplitUnits : kSplitUnitsTiles; }
+
 UnitView unit_view(const TileMap &tm, bool use_list, const int *count) {
     return UnitView{tm.unit_row.get(),
                     tm.unit_sys.get(),
@@ -1765,7 +1772,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                 try {
                     while (active > 0 && done_iters < max_launch_iters) {
                         const int grid = use_list ? rung(units) : 0;
-                        const bool split = M.val && units >= kSplitUnits;
+                        const bool split = M.val && units >= split_units(tm);
                         SPAI_HIP(hipGraphLaunch(graph_for(grid, split), st));
                         done_iters += gi;
                         SPAI_HIP(hipEventRecord(check_ev_[slot], st));
@@ -1790,7 +1797,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                 int parity = 0;
                 while (active > 0 && done_iters < max_launch_iters) {
                     const int grid = use_list ? units : 0;
-                    const bool split = M.val && units >= kSplitUnits;
+                    const bool split = M.val && units >= split_units(tm);
                     for (int k = 0; k < every; ++k) {
                         enqueue_iteration(A, M, tm, parity, grid, split, st);
                         parity ^= 1;
@@ -1934,7 +1941,7 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
     require(A.val && M.val, "time_iteration: needs A and M");
     require(reps >= 1, "time_iteration: reps must be >= 1");
     require(variant >= -1 && variant <= 1, "time_iteration: variant must be -1, 0 or 1");
-    const bool split = variant < 0 ? tm.nunits >= kSplitUnits : variant == 1;
+    const bool split = variant < 0 ? tm.nunits >= split_units(tm) : variant == 1;
     ensure(n, nsys, tm.nunits, tm.nnodes);
     hist_ptr_ = nullptr;
     hist_stride_ = 0;
