@@ -70,6 +70,19 @@ def test_tile_shape_candidates():
         D.tile_shape(3, 0)
 
 
+def test_tuner_entry_points_reject_bad_arguments():
+    # argument checks that run before any device work
+    lib = D.load_library()
+    shape = (ctypes.c_int * 3)(0, 0, 0)
+    t = D.Tuning()
+    assert lib.spai_system_autotune(None, D.PRECISION_MIXED, D.SCHEDULE_BALANCED_NNZ, 3, ctypes.byref(t)) != 0
+    assert lib.spai_system_set_tile_shapes(None, shape) != 0
+    assert lib.spai_system_get_tile_shapes(None, shape) != 0
+    assert lib.spai_tile_shape_get(0, 0, None) != 0
+    assert lib.spai_tile_shape_get(-1, 0, ctypes.byref(D.TileShape())) != 0
+    assert lib.spai_tile_shape_count(-1) == 0 and lib.spai_tile_shape_count(5) == 0
+
+
 def test_struct_layout_matches_reference():
     # dataType.h:24-42: int*, double*, int*, 4 ints
     assert D.CSC_Matrix.mIndex.offset == 0
