@@ -104,7 +104,7 @@ int spai_system_set_values(spai_system *sys, const double *values, int values_on
 /* ------------------------------------------------------------- build ---- */
 typedef struct {
     float ms;                /* device time: planning + column kernels + M in CSC/CSR */
-    int tier_columns[5];     /* columns per size tier (8/16/64-lane registers, LDS wave, global workgroup) */
+    int tier_columns[5];     /* columns per size tier (three register buckets, LDS wave, global workgroup) */
     int max_gathered;        /* max_k sum_{j in J_k} |A(:,j)| */
     int max_n2;              /* max_k |J_k| (the reference's n2max) */
     int empty_columns;

@@ -63,7 +63,7 @@ def test_random4096_mixed(ctx):
 
 @pytest.mark.parametrize("precision,tol", [(D.PRECISION_FP64, 1e-10), (D.PRECISION_MIXED, 2.0 ** -23 * 4)])
 def test_power_law_all_tiers(ctx, precision, tol):
-    # 60k rows of the power-law generator reach the 8/16/64-lane tiers and
+    # 60k rows of the power-law generator reach the three register tiers and
     # the LDS tier
     P = D.Problem.power_law(60000, 7)
     A = P.system(0)
