@@ -126,6 +126,29 @@ def test_autotune_batched(ctx):
     S.close()
 
 
+def test_autotune_static_schedule_with_overflow(ctx):
+    # balanced_columns: every column in the static tier (bucket 0 here), the
+    # default 16-row shape handing the wider row sets to bucket 1; whatever
+    # the tuner picks, M stays within the FP64 bar and the columns are counted
+    # where the chosen rows put them
+    A = scattered_columns()
+    n1_ref, _, _, _ = O.column_sets(A.n, A.colptr, A.rowidx)
+    m_ref = O.spai(A.n, A.colptr, A.rowidx, A.val, 0)
+    S = D.System(ctx, A)
+    tune = S.autotune(D.PRECISION_FP64, D.SCHEDULE_BALANCED_COLUMNS, reps=2)
+    print(f"static autotune: {tune}", flush=True)
+    assert tune["columns"][0] > 0 and tune["columns"][1] == int((n1_ref > 16).sum())
+    assert all(v is not None for v in tune["ms"][0]) and all(v is not None for v in tune["ms"][1])
+    assert all(v is None for v in tune["ms"][2])
+    err = column_rel_err(A, S.m_values(), m_ref)
+    assert err <= 1e-10
+    rep = S.build(D.PRECISION_FP64, D.SCHEDULE_BALANCED_COLUMNS)
+    rows = D.tile_shape(0, tune["shape"][0])["lanes"] * D.tile_shape(0, tune["shape"][0])["rows_per_lane"]
+    assert rep["static_tier"] == 0 and rep["tile_shape"] == tune["shape"]
+    assert rep["tier_columns"][1] == int((n1_ref > rows).sum())
+    S.close()
+
+
 def test_shape_arguments_checked(ctx):
     S = D.System(ctx, D.Problem.generate(1, 1).system(0))
     before = S.tile_shapes()
