@@ -347,7 +347,7 @@ const char *spai_status_string(int status) {
 
 const char *spai_last_error(void) { return g_last_error.c_str(); }
 
-const char *spai_version(void) { return "spai_mi355x 0.1 (gfx950)"; }
+const char *spai_version(void) { return "spai_mi355x 0.2 (gfx950)"; }
 
 int spai_context_create(int device, spai_context **out) {
     if (!out) return fail(SPAI_ERR_INVALID, "out is NULL");

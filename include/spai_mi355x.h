@@ -59,7 +59,8 @@ extern "C" {
 const char *spai_status_string(int status);
 /* Message of the last failing call on this thread. */
 const char *spai_last_error(void);
-/* Library version string. */
+/* Library version string.  0.2: spai_build_report gained tile_shape[3] (the
+ * struct grew; rebuild callers against this header). */
 const char *spai_version(void);
 
 /* ------------------------------------------------------------ context --- */
