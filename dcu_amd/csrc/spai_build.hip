@@ -36,6 +36,7 @@
 #include "spai_build.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <tuple>
 #include <type_traits>
@@ -191,10 +192,34 @@ __device__ __forceinline__ int locate_j(int e, int off, int n2) {
     return lo;
 }
 
-// Bitonic sort (ascending) of W*EPL 64-bit keys, lane l holding elements
+template <int W>
+__device__ __forceinline__ uint32_t shfl_xor_key(uint32_t v, int m) {
+    return (uint32_t)__shfl_xor((unsigned)v, m, W);
+}
+template <int W>
+__device__ __forceinline__ uint64_t shfl_xor_key(uint64_t v, int m) {
+    return shfl_xor_u64<W>(v, m);
+}
+
+// (row, entry) sort keys of the register tiers.  With 32-bit keys the entry
+// index takes EB = log2(T_max) low bits and the row the rest, which needs
+// n < 2^(32 - EB) - 1 (the all-ones key marks an empty slot); half the compare
+// and exchange work of 64-bit keys, same order.
+template <bool K32, int EB>
+struct SortKey {
+    using T = typename std::conditional<K32, uint32_t, uint64_t>::type;
+    static constexpr T kNone = ~T(0);
+    static constexpr int kShift = K32 ? EB : 32;
+    __device__ static T make(int row, int e) { return ((T)(unsigned)row << kShift) | (T)(unsigned)e; }
+    __device__ static int row(T k) { return (int)(k >> kShift); }
+    __device__ static int ent(T k) { return (int)(k & ((T(1) << kShift) - 1)); }
+};
+constexpr int log2_exact(int v) { return v <= 1 ? 0 : 1 + log2_exact(v / 2); }
+
+// Bitonic sort (ascending) of W*EPL keys, lane l holding elements
 // l*EPL .. l*EPL+EPL-1.
-template <int W, int EPL>
-__device__ __forceinline__ void bitonic_sort(uint64_t (&key)[EPL], int lane) {
+template <int W, int EPL, class KT>
+__device__ __forceinline__ void bitonic_sort(KT (&key)[EPL], int lane) {
     constexpr int N = W * EPL;
 #pragma unroll
     for (int size = 2; size <= N; size <<= 1) {
@@ -204,13 +229,13 @@ __device__ __forceinline__ void bitonic_sort(uint64_t (&key)[EPL], int lane) {
                 const int ls = stride / EPL;
 #pragma unroll
                 for (int i = 0; i < EPL; ++i) {
-                    uint64_t pk = shfl_xor_u64<W>(key[i], ls);
+                    KT pk = shfl_xor_key<W>(key[i], ls);
                     int e = lane * EPL + i;
                     bool up = (e & size) == 0;
                     bool lower = (e & stride) == 0;
                     bool take_min = (lower == up);
-                    uint64_t mn = pk < key[i] ? pk : key[i];
-                    uint64_t mx = pk < key[i] ? key[i] : pk;
+                    KT mn = pk < key[i] ? pk : key[i];
+                    KT mx = pk < key[i] ? key[i] : pk;
                     key[i] = take_min ? mn : mx;
                 }
             } else {
@@ -220,7 +245,7 @@ __device__ __forceinline__ void bitonic_sort(uint64_t (&key)[EPL], int lane) {
                         const int i2 = i | stride;
                         int e = lane * EPL + i;
                         bool up = (e & size) == 0;
-                        uint64_t a = key[i], b = key[i2];
+                        KT a = key[i], b = key[i2];
                         bool sw = up ? (a > b) : (a < b);
                         key[i] = sw ? b : a;
                         key[i2] = sw ? a : b;
@@ -242,9 +267,11 @@ __device__ __forceinline__ void store_m(const ColumnArgs &a, int pos, double x) 
 // ---------------------------------------------------------------------------
 // Register tier: one group of W lanes per column.
 // ---------------------------------------------------------------------------
-template <class TR, class QT, bool REFINE, class MT>
+template <class TR, class QT, bool REFINE, class MT, bool K32>
 __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *lds) {
     constexpr int W = TR::W, EPL = TR::EPL, NJ = TR::NJ, TS = TR::TS;
+    using SK = SortKey<K32, log2_exact(TR::TMAX)>;
+    using KT = typename SK::T;
     constexpr int RPL = TR::RPL;
     using L = TierLds<TR, QT, REFINE>;
     QT *tile = (QT *)lds;                                     // ROWS x TS
@@ -270,14 +297,14 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
     const int off = incl - cl;
 
     // gather (row, entry) keys, blocked layout e = lane*EPL + i
-    uint64_t key[EPL];
+    KT key[EPL];
 #pragma unroll
     for (int i = 0; i < EPL; ++i) {
         int e = lane * EPL + i;
         int j = locate_j<W, NJ>(e, off, n2);
         int src = __shfl(cs, j, W) + (e - __shfl(off, j, W));
-        key[i] = ~0ull;
-        if (e < T) key[i] = ((uint64_t)(unsigned)a.rowidx[src] << 32) | (unsigned)e;
+        key[i] = SK::kNone;
+        if (e < T) key[i] = SK::make(a.rowidx[src], e);
     }
     bitonic_sort<W, EPL>(key, lane);
 
@@ -285,12 +312,12 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
     int rowv[EPL];
     bool first[EPL];
     int cnt = 0;
-    int prev = __shfl_up((int)(key[EPL - 1] >> 32), 1, W);
+    int prev = __shfl_up(SK::row(key[EPL - 1]), 1, W);
     if (lane == 0) prev = -1;
 #pragma unroll
     for (int i = 0; i < EPL; ++i) {
-        bool valid = key[i] != ~0ull;
-        rowv[i] = (int)(key[i] >> 32);
+        bool valid = key[i] != SK::kNone;
+        rowv[i] = SK::row(key[i]);
         first[i] = valid && (rowv[i] != prev);
         prev = rowv[i];
         cnt += first[i] ? 1 : 0;
@@ -304,7 +331,7 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
     for (int i = 0; i < EPL; ++i) {
         run += first[i] ? 1 : 0;
         pos[i] = run - 1;
-        if (key[i] != ~0ull && rowv[i] == k) e_pos = pos[i];
+        if (key[i] != SK::kNone && rowv[i] == k) e_pos = pos[i];
     }
     const int E = group_max<W>(e_pos);
     if (TR::ROWS < TR::TMAX && n1 > TR::ROWS) {  // the next tier takes it
@@ -327,8 +354,8 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
 #pragma unroll
     for (int i = 0; i < EPL; ++i) {
         // shuffles run on every lane of the group; only the store is masked
-        const bool valid = key[i] != ~0ull;
-        const int e = valid ? (int)(unsigned)key[i] : 0;
+        const bool valid = key[i] != SK::kNone;
+        const int e = valid ? SK::ent(key[i]) : 0;
         const int j = locate_j<W, NJ>(e, off, n2);
         const int src = __shfl(cs, j, W) + (e - __shfl(off, j, W));
         vv[i] = valid ? a.val[src] : 0.0;
@@ -351,7 +378,7 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
 #pragma unroll
         for (int i = 0; i < EPL; ++i) {
             const int slot = lane * EPL + i;
-            if (key[i] != ~0ull) {
+            if (key[i] != SK::kNone) {
                 vslot[slot] = vv[i];
                 jslot[slot] = jv[i];
                 if (first[i]) rstart[pos[i]] = slot;
@@ -552,7 +579,7 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
 #else
 #define SPAI_TIER_OCCUPANCY
 #endif
-template <class TR, class QT, bool REFINE, class MT, bool DYNAMIC>
+template <class TR, class QT, bool REFINE, class MT, bool DYNAMIC, bool K32>
 __global__ __launch_bounds__(kBlock) SPAI_TIER_OCCUPANCY void spai_reg_tier_kernel(ColumnArgs a) {
     constexpr int W = TR::W;
     constexpr int G = 64 / W;    // groups per wavefront
@@ -576,7 +603,7 @@ __global__ __launch_bounds__(kBlock) SPAI_TIER_OCCUPANCY void spai_reg_tier_kern
             for (int it = 0; it < CH; ++it) {
                 int idx = base + it * G + wgrp;
                 if (idx < total)
-                    solve_column_reg<TR, QT, REFINE, MT>(a, idx < count ? a.list[idx] : a.ovf_in_list[idx - count],
+                    solve_column_reg<TR, QT, REFINE, MT, K32>(a, idx < count ? a.list[idx] : a.ovf_in_list[idx - count],
                                                          lane, lds);
             }
         }
@@ -589,7 +616,7 @@ __global__ __launch_bounds__(kBlock) SPAI_TIER_OCCUPANCY void spai_reg_tier_kern
         const int stop = min(start + chunk, a.col_end);
         for (int base = start; base < stop; base += G) {
             int k = base + wgrp;
-            if (k < stop) solve_column_reg<TR, QT, REFINE, MT>(a, k, lane, lds);
+            if (k < stop) solve_column_reg<TR, QT, REFINE, MT, K32>(a, k, lane, lds);
         }
     }
 }
@@ -1197,9 +1224,14 @@ constexpr int kOvfHead = kOvfCount + 2;
 constexpr int kCounters = kOvfHead + 2;
 constexpr int kLists = kTiers + 2;
 
+bool wide_keys_forced() {
+    const char *env = std::getenv("SPAI_MI355X_SORT_KEYS");
+    return env && std::atoi(env) == 64;
+}
+
 template <class MT, bool REFINE>
-void launch_tiers(const ColumnArgs &base, bool dynamic, int static_tier, const int *shape, int *counters, int *lists,
-                  int list_stride, int cus, hipStream_t st) {
+void launch_tiers(const ColumnArgs &base, int base_n, bool dynamic, int static_tier, const int *shape, int *counters,
+                  int *lists, int list_stride, int cus, hipStream_t st) {
     using QT = typename std::conditional<REFINE, float, double>::type;
     ColumnArgs a = base;
     // t: tier; ovf: a static run's follow-up over the tier's overflow list
@@ -1227,16 +1259,24 @@ void launch_tiers(const ColumnArgs &base, bool dynamic, int static_tier, const i
     auto run_reg = [&](auto tier_tag, int t, bool ovf = false) {
         using TR = decltype(tier_tag);
         setup(t, ovf);
-        if (dynamic || ovf) {
-            auto k = spai_reg_tier_kernel<TR, QT, REFINE, MT, true>;
-            hipLaunchKernelGGL(k, dim3(persistent_grid(k, kBlock, cus)), dim3(kBlock), 0, st, a);
-        } else {
-            auto k = spai_reg_tier_kernel<TR, QT, REFINE, MT, false>;
-            int ncols = base.col_end - base.col_begin;
-            int waves = ceil_div(ncols, 64 / TR::W * 4);  // ~4 columns per group
-            int grid = std::max(1, std::min(ceil_div(waves, kBlock / 64), persistent_grid(k, kBlock, cus) * 4));
-            hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, st, a);
-        }
+        // 32-bit sort keys when every row index fits beside the entry index
+        // (SPAI_MI355X_SORT_KEYS=64 keeps 64-bit keys; both give the same M)
+        const bool k32 = (long long)base_n + 1 < (1LL << (32 - log2_exact(TR::TMAX))) && !wide_keys_forced();
+        auto launch = [&](auto k32_tag) {
+            constexpr bool K32 = decltype(k32_tag)::value;
+            if (dynamic || ovf) {
+                auto k = spai_reg_tier_kernel<TR, QT, REFINE, MT, true, K32>;
+                hipLaunchKernelGGL(k, dim3(persistent_grid(k, kBlock, cus)), dim3(kBlock), 0, st, a);
+            } else {
+                auto k = spai_reg_tier_kernel<TR, QT, REFINE, MT, false, K32>;
+                int ncols = base.col_end - base.col_begin;
+                int waves = ceil_div(ncols, 64 / TR::W * 4);  // ~4 columns per group
+                int grid = std::max(1, std::min(ceil_div(waves, kBlock / 64), persistent_grid(k, kBlock, cus) * 4));
+                hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, st, a);
+            }
+        };
+        if (k32) launch(std::true_type{});
+        else launch(std::false_type{});
         SPAI_CHECK_LAUNCH();
     };
     auto run_lds = [&](int t) {
@@ -1449,11 +1489,11 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
     }
     if (ce > cb && static_tier < kTiers - 1) {
         if (p.precision == 1)
-            launch_tiers<float, true>(a, dynamic, static_tier, p.shape, counters_.get(), lists_.get(), list_stride,
-                                      cus_, st);
+            launch_tiers<float, true>(a, p.n, dynamic, static_tier, p.shape, counters_.get(), lists_.get(),
+                                      list_stride, cus_, st);
         else
-            launch_tiers<double, false>(a, dynamic, static_tier, p.shape, counters_.get(), lists_.get(), list_stride,
-                                        cus_, st);
+            launch_tiers<double, false>(a, p.n, dynamic, static_tier, p.shape, counters_.get(), lists_.get(),
+                                        list_stride, cus_, st);
     }
     if (p.tiers_ms) SPAI_HIP(hipEventRecord(ev[1], st));
     fetch();

@@ -149,6 +149,34 @@ def test_autotune_static_schedule_with_overflow(ctx):
     S.close()
 
 
+@pytest.mark.parametrize("precision", [D.PRECISION_FP64, D.PRECISION_MIXED])
+def test_sort_key_widths_agree(ctx, power_law, precision):
+    # the register tiers sort (row, entry) keys in 32 bits when the rows fit
+    # (every system here), in 64 bits otherwise; the order is the same, so M
+    # must be bit-identical, for every tile shape
+    import os
+    A, m_ref, (n1_ref, _, _, _) = power_law
+    for shape in shapes_to_try():
+        out = {}
+        for width in ("32", "64"):
+            old = os.environ.get("SPAI_MI355X_SORT_KEYS")
+            os.environ["SPAI_MI355X_SORT_KEYS"] = width
+            try:
+                S = D.System(ctx, A)
+                S.set_tile_shapes(shape)
+                rep = S.build(precision, D.SCHEDULE_BALANCED_NNZ)
+                out[width] = (S.m_values(), S.row_set_sizes(), rep["tier_columns"])
+                S.close()
+            finally:
+                if old is None:
+                    os.environ.pop("SPAI_MI355X_SORT_KEYS", None)
+                else:
+                    os.environ["SPAI_MI355X_SORT_KEYS"] = old
+        assert np.array_equal(out["32"][0], out["64"][0]), shape
+        assert np.array_equal(out["32"][1], n1_ref) and np.array_equal(out["64"][1], n1_ref)
+        assert out["32"][2] == out["64"][2]
+
+
 def test_shape_arguments_checked(ctx):
     S = D.System(ctx, D.Problem.generate(1, 1).system(0))
     before = S.tile_shapes()
