@@ -1260,8 +1260,12 @@ void launch_tiers(const ColumnArgs &base, int base_n, bool dynamic, int static_t
         using TR = decltype(tier_tag);
         setup(t, ovf);
         // 32-bit sort keys when every row index fits beside the entry index
-        // (SPAI_MI355X_SORT_KEYS=64 keeps 64-bit keys; both give the same M)
-        const bool k32 = (long long)base_n + 1 < (1LL << (32 - log2_exact(TR::TMAX))) && !wide_keys_forced();
+        // (SPAI_MI355X_SORT_KEYS=64 keeps 64-bit keys; both give the same M).
+        // Not for the FP64 16-column bucket: there the compiler trades the
+        // 64-bit keys' spill for 256 registers and half the occupancy.
+        constexpr bool k32_pays = REFINE || TR::NJ < 16;
+        const bool k32 = k32_pays && (long long)base_n + 1 < (1LL << (32 - log2_exact(TR::TMAX))) &&
+                         !wide_keys_forced();
         auto launch = [&](auto k32_tag) {
             constexpr bool K32 = decltype(k32_tag)::value;
             if (dynamic || ovf) {
