@@ -267,6 +267,12 @@ __device__ __forceinline__ void store_m(const ColumnArgs &a, int pos, double x) 
 // ---------------------------------------------------------------------------
 // Register tier: one group of W lanes per column.
 // ---------------------------------------------------------------------------
+// 1: a per-column LDS table maps each gathered entry to its source position
+// and J index (one read each); 0: binary search over the J offsets with
+// cross-lane permutes, twice per entry.
+#ifndef SPAI_TIER_ENTRY_TABLE
+#define SPAI_TIER_ENTRY_TABLE 1
+#endif
 template <class TR, class QT, bool REFINE, class MT, bool K32>
 __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *lds) {
     constexpr int W = TR::W, EPL = TR::EPL, NJ = TR::NJ, TS = TR::TS;
@@ -295,16 +301,33 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
     const int incl = group_inclusive_scan<W>(cl, lane);
     const int T = __shfl(incl, W - 1, W);
     const int off = incl - cl;
+#if SPAI_TIER_ENTRY_TABLE
+    // entry e -> its position in A's arrays and its J index, written by the
+    // lane that owns the column into the R / y area (dead until the QR ends)
+    int *tsrc = (int *)((char *)lds + L::RS_OFF);
+    unsigned char *tj = (unsigned char *)((char *)lds + L::YS_OFF);
+    static_assert(TR::TMAX * 4 <= TR::NJ * TR::NJ * (int)sizeof(QT) && TR::TMAX <= TR::NJ * 8 && TR::NJ <= 255,
+                  "the entry table fits the R and y areas");
+    for (int q = 0; q < cl; ++q) {
+        tsrc[off + q] = cs + q;
+        tj[off + q] = (unsigned char)lane;
+    }
+    wave_lds_fence();
+#endif
 
     // gather (row, entry) keys, blocked layout e = lane*EPL + i
     KT key[EPL];
 #pragma unroll
     for (int i = 0; i < EPL; ++i) {
         int e = lane * EPL + i;
+        key[i] = SK::kNone;
+#if SPAI_TIER_ENTRY_TABLE
+        if (e < T) key[i] = SK::make(a.rowidx[tsrc[e]], e);
+#else
         int j = locate_j<W, NJ>(e, off, n2);
         int src = __shfl(cs, j, W) + (e - __shfl(off, j, W));
-        key[i] = SK::kNone;
         if (e < T) key[i] = SK::make(a.rowidx[src], e);
+#endif
     }
     bitonic_sort<W, EPL>(key, lane);
 
@@ -356,8 +379,13 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
         // shuffles run on every lane of the group; only the store is masked
         const bool valid = key[i] != SK::kNone;
         const int e = valid ? SK::ent(key[i]) : 0;
+#if SPAI_TIER_ENTRY_TABLE
+        const int j = valid ? (int)tj[e] : 0;
+        const int src = valid ? tsrc[e] : 0;
+#else
         const int j = locate_j<W, NJ>(e, off, n2);
         const int src = __shfl(cs, j, W) + (e - __shfl(off, j, W));
+#endif
         vv[i] = valid ? a.val[src] : 0.0;
         jv[i] = j;
         if (valid) tile[pos[i] * TS + j] = (QT)vv[i];
