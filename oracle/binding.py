@@ -90,16 +90,18 @@ def set_rho_mode(mode: int) -> None:
     lib().oracle_set_rho_mode(int(mode))
 
 
-def iteration_spread(a_csr, m_csr, b, tol: float, max_iter: int, threads=(1, 2, 3, 4, 5, 6, 7, 8, 12, 16)):
+def iteration_spread(a_csr, m_csr, b, tol: float, max_iter: int, threads=(1, 2, 3, 4, 5, 6, 7, 8, 12, 16), x0=None):
     """Oracle iteration counts under different reduction orders (thread
     counts): BiCGSTAB's count is a chaotic function of rounding on
-    non-normal problems, so GPU parity is judged against this spread."""
+    non-normal problems, so GPU parity is judged against this spread.
+    x0: the initial guess (zeros by default)."""
     before = num_threads()
     its = []
+    start = np.zeros(len(b)) if x0 is None else x0
     try:
         for t in threads:
             set_threads(t)
-            its.append(bicgstab(a_csr, m_csr, b, np.zeros(len(b)), tol, max_iter)[1])
+            its.append(bicgstab(a_csr, m_csr, b, start, tol, max_iter)[1])
     finally:
         set_threads(before)
     return its
