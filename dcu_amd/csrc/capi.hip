@@ -1169,6 +1169,12 @@ float run_spai_api(MultiDCU_Context *ctx, const CSC_Matrix *CSC_A, CSC_Matrix *d
     if (cs < 0 || ce > CSC_A->nCol || cs >= ce) return 0.0f;  // the reference returns 0 for bad ranges
     spai_system *S = nullptr;
     if (spai_system_create(ctx->ctx, CSC_A, &S) != SPAI_OK) die("RunSPAI (upload)");
+    // SPAI_MI355X_AUTOTUNE=1: pick the tile shapes on this matrix first (the
+    // returned time is the build's alone)
+    const char *tune = std::getenv("SPAI_MI355X_AUTOTUNE");
+    if (tune && std::atoi(tune) == 1 &&
+        spai_system_autotune(S, SPAI_PRECISION_FP64, schedule, 3, nullptr) != SPAI_OK)
+        die("RunSPAI (autotune)");
     spai_build_report rep{};
     if (spai_system_build(S, SPAI_PRECISION_FP64, schedule, cs, ce, &rep) != SPAI_OK) die("RunSPAI (build)");
     int n = CSC_A->nCol, nnz = CSC_A->nonzeroes;

@@ -60,9 +60,12 @@ def cxx(lib):
     return f
 
 
+@pytest.mark.parametrize("tune", ["0", "1"])
 @pytest.mark.parametrize("entry", ["RunStaticSPAI_MultiDCU_MPI", "RunDynamicSPAI_MultiDCU_MPI"])
-def test_reference_driver_sequence(ctx, entry):
+def test_reference_driver_sequence(ctx, entry, tune, monkeypatch):
     import torch
+
+    monkeypatch.setenv("SPAI_MI355X_AUTOTUNE", tune)  # 1: tile shapes tuned inside Run*
 
     lib = D.load_library()
     f = cxx(lib)
@@ -119,8 +122,8 @@ def test_reference_driver_sequence(ctx, entry):
     Acsr = O.csc2csr(n, A.colptr, A.rowidx, A.val)[:3]
     Mcsr = O.csc2csr(n, A.colptr, A.rowidx, m_val)[:3]
     counts = oracle_counts(Acsr, Mcsr, np.ones(n), 1e-7, 10000, threads=(1, 4), x0=np.ones(n))
-    print(f"{entry}: build {ms:.3f} ms, M err {err:.2e}, iterations {its} (oracle {sorted(counts)}), "
-          f"||b - A x|| {r:.2e}", flush=True)
+    print(f"{entry} (autotune {tune}): build {ms:.3f} ms, M err {err:.2e}, iterations {its} "
+          f"(oracle {sorted(counts)}), ||b - A x|| {r:.2e}", flush=True)
     assert err <= 1e-10
     assert count_ok(its, counts)
     assert r < 1.5e-7
