@@ -47,7 +47,10 @@ namespace spai {
 
 namespace {
 
-constexpr int kBlock = 256;  // threads per workgroup for the column kernels
+#ifndef SPAI_TIER_BLOCK
+#define SPAI_TIER_BLOCK 256
+#endif
+constexpr int kBlock = SPAI_TIER_BLOCK;  // threads per workgroup for the column kernels
 
 // Per-tier group shapes: W lanes per column, EPL gathered entries per lane,
 // NJ = largest |J_k| the tier holds in registers, RPL tile rows per lane.
