@@ -177,6 +177,32 @@ def test_sort_key_widths_agree(ctx, power_law, precision):
         assert out["32"][2] == out["64"][2]
 
 
+def test_autotune_without_register_tier_columns(ctx):
+    # every column gathers ~324 entries from 18 columns: all in the LDS tier,
+    # so no bucket is timed and the shapes stay as they were
+    import scipy.sparse as sp
+    n, per = 300, 18
+    rng = np.random.default_rng(11)
+    rows, cols = [], []
+    for k in range(n):
+        rs = set(rng.choice(n, per - 1, replace=False).tolist()) | {k}
+        rows += sorted(rs)
+        cols += [k] * len(rs)
+    A = sp.csc_matrix((rng.uniform(-1, 1, len(rows)), (rows, cols)), shape=(n, n))
+    A.setdiag(30.0)
+    A.sort_indices()
+    A = D.CSC(n, A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64))
+    S = D.System(ctx, A)
+    S.set_tile_shapes([1, 2, 1])
+    tune = S.autotune(D.PRECISION_FP64, D.SCHEDULE_BALANCED_NNZ, reps=1)
+    print(f"no register-tier columns: {tune}", flush=True)
+    assert tune["columns"] == [0, 0, 0] and tune["shape"] == [1, 2, 1]
+    assert all(v is None for row in tune["ms"] for v in row)
+    m_ref = O.spai(A.n, A.colptr, A.rowidx, A.val, 0)
+    assert column_rel_err(A, S.m_values(), m_ref) <= 1e-10
+    S.close()
+
+
 def test_shape_arguments_checked(ctx):
     S = D.System(ctx, D.Problem.generate(1, 1).system(0))
     before = S.tile_shapes()
