@@ -691,7 +691,7 @@ struct OpInit {
         double b;
     };
     const double *x, *b;
-    double *r, *rs, *p0, *p1, *a, *s, *t, *y, *z;
+    double *r, *rs, *p0, *p1, *a, *s, *t, *y, *z, *y1, *z1;
     double rel_tol, abs_tol;
     int restart;  // 1: restart of an active system from its current x (iterations and tol kept)
     __device__ bool enabled(const SysState &S) const { return !restart || S.active; }
@@ -710,6 +710,8 @@ struct OpInit {
         t[i] = 0.0;
         y[i] = 0.0;
         z[i] = 0.0;
+        y1[i] = 0.0;
+        z1[i] = 0.0;
         acc[0] += ri * ri;
         acc[1] += pr.b * pr.b;
     }
@@ -720,8 +722,11 @@ struct OpInit {
         S.alpha = 0.0;
         S.omega = 0.0;
         S.beta = 0.0;
+        S.alpha_prev = 0.0;
+        S.omega_prev = 0.0;
         S.true_rr = 0.0;
         S.pending = 0;
+        S.xdefer = 0;
         if (restart) {
             S.restarts += 1;
             if (sqrt(tot[0]) < S.tol) {
@@ -760,13 +765,17 @@ struct OpMapplyP {
     struct Pre {
         double p;
     };
-    double *x, *r, *pout, *y;
+    double *x, *r, *pout, *y;  // y: this iteration's buffer (this parity)
     const double *s, *t, *pin, *a, *z;
-    double alpha, omega, nomega, beta, nbw;
+    const double *yb, *zb;  // the previous iteration's y and z (the other parity)
+    int defer;              // 1 (even iterations): x is left for the next K1
+    double alpha, omega, alpha_prev, omega_prev, nomega, beta, nbw;
     __device__ bool enabled(const SysState &S) const { return S.active; }
     __device__ void load(const SysState &S) {
         alpha = S.alpha;
         omega = S.omega;
+        alpha_prev = S.alpha_prev;
+        omega_prev = S.omega_prev;
         nomega = -S.omega;
         beta = S.beta;
         nbw = -S.beta * S.omega;
@@ -778,7 +787,14 @@ struct OpMapplyP {
     __device__ Pre pre(int i, double (&acc)[1]) const {
         const double ri = rval(i);
         const double pi = pval(i, ri);
-        x[i] = (x[i] + alpha * y[i]) + omega * z[i];
+        if (!defer) {
+            // the update of two iterations back (its y and z are this
+            // parity's buffers: y is overwritten only by this row's product
+            // below, z by K3), then the previous iteration's, in the order
+            // and rounding of one update per K1
+            const double xi = (x[i] + alpha_prev * y[i]) + omega_prev * z[i];
+            x[i] = (xi + alpha * yb[i]) + omega * zb[i];
+        }
         r[i] = ri;
         pout[i] = pi;
         acc[0] += ri * ri;
@@ -792,14 +808,47 @@ struct OpMapplyP {
     }
     __device__ void finish(const Ctx &c, int s_, const double (&tot)[1]) const {
         SysState &S = c.st[s_];
+        if (defer) {  // every workgroup has loaded alpha and omega by now
+            S.alpha_prev = S.alpha;
+            S.omega_prev = S.omega;
+        }
         if (!S.pending) {  // first K1 after OpInit: nothing to finish
             S.pending = 1;
             S.rr = tot[0];
             return;
         }
         finish_iteration(c, s_, S, tot[0]);
+        if (defer && !S.active) S.xdefer = 1;  // OpFlushX applies it
     }
 };
+
+// x += the update a system that stopped in a deferring K1 still owes:
+// (x + alpha_prev y) + omega_prev z with the odd-parity y and z.
+struct OpFlushX {
+    static constexpr int NRED = 0;
+    struct Pre {};
+    double *x;
+    const double *y1, *z1;
+    double alpha, omega;
+    __device__ bool enabled(const SysState &S) const { return S.xdefer != 0; }
+    __device__ void load(const SysState &S) {
+        alpha = S.alpha_prev;
+        omega = S.omega_prev;
+    }
+    __device__ Pre pre(int i, double (&)[1]) const {
+        x[i] = (x[i] + alpha * y1[i]) + omega * z1[i];
+        return {};
+    }
+    __device__ double gather(int) const { return 0.0; }
+    __device__ double stage(int, const Pre &) const { return 0.0; }
+    __device__ void row(int, double, const Pre &, double (&)[1]) const {}
+    __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
+};
+
+__global__ void k_clear_xdefer(SysState *st, int nsys) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsys) st[s].xdefer = 0;
+}
 
 // Second half of the split K1: y = M p' with p' stored by the first half
 // (one gathered vector instead of four).
@@ -1551,18 +1600,20 @@ Ctx BiCGStab::make_ctx(const TileMap &tm, int nsys) {
 void BiCGStab::ensure(int n, int nsys, int nunits, int nnodes) {
     if (n != n_) {
         drop_graph();
-        vec_.resize((size_t)9 * std::max(n, 1));
+        vec_.resize((size_t)11 * std::max(n, 1));
         double *v = vec_.get();
         size_t N = (size_t)std::max(n, 1);
         r_ = v;
         rs_ = v + N;
         p_[0] = v + 2 * N;
         p_[1] = v + 3 * N;
-        y_ = v + 4 * N;
+        y_[0] = v + 4 * N;
         a_ = v + 5 * N;
         s_ = v + 6 * N;
-        z_ = v + 7 * N;
+        z_[0] = v + 7 * N;
         t_ = v + 8 * N;
+        y_[1] = v + 9 * N;
+        z_[1] = v + 10 * N;
         n_ = n;
     }
     if (nsys != nsys_ || nnodes != nnodes_) {
@@ -1602,19 +1653,19 @@ void BiCGStab::enqueue_k1(const OperatorView &A, const OperatorView &M, const Ti
     double *pin = p_[parity], *pout = p_[parity ^ 1];
     if (!M.val) {
         // no preconditioner: y = p' through the streaming path
-        OpMapplyP<double, kYCopy> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
+        OpMapplyP<double, kYCopy> op{x_, r_, pout, y_[parity], s_, t_, pin, a_, z_[parity], y_[parity ^ 1], z_[parity ^ 1], parity == 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         launch_units<false, double>(A, tm, c, op, list, grid, st, flip());
     } else if (split) {
-        OpMapplyP<double, kYNone> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
+        OpMapplyP<double, kYNone> op{x_, r_, pout, y_[parity], s_, t_, pin, a_, z_[parity], y_[parity ^ 1], z_[parity ^ 1], parity == 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         launch_units<false, double>(A, tm, c, op, list, grid, st, flip());
-        OpApplyM om{pout, y_};
+        OpApplyM om{pout, y_[parity]};
         if (M.fp32) launch_units<true, float>(M, tm, c, om, list, grid, st, flip());
         else launch_units<true, double>(M, tm, c, om, list, grid, st, flip());
     } else if (M.fp32) {
-        OpMapplyP<float, kYProduct> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
+        OpMapplyP<float, kYProduct> op{x_, r_, pout, y_[parity], s_, t_, pin, a_, z_[parity], y_[parity ^ 1], z_[parity ^ 1], parity == 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         launch_units<true, float>(M, tm, c, op, list, grid, st, flip());
     } else {
-        OpMapplyP<double, kYProduct> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
+        OpMapplyP<double, kYProduct> op{x_, r_, pout, y_[parity], s_, t_, pin, a_, z_[parity], y_[parity ^ 1], z_[parity ^ 1], parity == 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         launch_units<true, double>(M, tm, c, op, list, grid, st, flip());
     }
 }
@@ -1627,23 +1678,23 @@ void BiCGStab::enqueue_iteration(const OperatorView &A, const OperatorView &M, c
     const int *list = grid > 0 ? tm.active.get() : nullptr;
     enqueue_k1(A, M, tm, parity, list, grid, split, st);
     {
-        OpSpmvAlpha op{y_, rs_, a_};
+        OpSpmvAlpha op{y_[parity], rs_, a_};
         launch_units<true, double>(A, tm, c, op, list, grid, st, flip());
     }
     if (M.val) {
         if (M.fp32) {
-            OpMapplyS<float> op{r_, a_, s_, z_, 0.0};
+            OpMapplyS<float> op{r_, a_, s_, z_[parity], 0.0};
             launch_units<true, float>(M, tm, c, op, list, grid, st, flip());
         } else {
-            OpMapplyS<double> op{r_, a_, s_, z_, 0.0};
+            OpMapplyS<double> op{r_, a_, s_, z_[parity], 0.0};
             launch_units<true, double>(M, tm, c, op, list, grid, st, flip());
         }
     } else {
-        OpCopyS op{r_, a_, s_, z_, 0.0};
+        OpCopyS op{r_, a_, s_, z_[parity], 0.0};
         launch_units<false, double>(A, tm, c, op, list, grid, st, flip());
     }
     {
-        OpSpmvOmega op{z_, s_, rs_, t_};
+        OpSpmvOmega op{z_[parity], s_, rs_, t_};
         launch_units<true, double>(A, tm, c, op, list, grid, st, flip());
     }
     SPAI_CHECK_LAUNCH();
@@ -1674,7 +1725,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     hipLaunchKernelGGL(k_state_init, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys, opt.max_iter,
                        counter_.get(), tm.nnodes);
     {
-        OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, s_, t_, y_, z_, opt.rel_tol, opt.abs_tol, 0};
+        OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, s_, t_, y_[0], z_[0], y_[1], z_[1], opt.rel_tol, opt.abs_tol, 0};
         launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
     }
     SPAI_CHECK_LAUNCH();
@@ -1808,7 +1859,15 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                 }
             }
         };
+        // x of a system that stopped in a deferring K1 still owes one update
+        auto flush_x = [&]() {
+            OpFlushX op{x, y_[1], z_[1], 0.0, 0.0};
+            launch_units<false, double>(A, tm, c, op, nullptr, 0, st, flip());
+            hipLaunchKernelGGL(k_clear_xdefer, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys);
+            SPAI_CHECK_LAUNCH();
+        };
         iterate();
+        flush_x();
         bool true_current = false;  // ||b - A x|| is in the states for the final x
         for (int round = 0; opt.true_residual_restart && round < 4; ++round) {
             OpTrueResidual tr{x, b};
@@ -1821,11 +1880,12 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                 true_current = true;
                 break;
             }
-            OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, s_, t_, y_, z_, opt.rel_tol, opt.abs_tol, 1};
+            OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, s_, t_, y_[0], z_[0], y_[1], z_[1], opt.rel_tol, opt.abs_tol, 1};
             launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
             SPAI_CHECK_LAUNCH();
             done_iters = 0;
             iterate();
+            flush_x();
         }
         if (!true_current) {
             OpTrueResidual op{x, b};
@@ -1879,6 +1939,7 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
     require((which != 1 && which != 3 && which != 5) || M.val, "time_kernel: no preconditioner");
     ensure(n, nsys, tm.nunits, tm.nnodes);
     Ctx c = make_ctx(tm, nsys);
+    const int parity = 0;  // the y and z buffers of an even iteration
     hipLaunchKernelGGL(k_state_activate, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys,
                        counter_.get(), tm.nnodes);
     hipEvent_t e0, e1;
@@ -1887,38 +1948,38 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
     auto once = [&]() {
         switch (which) {
         case 0: {
-            OpSpmvPlain op{p_[0], y_};
+            OpSpmvPlain op{p_[0], y_[0]};
             launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
         } break;
         case 5: {
-            OpSpmvPlain op{p_[0], y_};
+            OpSpmvPlain op{p_[0], y_[0]};
             if (M.fp32) launch_units<true, float>(M, tm, c, op, nullptr, 0, st, flip());
             else launch_units<true, double>(M, tm, c, op, nullptr, 0, st, flip());
         } break;
         case 1: {
             if (M.fp32) {
-                OpMapplyP<float, kYProduct> op{x_, r_, p_[1], y_, s_, t_, p_[0], a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
+                OpMapplyP<float, kYProduct> op{x_, r_, p_[1], y_[1], s_, t_, p_[0], a_, z_[1], y_[0], z_[0], 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
                 launch_units<true, float>(M, tm, c, op, nullptr, 0, st, flip());
             } else {
-                OpMapplyP<double, kYProduct> op{x_, r_, p_[1], y_, s_, t_, p_[0], a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
+                OpMapplyP<double, kYProduct> op{x_, r_, p_[1], y_[1], s_, t_, p_[0], a_, z_[1], y_[0], z_[0], 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
                 launch_units<true, double>(M, tm, c, op, nullptr, 0, st, flip());
             }
         } break;
         case 2: {
-            OpSpmvAlpha op{y_, rs_, a_};
+            OpSpmvAlpha op{y_[parity], rs_, a_};
             launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
         } break;
         case 3: {
             if (M.fp32) {
-                OpMapplyS<float> op{r_, a_, s_, z_, 0.0};
+                OpMapplyS<float> op{r_, a_, s_, z_[parity], 0.0};
                 launch_units<true, float>(M, tm, c, op, nullptr, 0, st, flip());
             } else {
-                OpMapplyS<double> op{r_, a_, s_, z_, 0.0};
+                OpMapplyS<double> op{r_, a_, s_, z_[parity], 0.0};
                 launch_units<true, double>(M, tm, c, op, nullptr, 0, st, flip());
             }
         } break;
         case 4: {
-            OpSpmvOmega op{z_, s_, rs_, t_};
+            OpSpmvOmega op{z_[parity], s_, rs_, t_};
             launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
         } break;
         }
@@ -1954,7 +2015,7 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
                        counter_.get(), tm.nnodes);
     Ctx c = make_ctx(tm, nsys);
     {
-        OpInit op{tx, tb, r_, rs_, p_[0], p_[1], a_, s_, t_, y_, z_, 0.0, 1e-300, 0};
+        OpInit op{tx, tb, r_, rs_, p_[0], p_[1], a_, s_, t_, y_[0], z_[0], y_[1], z_[1], 0.0, 1e-300, 0};
         launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
     }
     SPAI_CHECK_LAUNCH();
@@ -1967,42 +2028,43 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
     for (auto &e : ev) SPAI_HIP(hipEventCreate(&e));
     try {
         for (int it = 0; it < reps; ++it) {
-            double *pin = p_[it & 1], *pout = p_[(it & 1) ^ 1];
+            const int parity = it & 1;
+            double *pin = p_[parity], *pout = p_[parity ^ 1];
             hipEvent_t *e = &ev[(size_t)it * K];
             SPAI_HIP(hipEventRecord(e[0], st));
             if (split) {
-                OpMapplyP<double, kYNone> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
+                OpMapplyP<double, kYNone> op{x_, r_, pout, y_[parity], s_, t_, pin, a_, z_[parity], y_[parity ^ 1], z_[parity ^ 1], parity == 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
                 launch_units<false, double>(A, tm, c, op, nullptr, 0, st, flip());
                 SPAI_HIP(hipEventRecord(e[1], st));
-                OpApplyM om{pout, y_};
+                OpApplyM om{pout, y_[parity]};
                 if (M.fp32) launch_units<true, float>(M, tm, c, om, nullptr, 0, st, flip());
                 else launch_units<true, double>(M, tm, c, om, nullptr, 0, st, flip());
             } else {
                 if (M.fp32) {
-                    OpMapplyP<float, kYProduct> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
+                    OpMapplyP<float, kYProduct> op{x_, r_, pout, y_[parity], s_, t_, pin, a_, z_[parity], y_[parity ^ 1], z_[parity ^ 1], parity == 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
                     launch_units<true, float>(M, tm, c, op, nullptr, 0, st, flip());
                 } else {
-                    OpMapplyP<double, kYProduct> op{x_, r_, pout, y_, s_, t_, pin, a_, z_, 0.0, 0.0, 0.0, 0.0, 0.0};
+                    OpMapplyP<double, kYProduct> op{x_, r_, pout, y_[parity], s_, t_, pin, a_, z_[parity], y_[parity ^ 1], z_[parity ^ 1], parity == 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
                     launch_units<true, double>(M, tm, c, op, nullptr, 0, st, flip());
                 }
                 SPAI_HIP(hipEventRecord(e[1], st));
             }
             SPAI_HIP(hipEventRecord(e[2], st));
             {
-                OpSpmvAlpha op{y_, rs_, a_};
+                OpSpmvAlpha op{y_[parity], rs_, a_};
                 launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
             }
             SPAI_HIP(hipEventRecord(e[3], st));
             if (M.fp32) {
-                OpMapplyS<float> op{r_, a_, s_, z_, 0.0};
+                OpMapplyS<float> op{r_, a_, s_, z_[parity], 0.0};
                 launch_units<true, float>(M, tm, c, op, nullptr, 0, st, flip());
             } else {
-                OpMapplyS<double> op{r_, a_, s_, z_, 0.0};
+                OpMapplyS<double> op{r_, a_, s_, z_[parity], 0.0};
                 launch_units<true, double>(M, tm, c, op, nullptr, 0, st, flip());
             }
             SPAI_HIP(hipEventRecord(e[4], st));
             {
-                OpSpmvOmega op{z_, s_, rs_, t_};
+                OpSpmvOmega op{z_[parity], s_, rs_, t_};
                 launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
             }
         }

@@ -20,12 +20,14 @@ struct alignas(64) SysState {
     double tol;      // absolute threshold on ||r||_2
     double err0;     // ||r_0||_2
     double true_rr;  // ||b - A x||^2 after the solve
+    double alpha_prev, omega_prev;  // alpha, omega of the iteration whose x update is deferred
     int active;      // 1 while iterating
     int iters;       // the reference's iteration count once stopped
     int status;      // SolveStatus
     int max_iter;
     int restarts;    // restarts from the true residual (true_residual_restart)
     int pending;     // 1 once an iteration's x/r update is left for the next K1
+    int xdefer;      // 1: stopped with x missing the update held in the odd buffers
 };
 
 enum SolveStatus { kRunning = 0, kConverged = 1, kMaxIter = 2, kBreakdown = 3, kConvergedAtStart = 4 };
@@ -217,8 +219,10 @@ private:
     DeviceArray<int> counter_;      // [nsys] active systems, [nsys + 1] active units, [nsys + 2 + node] tree nodes
     DeviceArray<double> tvec_;      // b = 1 and x = 0 of time_iteration
     std::vector<SysState> h_states_;
-    double *r_ = nullptr, *rs_ = nullptr, *p_[2] = {nullptr, nullptr}, *y_ = nullptr, *a_ = nullptr, *s_ = nullptr,
-           *z_ = nullptr, *t_ = nullptr;
+    // y and z by iteration parity: x takes two iterations' updates in every
+    // other K1, so the previous iteration's y and z must survive one more
+    double *r_ = nullptr, *rs_ = nullptr, *p_[2] = {nullptr, nullptr}, *y_[2] = {nullptr, nullptr}, *a_ = nullptr,
+           *s_ = nullptr, *z_[2] = {nullptr, nullptr}, *t_ = nullptr;
     const double *b_ = nullptr;
     double *x_ = nullptr;
     // graph cache
