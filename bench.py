@@ -61,10 +61,13 @@ def kernel_bytes(kind, n, nnz, m_bytes):
     column block of a slice) and some padding; the measured traffic is the
     `traffic` figure."""
     rp = 4 * (n + 1)
-    if kind == "update_mapply_p":  # fused K1: x, r update + y = M p'; reads M, x, y, z, s, t, p, a; writes x, r, p', y
-        return (m_bytes + 4) * nnz + rp + 8 * n * 11
-    if kind == "update":          # split K1, streaming half: reads x, y, z, s, t, p, a; writes x, r, p'
-        return 8 * n * 10
+    # K1 updates x every other iteration from two iterations' y and z: one
+    # launch reads x, y, z, y', z' and writes x, the next touches none of
+    # them; the figures are the mean over the pair (x: 3 vectors per launch)
+    if kind == "update_mapply_p":  # fused K1: r update + y = M p'; reads M, s, t, p, a; writes r, p', y; + x
+        return (m_bytes + 4) * nnz + rp + 8 * n * 10
+    if kind == "update":          # split K1, streaming half: reads s, t, p, a; writes r, p'; + x
+        return 8 * n * 9
     if kind == "mapply_p":        # split K1, y = M p'; reads M, p' (gather, once); writes y
         return (m_bytes + 4) * nnz + rp + 8 * n * 2
     if kind == "spmv_alpha":      # a = A y; reads A, y (gather, once), r*; writes a
