@@ -755,10 +755,11 @@ struct OpInit {
 //   kYCopy     y = p' (no preconditioner)
 //   kYNone     no y: the streaming half of the split K1, OpApplyM follows
 enum { kYProduct = 0, kYCopy = 1, kYNone = 2 };
+enum { kXOne = 0, kXDefer = 1, kXTwo = 2 };
 #ifndef SPAI_STAGE_K1
 #define SPAI_STAGE_K1 1
 #endif
-template <class MT, int YMODE>
+template <class MT, int YMODE, int XMODE>
 struct OpMapplyP {
     static constexpr int NRED = 1;
     static constexpr bool kStage = SPAI_STAGE_K1 != 0;
@@ -768,7 +769,9 @@ struct OpMapplyP {
     double *x, *r, *pout, *y;  // y: this iteration's buffer (this parity)
     const double *s, *t, *pin, *a, *z;
     const double *yb, *zb;  // the previous iteration's y and z (the other parity)
-    int defer;              // 1 (even iterations): x is left for the next K1
+    // XMODE, the x update: kXOne the previous iteration's (every K1);
+    // kXDefer none, x is left for the next K1 (even iterations of a batch);
+    // kXTwo the two previous iterations' (odd iterations of a batch)
     double alpha, omega, alpha_prev, omega_prev, nomega, beta, nbw;
     __device__ bool enabled(const SysState &S) const { return S.active; }
     __device__ void load(const SysState &S) {
@@ -787,12 +790,13 @@ struct OpMapplyP {
     __device__ Pre pre(int i, double (&acc)[1]) const {
         const double ri = rval(i);
         const double pi = pval(i, ri);
-        if (!defer) {
-            // the update of two iterations back (its y and z are this
-            // parity's buffers: y is overwritten only by this row's product
-            // below, z by K3), then the previous iteration's, in the order
-            // and rounding of one update per K1
-            const double xi = (x[i] + alpha_prev * y[i]) + omega_prev * z[i];
+        if constexpr (XMODE != kXDefer) {
+            double xi = x[i];
+            // kXTwo: first the update of two iterations back (its y and z are
+            // this parity's buffers: y is overwritten only by this row's
+            // product below, z by K3), in the order and rounding of one
+            // update per K1
+            if constexpr (XMODE == kXTwo) xi = (xi + alpha_prev * y[i]) + omega_prev * z[i];
             x[i] = (xi + alpha * yb[i]) + omega * zb[i];
         }
         r[i] = ri;
@@ -808,6 +812,7 @@ struct OpMapplyP {
     }
     __device__ void finish(const Ctx &c, int s_, const double (&tot)[1]) const {
         SysState &S = c.st[s_];
+        constexpr bool defer = XMODE == kXDefer;
         if (defer) {  // every workgroup has loaded alpha and omega by now
             S.alpha_prev = S.alpha;
             S.omega_prev = S.omega;
@@ -1244,6 +1249,27 @@ void launch_units(const OperatorView &O, const TileMap &tm, const Ctx &c, const 
                            uv, c, op);
 }
 
+// Operands of a K1 launch (see OpMapplyP).
+struct K1Args {
+    double *x, *r, *pout, *y;
+    const double *s, *t, *pin, *a, *z, *yb, *zb;
+};
+
+// One K1 launch with the x update `xmode` selected at compile time.
+template <bool HM, class MT, int YMODE>
+void launch_k1(const OperatorView &O, const TileMap &tm, const Ctx &c, int xmode, const K1Args &k, const int *list,
+               int grid, hipStream_t st, bool reverse) {
+    auto go = [&](auto xtag) {
+        constexpr int XM = decltype(xtag)::value;
+        OpMapplyP<MT, YMODE, XM> op{k.x, k.r, k.pout, k.y, k.s, k.t, k.pin, k.a, k.z, k.yb, k.zb,
+                                    0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        launch_units<HM, MT>(O, tm, c, op, list, grid, st, reverse);
+    };
+    if (xmode == kXOne) go(std::integral_constant<int, kXOne>{});
+    else if (xmode == kXDefer) go(std::integral_constant<int, kXDefer>{});
+    else go(std::integral_constant<int, kXTwo>{});
+}
+
 }  // namespace
 
 // ================================================================ host ====
@@ -1651,22 +1677,19 @@ void BiCGStab::enqueue_k1(const OperatorView &A, const OperatorView &M, const Ti
                           const int *list, int grid, bool split, hipStream_t st) {
     Ctx c = make_ctx(tm, nsys_);
     double *pin = p_[parity], *pout = p_[parity ^ 1];
+    const K1Args k1{x_, r_, pout, ybuf(parity), s_, t_, pin, a_, zbuf(parity), ybuf(parity ^ 1), zbuf(parity ^ 1)};
     if (!M.val) {
         // no preconditioner: y = p' through the streaming path
-        OpMapplyP<double, kYCopy> op{x_, r_, pout, y_[parity], s_, t_, pin, a_, z_[parity], y_[parity ^ 1], z_[parity ^ 1], parity == 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        launch_units<false, double>(A, tm, c, op, list, grid, st, flip());
+        launch_k1<false, double, kYCopy>(A, tm, c, x_mode(parity), k1, list, grid, st, flip());
     } else if (split) {
-        OpMapplyP<double, kYNone> op{x_, r_, pout, y_[parity], s_, t_, pin, a_, z_[parity], y_[parity ^ 1], z_[parity ^ 1], parity == 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        launch_units<false, double>(A, tm, c, op, list, grid, st, flip());
-        OpApplyM om{pout, y_[parity]};
+        launch_k1<false, double, kYNone>(A, tm, c, x_mode(parity), k1, list, grid, st, flip());
+        OpApplyM om{pout, ybuf(parity)};
         if (M.fp32) launch_units<true, float>(M, tm, c, om, list, grid, st, flip());
         else launch_units<true, double>(M, tm, c, om, list, grid, st, flip());
     } else if (M.fp32) {
-        OpMapplyP<float, kYProduct> op{x_, r_, pout, y_[parity], s_, t_, pin, a_, z_[parity], y_[parity ^ 1], z_[parity ^ 1], parity == 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        launch_units<true, float>(M, tm, c, op, list, grid, st, flip());
+        launch_k1<true, float, kYProduct>(M, tm, c, x_mode(parity), k1, list, grid, st, flip());
     } else {
-        OpMapplyP<double, kYProduct> op{x_, r_, pout, y_[parity], s_, t_, pin, a_, z_[parity], y_[parity ^ 1], z_[parity ^ 1], parity == 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        launch_units<true, double>(M, tm, c, op, list, grid, st, flip());
+        launch_k1<true, double, kYProduct>(M, tm, c, x_mode(parity), k1, list, grid, st, flip());
     }
 }
 
@@ -1678,23 +1701,23 @@ void BiCGStab::enqueue_iteration(const OperatorView &A, const OperatorView &M, c
     const int *list = grid > 0 ? tm.active.get() : nullptr;
     enqueue_k1(A, M, tm, parity, list, grid, split, st);
     {
-        OpSpmvAlpha op{y_[parity], rs_, a_};
+        OpSpmvAlpha op{ybuf(parity), rs_, a_};
         launch_units<true, double>(A, tm, c, op, list, grid, st, flip());
     }
     if (M.val) {
         if (M.fp32) {
-            OpMapplyS<float> op{r_, a_, s_, z_[parity], 0.0};
+            OpMapplyS<float> op{r_, a_, s_, zbuf(parity), 0.0};
             launch_units<true, float>(M, tm, c, op, list, grid, st, flip());
         } else {
-            OpMapplyS<double> op{r_, a_, s_, z_[parity], 0.0};
+            OpMapplyS<double> op{r_, a_, s_, zbuf(parity), 0.0};
             launch_units<true, double>(M, tm, c, op, list, grid, st, flip());
         }
     } else {
-        OpCopyS op{r_, a_, s_, z_[parity], 0.0};
+        OpCopyS op{r_, a_, s_, zbuf(parity), 0.0};
         launch_units<false, double>(A, tm, c, op, list, grid, st, flip());
     }
     {
-        OpSpmvOmega op{z_[parity], s_, rs_, t_};
+        OpSpmvOmega op{zbuf(parity), s_, rs_, t_};
         launch_units<true, double>(A, tm, c, op, list, grid, st, flip());
     }
     SPAI_CHECK_LAUNCH();
@@ -1709,6 +1732,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     ensure(n, nsys, tm.nunits, tm.nnodes);
     b_ = b;
     x_ = x;
+    defer_x_ = nsys > 1;
     if (opt.record_history) {
         hist_.reserve((size_t)nsys * (size_t)opt.max_iter);
         hist_ptr_ = hist_.get();
@@ -1762,7 +1786,8 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
             if (opt.use_graph) {
                 const int gi = std::max(2, opt.graph_iters + (opt.graph_iters & 1));
                 const void *key[kGraphKey] = {A.val, A.idx, A.off, A.diag, M.val, M.idx, M.off, M.diag, M.blk,
-                                              (const void *)(size_t)M.fp32, tm.unit_row.get(), b, x, hist_ptr_};
+                                              (const void *)(size_t)M.fp32, tm.unit_row.get(), b, x, hist_ptr_,
+                                              (const void *)(size_t)defer_x_};
                 // One graph runs ahead of the host check: graph k + 1 is
                 // queued before the counts graph k published are read, so the
                 // device does not wait for the host between graphs.  The
@@ -1957,29 +1982,28 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
             else launch_units<true, double>(M, tm, c, op, nullptr, 0, st, flip());
         } break;
         case 1: {
+            const K1Args k1{x_, r_, p_[1], y_[1], s_, t_, p_[0], a_, z_[1], y_[0], z_[0]};
             if (M.fp32) {
-                OpMapplyP<float, kYProduct> op{x_, r_, p_[1], y_[1], s_, t_, p_[0], a_, z_[1], y_[0], z_[0], 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-                launch_units<true, float>(M, tm, c, op, nullptr, 0, st, flip());
+                launch_k1<true, float, kYProduct>(M, tm, c, kXTwo, k1, nullptr, 0, st, flip());
             } else {
-                OpMapplyP<double, kYProduct> op{x_, r_, p_[1], y_[1], s_, t_, p_[0], a_, z_[1], y_[0], z_[0], 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-                launch_units<true, double>(M, tm, c, op, nullptr, 0, st, flip());
+                launch_k1<true, double, kYProduct>(M, tm, c, kXTwo, k1, nullptr, 0, st, flip());
             }
         } break;
         case 2: {
-            OpSpmvAlpha op{y_[parity], rs_, a_};
+            OpSpmvAlpha op{ybuf(parity), rs_, a_};
             launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
         } break;
         case 3: {
             if (M.fp32) {
-                OpMapplyS<float> op{r_, a_, s_, z_[parity], 0.0};
+                OpMapplyS<float> op{r_, a_, s_, zbuf(parity), 0.0};
                 launch_units<true, float>(M, tm, c, op, nullptr, 0, st, flip());
             } else {
-                OpMapplyS<double> op{r_, a_, s_, z_[parity], 0.0};
+                OpMapplyS<double> op{r_, a_, s_, zbuf(parity), 0.0};
                 launch_units<true, double>(M, tm, c, op, nullptr, 0, st, flip());
             }
         } break;
         case 4: {
-            OpSpmvOmega op{z_[parity], s_, rs_, t_};
+            OpSpmvOmega op{zbuf(parity), s_, rs_, t_};
             launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
         } break;
         }
@@ -2003,6 +2027,7 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
     require(reps >= 1, "time_iteration: reps must be >= 1");
     require(variant >= -1 && variant <= 1, "time_iteration: variant must be -1, 0 or 1");
     const bool split = variant < 0 ? tm.nunits >= split_units(tm) : variant == 1;
+    defer_x_ = nsys > 1;  // as the solve runs it
     ensure(n, nsys, tm.nunits, tm.nnodes);
     hist_ptr_ = nullptr;
     hist_stride_ = 0;
@@ -2030,41 +2055,39 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
         for (int it = 0; it < reps; ++it) {
             const int parity = it & 1;
             double *pin = p_[parity], *pout = p_[parity ^ 1];
+            const K1Args k1{x_, r_, pout, ybuf(parity), s_, t_, pin, a_, zbuf(parity), ybuf(parity ^ 1), zbuf(parity ^ 1)};
             hipEvent_t *e = &ev[(size_t)it * K];
             SPAI_HIP(hipEventRecord(e[0], st));
             if (split) {
-                OpMapplyP<double, kYNone> op{x_, r_, pout, y_[parity], s_, t_, pin, a_, z_[parity], y_[parity ^ 1], z_[parity ^ 1], parity == 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-                launch_units<false, double>(A, tm, c, op, nullptr, 0, st, flip());
+                launch_k1<false, double, kYNone>(A, tm, c, x_mode(parity), k1, nullptr, 0, st, flip());
                 SPAI_HIP(hipEventRecord(e[1], st));
-                OpApplyM om{pout, y_[parity]};
+                OpApplyM om{pout, ybuf(parity)};
                 if (M.fp32) launch_units<true, float>(M, tm, c, om, nullptr, 0, st, flip());
                 else launch_units<true, double>(M, tm, c, om, nullptr, 0, st, flip());
             } else {
                 if (M.fp32) {
-                    OpMapplyP<float, kYProduct> op{x_, r_, pout, y_[parity], s_, t_, pin, a_, z_[parity], y_[parity ^ 1], z_[parity ^ 1], parity == 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-                    launch_units<true, float>(M, tm, c, op, nullptr, 0, st, flip());
+                    launch_k1<true, float, kYProduct>(M, tm, c, x_mode(parity), k1, nullptr, 0, st, flip());
                 } else {
-                    OpMapplyP<double, kYProduct> op{x_, r_, pout, y_[parity], s_, t_, pin, a_, z_[parity], y_[parity ^ 1], z_[parity ^ 1], parity == 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-                    launch_units<true, double>(M, tm, c, op, nullptr, 0, st, flip());
+                    launch_k1<true, double, kYProduct>(M, tm, c, x_mode(parity), k1, nullptr, 0, st, flip());
                 }
                 SPAI_HIP(hipEventRecord(e[1], st));
             }
             SPAI_HIP(hipEventRecord(e[2], st));
             {
-                OpSpmvAlpha op{y_[parity], rs_, a_};
+                OpSpmvAlpha op{ybuf(parity), rs_, a_};
                 launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
             }
             SPAI_HIP(hipEventRecord(e[3], st));
             if (M.fp32) {
-                OpMapplyS<float> op{r_, a_, s_, z_[parity], 0.0};
+                OpMapplyS<float> op{r_, a_, s_, zbuf(parity), 0.0};
                 launch_units<true, float>(M, tm, c, op, nullptr, 0, st, flip());
             } else {
-                OpMapplyS<double> op{r_, a_, s_, z_[parity], 0.0};
+                OpMapplyS<double> op{r_, a_, s_, zbuf(parity), 0.0};
                 launch_units<true, double>(M, tm, c, op, nullptr, 0, st, flip());
             }
             SPAI_HIP(hipEventRecord(e[4], st));
             {
-                OpSpmvOmega op{z_[parity], s_, rs_, t_};
+                OpSpmvOmega op{zbuf(parity), s_, rs_, t_};
                 launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
             }
         }

@@ -233,7 +233,7 @@ private:
         hipGraphExec_t exec;
     };
     std::vector<GraphEntry> graphs_;
-    static constexpr int kGraphKey = 14;  // operand pointers the graphs were captured with
+    static constexpr int kGraphKey = 15;  // operand pointers the graphs were captured with
     const void *graph_key_[kGraphKey] = {};
     hipEvent_t check_ev_[2] = {nullptr, nullptr};  // host checks of the graph loop
     int *h_pub_ = nullptr, *d_pub_ = nullptr;      // counts published by the graphs (host-coherent)
@@ -244,6 +244,14 @@ private:
     // were captured with.
     int sweep_ = 0;
     bool flip() { return (sweep_++ & 1) != 0; }
+    // A batch updates x in every other K1 (fewer bytes where bandwidth
+    // bounds the launches); a single system, latency-bound, in every K1.
+    bool defer_x_ = false;
+    int x_mode(int parity) const { return defer_x_ ? (parity == 0 ? 1 : 2) : 0; }
+    // without deferral one y and one z serve every iteration (read for x,
+    // then overwritten, row by row, as before the double buffers)
+    double *ybuf(int parity) const { return y_[defer_x_ ? parity : 0]; }
+    double *zbuf(int parity) const { return z_[defer_x_ ? parity : 0]; }
     DeviceArray<double> hist_;
     double *hist_ptr_ = nullptr;
     int hist_stride_ = 0;
