@@ -52,7 +52,7 @@ def cpu_model():
     return platform.processor()
 
 
-def kernel_bytes(kind, n, nnz, m_bytes):
+def kernel_bytes(kind, n, nnz, m_bytes, batched=True):
     """Algorithmic HBM bytes of one launch of an iteration kernel with every
     system active, in the CSR terms of the reference's operators: per
     nonzero a 4 B column index and the value (A: 8 B, M: m_bytes), 4 B per
@@ -61,13 +61,15 @@ def kernel_bytes(kind, n, nnz, m_bytes):
     column block of a slice) and some padding; the measured traffic is the
     `traffic` figure."""
     rp = 4 * (n + 1)
-    # K1 updates x every other iteration from two iterations' y and z: one
-    # launch reads x, y, z, y', z' and writes x, the next touches none of
-    # them; the figures are the mean over the pair (x: 3 vectors per launch)
+    # x: a batch's K1 updates x every other iteration from two iterations' y
+    # and z (one launch reads x, y, z, y', z' and writes x, the next touches
+    # none of them: 3 vectors per launch on average); a single system's K1
+    # reads x, y, z and writes x every iteration (4)
+    xv = 3 if batched else 4
     if kind == "update_mapply_p":  # fused K1: r update + y = M p'; reads M, s, t, p, a; writes r, p', y; + x
-        return (m_bytes + 4) * nnz + rp + 8 * n * 10
+        return (m_bytes + 4) * nnz + rp + 8 * n * (7 + xv)
     if kind == "update":          # split K1, streaming half: reads s, t, p, a; writes r, p'; + x
-        return 8 * n * 9
+        return 8 * n * (6 + xv)
     if kind == "mapply_p":        # split K1, y = M p'; reads M, p' (gather, once); writes y
         return (m_bytes + 4) * nnz + rp + 8 * n * 2
     if kind == "spmv_alpha":      # a = A y; reads A, y (gather, once), r*; writes a
@@ -156,7 +158,7 @@ def main():
         m_bytes = 4 if precision == D.PRECISION_MIXED else 8
         print(json.dumps({"profile_kernels": True, "n": n_tot, "nnz": nnz_tot, "layout": layout,
                           "iteration_kernel_ms": kms,
-                          "algorithmic_bytes": {k: kernel_bytes(k, n_tot, nnz_tot, m_bytes) for k in kms},
+                          "algorithmic_bytes": {k: kernel_bytes(k, n_tot, nnz_tot, m_bytes, batched) for k in kms},
                           "copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes}))
         S.close()
         ctx.close()
@@ -195,14 +197,14 @@ def main():
     m_bytes = 4 if precision == D.PRECISION_MIXED else 8
     kernel_ms = iteration_kernels(S.time_iteration(20))
     dominant = max(kernel_ms, key=kernel_ms.get)
-    kb = kernel_bytes(dominant, n_tot, nnz_tot, m_bytes)
+    kb = kernel_bytes(dominant, n_tot, nnz_tot, m_bytes, batched)
     achieved = kb / (kernel_ms[dominant] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_launch": int(kb), "launch_ms": round(kernel_ms[dominant], 5),
                 "layout": layout, "stored_entries": layout_entries,
                 "all_kernels_ms": {k: round(v, 5) for k, v in kernel_ms.items()},
-                "all_kernels_frac": {k: round(kernel_bytes(k, n_tot, nnz_tot, m_bytes) / (v * 1e-3) /
+                "all_kernels_frac": {k: round(kernel_bytes(k, n_tot, nnz_tot, m_bytes, batched) / (v * 1e-3) /
                                               1e9 / HBM_PEAK_GBS, 4) for k, v in kernel_ms.items()}}
     # HBM traffic per launch from the committed counter summary of this
     # workload (two rocprofv3 --pmc passes, calibrated against a copy kernel)
@@ -221,7 +223,7 @@ def main():
     per_iter = 0.0
     for A, o in zip(mats, oc):
         it = o["iterations"] if o["status"] != D.SOLVE_MAX_ITER else o["iterations"] - 1
-        per_iter += it * sum(kernel_bytes(k, A.n, A.nnz, m_bytes)
+        per_iter += it * sum(kernel_bytes(k, A.n, A.nnz, m_bytes, batched)
                              for k in ("update_mapply_p", "spmv_alpha", "mapply_s", "spmv_omega"))
     solve_gbs = per_iter / (float(np.median(solve_ms)) * 1e-3) / 1e9
     roofline["solve_achieved"] = round(solve_gbs, 1)
