@@ -175,7 +175,7 @@ void finish_setup(spai_system *S, hipStream_t st) {
     S->colidx.resize((size_t)std::max(S->nnz, 1));
     S->csrval.resize((size_t)std::max(S->nnz, 1));
     S->perm.resize((size_t)std::max(S->nnz, 1));
-    csc_to_csr_device(S->n, S->nnz, S->colptr.get(), S->rowidx.get(), S->cscval.get(), S->rowptr.get(),
+    csc_to_csr_device(S->n, S->n, S->nnz, S->colptr.get(), S->rowidx.get(), S->cscval.get(), S->rowptr.get(),
                       S->colidx.get(), S->csrval.get(), S->perm.get(), st);
     build_layout(S->tiles, S->rowptr.get(), S->colidx.get(), S->h_row_off, S->h_nnz_off, true, st);
     S->sell_layout = S->tiles.kind == kLayoutSell;
@@ -461,6 +461,9 @@ int spai_system_destroy(spai_system *sys) {
         spai_context *ctx = sys->ctx;
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
+        // graphs captured on this system's layouts are never replayed again
+        ctx->solver.forget_layout(sys->tiles.generation);
+        ctx->adjoint_solver.forget_layout(sys->tiles_t.generation);
         delete sys;
         if (--ctx->live_systems == 0 && ctx->release_pending) release_context(ctx);
     });
@@ -712,7 +715,7 @@ int spai_system_set_pattern(spai_system *sys, int pattern) {
         sys->mrowptr.resize((size_t)n + 1);
         sys->mcolidx.resize((size_t)std::max(mn, 1));
         sys->mperm.resize((size_t)std::max(mn, 1));
-        csc_to_csr_device(n, mn, sys->pcolptr.get(), sys->prowidx.get(), nullptr, sys->mrowptr.get(),
+        csc_to_csr_device(n, n, mn, sys->pcolptr.get(), sys->prowidx.get(), nullptr, sys->mrowptr.get(),
                           sys->mcolidx.get(), nullptr, sys->mperm.get(), st);
         sys->perm_lay.resize((size_t)std::max(mn, 1));
         if (sys->sell_layout) {
@@ -1044,11 +1047,11 @@ int spai_calibration_copy(spai_context *ctx, long long bytes, int reps, float *m
 
 int spai_csc_to_csr_device(int m, int n, int nnz, const double *cscData, const int *cscIndex, const int *cscPtr,
                            double *csrData, int *csrIndex, int *csrPtr) {
-    if (m <= 0 || n <= 0 || m != n || nnz < 0 || !cscIndex || !cscPtr || !csrIndex || !csrPtr)
-        return fail(SPAI_ERR_INVALID, "csc_to_csr: bad arguments (square matrices only)");
+    if (m <= 0 || n <= 0 || nnz < 0 || !cscIndex || !cscPtr || !csrIndex || !csrPtr)
+        return fail(SPAI_ERR_INVALID, "csc_to_csr: bad arguments");
     return guarded([&] {
         DeviceArray<int> perm((size_t)std::max(nnz, 1));
-        csc_to_csr_device(n, nnz, cscPtr, cscIndex, cscData, csrPtr, csrIndex, csrData, perm.get(), nullptr);
+        csc_to_csr_device(m, n, nnz, cscPtr, cscIndex, cscData, csrPtr, csrIndex, csrData, perm.get(), nullptr);
     });
 }
 

@@ -41,6 +41,10 @@ enum Status {
 inline void require(bool ok, const std::string &msg, int st = kErrInvalid) {
     if (!ok) throw Error(st, msg);
 }
+// literal messages: no string is built unless the check fails
+inline void require(bool ok, const char *msg, int st = kErrInvalid) {
+    if (!ok) throw Error(st, msg);
+}
 
 // Owning device allocation.
 template <class T>

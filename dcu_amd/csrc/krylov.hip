@@ -35,6 +35,8 @@
 #include "krylov.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -710,8 +712,10 @@ struct OpInit {
         t[i] = 0.0;
         y[i] = 0.0;
         z[i] = 0.0;
-        y1[i] = 0.0;
-        z1[i] = 0.0;
+        if (y1) {  // the second y and z buffers exist only when x updates are deferred
+            y1[i] = 0.0;
+            z1[i] = 0.0;
+        }
         acc[0] += ri * ri;
         acc[1] += pr.b * pr.b;
     }
@@ -742,6 +746,13 @@ struct OpInit {
         if (S.err0 < S.tol) {
             // r0 already below the threshold: the reference would divide 0/0
             S.status = kConvergedAtStart;
+            S.active = 0;
+            atomicSub(&c.counter[c.nsys], 1);
+        } else if (S.max_iter <= 1) {
+            // the reference's loop (iter_times = 1; while (iter_times <
+            // MAX_ITER)) runs no iteration: MAX_ITER is returned with x = x0
+            S.status = kMaxIter;
+            S.iters = S.max_iter;
             S.active = 0;
             atomicSub(&c.counter[c.nsys], 1);
         }
@@ -1273,23 +1284,23 @@ void launch_k1(const OperatorView &O, const TileMap &tm, const Ctx &c, int xmode
 }  // namespace
 
 // ================================================================ host ====
-void csc_to_csr_device(int n, int nnz, const int *colptr, const int *rowidx, const double *cscval, int *rowptr,
-                       int *colidx, double *csrval, int *perm, hipStream_t st) {
-    require(n >= 0 && nnz >= 0, "csc_to_csr: negative size");
-    DeviceArray<int> cnt((size_t)n + 1), src((size_t)std::max(nnz, 1)), tmp;
+void csc_to_csr_device(int m, int n, int nnz, const int *colptr, const int *rowidx, const double *cscval,
+                       int *rowptr, int *colidx, double *csrval, int *perm, hipStream_t st) {
+    require(m >= 0 && n >= 0 && nnz >= 0, "csc_to_csr: negative size");
+    DeviceArray<int> cnt((size_t)m + 1), src((size_t)std::max(nnz, 1)), tmp;
     const int B = 256;
-    hipLaunchKernelGGL(k_fill_int, dim3(std::max(1, ceil_div(n + 1, B))), dim3(B), 0, st, cnt.get(), 0,
-                       (long long)n + 1);
+    hipLaunchKernelGGL(k_fill_int, dim3(std::max(1, ceil_div(m + 1, B))), dim3(B), 0, st, cnt.get(), 0,
+                       (long long)m + 1);
     if (nnz > 0) hipLaunchKernelGGL(k_count_rows, dim3(ceil_div(nnz, B)), dim3(B), 0, st, rowidx, nnz, cnt.get());
-    exclusive_scan(cnt.get(), rowptr, n, tmp, st);
-    hipLaunchKernelGGL(k_fill_int, dim3(std::max(1, ceil_div(n + 1, B))), dim3(B), 0, st, cnt.get(), 0,
-                       (long long)n + 1);
-    if (n > 0) {
+    exclusive_scan(cnt.get(), rowptr, m, tmp, st);
+    hipLaunchKernelGGL(k_fill_int, dim3(std::max(1, ceil_div(m + 1, B))), dim3(B), 0, st, cnt.get(), 0,
+                       (long long)m + 1);
+    if (n > 0)  // one thread per column appends its entries to their rows
         hipLaunchKernelGGL(k_fill_rows, dim3(ceil_div(n, B)), dim3(B), 0, st, colptr, rowidx, n, rowptr, cnt.get(),
                            colidx, src.get());
-        hipLaunchKernelGGL(k_sort_rows, dim3(ceil_div(n, B)), dim3(B), 0, st, n, rowptr, colidx, src.get(), cscval,
+    if (m > 0)  // one thread per row orders it by column
+        hipLaunchKernelGGL(k_sort_rows, dim3(ceil_div(m, B)), dim3(B), 0, st, m, rowptr, colidx, src.get(), cscval,
                            csrval, perm);
-    }
     SPAI_CHECK_LAUNCH();
     SPAI_HIP(hipStreamSynchronize(st));  // temporaries are freed on return
 }
@@ -1373,6 +1384,8 @@ void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector
                   const std::vector<long long> &h_nnz_off, bool allow_sell, hipStream_t st, const int *ptr2) {
     const int nsys = (int)h_row_off.size() - 1;
     require(nsys >= 1, "build_layout: no systems");
+    static std::atomic<unsigned long long> generations{0};
+    tm.generation = ++generations;
     const int n = h_row_off[(size_t)nsys];
     std::vector<int> hp((size_t)n + 1);
     SPAI_HIP(hipMemcpyAsync(hp.data(), ptr, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost, st));
@@ -1618,28 +1631,85 @@ void BiCGStab::drop_graph() {
     for (auto &k : graph_key_) k = nullptr;
 }
 
+void BiCGStab::forget_layout(unsigned long long generation) {
+    if (!graphs_.empty() && graph_key_[kGraphKeyGeneration] == (const void *)(size_t)generation) drop_graph();
+}
+
 Ctx BiCGStab::make_ctx(const TileMap &tm, int nsys) {
-    return Ctx{state_.get(), tm.sys_units.get(), tm.sys_nodes.get(), counter_.get(), partials_.get(),
+    return Ctx{st_, tm.sys_units.get(), tm.sys_nodes.get(), cnt_, part_,
                nsys,         hist_ptr_,          hist_stride_};
 }
 
+namespace {
+// the eleven work vectors: r, r*, p (two), y, a, s, z, t, and the second y
+// and z of deferred x updates
+void carve_vectors(double *v, size_t N, double *&r, double *&rs, double *(&p)[2], double *(&y)[2], double *&a,
+                   double *&s, double *(&z)[2], double *&t) {
+    r = v;
+    rs = v + N;
+    p[0] = v + 2 * N;
+    p[1] = v + 3 * N;
+    y[0] = v + 4 * N;
+    a = v + 5 * N;
+    s = v + 6 * N;
+    z[0] = v + 7 * N;
+    t = v + 8 * N;
+    y[1] = v + 9 * N;
+    z[1] = v + 10 * N;
+}
+
+size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+}  // namespace
+
+size_t BiCGStab::workspace_bytes(int n, int nsys, int nnodes) {
+    const size_t N = (size_t)std::max(n, 1);
+    // nine vectors (no second y and z: a bound workspace never defers x)
+    return round256(sizeof(double) * 9 * N) + round256(sizeof(SysState) * (size_t)nsys) +
+           round256(sizeof(int) * ((size_t)nsys + 2 + (size_t)nnodes)) +
+           round256(sizeof(double) * kMaxRed * (size_t)std::max(nnodes, 1));
+}
+
+void BiCGStab::bind_workspace(void *base, size_t bytes) {
+    require(base != nullptr && ((uintptr_t)base & 255) == 0, "bind_workspace: base must be 256-byte aligned");
+    drop_graph();
+    ws_base_ = base;
+    ws_bytes_ = bytes;
+    n_ = 0;
+    nsys_ = 0;
+    nnodes_ = -1;
+    nunits_ = 0;
+}
+
 void BiCGStab::ensure(int n, int nsys, int nunits, int nnodes) {
+    if (ws_base_) {
+        // a bound workspace: carve, never allocate (the same pointers for
+        // the same shape, so nothing is cached across shapes)
+        const size_t need = workspace_bytes(n, nsys, nnodes);
+        if (need > ws_bytes_)
+            throw Error(kErrUnsupported, "the solve needs " + std::to_string(need) + " bytes of workspace, " +
+                                             std::to_string(ws_bytes_) + " are available");
+        const size_t N = (size_t)std::max(n, 1);
+        char *p = (char *)ws_base_;
+        double *v = (double *)p;
+        p += round256(sizeof(double) * 9 * N);
+        carve_vectors(v, N, r_, rs_, p_, y_, a_, s_, z_, t_);
+        y_[1] = y_[0];  // unused: x updates are never deferred here
+        z_[1] = z_[0];
+        st_ = (SysState *)p;
+        p += round256(sizeof(SysState) * (size_t)nsys);
+        cnt_ = (int *)p;
+        p += round256(sizeof(int) * ((size_t)nsys + 2 + (size_t)nnodes));
+        part_ = (double *)p;
+        n_ = n;
+        nsys_ = nsys;
+        nnodes_ = nnodes;
+        nunits_ = nunits;
+        return;
+    }
     if (n != n_) {
         drop_graph();
         vec_.resize((size_t)11 * std::max(n, 1));
-        double *v = vec_.get();
-        size_t N = (size_t)std::max(n, 1);
-        r_ = v;
-        rs_ = v + N;
-        p_[0] = v + 2 * N;
-        p_[1] = v + 3 * N;
-        y_[0] = v + 4 * N;
-        a_ = v + 5 * N;
-        s_ = v + 6 * N;
-        z_[0] = v + 7 * N;
-        t_ = v + 8 * N;
-        y_[1] = v + 9 * N;
-        z_[1] = v + 10 * N;
+        carve_vectors(vec_.get(), (size_t)std::max(n, 1), r_, rs_, p_, y_, a_, s_, z_, t_);
         n_ = n;
     }
     if (nsys != nsys_ || nnodes != nnodes_) {
@@ -1647,6 +1717,9 @@ void BiCGStab::ensure(int n, int nsys, int nunits, int nnodes) {
         state_.resize((size_t)nsys);
         counter_.resize((size_t)nsys + 2 + nnodes);
         partials_.resize((size_t)kMaxRed * std::max(nnodes, 1));
+        st_ = state_.get();
+        cnt_ = counter_.get();
+        part_ = partials_.get();
         nsys_ = nsys;
         nnodes_ = nnodes;
     }
@@ -1659,13 +1732,13 @@ void BiCGStab::ensure(int n, int nsys, int nunits, int nnodes) {
 // Rebuilds the active-unit list from the system states (stream-ordered).
 void BiCGStab::compact(const TileMap &tm, int nsys, hipStream_t st, int *pub) {
     if (nsys <= kCompactBlock) {
-        hipLaunchKernelGGL(k_compact_small, dim3(1), dim3(kCompactBlock), 0, st, state_.get(), tm.unit_sys.get(),
-                           tm.sys_units.get(), nsys, tm.nunits, counter_.get() + nsys, tm.active.get(), pub);
+        hipLaunchKernelGGL(k_compact_small, dim3(1), dim3(kCompactBlock), 0, st, st_, tm.unit_sys.get(),
+                           tm.sys_units.get(), nsys, tm.nunits, cnt_ + nsys, tm.active.get(), pub);
     } else {
-        hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(kCompactBlock), 0, st, state_.get(), tm.sys_units.get(),
-                           nsys, tm.sys_base.get(), counter_.get() + nsys, pub);
-        hipLaunchKernelGGL(k_compact_fill, dim3(ceil_div(tm.nunits, 256)), dim3(256), 0, st, state_.get(),
-                           tm.unit_sys.get(), tm.sys_units.get(), tm.sys_base.get(), counter_.get() + nsys + 1,
+        hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(kCompactBlock), 0, st, st_, tm.sys_units.get(),
+                           nsys, tm.sys_base.get(), cnt_ + nsys, pub);
+        hipLaunchKernelGGL(k_compact_fill, dim3(ceil_div(tm.nunits, 256)), dim3(256), 0, st, st_,
+                           tm.unit_sys.get(), tm.sys_units.get(), tm.sys_base.get(), cnt_ + nsys + 1,
                            tm.nunits, tm.active.get());
     }
     SPAI_CHECK_LAUNCH();
@@ -1724,15 +1797,21 @@ void BiCGStab::enqueue_iteration(const OperatorView &A, const OperatorView &M, c
 }
 
 void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap &tm, int n, int nsys,
-                     const double *b, double *x, const SolveOptions &opt, hipStream_t st, SolveResult *res) {
+                     const double *b, double *x, const SolveOptions &opt, hipStream_t st, SolveResult *res,
+                     SysState *states_out) {
     require(n > 0 && nsys >= 1 && (A.idx || A.diag) && A.val && b && x, "bicgstab: bad arguments");
     require(tm.kind == kLayoutSell || A.ptr, "bicgstab: row tiles need a pointer array");
     require(tm.nunits > 0 && (int)tm.h_sys_units.size() == nsys + 1, "bicgstab: layout does not match the systems");
     require(opt.max_iter >= 1, "bicgstab: max_iter must be >= 1");
+    // a bound workspace allocates nothing: no graph, no history, no events,
+    // no pinned host memory, and the states go to the caller's host array
+    const bool bound = ws_base_ != nullptr;
+    require(!bound || (!opt.use_graph && !opt.record_history && states_out),
+            "bicgstab: a bound workspace supports neither graphs nor histories and needs a states array");
     ensure(n, nsys, tm.nunits, tm.nnodes);
     b_ = b;
     x_ = x;
-    defer_x_ = nsys > 1;
+    defer_x_ = nsys > 1 && !bound;
     if (opt.record_history) {
         hist_.reserve((size_t)nsys * (size_t)opt.max_iter);
         hist_ptr_ = hist_.get();
@@ -1741,33 +1820,41 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
         hist_ptr_ = nullptr;
         hist_stride_ = 0;
     }
-    hipEvent_t e0, e1;
-    SPAI_HIP(hipEventCreate(&e0));
-    SPAI_HIP(hipEventCreate(&e1));
-    SPAI_HIP(hipEventRecord(e0, st));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!bound) {
+        SPAI_HIP(hipEventCreate(&e0));
+        SPAI_HIP(hipEventCreate(&e1));
+        SPAI_HIP(hipEventRecord(e0, st));
+    }
     Ctx c = make_ctx(tm, nsys);
-    hipLaunchKernelGGL(k_state_init, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys, opt.max_iter,
-                       counter_.get(), tm.nnodes);
+    hipLaunchKernelGGL(k_state_init, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, st_, nsys, opt.max_iter,
+                       cnt_, tm.nnodes);
     {
-        OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, s_, t_, y_[0], z_[0], y_[1], z_[1], opt.rel_tol, opt.abs_tol, 0};
+        OpInit op{x,     b,         r_,          rs_,         p_[0],      p_[1], a_, s_, t_, y_[0], z_[0],
+                  y1(),  z1(),      opt.rel_tol, opt.abs_tol, 0};
         launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
     }
     SPAI_CHECK_LAUNCH();
     // A batch walks the active-unit list, rebuilt after every host check; a
     // single system walks every unit.
     const bool use_list = nsys > 1;
-    for (auto &e : check_ev_)
-        if (!e) SPAI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    if (!h_pub_) {
-        SPAI_HIP(hipHostMalloc((void **)&h_pub_, 64, hipHostMallocMapped | hipHostMallocCoherent));
-        SPAI_HIP(hipHostGetDevicePointer((void **)&d_pub_, h_pub_, 0));
+    if (!bound) {
+        for (auto &e : check_ev_)
+            if (!e) SPAI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        if (!h_pub_) {
+            SPAI_HIP(hipHostMalloc((void **)&h_pub_, 64, hipHostMallocMapped | hipHostMallocCoherent));
+            SPAI_HIP(hipHostGetDevicePointer((void **)&d_pub_, h_pub_, 0));
+        }
     }
     int active = nsys;
     int units = tm.nunits;
     int checks = 0;
-    int *h_counts = h_pub_ + 4;  // copied counts (the graphs publish into h_pub_[0..1])
+    int local_counts[2] = {0, 0};
+    // copied counts (the graphs publish into h_pub_[0..1])
+    int *h_counts = bound ? local_counts : h_pub_ + 4;
     auto read_counts = [&]() {
-        SPAI_HIP(hipMemcpyAsync(h_counts, counter_.get() + nsys, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+        SPAI_HIP(hipMemcpyAsync(h_counts, cnt_ + nsys, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
         SPAI_HIP(hipStreamSynchronize(st));
         ++checks;
         active = h_counts[0];
@@ -1785,9 +1872,10 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
             units = tm.nunits;
             if (opt.use_graph) {
                 const int gi = std::max(2, opt.graph_iters + (opt.graph_iters & 1));
-                const void *key[kGraphKey] = {A.val, A.idx, A.off, A.diag, M.val, M.idx, M.off, M.diag, M.blk,
-                                              (const void *)(size_t)M.fp32, tm.unit_row.get(), b, x, hist_ptr_,
-                                              (const void *)(size_t)defer_x_};
+                const void *key[kGraphKey] = {A.ptr, A.val, A.idx, A.off, A.diag, M.ptr, M.val, M.idx, M.off,
+                                              M.diag, M.blk, (const void *)(size_t)M.fp32,
+                                              (const void *)(size_t)tm.generation, tm.unit_row.get(), b, x,
+                                              hist_ptr_, (const void *)(size_t)defer_x_};
                 // One graph runs ahead of the host check: graph k + 1 is
                 // queued before the counts graph k published are read, so the
                 // device does not wait for the host between graphs.  The
@@ -1829,7 +1917,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                     }
                     if (use_list) compact(tm, nsys, cap, d_pub_);
                     else
-                        hipLaunchKernelGGL(k_publish_counts, dim3(1), dim3(64), 0, cap, counter_.get() + nsys,
+                        hipLaunchKernelGGL(k_publish_counts, dim3(1), dim3(64), 0, cap, cnt_ + nsys,
                                            d_pub_);
                     SPAI_HIP(hipStreamEndCapture(cap, &graph));
                     SPAI_HIP(hipStreamDestroy(cap));
@@ -1885,10 +1973,12 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
             }
         };
         // x of a system that stopped in a deferring K1 still owes one update
+        // (only batches defer)
         auto flush_x = [&]() {
+            if (!defer_x_) return;
             OpFlushX op{x, y_[1], z_[1], 0.0, 0.0};
             launch_units<false, double>(A, tm, c, op, nullptr, 0, st, flip());
-            hipLaunchKernelGGL(k_clear_xdefer, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys);
+            hipLaunchKernelGGL(k_clear_xdefer, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, st_, nsys);
             SPAI_CHECK_LAUNCH();
         };
         iterate();
@@ -1897,15 +1987,16 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
         for (int round = 0; opt.true_residual_restart && round < 4; ++round) {
             OpTrueResidual tr{x, b};
             launch_units<true, double>(A, tm, c, tr, nullptr, 0, st, flip());
-            hipLaunchKernelGGL(k_check_true, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys,
-                               counter_.get());
+            hipLaunchKernelGGL(k_check_true, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, st_, nsys,
+                               cnt_);
             SPAI_CHECK_LAUNCH();
             read_counts();
             if (active == 0) {
                 true_current = true;
                 break;
             }
-            OpInit op{x, b, r_, rs_, p_[0], p_[1], a_, s_, t_, y_[0], z_[0], y_[1], z_[1], opt.rel_tol, opt.abs_tol, 1};
+            OpInit op{x,    b,    r_,          rs_,         p_[0], p_[1], a_, s_, t_, y_[0], z_[0],
+                      y1(), z1(), opt.rel_tol, opt.abs_tol, 1};
             launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
             SPAI_CHECK_LAUNCH();
             done_iters = 0;
@@ -1917,19 +2008,36 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
             launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
             SPAI_CHECK_LAUNCH();
         }
-        SPAI_HIP(hipEventRecord(e1, st));
-        SPAI_HIP(hipEventSynchronize(e1));
+        if (!bound) {
+            SPAI_HIP(hipEventRecord(e1, st));
+            SPAI_HIP(hipEventSynchronize(e1));
+        } else {
+            SPAI_HIP(hipStreamSynchronize(st));
+        }
     } catch (...) {
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
         throw;
     }
     float ms = 0.0f;
-    SPAI_HIP(hipEventElapsedTime(&ms, e0, e1));
-    SPAI_HIP(hipEventDestroy(e0));
-    SPAI_HIP(hipEventDestroy(e1));
-    h_states_.resize((size_t)nsys);
-    SPAI_HIP(hipMemcpy(h_states_.data(), state_.get(), sizeof(SysState) * nsys, hipMemcpyDeviceToHost));
+    if (!bound) {
+        SPAI_HIP(hipEventElapsedTime(&ms, e0, e1));
+        SPAI_HIP(hipEventDestroy(e0));
+        SPAI_HIP(hipEventDestroy(e1));
+    } else {
+        // no events in a bound workspace: host time from the first enqueue
+        ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    const SysState *states = states_out;
+    if (states_out) {
+        SPAI_HIP(hipMemcpyAsync(states_out, st_, sizeof(SysState) * nsys, hipMemcpyDeviceToHost, st));
+        SPAI_HIP(hipStreamSynchronize(st));
+    }
+    if (!bound) {
+        h_states_.resize((size_t)nsys);
+        SPAI_HIP(hipMemcpy(h_states_.data(), st_, sizeof(SysState) * nsys, hipMemcpyDeviceToHost));
+        states = h_states_.data();
+    }
     h_hist_.clear();
     if (hist_ptr_) {
         h_hist_.resize((size_t)nsys * hist_stride_);
@@ -1941,7 +2049,8 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
         res->iterations_max = 0;
         res->iterations_sum = 0;
         res->all_converged = 1;
-        for (auto &S : h_states_) {
+        for (int q = 0; q < nsys; ++q) {
+            const SysState &S = states[q];
             res->iterations_max = std::max(res->iterations_max, S.iters);
             res->iterations_sum += S.iters;
             if (!(S.status == kConverged || S.status == kConvergedAtStart)) res->all_converged = 0;
@@ -1958,15 +2067,37 @@ int BiCGStab::history(int s, double *out, int maxlen) const {
     return k;
 }
 
+double *BiCGStab::seed_timing_state(const OperatorView &A, const TileMap &tm, int n, int nsys, int max_iter,
+                                     hipStream_t st) {
+    tvec_.reserve((size_t)2 * n);
+    double *tb = tvec_.get(), *tx = tvec_.get() + n;
+    hipLaunchKernelGGL(k_fill_double, dim3(ceil_div(n, 256)), dim3(256), 0, st, tb, 1.0, (long long)n);
+    hipLaunchKernelGGL(k_fill_double, dim3(ceil_div(n, 256)), dim3(256), 0, st, tx, 0.0, (long long)n);
+    hipLaunchKernelGGL(k_state_init, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, st_, nsys, max_iter,
+                       cnt_, tm.nnodes);
+    Ctx c = make_ctx(tm, nsys);
+    OpInit op{tx, tb, r_, rs_, p_[0], p_[1], a_, s_, t_, y_[0], z_[0], y1(), z1(), 0.0, 1e-300, 0};
+    launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
+    SPAI_CHECK_LAUNCH();
+    return tx;
+}
+
 float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView &M, const TileMap &tm, int n,
                             int nsys, int reps, hipStream_t st) {
     require(which >= 0 && which <= 5, "time_kernel: which must be 0..5");
     require((which != 1 && which != 3 && which != 5) || M.val, "time_kernel: no preconditioner");
+    defer_x_ = nsys > 1;  // as the solve runs it
     ensure(n, nsys, tm.nunits, tm.nnodes);
+    hist_ptr_ = nullptr;
+    hist_stride_ = 0;
     Ctx c = make_ctx(tm, nsys);
     const int parity = 0;  // the y and z buffers of an even iteration
-    hipLaunchKernelGGL(k_state_activate, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys,
-                       counter_.get(), tm.nnodes);
+    // The kernels run on the solver's own vectors and states, seeded as a
+    // solve from x0 = 0, b = 1 starts (K1 updates the solver-owned x, never a
+    // caller's); every system stays active.
+    double *tx = seed_timing_state(A, tm, n, nsys, 1 << 30, st);
+    hipLaunchKernelGGL(k_state_activate, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, st_, nsys,
+                       cnt_, tm.nnodes);
     hipEvent_t e0, e1;
     SPAI_HIP(hipEventCreate(&e0));
     SPAI_HIP(hipEventCreate(&e1));
@@ -1982,11 +2113,12 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
             else launch_units<true, double>(M, tm, c, op, nullptr, 0, st, flip());
         } break;
         case 1: {
-            const K1Args k1{x_, r_, p_[1], y_[1], s_, t_, p_[0], a_, z_[1], y_[0], z_[0]};
+            // an odd iteration's fused K1 (a batch's two-update form)
+            const K1Args k1{tx, r_, p_[0], ybuf(1), s_, t_, p_[1], a_, zbuf(1), ybuf(0), zbuf(0)};
             if (M.fp32) {
-                launch_k1<true, float, kYProduct>(M, tm, c, kXTwo, k1, nullptr, 0, st, flip());
+                launch_k1<true, float, kYProduct>(M, tm, c, x_mode(1), k1, nullptr, 0, st, flip());
             } else {
-                launch_k1<true, double, kYProduct>(M, tm, c, kXTwo, k1, nullptr, 0, st, flip());
+                launch_k1<true, double, kYProduct>(M, tm, c, x_mode(1), k1, nullptr, 0, st, flip());
             }
         } break;
         case 2: {
@@ -2031,22 +2163,12 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
     ensure(n, nsys, tm.nunits, tm.nnodes);
     hist_ptr_ = nullptr;
     hist_stride_ = 0;
-    tvec_.reserve((size_t)2 * n);
-    double *tb = tvec_.get(), *tx = tvec_.get() + n;
-    hipLaunchKernelGGL(k_fill_double, dim3(ceil_div(n, 256)), dim3(256), 0, st, tb, 1.0, (long long)n);
-    hipLaunchKernelGGL(k_fill_double, dim3(ceil_div(n, 256)), dim3(256), 0, st, tx, 0.0, (long long)n);
-    // no threshold is reached: every system stays active for the timed iterations
-    hipLaunchKernelGGL(k_state_init, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, state_.get(), nsys, reps + 3,
-                       counter_.get(), tm.nnodes);
     Ctx c = make_ctx(tm, nsys);
-    {
-        OpInit op{tx, tb, r_, rs_, p_[0], p_[1], a_, s_, t_, y_[0], z_[0], y_[1], z_[1], 0.0, 1e-300, 0};
-        launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
-    }
-    SPAI_CHECK_LAUNCH();
+    // no threshold is reached: every system stays active for the timed iterations
+    double *tx = seed_timing_state(A, tm, n, nsys, reps + 3, st);
     const double *b_keep = b_;
     double *x_keep = x_;
-    b_ = tb;
+    b_ = tvec_.get();
     x_ = tx;
     constexpr int K = 5;  // event slots per iteration: K1 (or its halves), K2, K3, K4
     std::vector<hipEvent_t> ev((size_t)reps * K + 1);

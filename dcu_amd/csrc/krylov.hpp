@@ -66,6 +66,10 @@ constexpr int kSellWaves = SPAI_SELL_WAVES;  // wavefronts per SELL workgroup
 constexpr int kSlicesPerUnit = kSellWaves * kSellSlicesPerWave;
 
 struct TileMap {
+    // distinct for every build_layout call in the process: captured solver
+    // graphs are keyed by it, so a graph is never replayed on the device
+    // arrays of another (or a rebuilt) layout that happen to reuse addresses
+    unsigned long long generation = 0;
     int kind = kLayoutTiles;
     int target_nnz = 0;
     int nunits = 0;
@@ -96,10 +100,12 @@ struct TileMap {
     long long value_len(long long nnz) const { return kind == kLayoutSell ? sell_len : nnz; }
 };
 
-// Device CSC -> CSR (rows ordered by column) plus perm[csc_pos] = csr_pos.
-// Deterministic: counts by atomics, rows sorted by column afterwards.
-void csc_to_csr_device(int n, int nnz, const int *colptr, const int *rowidx, const double *cscval, int *rowptr,
-                       int *colidx, double *csrval, int *perm, hipStream_t st);
+// Device CSC -> CSR of an m x n matrix (m rows: rowptr holds m + 1 entries;
+// n columns: colptr holds n + 1), rows ordered by column, plus
+// perm[csc_pos] = csr_pos.  Deterministic: counts by atomics, rows sorted by
+// column afterwards.
+void csc_to_csr_device(int m, int n, int nnz, const int *colptr, const int *rowidx, const double *cscval,
+                       int *rowptr, int *colidx, double *csrval, int *perm, hipStream_t st);
 
 // Builds the row layout over a CSR-like pointer/index pair (`ptr` n+1 and
 // `idx` entries, on the device) for systems with row offsets h_row_off
@@ -183,16 +189,33 @@ public:
     ~BiCGStab();
     // Solves every system of the batch: A x = b with right preconditioner M,
     // x holds x0 on entry.  Rows of system s are [row_off[s], row_off[s+1]).
+    // states_out (host, nsys entries, nullable) receives the per-system
+    // states; a solver bound to a workspace requires it.
     void solve(const OperatorView &A, const OperatorView &M, const TileMap &tiles, int n, int nsys, const double *b,
-               double *x, const SolveOptions &opt, hipStream_t st, SolveResult *res);
+               double *x, const SolveOptions &opt, hipStream_t st, SolveResult *res, SysState *states_out = nullptr);
+    // Binds the solver to caller-owned device memory (256-byte aligned)
+    // instead of allocations of its own: the work vectors, states, counters
+    // and reduction partials of every later solve are carved from
+    // [base, base + bytes), and a solve that does not fit is refused.  Such
+    // a solver allocates nothing: it never captures graphs, records no
+    // history, creates no events and never defers x updates (batches
+    // included; the results are bit-identical to the deferring form).
+    void bind_workspace(void *base, size_t bytes);
+    // Device bytes a bound workspace needs for a solve of n rows, nsys
+    // systems and nnodes reduction tree nodes.
+    static size_t workspace_bytes(int n, int nsys, int nnodes);
+    // Drops the captured graphs of the layout `generation` (a system that
+    // is destroyed or laid out again).
+    void forget_layout(unsigned long long generation);
     // Per-system results of the last solve (host copies).
     const std::vector<SysState> &states() const { return h_states_; }
     // ||r_k||_2 after iteration k = 1.. of system s in the last solve that
     // recorded a history; returns the number of values written.
     int history(int s, double *out, int maxlen) const;
     // Average device time of one launch of a single kernel repeated `reps`
-    // times on the solver's stream (which: 0 bare A*x, 5 bare M*x, 1..4 K1..K4
-    // with the vectors and scalars of the last solve, every system active).
+    // times on the solver's stream (which: 0 bare A*x, 5 bare M*x, 1..4
+    // K1..K4), on the solver's own vectors and states seeded as a solve from
+    // x0 = 0, b = 1 starts, every system active.
     float time_kernel(int which, const OperatorView &A, const OperatorView &M, const TileMap &tiles, int n, int nsys,
                       int reps, hipStream_t st);
     // Average device time of each iteration kernel over `reps` consecutive
@@ -206,6 +229,10 @@ public:
 
 private:
     void ensure(int n, int nsys, int nunits, int nnodes);
+    // b = 1 and x = 0 in tvec_, the states and work vectors of a solve from
+    // them (OpInit); returns x
+    double *seed_timing_state(const OperatorView &A, const TileMap &tm, int n, int nsys, int max_iter,
+                              hipStream_t st);
     void compact(const TileMap &tm, int nsys, hipStream_t st, int *pub = nullptr);
     Ctx make_ctx(const TileMap &tm, int nsys);
     void enqueue_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tiles, int parity, int grid,
@@ -218,6 +245,12 @@ private:
     DeviceArray<SysState> state_;
     DeviceArray<int> counter_;      // [nsys] active systems, [nsys + 1] active units, [nsys + 2 + node] tree nodes
     DeviceArray<double> tvec_;      // b = 1 and x = 0 of time_iteration
+    // what the kernels use: the arrays above, or a bound workspace
+    SysState *st_ = nullptr;
+    int *cnt_ = nullptr;
+    double *part_ = nullptr;
+    void *ws_base_ = nullptr;
+    size_t ws_bytes_ = 0;
     std::vector<SysState> h_states_;
     // y and z by iteration parity: x takes two iterations' updates in every
     // other K1, so the previous iteration's y and z must survive one more
@@ -233,7 +266,8 @@ private:
         hipGraphExec_t exec;
     };
     std::vector<GraphEntry> graphs_;
-    static constexpr int kGraphKey = 15;  // operand pointers the graphs were captured with
+    static constexpr int kGraphKey = 18;  // operands and layout the graphs were captured with
+    static constexpr int kGraphKeyGeneration = 12;  // the slot holding the layout's generation
     const void *graph_key_[kGraphKey] = {};
     hipEvent_t check_ev_[2] = {nullptr, nullptr};  // host checks of the graph loop
     int *h_pub_ = nullptr, *d_pub_ = nullptr;      // counts published by the graphs (host-coherent)
@@ -252,6 +286,9 @@ private:
     // then overwritten, row by row, as before the double buffers)
     double *ybuf(int parity) const { return y_[defer_x_ ? parity : 0]; }
     double *zbuf(int parity) const { return z_[defer_x_ ? parity : 0]; }
+    // the second buffers, for OpInit to clear (null when not deferring)
+    double *y1() const { return defer_x_ ? y_[1] : nullptr; }
+    double *z1() const { return defer_x_ ? z_[1] : nullptr; }
     DeviceArray<double> hist_;
     double *hist_ptr_ = nullptr;
     int hist_stride_ = 0;

@@ -232,6 +232,50 @@ def test_bicgstab_rho_by_linearity_matches_scipy_golden():
         O.set_rho_mode(0)
 
 
+def test_true_residual_matches_numpy():
+    # oracle_true_residual judges every "true residual < 1e-8" GPU assertion:
+    # it must equal numpy's ||b - A x||_2 on the golden systems (x = 0, the
+    # scipy iterates, random x), and see a planted error in x
+    rng = np.random.default_rng(11)
+    for name, n, cp, ri, va, m, b, xs, _ in _golden_systems():
+        A = sp.csc_matrix((va, ri, cp), shape=(n, n))
+        Acsr = O.csc2csr(n, cp, ri, va)[:3]
+        for x in (np.zeros(n), xs[0], xs[-1], rng.standard_normal(n)):
+            ref = np.linalg.norm(b - A @ x)
+            got = O.true_residual(Acsr, b, x)
+            assert abs(got - ref) <= 1e-13 * max(ref, np.linalg.norm(b)), (name, got, ref)
+        assert O.true_residual(Acsr, b, np.zeros(n)) == pytest.approx(np.linalg.norm(b), rel=1e-15)
+        # the exact solution has a residual at rounding level; one entry off
+        # by d moves it to |d| ||A(:, i)||
+        xe = spl.spsolve(A.tocsc(), b)
+        assert O.true_residual(Acsr, b, xe) <= 1e-13 * np.linalg.norm(b)
+        i = int(rng.integers(n))
+        xb = xe.copy()
+        xb[i] += 1e-3
+        col = np.linalg.norm(A[:, i].toarray())
+        assert O.true_residual(Acsr, b, xb) == pytest.approx(1e-3 * col, rel=1e-6), name
+
+
+def test_residual_history_matches_scipy_iterates():
+    # ||r_k|| after iteration k (the recursive residual the stop test reads)
+    # equals ||b - A x_k|| of scipy's k-th iterate to rounding; the history
+    # has one entry per executed iteration
+    for name, n, cp, ri, va, m, b, xs, _ in _golden_systems():
+        A = sp.csc_matrix((va, ri, cp), shape=(n, n))
+        Acsr = O.csc2csr(n, cp, ri, va)[:3]
+        Mcsr = O.csc2csr(n, cp, ri, m)[:3]
+        x, it, info, hist = O.bicgstab(Acsr, Mcsr, b, np.zeros(n), 0.0, 7, history=True)
+        assert it == 7 and len(hist) == 6
+        for k in range(1, 7):
+            ref = np.linalg.norm(b - A @ xs[k - 1])
+            assert abs(hist[k - 1] - ref) <= 1e-10 * np.linalg.norm(b) + 1e-8 * ref, (name, k, hist[k - 1], ref)
+        assert hist[-1] == info[0]
+        # on convergence the last entry is the one below the threshold
+        tol = 1e-6 * np.linalg.norm(b)
+        x, it, info, hist = O.bicgstab(Acsr, Mcsr, b, np.zeros(n), tol, 1000, history=True)
+        assert info[2] == 1 and len(hist) == it and hist[-1] < tol and np.all(hist[:-1] >= tol)
+
+
 def test_bicgstab_exact_inverse_one_iteration():
     rng = np.random.default_rng(2)
     Ad = rng.standard_normal((20, 20)) + 20 * np.eye(20)
