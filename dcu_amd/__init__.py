@@ -37,6 +37,14 @@ PATTERN_AT = 0
 PATTERN_I_PLUS_A = 1
 PATTERN_I_PLUS_A_PLUS_AT = 2
 
+OK = 0
+ERR_INVALID = -1
+ERR_HIP = -2
+ERR_NOMEM = -3
+ERR_UNSUPPORTED = -4
+ERR_BREAKDOWN = -5
+ERR_NOT_BUILT = -6
+
 SOLVE_RUNNING = 0
 SOLVE_CONVERGED = 1
 SOLVE_MAX_ITER = 2
@@ -51,7 +59,7 @@ C_SYMBOLS = [
     "spai_system_destroy", "spai_system_get_info", "spai_system_row_offsets",
     "spai_system_set_values", "spai_system_build", "spai_system_get_m",
     "spai_system_get_row_set_sizes", "spai_solve_options_default", "spai_system_solve",
-    "spai_system_outcomes", "spai_system_residual_history", "spai_system_adjoint", "spai_system_time_spmv",
+    "spai_system_outcomes", "spai_system_solve_noalloc", "spai_system_residual_history", "spai_system_adjoint", "spai_system_time_spmv",
     "spai_system_time_iteration", "spai_system_layout", "spai_system_set_pattern", "spai_system_m_pattern",
     "spai_system_quality", "spai_system_product",
     "spai_tile_shape_count", "spai_tile_shape_get", "spai_system_autotune", "spai_system_set_tile_shapes",
@@ -184,6 +192,8 @@ def load_library() -> ctypes.CDLL:
         "spai_solve_options_default": (None, [ctypes.POINTER(SolveOptions)]),
         "spai_system_solve": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.POINTER(SolveOptions),
                                              ctypes.POINTER(SolveReport)]),
+        "spai_system_solve_noalloc": (ctypes.c_int, [vp, vp, vp, ctypes.POINTER(SolveOptions),
+                                                     ctypes.POINTER(SolveReport)]),
         "spai_system_outcomes": (ctypes.c_int, [vp, ctypes.POINTER(Outcome)]),
         "spai_system_residual_history": (ctypes.c_int, [vp, ctypes.c_int, vp, ctypes.c_int]),
         "spai_system_adjoint": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_int, ctypes.POINTER(SolveOptions),
@@ -588,6 +598,16 @@ class System:
         rep = SolveReport()
         _check(self._lib.spai_system_solve(self._h, ctypes.c_void_p(int(b_ptr)), ctypes.c_void_p(int(x_ptr)), 1,
                                            ctypes.byref(opt), ctypes.byref(rep)))
+        return _report(rep)
+
+    def solve_noalloc(self, b_ptr: int, x_ptr: int, **kw) -> dict:
+        """spai_system_solve_noalloc on device pointers: the solve with its
+        work memory in the library's static device scratch (no device
+        allocation); no graphs or histories."""
+        opt = _options(**kw)
+        rep = SolveReport()
+        _check(self._lib.spai_system_solve_noalloc(self._h, ctypes.c_void_p(int(b_ptr)), ctypes.c_void_p(int(x_ptr)),
+                                                   ctypes.byref(opt), ctypes.byref(rep)))
         return _report(rep)
 
     def adjoint(self, x: np.ndarray, g: np.ndarray, **kw) -> Tuple[np.ndarray, np.ndarray, dict]:

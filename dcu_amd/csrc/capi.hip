@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -120,6 +121,40 @@ void validate_host_csc(const CSC_Matrix *A) {
 }
 }  // namespace
 
+// Device scratch of the allocation-free solve (spai_system_solve_noalloc):
+// the solver's work vectors, per-system states, counters and reduction
+// partials for callers that cannot allocate device memory.  Each device the
+// library runs on holds its own copy from its first context until the
+// process ends; every allocation-free solve on a device shares that copy.
+// In host code `scratch` is only a stand-in: its device address comes from
+// hipGetSymbolAddress (scratch_address below), once per device.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_rc[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current; looked
+// up on the first call for each device and kept.
+hipError_t scratch_address(int device, void **out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_done[device]) {
+        void *ptr = nullptr;
+        g_scratch_rc[device] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        g_scratch_ptr[device] = g_scratch_rc[device] == hipSuccess ? ptr : nullptr;
+        g_scratch_done[device] = true;
+    }
+    *out = g_scratch_ptr[device];
+    return g_scratch_rc[device];
+}
+}  // namespace
+
 struct spai_context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -132,6 +167,10 @@ struct spai_context {
     SpaiBuilder builder;
     BiCGStab solver;
     BiCGStab adjoint_solver;
+    // the allocation-free solve: a solver bound to this device's scratch
+    BiCGStab noalloc_solver;
+    void *scratch = nullptr;
+    std::string scratch_error;  // why the scratch address is missing
 };
 
 struct spai_system {
@@ -160,7 +199,8 @@ struct spai_system {
     DeviceArray<double> stage_b, stage_x, stage_g, stage_dA, stage_db;
     float setup_ms = 0.0f;
     int tile_shape[kTunedTiers] = {0, 0, 0};  // register-tier group shapes of the next builds
-    std::vector<SysState> last_states;
+    std::vector<SysState> last_states;  // nsys entries from creation on
+    bool have_states = false;           // a solve or adjoint has filled them
     const BiCGStab *last_solver = nullptr;
 };
 
@@ -266,6 +306,7 @@ spai_system *create_from_host(spai_context *ctx, const std::vector<const CSC_Mat
     SPAI_HIP(hipMemcpyAsync(S->cscval.get(), va.data(), sizeof(double) * (size_t)S->nnz, hipMemcpyHostToDevice, st));
     finish_setup(S.get(), st);
     S->setup_ms = tm.stop();
+    S->last_states.resize((size_t)S->nsys);
     return S.release();
 }
 
@@ -365,6 +406,16 @@ int spai_context_create(int device, spai_context **out) {
         if (c->name.empty()) c->name = prop.gcnArchName;
         c->cus = prop.multiProcessorCount;
         SPAI_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        // set-up of the allocation-free solve: this device's scratch address
+        // (a failed lookup only disables that entry point)
+        void *ptr = nullptr;
+        const hipError_t rc = scratch_address(device, &ptr);
+        if (rc == hipSuccess && ptr) {
+            c->scratch = ptr;
+            c->noalloc_solver.bind_workspace(ptr, kNoAllocScratchBytes);
+        } else {
+            c->scratch_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(rc);
+        }
         *out = c.release();
     });
 }
@@ -450,6 +501,7 @@ int spai_system_create_device(spai_context *ctx, const CSC_Matrix *dev_A, spai_s
                                 hipMemcpyDeviceToDevice, st));
         finish_setup(S.get(), st);
         S->setup_ms = tm.stop();
+        S->last_states.resize(1);
         *out = S.release();
         ctx->live_systems++;
     });
@@ -919,6 +971,7 @@ int spai_system_solve(spai_system *sys, const double *b, double *x, int on_devic
         SolveResult res;
         sys->ctx->solver.solve(A, M, sys->tiles, sys->n, sys->nsys, db, dx, to_options(opt), st, &res);
         sys->last_states = sys->ctx->solver.states();
+        sys->have_states = true;
         sys->last_solver = &sys->ctx->solver;
         if (!on_device) {
             SPAI_HIP(hipMemcpyAsync(x, dx, sizeof(double) * sys->n, hipMemcpyDeviceToHost, st));
@@ -928,9 +981,39 @@ int spai_system_solve(spai_system *sys, const double *b, double *x, int on_devic
     });
 }
 
+int spai_system_solve_noalloc(spai_system *sys, const double *b, double *x, const spai_solve_options *opt,
+                              spai_solve_report *report) {
+    if (!sys || !b || !x) return fail(SPAI_ERR_INVALID, "sys/b/x is NULL");
+    spai_solve_options dflt;
+    spai_solve_options_default(&dflt);
+    if (!opt) opt = &dflt;
+    if (opt->use_graph || opt->record_history)
+        return fail(SPAI_ERR_INVALID, "the allocation-free solve runs without graphs and residual histories");
+    if (opt->precondition && sys->m_precision < 0) return fail(SPAI_ERR_NOT_BUILT, "M has not been built");
+    spai_context *ctx = sys->ctx;
+    if (!ctx->scratch) return fail(SPAI_ERR_HIP, "no device scratch: " + ctx->scratch_error);
+    const size_t need = BiCGStab::workspace_bytes(sys->n, sys->nsys, sys->tiles.nnodes);
+    if (need > kNoAllocScratchBytes)
+        return fail(SPAI_ERR_UNSUPPORTED, "the allocation-free solve needs " + std::to_string(need) +
+                                              " bytes of scratch; the device scratch holds " +
+                                              std::to_string(kNoAllocScratchBytes) + " (64 MiB)");
+    return guarded([&] {
+        SPAI_HIP(hipSetDevice(ctx->device));
+        OperatorView A = a_view(sys);
+        OperatorView M = opt->precondition ? m_view(sys, false) : OperatorView{};
+        SolveResult res;
+        ctx->noalloc_solver.solve(A, M, sys->tiles, sys->n, sys->nsys, b, x, to_options(opt), ctx->stream, &res,
+                                  sys->last_states.data());
+        sys->have_states = true;
+        sys->last_solver = &ctx->noalloc_solver;
+        fill_report(report, res);
+    });
+}
+
 int spai_system_outcomes(spai_system *sys, spai_system_outcome *out) {
     if (!sys || !out) return fail(SPAI_ERR_INVALID, "sys/out is NULL");
-    if ((int)sys->last_states.size() != sys->nsys) return fail(SPAI_ERR_INVALID, "no solve has run");
+    if (!sys->have_states || (int)sys->last_states.size() != sys->nsys)
+        return fail(SPAI_ERR_INVALID, "no solve has run");
     for (int s = 0; s < sys->nsys; ++s) {
         const SysState &S = sys->last_states[(size_t)s];
         out[s].status = S.status;
@@ -982,6 +1065,7 @@ int spai_system_adjoint(spai_system *sys, const double *x, const double *g, doub
         SolveResult res;
         sys->ctx->adjoint_solver.solve(At, Mt, sys->tiles_t, sys->n, sys->nsys, dg, ddb, to_options(opt), st, &res);
         sys->last_states = sys->ctx->adjoint_solver.states();
+        sys->have_states = true;
         sys->last_solver = &sys->ctx->adjoint_solver;
         adjoint_scatter(sys->n, sys->colptr.get(), sys->rowidx.get(), ddb, dx, ddA, st);
         if (!on_device) {

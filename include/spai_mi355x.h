@@ -233,6 +233,23 @@ typedef struct {
  * `on_device` = 1: b and x are device pointers (n doubles each). */
 int spai_system_solve(spai_system *sys, const double *b, double *x, int on_device, const spai_solve_options *opt,
                       spai_solve_report *report);
+/* Allocation-free variant of spai_system_solve.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc/hipFree or other allocating routine:
+ * the solver's work vectors, per-system states, counters and reduction
+ * partials live in a static 64 MiB device array, one per device, looked up
+ * when the device's first context is created and held until the process
+ * ends.  b and x are device pointers (x holds x0 on entry).  Limit: 72 bytes
+ * per row, 132 per system and 36 per reduction tree node (about 1 node per
+ * 450 rows), plus under 1 KiB of alignment, must fit in 64 MiB: about
+ * 930,000 rows for one system; larger systems return SPAI_ERR_UNSUPPORTED.  opt->use_graph and opt->record_history must be 0
+ * (SPAI_ERR_INVALID otherwise); x updates are never deferred, so results
+ * equal spai_system_solve's bit for bit.  The report's ms is host time.
+ * Every call on one device shares that device's array: calls that overlap
+ * on one device (from any context or thread) race. */
+int spai_system_solve_noalloc(spai_system *sys, const double *b, double *x, const spai_solve_options *opt,
+                              spai_solve_report *report);
 /* Per-system outcome of the last solve (nsys entries). */
 int spai_system_outcomes(spai_system *sys, spai_system_outcome *out);
 /* ||r_k||_2 after iterations k = 1.. of system s in the last solve or adjoint
