@@ -1295,12 +1295,18 @@ spai_context *default_context() {
 }
 }  // namespace
 
+// The reference takes devices 0 .. numDCUs - 1 (src/spai_multi_dcu.cpp:1309-1342)
+// and exits when there is none; with one device that is device 0, which is
+// left current.
 MultiDCU_Context *initMultiDCU(int requestedDCUs) {
-    (void)requestedDCUs;
+    (void)requestedDCUs;  // one device
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
+        std::printf("initMultiDCU: no HIP device found\n");
+        std::exit(EXIT_FAILURE);
+    }
     auto *c = new MultiDCU_Context();
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    if (spai_context_create(dev, &c->ctx) != SPAI_OK) die("initMultiDCU");
+    if (spai_context_create(0, &c->ctx) != SPAI_OK) die("initMultiDCU");
     c->numDCUs = 1;
     return c;
 }

@@ -16,6 +16,9 @@
 //               --graph                    replay iterations from a hipGraph
 //               --autotune                 time the tile-shape candidates of each
 //                                          size bucket first and build with the fastest
+//               --device D                 HIP device to run on (the single-device
+//                                          driver's flag, src/spai_single_dcu.cpp:2697-2717:
+//                                          out of range means device 0)
 //
 // With --matrix the solve defaults follow the reference driver
 // (src/spai_multi_dcu.cpp:1920-1934: b = 1, x0 = 1, TOL = 1e-7 absolute,
@@ -28,6 +31,8 @@
 #include <string>
 #include <vector>
 
+#include <hip/hip_runtime.h>
+
 #include "spai_mi355x.h"
 
 static void usage(const char *prog) {
@@ -35,7 +40,7 @@ static void usage(const char *prog) {
     std::printf("           [--matrix <path> | --gen <1..5> [--seed S]] [--precondition-only]\n");
     std::printf("           [--precision fp64|mixed] [--tol T | --rel-tol R] [--max-iter K]\n");
     std::printf("           [--x0 ones|zero] [--rhs ones|problem] [--graph] [--pattern a|at|i+a|i+a+at]\n");
-    std::printf("           [--autotune] [--help]\n");
+    std::printf("           [--autotune] [--device D] [--help]\n");
 }
 
 static void check(int st, const char *what) {
@@ -54,7 +59,7 @@ int main(int argc, char **argv) {
     unsigned long long seed = 0;
     double tol = -1.0, rel_tol = -1.0;
     std::string x0_mode, rhs_mode;
-    int requested = -1;
+    int requested = -1, device = 0;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         auto next = [&]() -> const char * {
@@ -83,6 +88,7 @@ int main(int argc, char **argv) {
         else if (a == "--rhs") rhs_mode = next();
         else if (a == "--graph") use_graph = true;
         else if (a == "--autotune") autotune = true;
+        else if (a == "--device") device = std::atoi(next());
         else if (a == "--pattern") {
             const std::string v = next();
             if (v == "a") pattern = SPAI_PATTERN_A;
@@ -119,12 +125,18 @@ int main(int argc, char **argv) {
     std::printf("matrix: %s  systems: %d  n(first): %d  nnz(first): %d\n", spai_problem_name(P), nsys, parts[0].n,
                 parts[0].nonzeroes);
 
+    int device_count = 0;
+    if (hipGetDeviceCount(&device_count) != hipSuccess || device_count < 1) {
+        std::printf("error: no HIP device found\n");
+        return EXIT_FAILURE;
+    }
+    if (device < 0 || device >= device_count) device = 0;
     spai_context *ctx = nullptr;
-    check(spai_context_create(0, &ctx), "context");
+    check(spai_context_create(device, &ctx), "context");
     char name[128];
     int cus = 0;
     spai_context_device_info(ctx, name, sizeof name, &cus);
-    std::printf("device: %s (%d CUs)\n", name, cus);
+    std::printf("device %d of %d: %s (%d CUs)\n", device, device_count, name, cus);
 
     spai_system *S = nullptr;
     if (nsys == 1) check(spai_system_create(ctx, &parts[0], &S), "upload");

@@ -127,8 +127,11 @@ def test_cli_fixture_report(tmp_path, name):
     print(f"CLI {name}: {line[0]}  oracle spread {list(spread)}", flush=True)
     assert spread.min() - w <= its <= spread.max() + w
     print(f"CLI {name} static, preconditioner only", flush=True)
-    out = subprocess.run([D.CLI_PATH, "1", "--algo", "static", "--matrix", path, "--precondition-only"],
-                         capture_output=True, text=True, timeout=300)
+    # --device as the single-device driver: a device past the last one means
+    # device 0 (src/spai_single_dcu.cpp:2715)
+    out = subprocess.run([D.CLI_PATH, "1", "--algo", "static", "--matrix", path, "--precondition-only",
+                          "--device", "4096"], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
     m = re.search(REPORT.format(mode="static", part="balanced_columns"), out.stdout)
     assert m and int(m.group(2)) == 0, out.stdout
+    assert re.search(r"^device 0 of \d+:", out.stdout, re.M), out.stdout
