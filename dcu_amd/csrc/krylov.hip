@@ -395,21 +395,34 @@ __device__ __forceinline__ T stream_load(const T *p) {
 
 constexpr int kRowsPerTile = 512;  // row cap of a tile (host-built)
 constexpr int kRowsPerThread = kRowsPerTile / kThreads;
+// Rows of a tile with more entries than this are summed by a whole wavefront
+// (lane-strided partial sums, then a wave sum); shorter ones by the thread
+// that owns the row, in storage order.
+#ifndef SPAI_TILE_SHORT_ROW
+#define SPAI_TILE_SHORT_ROW 16
+#endif
+constexpr int kTileShortRow = SPAI_TILE_SHORT_ROW;
 
 // Row tiles.  For every row i of the unit's tile: y_i = sum_p val[p] *
-// op.gather(idx[p]) (stored as products in LDS, then summed per row in
-// storage order), then op.row(i, y_i, pre_i, acc) where pre_i = op.pre(i, acc)
-// is the row's own work that does not need the product (operand loads, and
-// outputs that can be stored at once), done before the product so that it
-// overlaps with it and holds few registers across the gathers.  One tile needs two dependent global round trips: the
-// index/value/row-pointer/operand loads, then the gathers.  HAS_MATRIX =
-// false skips the product (streaming ops).
+// op.gather(idx[p]) (stored as products in LDS, then summed per row), then
+// op.row(i, y_i, pre_i, acc) where pre_i = op.pre(i, acc) is the row's own
+// work that does not need the product (operand loads, and outputs that can be
+// stored at once), done before the product so that it overlaps with it and
+// holds few registers across the gathers.  One tile needs two dependent
+// global round trips: the index/value/row-pointer/operand loads, then the
+// gathers.  A row of at most kTileShortRow entries is summed in storage order
+// by the thread that owns it; a longer one (power-law rows reach 1000) by one
+// wavefront: lane l sums entries l, l + 64, ... in order, then a fixed DPP
+// tree adds the lanes, so that no thread walks a long row alone while the
+// workgroup waits.  Every order is fixed: results are reproducible.
+// HAS_MATRIX = false skips the product (streaming ops).
 template <bool HAS_MATRIX, class MT, class Op>
 __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr, const int *__restrict__ idx,
                                                     const MT *__restrict__ val, UnitView uv, Ctx c, Op op) {
     constexpr int NRED = Op::NRED;
     __shared__ double prod[HAS_MATRIX ? kLdsProducts : 1];
     __shared__ int rp[HAS_MATRIX ? kRowsPerTile + 1 : 1];
+    __shared__ double rowsum[HAS_MATRIX ? kRowsPerTile : 1];  // sums of the long rows
     __shared__ double sh[(NRED > 0 ? NRED : 1) * 4];
     const int unit = unit_of(uv);
     if (unit < 0) return;
@@ -456,13 +469,37 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
                 if (q < cnt) prod[q] = (double)vv[u] * gv[u];
             }
             __syncthreads();
+            // long rows: wavefront w takes the rows of [64 w, 64 w + 64),
+            // [64 w + 256, ...) that have more than kTileShortRow entries
+            {
+                const int lane = tid & 63;
+                for (int base = tid - lane; base < nrows; base += kThreads) {
+                    const int li = base + lane;
+                    const bool lng = li < nrows && rp[li + 1] - rp[li] > kTileShortRow;
+                    unsigned long long m = __ballot(lng);
+                    while (m) {
+                        const int j = __ffsll((long long)m) - 1;
+                        m &= m - 1;
+                        const int b = rp[base + j], e = rp[base + j + 1];
+                        double part = 0.0;
+                        for (int q = b + lane; q < e; q += 64) part += prod[q];
+                        part = group_allsum<64>(part);
+                        if (lane == 0) rowsum[base + j] = part;
+                    }
+                }
+            }
+            __syncthreads();
 #pragma unroll
             for (int k = 0; k < kRowsPerThread; ++k) {
                 const int li = tid + k * kThreads;
                 if (li < nrows) {
                     const int b = rp[li], e = rp[li + 1];
                     double sum = 0.0;
-                    for (int q = b; q < e; ++q) sum += prod[q];
+                    if (e - b > kTileShortRow) {
+                        sum = rowsum[li];
+                    } else {
+                        for (int q = b; q < e; ++q) sum += prod[q];
+                    }
                     op.row(r0 + li, sum, pre[k], acc);
                 }
             }
@@ -925,6 +962,29 @@ struct OpMapplyS {  // K3: z = M s, s = r - alpha a (stored before the product; 
     __device__ double gather(int j) const { return sval(j); }
     __device__ double stage(int, const Pre &pr) const { return pr.s; }
     __device__ void row(int i, double mx, const Pre &, double (&)[1]) const { z[i] = mx; }
+    __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
+};
+
+// K3 split in two for row tiles: s = r - alpha a as a streaming pass, then
+// z = M s (OpApplyM) gathering the stored s alone.  The random gathers of a
+// row tile mostly miss L2 (a 64-byte line fetched per 8-byte value), so one
+// gathered vector instead of two halves the M-apply's fetched bytes.  s is
+// formed by the same expression as OpMapplyS's, so z is the same.
+struct OpUpdateS {
+    static constexpr int NRED = 0;
+    struct Pre {};
+    const double *r, *a;
+    double *s;
+    double nalpha;
+    __device__ bool enabled(const SysState &S) const { return S.active; }
+    __device__ void load(const SysState &S) { nalpha = -S.alpha; }
+    __device__ Pre pre(int i, double (&)[1]) const {
+        s[i] = r[i] + nalpha * a[i];
+        return {};
+    }
+    __device__ double gather(int) const { return 0.0; }
+    __device__ double stage(int, const Pre &) const { return 0.0; }
+    __device__ void row(int, double, const Pre &, double (&)[1]) const {}
     __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
 
@@ -1766,7 +1826,28 @@ void BiCGStab::enqueue_k1(const OperatorView &A, const OperatorView &M, const Ti
     }
 }
 
-// One iteration: K1 (one or two launches), K2, K3, K4.  grid > 0: walk the
+// K3: z = M s with s = r - alpha a, fused (s formed while gathering) or, for
+// row tiles while K1 is split too, a streaming s pass and an M-apply of the
+// stored s.
+void BiCGStab::enqueue_k3(const OperatorView &M, const TileMap &tm, int parity, const int *list, int grid,
+                          bool split, hipStream_t st) {
+    Ctx c = make_ctx(tm, nsys_);
+    if (split && tm.kind == kLayoutTiles) {
+        OpUpdateS us{r_, a_, s_, 0.0};
+        launch_units<false, double>(M, tm, c, us, list, grid, st, flip());
+        OpApplyM om{s_, zbuf(parity)};
+        if (M.fp32) launch_units<true, float>(M, tm, c, om, list, grid, st, flip());
+        else launch_units<true, double>(M, tm, c, om, list, grid, st, flip());
+    } else if (M.fp32) {
+        OpMapplyS<float> op{r_, a_, s_, zbuf(parity), 0.0};
+        launch_units<true, float>(M, tm, c, op, list, grid, st, flip());
+    } else {
+        OpMapplyS<double> op{r_, a_, s_, zbuf(parity), 0.0};
+        launch_units<true, double>(M, tm, c, op, list, grid, st, flip());
+    }
+}
+
+// One iteration: K1 (one or two launches), K2, K3 (one or two), K4.  grid > 0: walk the
 // first `grid` entries of the active-unit list; grid == 0: every unit.
 void BiCGStab::enqueue_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tm, int parity,
                                  int grid, bool split, hipStream_t st) {
@@ -1778,13 +1859,7 @@ void BiCGStab::enqueue_iteration(const OperatorView &A, const OperatorView &M, c
         launch_units<true, double>(A, tm, c, op, list, grid, st, flip());
     }
     if (M.val) {
-        if (M.fp32) {
-            OpMapplyS<float> op{r_, a_, s_, zbuf(parity), 0.0};
-            launch_units<true, float>(M, tm, c, op, list, grid, st, flip());
-        } else {
-            OpMapplyS<double> op{r_, a_, s_, zbuf(parity), 0.0};
-            launch_units<true, double>(M, tm, c, op, list, grid, st, flip());
-        }
+        enqueue_k3(M, tm, parity, list, grid, split, st);
     } else {
         OpCopyS op{r_, a_, s_, zbuf(parity), 0.0};
         launch_units<false, double>(A, tm, c, op, list, grid, st, flip());
@@ -2125,15 +2200,9 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
             OpSpmvAlpha op{ybuf(parity), rs_, a_};
             launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
         } break;
-        case 3: {
-            if (M.fp32) {
-                OpMapplyS<float> op{r_, a_, s_, zbuf(parity), 0.0};
-                launch_units<true, float>(M, tm, c, op, nullptr, 0, st, flip());
-            } else {
-                OpMapplyS<double> op{r_, a_, s_, zbuf(parity), 0.0};
-                launch_units<true, double>(M, tm, c, op, nullptr, 0, st, flip());
-            }
-        } break;
+        case 3:  // the form the solver uses with every unit active
+            enqueue_k3(M, tm, parity, nullptr, 0, tm.nunits >= split_units(tm), st);
+            break;
         case 4: {
             OpSpmvOmega op{zbuf(parity), s_, rs_, t_};
             launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
@@ -2200,13 +2269,7 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
                 launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
             }
             SPAI_HIP(hipEventRecord(e[3], st));
-            if (M.fp32) {
-                OpMapplyS<float> op{r_, a_, s_, zbuf(parity), 0.0};
-                launch_units<true, float>(M, tm, c, op, nullptr, 0, st, flip());
-            } else {
-                OpMapplyS<double> op{r_, a_, s_, zbuf(parity), 0.0};
-                launch_units<true, double>(M, tm, c, op, nullptr, 0, st, flip());
-            }
+            enqueue_k3(M, tm, parity, nullptr, 0, split, st);
             SPAI_HIP(hipEventRecord(e[4], st));
             {
                 OpSpmvOmega op{zbuf(parity), s_, rs_, t_};

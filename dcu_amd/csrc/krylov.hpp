@@ -239,6 +239,8 @@ private:
                            bool split, hipStream_t st);
     void enqueue_k1(const OperatorView &A, const OperatorView &M, const TileMap &tiles, int parity, const int *list,
                     int grid, bool split, hipStream_t st);
+    void enqueue_k3(const OperatorView &M, const TileMap &tiles, int parity, const int *list, int grid, bool split,
+                    hipStream_t st);
     int n_ = 0, nsys_ = 0, nunits_ = 0, nnodes_ = -1;
     DeviceArray<double> vec_;       // x-independent work vectors
     DeviceArray<double> partials_;
