@@ -111,7 +111,7 @@ class SystemInfo(ctypes.Structure):
 class BuildReport(ctypes.Structure):
     _fields_ = [("ms", ctypes.c_float), ("tier_columns", ctypes.c_int * 5), ("max_gathered", ctypes.c_int),
                 ("max_n2", ctypes.c_int), ("empty_columns", ctypes.c_int), ("sum_gathered", ctypes.c_longlong),
-                ("static_tier", ctypes.c_int), ("tile_shape", ctypes.c_int * 3)]
+                ("static_tier", ctypes.c_int), ("tile_shape", ctypes.c_int * 3), ("phase_ms", ctypes.c_float * 4)]
 
 
 TUNE_TIERS = 3
@@ -502,7 +502,9 @@ class System:
                                            ctypes.byref(rep)))
         return {"ms": rep.ms, "tier_columns": list(rep.tier_columns), "max_gathered": rep.max_gathered,
                 "max_n2": rep.max_n2, "empty_columns": rep.empty_columns, "sum_gathered": rep.sum_gathered,
-                "static_tier": rep.static_tier, "tile_shape": list(rep.tile_shape)}
+                "static_tier": rep.static_tier, "tile_shape": list(rep.tile_shape),
+                "phase_ms": dict(zip(("plan", "register_tiers", "lds_tier", "global_tier"),
+                                     [float(v) for v in rep.phase_ms]))}
 
     def autotune(self, precision: int = PRECISION_MIXED, schedule: int = SCHEDULE_BALANCED_NNZ,
                  reps: int = 5) -> dict:

@@ -388,7 +388,7 @@ const char *spai_status_string(int status) {
 
 const char *spai_last_error(void) { return g_last_error.c_str(); }
 
-const char *spai_version(void) { return "spai_mi355x 0.2 (gfx950)"; }
+const char *spai_version(void) { return "spai_mi355x 0.3 (gfx950)"; }
 
 int spai_context_create(int device, spai_context **out) {
     if (!out) return fail(SPAI_ERR_INVALID, "out is NULL");
@@ -631,6 +631,7 @@ void build_m(spai_system *sys, int precision, int schedule, int col_begin, int c
         report->sum_gathered = stats.sum_gathered;
         report->static_tier = stats.static_tier;
         for (int t = 0; t < kTunedTiers; ++t) report->tile_shape[t] = shape[t];
+        for (int q = 0; q < 4; ++q) report->phase_ms[q] = stats.phase_ms[q];
     }
 }
 

@@ -163,6 +163,8 @@ int main(int argc, char **argv) {
     std::printf("SPAI build: %.4f ms  max|J|=%d  max gathered=%d  tiers=[%d %d %d %d %d]\n", br.ms, br.max_n2,
                 br.max_gathered, br.tier_columns[0], br.tier_columns[1], br.tier_columns[2], br.tier_columns[3],
                 br.tier_columns[4]);
+    std::printf("SPAI phases: plan %.4f ms, register tiers %.4f ms, LDS tier %.4f ms, global tier %.4f ms\n",
+                br.phase_ms[0], br.phase_ms[1], br.phase_ms[2], br.phase_ms[3]);
 
     int iterations = 0;
     if (!precondition_only) {

@@ -131,17 +131,34 @@ TileShape shape_info(int c, std::index_sequence<I...>) {
     return r;
 }
 
-// LDS of one column group, in bytes.  The tile and R are held in the QR's
-// type (FP32 in mixed mode); mixed mode's FP64 refinement reads the column's
-// gathered entries (values, columns, row starts) over the tile, which is dead
-// once its rows are in registers.
+// LDS of one column group, in bytes: one area that holds, one after the
+// other, what each step needs:
+//   gather   the entry table (source position and J index of every
+//            gathered entry), read into registers before the tile is
+//            cleared;
+//   scatter  the tile A(I_k, J_k), ROWS x TS in the QR's type (FP32 in mixed
+//            mode), dead once its rows are in registers;
+//   solve    mixed mode's refinement data (the entries' FP64 values, their
+//            J indices and each row's first entry, in sorted order), then R,
+//            y and 1 / R_cc.
+// Sharing the area keeps the group to the tile's size, so the 3D shapes fit
+// four workgroups per CU instead of three.
 template <class TR, class QT, bool REFINE>
 struct TierLds {
+    static constexpr int al8(int b) { return (b + 7) / 8 * 8; }
+    static constexpr int al4(int b) { return (b + 3) / 4 * 4; }
     static constexpr int TILE = TR::ROWS * TR::TS * (int)sizeof(QT);
-    static constexpr int REF = REFINE ? TR::TMAX * 8 + TR::TMAX * 4 + (TR::ROWS + 1) * 4 : 0;
-    static constexpr int RS_OFF = ((TILE > REF ? TILE : REF) + 7) / 8 * 8;
-    static constexpr int YS_OFF = RS_OFF + (TR::NJ * TR::NJ * (int)sizeof(QT) + 7) / 8 * 8;
-    static constexpr int DOUBLES = YS_OFF / 8 + TR::NJ;
+    static constexpr int TJ_OFF = TR::TMAX * 4;  // table: int source position, then byte J index
+    static constexpr int TABLE = TJ_OFF + TR::TMAX;
+    static constexpr int JS_OFF = TR::TMAX * 8;  // refinement: double value, byte J index, int row start
+    static constexpr int RST_OFF = al4(JS_OFF + TR::TMAX);
+    static constexpr int REF = REFINE ? RST_OFF + (TR::ROWS + 1) * 4 : 0;
+    static constexpr int RS_OFF = al8(REF);
+    static constexpr int YS_OFF = al8(RS_OFF + TR::NJ * TR::NJ * (int)sizeof(QT));
+    static constexpr int RI_OFF = YS_OFF + TR::NJ * 8;  // 1 / R_cc
+    static constexpr int END = RI_OFF + TR::NJ * 8;
+    static constexpr int AREA = TILE > TABLE ? (TILE > END ? TILE : END) : (TABLE > END ? TABLE : END);
+    static constexpr int DOUBLES = al8(AREA) / 8;
 };
 
 // The LDS tier: one wavefront per column, tile in LDS, FP64 arithmetic.
@@ -286,9 +303,10 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
     QT *tile = (QT *)lds;                                     // ROWS x TS
     QT *Rs = (QT *)((char *)lds + L::RS_OFF);                 // NJ x NJ
     double *ys = (double *)((char *)lds + L::YS_OFF);         // NJ (then x in the refinement)
+    double *rinv = (double *)((char *)lds + L::RI_OFF);       // NJ: 1 / R_cc
     double *vslot = lds;                                      // refinement: entry values (sorted order)
-    int *jslot = (int *)(vslot + TR::TMAX);                   //   their columns
-    int *rstart = jslot + TR::TMAX;                           //   first entry of each row, n1 + 1
+    unsigned char *jslot = (unsigned char *)lds + L::JS_OFF;  //   their J indices
+    int *rstart = (int *)((char *)lds + L::RST_OFF);          //   first entry of each row, n1 + 1
 
     const int c0 = a.pcolptr[k];
     const int n2 = a.pcolptr[k + 1] - c0;
@@ -306,11 +324,10 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
     const int off = incl - cl;
 #if SPAI_TIER_ENTRY_TABLE
     // entry e -> its position in A's arrays and its J index, written by the
-    // lane that owns the column into the R / y area (dead until the QR ends)
-    int *tsrc = (int *)((char *)lds + L::RS_OFF);
-    unsigned char *tj = (unsigned char *)((char *)lds + L::YS_OFF);
-    static_assert(TR::TMAX * 4 <= TR::NJ * TR::NJ * (int)sizeof(QT) && TR::TMAX <= TR::NJ * 8 && TR::NJ <= 255,
-                  "the entry table fits the R and y areas");
+    // lane that owns the column (the area is free until the tile is cleared)
+    int *tsrc = (int *)lds;
+    unsigned char *tj = (unsigned char *)lds + L::TJ_OFF;
+    static_assert(TR::NJ <= 255, "J indices are bytes");
     for (int q = 0; q < cl; ++q) {
         tsrc[off + q] = cs + q;
         tj[off + q] = (unsigned char)lane;
@@ -365,21 +382,13 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
         return;
     }
 
-    // zero the rows this lane owns, then scatter A(I_k, J_k) into LDS
-#pragma unroll
-    for (int r = 0; r < RPL; ++r) {
-        int rho = lane + W * r;
-        if (rho < n1) {
-#pragma unroll
-            for (int c = 0; c < NJ; ++c) tile[rho * TS + c] = (QT)0;
-        }
-    }
-    wave_lds_fence();
-    double vv[EPL];  // the entries' FP64 values and columns, for the refinement
+    // the entries' FP64 values and J indices into registers (the entry
+    // table dies here; the values also feed the refinement)
+    double vv[EPL];
     int jv[EPL];
 #pragma unroll
     for (int i = 0; i < EPL; ++i) {
-        // shuffles run on every lane of the group; only the store is masked
+        // shuffles run on every lane of the group; only the loads are masked
         const bool valid = key[i] != SK::kNone;
         const int e = valid ? SK::ent(key[i]) : 0;
 #if SPAI_TIER_ENTRY_TABLE
@@ -391,8 +400,21 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
 #endif
         vv[i] = valid ? a.val[src] : 0.0;
         jv[i] = j;
-        if (valid) tile[pos[i] * TS + j] = (QT)vv[i];
     }
+    wave_lds_fence();
+    // zero the rows this lane owns, then scatter A(I_k, J_k) into LDS
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+        int rho = lane + W * r;
+        if (rho < n1) {
+#pragma unroll
+            for (int c = 0; c < NJ; ++c) tile[rho * TS + c] = (QT)0;
+        }
+    }
+    wave_lds_fence();
+#pragma unroll
+    for (int i = 0; i < EPL; ++i)
+        if (key[i] != SK::kNone) tile[pos[i] * TS + jv[i]] = (QT)vv[i];
     wave_lds_fence();
 
     // rows rho = lane + W*r into registers
@@ -411,7 +433,7 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
             const int slot = lane * EPL + i;
             if (key[i] != SK::kNone) {
                 vslot[slot] = vv[i];
-                jslot[slot] = jv[i];
+                jslot[slot] = (unsigned char)jv[i];
                 if (first[i]) rstart[pos[i]] = slot;
             }
         }
@@ -491,7 +513,8 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
         }
     }
 
-    // R and y to LDS (rows < n2 only), then a uniform back substitution
+    // R and y to LDS (rows < n2 only), 1 / R_cc once per column (lane c),
+    // then a uniform back substitution
 #pragma unroll
     for (int r = 0; r < RPL; ++r) {
         int rho = lane + W * r;
@@ -500,6 +523,13 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
             for (int c = 0; c < NJ; ++c) Rs[rho * NJ + c] = A[r][c];
             ys[rho] = (double)y[r];
         }
+    }
+    if (lane < n2) {  // W >= NJ: row c < n2 of R is lane c's first row
+        QT d = 0;
+#pragma unroll
+        for (int c = 0; c < NJ; ++c)
+            if (c == lane) d = A[0][c];
+        rinv[lane] = 1.0 / (double)d;
     }
     wave_lds_fence();
     double x[NJ];
@@ -511,7 +541,7 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
 #pragma unroll
             for (int c2 = c + 1; c2 < NJ; ++c2)
                 if (c2 < n2) t -= (double)Rs[c * NJ + c2] * x[c2];
-            x[c] = t / (double)Rs[c * NJ + c];
+            x[c] = t * rinv[c];
         }
     }
     if (E < 0) {
@@ -571,7 +601,7 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
                     double t = g[c];
 #pragma unroll
                     for (int c2 = 0; c2 < c; ++c2) t -= (double)Rs[c2 * NJ + c] * g[c2];
-                    g[c] = t / (double)Rs[c * NJ + c];
+                    g[c] = t * rinv[c];
                 }
             }
 #pragma unroll
@@ -581,7 +611,7 @@ __device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *l
 #pragma unroll
                     for (int c2 = c + 1; c2 < NJ; ++c2)
                         if (c2 < n2) t -= (double)Rs[c * NJ + c2] * g[c2];
-                    g[c] = t / (double)Rs[c * NJ + c];
+                    g[c] = t * rinv[c];
                     x[c] += g[c];
                 }
             }
@@ -1262,7 +1292,7 @@ bool wide_keys_forced() {
 
 template <class MT, bool REFINE>
 void launch_tiers(const ColumnArgs &base, int base_n, bool dynamic, int static_tier, const int *shape, int *counters,
-                  int *lists, int list_stride, int cus, hipStream_t st) {
+                  int *lists, int list_stride, int cus, hipStream_t st, hipEvent_t after_reg) {
     using QT = typename std::conditional<REFINE, float, double>::type;
     ColumnArgs a = base;
     // t: tier; ovf: a static run's follow-up over the tier's overflow list
@@ -1339,12 +1369,16 @@ void launch_tiers(const ColumnArgs &base, int base_n, bool dynamic, int static_t
         // columns the tier before it handed on (the launches run one after
         // another on the stream either way)
         for (int t = 0; t < kRegTiers; ++t) run_bucket(t);
+        SPAI_HIP(hipEventRecord(after_reg, st));
         run_lds(3);
-    } else {
-        if (static_tier < kRegTiers) run_bucket(static_tier);
-        else run_lds(3);
+    } else if (static_tier < kRegTiers) {
+        run_bucket(static_tier);
         // the overflow lists of the tiers after the static one
         for (int t = static_tier + 1; t < kRegTiers; ++t) run_bucket(t, true);
+        SPAI_HIP(hipEventRecord(after_reg, st));
+    } else {
+        SPAI_HIP(hipEventRecord(after_reg, st));
+        run_lds(3);
     }
 }
 
@@ -1457,7 +1491,10 @@ __global__ __launch_bounds__(kGlobThreads) void spai_product_kernel(ProductArgs 
 }  // namespace
 
 SpaiBuilder::SpaiBuilder() = default;
-SpaiBuilder::~SpaiBuilder() = default;
+SpaiBuilder::~SpaiBuilder() {
+    for (auto &e : ev_)
+        if (e) (void)hipEventDestroy(e);
+}
 
 void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats *stats) {
     require(p.n >= 0 && p.colptr && p.rowidx && p.val && p.perm && p.m_csc && p.m_csr, "spai build: bad arguments");
@@ -1467,6 +1504,8 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
     SPAI_HIP(hipGetDevice(&dev));
     if (cus_ == 0) SPAI_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
     const bool dynamic = p.schedule == 1;
+    for (auto &e : ev_)
+        if (!e) SPAI_HIP(hipEventCreate(&e));
     // J_k from M's pattern (A's own unless another pattern is given)
     const int *pc = p.pcolptr ? p.pcolptr : p.colptr;
     const int *pr = p.pcolptr ? p.prowidx : p.rowidx;
@@ -1475,6 +1514,7 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
     stats64_.reserve(3);
     const int list_stride = std::max(1, ce - cb);
     lists_.reserve((size_t)kLists * list_stride);
+    SPAI_HIP(hipEventRecord(ev_[0], st));
     hipLaunchKernelGGL(spai_zero_counters, dim3(1), dim3(64), 0, st, counters_.get(), nc, stats64_.get(), 3);
     SPAI_CHECK_LAUNCH();
     if (ce > cb) {
@@ -1487,6 +1527,7 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
                                ce, counters_.get(), stats64_.get(), (int *)nullptr, list_stride);
         SPAI_CHECK_LAUNCH();
     }
+    SPAI_HIP(hipEventRecord(ev_[1], st));
     int hc[kCounters] = {0};
     unsigned long long h64[3] = {0, 0, 0};
     auto fetch = [&]() {
@@ -1516,27 +1557,21 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
     }
     for (int t = 0; t < kTunedTiers; ++t)
         require(p.shape[t] >= 0 && p.shape[t] < tile_shape_count(t), "spai build: no such tile shape");
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    if (p.tiers_ms) {
-        SPAI_HIP(hipEventCreate(&ev[0]));
-        SPAI_HIP(hipEventCreate(&ev[1]));
-        SPAI_HIP(hipEventRecord(ev[0], st));
-    }
+    SPAI_HIP(hipEventRecord(ev_[2], st));
     if (ce > cb && static_tier < kTiers - 1) {
         if (p.precision == 1)
             launch_tiers<float, true>(a, p.n, dynamic, static_tier, p.shape, counters_.get(), lists_.get(),
-                                      list_stride, cus_, st);
+                                      list_stride, cus_, st, ev_[3]);
         else
             launch_tiers<double, false>(a, p.n, dynamic, static_tier, p.shape, counters_.get(), lists_.get(),
-                                        list_stride, cus_, st);
+                                        list_stride, cus_, st, ev_[3]);
+    } else {
+        SPAI_HIP(hipEventRecord(ev_[3], st));
     }
-    if (p.tiers_ms) SPAI_HIP(hipEventRecord(ev[1], st));
+    SPAI_HIP(hipEventRecord(ev_[4], st));
     fetch();
-    if (p.tiers_ms) {
-        SPAI_HIP(hipEventElapsedTime(p.tiers_ms, ev[0], ev[1]));
-        SPAI_HIP(hipEventDestroy(ev[0]));
-        SPAI_HIP(hipEventDestroy(ev[1]));
-    }
+    if (p.tiers_ms) SPAI_HIP(hipEventElapsedTime(p.tiers_ms, ev_[2], ev_[4]));
+    SPAI_HIP(hipEventRecord(ev_[5], st));
     const int global_cols = dynamic ? hc[kTiers - 1] : (static_tier == kTiers - 1 ? ce - cb : 0);
     if (global_cols > 0) {
         // very wide columns: one workgroup per column with a global scratch
@@ -1581,9 +1616,12 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
         if (p.precision == 1) launch(float{});
         else launch(double{});
         SPAI_CHECK_LAUNCH();
-        SPAI_HIP(hipStreamSynchronize(st));
     }
+    SPAI_HIP(hipEventRecord(ev_[6], st));
+    SPAI_HIP(hipEventSynchronize(ev_[6]));
     if (stats) {
+        const int from[4] = {0, 2, 3, 5}, to[4] = {1, 3, 4, 6};
+        for (int q = 0; q < 4; ++q) SPAI_HIP(hipEventElapsedTime(&stats->phase_ms[q], ev_[from[q]], ev_[to[q]]));
         for (int t = 0; t < kTiers; ++t) stats->tier_columns[t] = dynamic ? hc[t] : 0;
         if (!dynamic && ce > cb) stats->tier_columns[static_tier] = ce - cb - hc[2 * kTiers + 2];
         // columns handed on to the next tier are counted where they were solved

@@ -48,6 +48,8 @@ struct SpaiBuildStats {
     int empty_columns = 0;
     long long sum_gathered = 0;    // sum_k T_k (entries of A read by the gather)
     int static_tier = -1;
+    // device ms of the phases: planner, register tiers, LDS tier, global tier
+    float phase_ms[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 };
 
 int classify_host(int T, int n2);
@@ -89,6 +91,9 @@ private:
     DeviceArray<uint64_t> glob_keys_;
     DeviceArray<int> glob_ints_;
     int cus_ = 0;
+    // phase boundaries: planner start/end, tiers start, after the register
+    // tiers, after the LDS tier, global tier start/end
+    hipEvent_t ev_[7] = {};
 };
 
 }  // namespace spai

@@ -59,8 +59,8 @@ extern "C" {
 const char *spai_status_string(int status);
 /* Message of the last failing call on this thread. */
 const char *spai_last_error(void);
-/* Library version string.  0.2: spai_build_report gained tile_shape[3] (the
- * struct grew; rebuild callers against this header). */
+/* Library version string.  0.2: spai_build_report gained tile_shape[3];
+ * 0.3: phase_ms[4] (the struct grew; rebuild callers against this header). */
 const char *spai_version(void);
 
 /* ------------------------------------------------------------ context --- */
@@ -112,6 +112,9 @@ typedef struct {
     long long sum_gathered;  /* entries of A gathered over all columns */
     int static_tier;         /* tier used by balanced_columns, -1 for balanced_nnz */
     int tile_shape[3];       /* candidate group shape of each register tier (see spai_system_autotune) */
+    float phase_ms[4];       /* device ms of the phases: planner, register tiers, LDS tier, global tier
+                                (the reference times its SPAI phases with events,
+                                src/spai_single_dcu.cpp:2153-2190) */
 } spai_build_report;
 
 /* Sparsity pattern of M.  SPAI_PATTERN_A (the default) is pattern(A), as

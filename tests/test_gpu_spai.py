@@ -69,6 +69,13 @@ def test_power_law_all_tiers(ctx, precision, tol):
     A = P.system(0)
     S, rep, _, _ = build_and_compare(ctx, A, precision, D.SCHEDULE_BALANCED_NNZ, tol)
     assert rep["tier_columns"][0] > 0 and rep["tier_columns"][1] > 0 and rep["tier_columns"][2] > 0
+    # per-phase device times: every phase that had columns took time, and
+    # the phases fit inside the whole build
+    ph = rep["phase_ms"]
+    print(f"phases {ph}, build {rep['ms']:.3f} ms", flush=True)
+    assert ph["plan"] > 0 and ph["register_tiers"] > 0 and ph["global_tier"] >= 0
+    assert (ph["lds_tier"] > 0) == (rep["tier_columns"][3] > 0)
+    assert sum(ph.values()) <= rep["ms"] * 1.001 + 1e-3
 
 
 def test_power_law_lds_tier_static(ctx):
