@@ -294,7 +294,7 @@ __device__ __forceinline__ void store_m(const ColumnArgs &a, int pos, double x) 
 #define SPAI_TIER_ENTRY_TABLE 1
 #endif
 template <class TR, class QT, bool REFINE, class MT, bool K32>
-__device__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *lds) {
+__device__ __forceinline__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *lds) {
     constexpr int W = TR::W, EPL = TR::EPL, NJ = TR::NJ, TS = TR::TS;
     using SK = SortKey<K32, log2_exact(TR::TMAX)>;
     using KT = typename SK::T;
@@ -871,10 +871,12 @@ __global__ __launch_bounds__(64) void spai_lds_tier_kernel(ColumnArgs a) {
 constexpr int kGlobThreads = 512;
 constexpr int kGlobCols = 32;                           // columns per pass
 constexpr int kGlobRowGroups = kGlobThreads / kGlobCols;  // 16 row groups
+constexpr int kGlobNB = 16;                             // panel width of the blocked QR
+static_assert(kGlobNB <= kGlobRowGroups, "one row group per reflector when forming T^T W");
 
 struct GlobalScratch {
-    double *tile;       // slots * tile_cap: the n1 x n2 tile, row-major (stride n2)
-    double *vec;        // slots * vec_cap: y (n1), tau, x, w (n2 each)
+    double *tile;       // slots * tile_cap: max(n1, n2) x (n2 + 1), row-major (stride n2 + 1)
+    double *vec;        // slots * vec_cap: x (n2)
     uint64_t *keys;     // slots * key_cap: sorted (row, entry) keys
     int *ints;          // slots * int_cap: positions (key_cap + 1), J offsets (n2 + 1)
     long long tile_cap, vec_cap, key_cap, int_cap;
@@ -935,18 +937,187 @@ __device__ __forceinline__ int glob_find_j(const int *offs, int n2, int e) {
     return lo;
 }
 
+// Sums N values across the workgroup (each thread's v[q], q < cnt); every
+// thread receives the totals in v.  Uses red[64 .. 64 + 8 N).
+template <int N>
+__device__ __forceinline__ void glob_block_sum_vec(double (&v)[N], int cnt, double *red) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        if (q < cnt) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+        }
+    }
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < N; ++q)
+            if (q < cnt) red[64 + q * 8 + w] = v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        double t = 0.0;
+        if (q < cnt) {
+#pragma unroll
+            for (int i = 0; i < kGlobThreads / 64; ++i) t += red[64 + q * 8 + i];
+        }
+        v[q] = t;
+    }
+    __syncthreads();
+}
+
+// Blocked Householder QR of the m x (n + 1) row-major tile (leading dimension
+// ld), reflectors from the first n columns, applied to all n + 1: on return
+// R is on and above the diagonal of the first n columns and column n holds
+// Q^T times its input.  Panels of kGlobNB columns: each panel is factored
+// column by column (the reflector, then its dot products with the panel's
+// later columns and with the panel's earlier reflectors, one workgroup sum
+// for all of them), which gives the compact-WY factor T (Q_panel = I - V T
+// V^T, the reference LAPACK dlarft's forward columnwise recurrence); the
+// trailing columns are then updated 32 at a time as A2 -= V (T^T (V^T A2)),
+// reading V and A2 twice per panel instead of twice per column.
+__device__ __forceinline__ void glob_householder(double *tile, int m, int n, int ld, double *red, double *wsh, double *tsh,
+                                 double *wp) {
+    const int tid = threadIdx.x;
+    constexpr int NB = kGlobNB;
+    for (int j0 = 0; j0 < n; j0 += NB) {
+        const int nb = min(NB, n - j0);
+        // ---- panel: columns j0 .. j0 + nb - 1
+        for (int jj = 0; jj < nb; ++jj) {
+            const int c = j0 + jj;
+            double sl = 0.0;
+            for (int r = c + 1 + tid; r < m; r += kGlobThreads) {
+                const double v = tile[(size_t)r * ld + c];
+                sl += v * v;
+            }
+            const double sigma = glob_block_sum(sl, red);
+            const double alpha = tile[(size_t)c * ld + c];
+            double beta = alpha, tau = 0.0, scale = 0.0;
+            if (sigma != 0.0) {
+                const double nrm = sqrt(alpha * alpha + sigma);
+                beta = alpha >= 0.0 ? -nrm : nrm;
+                tau = (beta - alpha) / beta;
+                scale = 1.0 / (alpha - beta);
+            }
+            for (int r = c + 1 + tid; r < m; r += kGlobThreads) tile[(size_t)r * ld + c] *= scale;
+            __syncthreads();
+            // acc[q] = (v_jj, A(:, j0 + q)) over rows >= c: for q < jj the
+            // stored reflector v_q (for T), for q > jj a later panel column
+            double acc[NB];
+#pragma unroll
+            for (int q = 0; q < NB; ++q) acc[q] = 0.0;
+            for (int r = c + tid; r < m; r += kGlobThreads) {
+                const double *row = tile + (size_t)r * ld + j0;
+                const double vr = r == c ? 1.0 : row[jj];
+#pragma unroll
+                for (int q = 0; q < NB; ++q)
+                    if (q < nb && q != jj) acc[q] += vr * row[q];
+            }
+            glob_block_sum_vec<NB>(acc, nb, red);
+            if (tau != 0.0) {
+                for (int r = c + tid; r < m; r += kGlobThreads) {
+                    double *row = tile + (size_t)r * ld + j0;
+                    const double tv = tau * (r == c ? 1.0 : row[jj]);
+#pragma unroll
+                    for (int q = 0; q < NB; ++q)
+                        if (q > jj && q < nb) row[q] -= acc[q] * tv;
+                }
+            }
+            if (tid == 0) {
+                tile[(size_t)c * ld + c] = beta;
+                // T(0:jj, jj) = -tau T(0:jj, 0:jj) (V(:, 0:jj)^T v_jj), T(jj, jj) = tau;
+                // the sums go through LDS (red[200..]) so that the loops index
+                // memory, not registers
+                double *gq = red + 200;
+#pragma unroll
+                for (int q = 0; q < NB; ++q)
+                    if (q < jj) gq[q] = acc[q];
+                for (int i = 0; i < jj; ++i) {
+                    double t = 0.0;
+                    for (int q = i; q < jj; ++q) t += tsh[i * NB + q] * gq[q];
+                    tsh[i * NB + jj] = -tau * t;
+                }
+                tsh[jj * NB + jj] = tau;
+            }
+            __syncthreads();
+        }
+        // ---- trailing columns j0 + nb .. n (n: the right-hand side)
+        const int cc = tid % kGlobCols, rg = tid / kGlobCols, lane = tid & 63, wv = tid >> 6;
+        for (int cb = j0 + nb; cb <= n; cb += kGlobCols) {
+            const int col = cb + cc;
+            const bool live = col <= n;
+            // W = V^T A2 (nb x 32): rows r >= j0, V(r, i) = 0 above row j0 + i,
+            // 1 on it, the stored reflector below
+            double w[NB];
+#pragma unroll
+            for (int i = 0; i < NB; ++i) w[i] = 0.0;
+            if (live) {
+                for (int r = j0 + rg; r < m; r += kGlobRowGroups) {
+                    const double *row = tile + (size_t)r * ld;
+                    const double ar = row[col];
+#pragma unroll
+                    for (int i = 0; i < NB; ++i) {
+                        if (i < nb) {
+                            const int d = r - (j0 + i);
+                            const double vri = d < 0 ? 0.0 : (d == 0 ? 1.0 : row[j0 + i]);
+                            w[i] += vri * ar;
+                        }
+                    }
+                }
+            }
+            // the two row groups of a wavefront, then the 8 wavefronts
+#pragma unroll
+            for (int i = 0; i < NB; ++i) w[i] += __shfl_xor(w[i], 32, 64);
+            if (lane < 32) {
+#pragma unroll
+                for (int i = 0; i < NB; ++i) wsh[(wv * NB + i) * kGlobCols + cc] = w[i];
+            }
+            __syncthreads();
+            // W' = T^T W: thread (i, cc) of the first NB row groups
+            if (rg < nb) {
+                const int i = rg;
+                double t = 0.0;
+                for (int q = 0; q <= i; ++q) {
+                    double wq = 0.0;
+#pragma unroll
+                    for (int v8 = 0; v8 < kGlobThreads / 64; ++v8) wq += wsh[(v8 * NB + q) * kGlobCols + cc];
+                    t += tsh[q * NB + i] * wq;
+                }
+                wp[i * kGlobCols + cc] = t;
+            }
+            __syncthreads();
+            // A2 -= V W'
+            if (live) {
+                for (int r = j0 + rg; r < m; r += kGlobRowGroups) {
+                    double *row = tile + (size_t)r * ld;
+                    double u = 0.0;
+#pragma unroll
+                    for (int i = 0; i < NB; ++i) {
+                        if (i < nb) {
+                            const int d = r - (j0 + i);
+                            const double vri = d < 0 ? 0.0 : (d == 0 ? 1.0 : row[j0 + i]);
+                            u += vri * wp[i * kGlobCols + cc];
+                        }
+                    }
+                    row[col] -= u;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 template <class MT>
-__device__ void solve_column_global(const ColumnArgs &a, int k, const GlobalScratch &g, int slot, double *red,
-                                    int *ired, int *sh) {
+__device__ __forceinline__ void solve_column_global(const ColumnArgs &a, int k, const GlobalScratch &g, int slot, double *red,
+                                    int *ired, int *sh, double *wsh, double *tsh, double *wp) {
     const int tid = threadIdx.x;
     const int c0 = a.pcolptr[k];
     const int n2 = a.pcolptr[k + 1] - c0;
     if (n2 <= 0) return;
     double *tile = g.tile + (size_t)slot * g.tile_cap;
-    double *yv = g.vec + (size_t)slot * g.vec_cap;
-    double *tau = yv + g.n1_cap;
-    double *xv = tau + g.n2_cap;
-    double *wv = xv + g.n2_cap;
+    double *xv = g.vec + (size_t)slot * g.vec_cap;  // n2
     uint64_t *keys = g.keys + (size_t)slot * g.key_cap;
     int *pos = g.ints + (size_t)slot * g.int_cap;  // key_cap + 1 (scan total at [T])
     int *offs = pos + g.key_cap + 1;                // n2 + 1
@@ -1002,88 +1173,35 @@ __device__ void solve_column_global(const ColumnArgs &a, int k, const GlobalScra
         pos[e] = pos[e] + (first ? 1 : 0) - 1;
     }
     __syncthreads();
-    for (long long idx = tid; idx < (long long)n1 * n2; idx += kGlobThreads) tile[idx] = 0.0;
+    if (E < 0) {  // k not in I_k: M(:, k) = 0, as the reference's Sol_SSPAIv10
+        for (int c = tid; c < n2; c += kGlobThreads) store_m<MT>(a, c0 + c, 0.0);
+        if (a.n1_out && tid == 0) a.n1_out[k] = n1;
+        __syncthreads();
+        return;
+    }
+    // The tile: m = max(n1, n2) rows (a column with n1 < n2 still has an R_jj
+    // for every j), n2 columns of A(I_k, J_k) and, as column n2, the right-
+    // hand side e_E, so that every panel's reflectors reach Q^T e_E with the
+    // trailing columns.  Row-major, leading dimension ld = n2 + 1.
+    const int m = n1 > n2 ? n1 : n2;
+    const int ld = n2 + 1;
+    for (long long idx = tid; idx < (long long)m * ld; idx += kGlobThreads) {
+        const int r = (int)(idx / ld), c = (int)(idx - (long long)r * ld);
+        tile[idx] = (c == n2 && r == E) ? 1.0 : 0.0;
+    }
     __syncthreads();
     for (int e = tid; e < T; e += kGlobThreads) {
         const int ee = (int)(unsigned)keys[e];
         const int j = glob_find_j(offs, n2, ee);
         const int jr = a.prowidx[c0 + j];
-        tile[(size_t)pos[e] * n2 + j] = a.val[a.colptr[jr] + ee - offs[j]];
+        tile[(size_t)pos[e] * ld + j] = a.val[a.colptr[jr] + ee - offs[j]];
     }
     __syncthreads();
-    // Householder QR, H_j = I - tau_j v v^T with v_j = 1
-    const int lane32 = tid % kGlobCols, rg = tid / kGlobCols;
-    for (int j = 0; j < n2; ++j) {
-        const double alpha = tile[(size_t)j * n2 + j];
-        double sl = 0.0;
-        for (int i = j + 1 + tid; i < n1; i += kGlobThreads) {
-            const double v = tile[(size_t)i * n2 + j];
-            sl += v * v;
-        }
-        const double sigma = glob_block_sum(sl, red);
-        double beta = alpha, t = 0.0, scale = 0.0;
-        if (sigma != 0.0) {
-            const double nrm = sqrt(alpha * alpha + sigma);
-            beta = alpha >= 0.0 ? -nrm : nrm;
-            t = (beta - alpha) / beta;
-            scale = 1.0 / (alpha - beta);
-        }
-        for (int i = j + 1 + tid; i < n1; i += kGlobThreads) tile[(size_t)i * n2 + j] *= scale;
-        if (tid == 0) {
-            tile[(size_t)j * n2 + j] = beta;
-            tau[j] = t;
-        }
-        __syncthreads();
-        if (t == 0.0) continue;
-        // w_c = tau * (A[j][c] + sum_{i>j} v_i A[i][c]) for c > j, 32 columns a pass
-        for (int cb = j + 1; cb < n2; cb += kGlobCols) {
-            const int c = cb + lane32;
-            double part = 0.0;
-            if (c < n2) {
-                for (int i = j + rg; i < n1; i += kGlobRowGroups) {
-                    const double v = i == j ? 1.0 : tile[(size_t)i * n2 + j];
-                    part += v * tile[(size_t)i * n2 + c];
-                }
-            }
-            red[tid + 64] = part;  // red[64 .. 64 + kGlobThreads)
-            __syncthreads();
-            if (rg == 0 && c < n2) {
-                double w = 0.0;
-                for (int q = 0; q < kGlobRowGroups; ++q) w += red[64 + q * kGlobCols + lane32];
-                wv[c] = w * t;
-            }
-            __syncthreads();
-        }
-        // A[i][c] -= w_c v_i for i >= j, c > j
-        const int wd = n2 - j - 1;
-        for (long long idx = tid; idx < (long long)(n1 - j) * wd; idx += kGlobThreads) {
-            const int i = j + (int)(idx / wd), c = j + 1 + (int)(idx % wd);
-            const double v = i == j ? 1.0 : tile[(size_t)i * n2 + j];
-            tile[(size_t)i * n2 + c] -= wv[c] * v;
-        }
-        __syncthreads();
-    }
-    // y = Q^T e_E
-    for (int i = tid; i < n1; i += kGlobThreads) yv[i] = (i == E) ? 1.0 : 0.0;
-    __syncthreads();
-    for (int j = 0; j < n2 && E >= 0; ++j) {
-        const double t = tau[j];
-        double sl = 0.0;
-        for (int i = j + tid; i < n1; i += kGlobThreads) {
-            const double v = i == j ? 1.0 : tile[(size_t)i * n2 + j];
-            sl += v * yv[i];
-        }
-        const double w = glob_block_sum(sl, red) * t;
-        for (int i = j + tid; i < n1; i += kGlobThreads) {
-            const double v = i == j ? 1.0 : tile[(size_t)i * n2 + j];
-            yv[i] -= w * v;
-        }
-        __syncthreads();
-    }
+    glob_householder(tile, m, n2, ld, red, wsh, tsh, wp);
     // back substitution, column oriented: x_c = y_c / R_cc, then y_q -= R_qc x_c
     for (int c = n2 - 1; c >= 0; --c) {
-        const double xc = E < 0 ? 0.0 : yv[c] / tile[(size_t)c * n2 + c];
-        for (int q = tid; q < c; q += kGlobThreads) yv[q] -= tile[(size_t)q * n2 + c] * xc;
+        const double xc = tile[(size_t)c * ld + n2] / tile[(size_t)c * ld + c];
+        for (int q = tid; q < c; q += kGlobThreads) tile[(size_t)q * ld + n2] -= tile[(size_t)q * ld + c] * xc;
         if (tid == 0) xv[c] = xc;
         __syncthreads();
     }
@@ -1097,6 +1215,9 @@ __global__ __launch_bounds__(kGlobThreads) void spai_global_tier_kernel(ColumnAr
     __shared__ double red[64 + kGlobThreads];
     __shared__ int ired[kGlobThreads / 64];
     __shared__ int sh[2];
+    __shared__ double wsh[kGlobThreads / 64 * kGlobNB * kGlobCols];  // W partials per wavefront
+    __shared__ double tsh[kGlobNB * kGlobNB];                        // T of the panel
+    __shared__ double wp[kGlobNB * kGlobCols];                       // T^T W
     const int slot = blockIdx.x;
     if (DYNAMIC) {
         const int count = *a.count;
@@ -1106,11 +1227,11 @@ __global__ __launch_bounds__(kGlobThreads) void spai_global_tier_kernel(ColumnAr
             const int idx = sh[1];
             __syncthreads();
             if (idx >= count) break;
-            solve_column_global<MT>(a, a.list[idx], g, slot, red, ired, sh);
+            solve_column_global<MT>(a, a.list[idx], g, slot, red, ired, sh, wsh, tsh, wp);
         }
     } else {
         for (int k = a.col_begin + blockIdx.x; k < a.col_end; k += gridDim.x)
-            solve_column_global<MT>(a, k, g, slot, red, ired, sh);
+            solve_column_global<MT>(a, k, g, slot, red, ired, sh, wsh, tsh, wp);
     }
 }
 
@@ -1580,11 +1701,14 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
         const long long maxT = std::max(1, hc[2 * kTiers]);
         g.n2_cap = std::max(1, hc[2 * kTiers + 1]);
         g.n1_cap = (int)std::max<long long>(std::min<long long>(maxT, std::max(p.n, 1)), g.n2_cap);
-        g.tile_cap = (long long)std::max<unsigned long long>(1ull, dynamic ? h64[1] : h64[2]);
+        // max(n1, n2) x (n2 + 1) <= the planner's bound max(min(T, n), n2) x n2
+        // plus one column of max(min(T, n), n2) rows
+        g.tile_cap = (long long)std::max<unsigned long long>(1ull, dynamic ? h64[1] : h64[2]) +
+                     std::max(g.n1_cap, g.n2_cap);
         long long kp = 1;
         while (kp < maxT) kp <<= 1;
         g.key_cap = kp;
-        g.vec_cap = (long long)g.n1_cap + 3LL * g.n2_cap;
+        g.vec_cap = (long long)g.n2_cap;
         g.int_cap = g.key_cap + 1 + g.n2_cap + 1;
         const long long slot_bytes = 8 * (g.tile_cap + g.vec_cap + g.key_cap) + 4 * g.int_cap;
         const long long budget = 8LL << 30;  // scratch for concurrent wide columns
