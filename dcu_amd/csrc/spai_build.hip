@@ -872,6 +872,7 @@ constexpr int kGlobThreads = 512;
 constexpr int kGlobCols = 32;                           // columns per pass
 constexpr int kGlobRowGroups = kGlobThreads / kGlobCols;  // 16 row groups
 constexpr int kGlobNB = 16;                             // panel width of the blocked QR
+constexpr int kGlobChunk = 128;                         // rows of V staged in LDS at a time
 static_assert(kGlobNB <= kGlobRowGroups, "one row group per reflector when forming T^T W");
 
 struct GlobalScratch {
@@ -978,8 +979,8 @@ __device__ __forceinline__ void glob_block_sum_vec(double (&v)[N], int cnt, doub
 // V^T, the reference LAPACK dlarft's forward columnwise recurrence); the
 // trailing columns are then updated 32 at a time as A2 -= V (T^T (V^T A2)),
 // reading V and A2 twice per panel instead of twice per column.
-__device__ __forceinline__ void glob_householder(double *tile, int m, int n, int ld, double *red, double *wsh, double *tsh,
-                                 double *wp) {
+__device__ __forceinline__ void glob_householder(double *tile, int m, int n, int ld, double *red, double *wsh,
+                                                 double *tsh, double *wp, double *vsh) {
     const int tid = threadIdx.x;
     constexpr int NB = kGlobNB;
     for (int j0 = 0; j0 < n; j0 += NB) {
@@ -1043,28 +1044,45 @@ __device__ __forceinline__ void glob_householder(double *tile, int m, int n, int
             }
             __syncthreads();
         }
-        // ---- trailing columns j0 + nb .. n (n: the right-hand side)
+        // ---- trailing columns j0 + nb .. n (n: the right-hand side), 32 at a
+        // time.  The rows go in chunks of kGlobChunk: the chunk's V (nb
+        // columns, 0 above the diagonal, 1 on it) is staged in LDS once for the
+        // whole workgroup, and each thread has its kGlobChunk / 16 rows of A2
+        // in flight at once.
         const int cc = tid % kGlobCols, rg = tid / kGlobCols, lane = tid & 63, wv = tid >> 6;
+        constexpr int RPT = kGlobChunk / kGlobRowGroups;  // rows per thread and chunk
+        auto stage_v = [&](int r0, int rn) {
+            __syncthreads();  // the previous chunk's readers are done
+            for (int idx = tid; idx < kGlobChunk * NB; idx += kGlobThreads) {
+                const int rr = idx / NB, i = idx - rr * NB;
+                const int r = r0 + rr, d = r - (j0 + i);
+                double v = 0.0;
+                if (rr < rn && i < nb && d >= 0) v = d == 0 ? 1.0 : tile[(size_t)r * ld + j0 + i];
+                vsh[idx] = v;
+            }
+            __syncthreads();
+        };
         for (int cb = j0 + nb; cb <= n; cb += kGlobCols) {
             const int col = cb + cc;
             const bool live = col <= n;
-            // W = V^T A2 (nb x 32): rows r >= j0, V(r, i) = 0 above row j0 + i,
-            // 1 on it, the stored reflector below
+            // W = V^T A2 (nb x 32)
             double w[NB];
 #pragma unroll
             for (int i = 0; i < NB; ++i) w[i] = 0.0;
-            if (live) {
-                for (int r = j0 + rg; r < m; r += kGlobRowGroups) {
-                    const double *row = tile + (size_t)r * ld;
-                    const double ar = row[col];
+            for (int r0 = j0; r0 < m; r0 += kGlobChunk) {
+                const int rn = min(kGlobChunk, m - r0);
+                stage_v(r0, rn);
+                double ar[RPT];
 #pragma unroll
-                    for (int i = 0; i < NB; ++i) {
-                        if (i < nb) {
-                            const int d = r - (j0 + i);
-                            const double vri = d < 0 ? 0.0 : (d == 0 ? 1.0 : row[j0 + i]);
-                            w[i] += vri * ar;
-                        }
-                    }
+                for (int u = 0; u < RPT; ++u) {
+                    const int rr = rg + u * kGlobRowGroups;
+                    ar[u] = (live && rr < rn) ? tile[(size_t)(r0 + rr) * ld + col] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < RPT; ++u) {
+                    const double *vr = vsh + (rg + u * kGlobRowGroups) * NB;
+#pragma unroll
+                    for (int i = 0; i < NB; ++i) w[i] += vr[i] * ar[u];
                 }
             }
             // the two row groups of a wavefront, then the 8 wavefronts
@@ -1089,29 +1107,38 @@ __device__ __forceinline__ void glob_householder(double *tile, int m, int n, int
             }
             __syncthreads();
             // A2 -= V W'
-            if (live) {
-                for (int r = j0 + rg; r < m; r += kGlobRowGroups) {
-                    double *row = tile + (size_t)r * ld;
-                    double u = 0.0;
+            double wc[NB];
 #pragma unroll
-                    for (int i = 0; i < NB; ++i) {
-                        if (i < nb) {
-                            const int d = r - (j0 + i);
-                            const double vri = d < 0 ? 0.0 : (d == 0 ? 1.0 : row[j0 + i]);
-                            u += vri * wp[i * kGlobCols + cc];
-                        }
+            for (int i = 0; i < NB; ++i) wc[i] = i < nb ? wp[i * kGlobCols + cc] : 0.0;
+            for (int r0 = j0; r0 < m; r0 += kGlobChunk) {
+                const int rn = min(kGlobChunk, m - r0);
+                stage_v(r0, rn);
+                if (live) {
+                    double ar[RPT];
+#pragma unroll
+                    for (int u = 0; u < RPT; ++u) {
+                        const int rr = rg + u * kGlobRowGroups;
+                        ar[u] = rr < rn ? tile[(size_t)(r0 + rr) * ld + col] : 0.0;
                     }
-                    row[col] -= u;
+#pragma unroll
+                    for (int u = 0; u < RPT; ++u) {
+                        const int rr = rg + u * kGlobRowGroups;
+                        const double *vr = vsh + rr * NB;
+                        double x = 0.0;
+#pragma unroll
+                        for (int i = 0; i < NB; ++i) x += vr[i] * wc[i];
+                        if (rr < rn) tile[(size_t)(r0 + rr) * ld + col] = ar[u] - x;
+                    }
                 }
             }
-            __syncthreads();
         }
+        __syncthreads();  // the next panel (or the back substitution) reads the updated columns
     }
 }
 
 template <class MT>
 __device__ __forceinline__ void solve_column_global(const ColumnArgs &a, int k, const GlobalScratch &g, int slot, double *red,
-                                    int *ired, int *sh, double *wsh, double *tsh, double *wp) {
+                                    int *ired, int *sh, double *wsh, double *tsh, double *wp, double *vsh) {
     const int tid = threadIdx.x;
     const int c0 = a.pcolptr[k];
     const int n2 = a.pcolptr[k + 1] - c0;
@@ -1197,7 +1224,7 @@ __device__ __forceinline__ void solve_column_global(const ColumnArgs &a, int k, 
         tile[(size_t)pos[e] * ld + j] = a.val[a.colptr[jr] + ee - offs[j]];
     }
     __syncthreads();
-    glob_householder(tile, m, n2, ld, red, wsh, tsh, wp);
+    glob_householder(tile, m, n2, ld, red, wsh, tsh, wp, vsh);
     // back substitution, column oriented: x_c = y_c / R_cc, then y_q -= R_qc x_c
     for (int c = n2 - 1; c >= 0; --c) {
         const double xc = tile[(size_t)c * ld + n2] / tile[(size_t)c * ld + c];
@@ -1210,6 +1237,52 @@ __device__ __forceinline__ void solve_column_global(const ColumnArgs &a, int k, 
     __syncthreads();
 }
 
+// Orders the global tier's column list by decreasing work |J_k| x T_k (ties
+// by column), so that the widest columns start first, each on a workgroup of
+// its own, rather than wherever they fall in the queue.  One workgroup; a
+// list longer than kWideSortMax keeps the planner's order.
+constexpr int kWideSortMax = 4096;
+__global__ __launch_bounds__(1024) void spai_sort_wide_columns(int *list, const int *count, const int *colptr,
+                                                               const int *pcolptr, const int *prowidx) {
+    __shared__ unsigned long long key[kWideSortMax];
+    const int cnt = *count;
+    if (cnt < 2 || cnt > kWideSortMax) return;
+    int P = 1;
+    while (P < cnt) P <<= 1;
+    for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        unsigned long long kk = ~0ull;
+        if (i < cnt) {
+            const int k = list[i];
+            const int c0 = pcolptr[k], n2 = pcolptr[k + 1] - c0;
+            unsigned long long t = 0ull;
+            for (int j = 0; j < n2; ++j) {
+                const int r = prowidx[c0 + j];
+                t += (unsigned long long)(colptr[r + 1] - colptr[r]);
+            }
+            const unsigned long long work = min((unsigned long long)n2 * t, 0xffffffffull);
+            kk = ((0xffffffffull - work) << 32) | (unsigned)k;  // ascending key: descending work
+        }
+        key[i] = kk;
+    }
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int q = threadIdx.x; q < P / 2; q += blockDim.x) {
+                const int i = 2 * stride * (q / stride) + (q % stride);
+                const int l = i + stride;
+                const bool up = (i & size) == 0;
+                const unsigned long long x = key[i], y = key[l];
+                if ((x > y) == up) {
+                    key[i] = y;
+                    key[l] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) list[i] = (int)(unsigned)(key[i] & 0xffffffffull);
+}
+
 template <class MT, bool DYNAMIC>
 __global__ __launch_bounds__(kGlobThreads) void spai_global_tier_kernel(ColumnArgs a, GlobalScratch g) {
     __shared__ double red[64 + kGlobThreads];
@@ -1218,6 +1291,7 @@ __global__ __launch_bounds__(kGlobThreads) void spai_global_tier_kernel(ColumnAr
     __shared__ double wsh[kGlobThreads / 64 * kGlobNB * kGlobCols];  // W partials per wavefront
     __shared__ double tsh[kGlobNB * kGlobNB];                        // T of the panel
     __shared__ double wp[kGlobNB * kGlobCols];                       // T^T W
+    __shared__ double vsh[kGlobChunk * kGlobNB];                     // a chunk of V
     const int slot = blockIdx.x;
     if (DYNAMIC) {
         const int count = *a.count;
@@ -1227,11 +1301,11 @@ __global__ __launch_bounds__(kGlobThreads) void spai_global_tier_kernel(ColumnAr
             const int idx = sh[1];
             __syncthreads();
             if (idx >= count) break;
-            solve_column_global<MT>(a, a.list[idx], g, slot, red, ired, sh, wsh, tsh, wp);
+            solve_column_global<MT>(a, a.list[idx], g, slot, red, ired, sh, wsh, tsh, wp, vsh);
         }
     } else {
         for (int k = a.col_begin + blockIdx.x; k < a.col_end; k += gridDim.x)
-            solve_column_global<MT>(a, k, g, slot, red, ired, sh, wsh, tsh, wp);
+            solve_column_global<MT>(a, k, g, slot, red, ired, sh, wsh, tsh, wp, vsh);
     }
 }
 
@@ -1727,6 +1801,11 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
             ga.list = lists_.get() + (size_t)(kTiers - 1) * list_stride;
             ga.count = counters_.get() + (kTiers - 1);
             ga.head = counters_.get() + kTiers + (kTiers - 1);
+            if (global_cols > 1 && global_cols <= kWideSortMax) {
+                hipLaunchKernelGGL(spai_sort_wide_columns, dim3(1), dim3(1024), 0, st, (int *)ga.list, ga.count,
+                                   p.colptr, pc, pr);
+                SPAI_CHECK_LAUNCH();
+            }
         }
         auto launch = [&](auto mt_tag) {
             using MT = decltype(mt_tag);

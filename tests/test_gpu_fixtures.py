@@ -87,6 +87,49 @@ def test_global_tier_synthetic(ctx, schedule):
     assert np.array_equal(S.row_set_sizes(), n1_ref)
     rep = S.build(D.PRECISION_MIXED, schedule)
     assert col_err(A, S.m_values(), ref) <= 4 * 2.0 ** -23
+    # the global tier's blocked QR runs several panels per column here: two
+    # builds give the same M bit for bit (every workgroup sum has a fixed order)
+    S.build(D.PRECISION_FP64, schedule)
+    assert np.array_equal(S.m_values(), m)
+
+
+def test_global_tier_panels_static(ctx):
+    """Every column through the global tier (balanced_columns on a matrix
+    whose widest column needs it), with |J_k| of 1..40 so that columns have
+    one, two and three QR panels: M within 1e-10 of the oracle, and bit-equal
+    over repeated builds."""
+    D_ = D
+    rng = np.random.default_rng(17)
+    n = 3000
+    cols = [dict() for _ in range(n)]
+    for j in range(n):
+        width = 80 if j == n - 1 else int(rng.integers(1, 41))
+        for i in rng.choice(n, size=min(width, n), replace=False):
+            cols[j][int(i)] = rng.uniform(-1, 1)
+    rowsum = np.zeros(n)
+    for j in range(n):
+        for i, v in cols[j].items():
+            rowsum[i] += abs(v)
+    for i in range(n):
+        cols[i][i] = rowsum[i] + 1.0
+    cp, ri, va = [0], [], []
+    for j in range(n):
+        for i in sorted(cols[j]):
+            ri.append(i)
+            va.append(cols[j][i])
+        cp.append(len(ri))
+    A = D_.CSC(n, np.array(cp, dtype=np.int32), np.array(ri, dtype=np.int32), np.array(va))
+    S = D_.System(ctx, A)
+    ref = O.spai(A.n, A.colptr, A.rowidx, A.val, 0)
+    rep = S.build(D_.PRECISION_FP64, D_.SCHEDULE_BALANCED_COLUMNS)
+    m = S.m_values()
+    err = col_err(A, m, ref)
+    print(f"static global tier: static tier {rep['static_tier']} err {err:.2e}", flush=True)
+    assert rep["static_tier"] == 4 and err <= 1e-10
+    for _ in range(3):
+        S.build(D_.PRECISION_FP64, D_.SCHEDULE_BALANCED_COLUMNS)
+        assert np.array_equal(S.m_values(), m)
+    S.close()
 
 
 @pytest.mark.parametrize("name", ["fill_3elt", "circuit_2"])
