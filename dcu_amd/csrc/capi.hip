@@ -993,7 +993,7 @@ int spai_system_solve_noalloc(spai_system *sys, const double *b, double *x, cons
     if (opt->precondition && sys->m_precision < 0) return fail(SPAI_ERR_NOT_BUILT, "M has not been built");
     spai_context *ctx = sys->ctx;
     if (!ctx->scratch) return fail(SPAI_ERR_HIP, "no device scratch: " + ctx->scratch_error);
-    const size_t need = BiCGStab::workspace_bytes(sys->n, sys->nsys, sys->tiles.nnodes);
+    const size_t need = BiCGStab::workspace_bytes(sys->n, sys->nsys, sys->tiles);
     if (need > kNoAllocScratchBytes)
         return fail(SPAI_ERR_UNSUPPORTED, "the allocation-free solve needs " + std::to_string(need) +
                                               " bytes of scratch; the device scratch holds " +

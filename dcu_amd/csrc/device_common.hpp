@@ -131,6 +131,14 @@ SPAI_DEV T group_allsum(T v) {
     return v;
 }
 
+// A value every lane of the wavefront holds alike, marked uniform (scalar
+// registers) for the code that uses it.
+SPAI_DEV double wave_uniform(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)b), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 template <int W>
 SPAI_DEV int group_max(int v) {
 #pragma unroll

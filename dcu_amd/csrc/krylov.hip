@@ -44,11 +44,20 @@
 
 namespace spai {
 
-// Reductions run up a tree per system: level 0 holds one partial per unit,
-// each further level one partial per group of up to kFanIn consecutive nodes
-// of the level below, and the root's value goes to the op's finish step.
-// Same-address atomics serialise at the memory side, so a counter that sees
-// few arrivals (kFanIn) finishes sooner than one shared by many units.
+// Two kinds of reduction.  The iteration kernels (K1, K2, K4) publish one
+// partial per group of units and leave the total to the kernel that needs it
+// next: each of its workgroups sums the system's group partials in a fixed
+// order, and the system's first unit commits the scalars formed from them to
+// the system's state.  So no iteration kernel waits for the last of its
+// workgroups.  A group of one unit (systems of at most kMaxGroups units)
+// stores its partial directly; larger groups count arrivals and the last one
+// sums the group's unit partials.
+// The set-up reductions (r0 and the true residual) run up a tree per system:
+// level 0 holds one partial per unit, each further level one partial per
+// group of up to kFanIn consecutive nodes of the level below, and the root's
+// value goes to the op's finish step.  Same-address atomics serialise at the
+// memory side, so a counter that sees few arrivals (kFanIn) finishes sooner
+// than one shared by many units.
 struct Ctx {
     SysState *st;
     const int *sys_units;  // nsys + 1: first unit of each system
@@ -59,6 +68,11 @@ struct Ctx {
     int nsys;
     double *hist;          // optional ||r_k|| history, hist_stride entries per system
     int hist_stride;
+    const int *sys_groups;  // nsys + 1: first partial-sum group of each system
+    double *gpart;          // kSlots x ngroups group partials (slot-major)
+    double *upart;          // kSlots x nunits unit partials of groups of several units (slot-major)
+    int *gcount;            // ngroups arrival counters
+    int ngroups, nunits;
 };
 
 namespace {
@@ -86,6 +100,10 @@ constexpr int kSplitUnitsTiles = SPAI_SPLIT_UNITS_TILES;
 #endif
 constexpr int kCompactEvery = SPAI_COMPACT_EVERY;  // iterations between active-list rebuilds inside a graph
 constexpr int kMaxRed = 4;          // most values any kernel reduces (partials per tree node)
+// Slots of the iteration partials: K1's (r, r), K2's (r*, a), K4's (t, s),
+// (t, t), (r*, s), (r*, t).
+enum { kSlotRR = 0, kSlotRsA = 1, kSlotOmega = 2, kSlots = 6 };
+constexpr int kTakePer = kMaxGroups / kThreads;  // group partials per thread a consuming workgroup loads
 
 // ------------------------------------------------------------ transpose ----
 __global__ void k_fill_int(int *p, int v, long long n) {
@@ -260,11 +278,13 @@ __device__ __forceinline__ void wave_sum(double (&v)[NRED]) {
 // counts; returns (to the whole wavefront) whether this was the last of
 // `expect` arrivals, in which case the counter is reset for the next launch.
 template <int NRED>
-__device__ __forceinline__ bool publish_and_count(double *dst, const double (&v)[NRED], int *counter, int expect) {
+__device__ __forceinline__ bool publish_and_count(double *dst, const double (&v)[NRED], int *counter, int expect,
+                                                  size_t stride = 1) {
     int last = 0;
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-        for (int q = 0; q < NRED; ++q) __hip_atomic_store(&dst[q], v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int q = 0; q < NRED; ++q)
+            __hip_atomic_store(&dst[q * stride], v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
         const int prev = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         last = prev == expect - 1;
@@ -328,6 +348,192 @@ __device__ __forceinline__ void reduce_system(const Ctx &c, int unit, int s, dou
         n = (n + kFanIn - 1) / kFanIn;
     }
     if (lane == 0) op.finish(c, s, val);
+}
+
+// ------------------------------------------------ iteration reductions ----
+// What an op does with the iteration partials (members it may declare):
+//   kTree     reduces up the tree (reduce_system) and finishes there
+//   kPublish  leading values of its NRED sums published in slots
+//             [kSlot, kSlot + kPublish) for the next kernels
+//   kTake     consumes slots [kTakeSlot, kTakeSlot + kTake): every workgroup
+//             sums the system's group partials before its rows (need_take()
+//             and take(v))
+//   kFold     the committing workgroup folds values preloaded by preload()
+//             into its sums (the last of its NRED values) before they are
+//             reduced, so that the loads overlap the rows
+//   kCommit   the system's first unit calls finish(c, s, sums) once its
+//             workgroup's sums are reduced
+#define SPAI_OP_TRAIT(Name, Member, Type, Default)                                                  \
+    template <class Op, class = void>                                                               \
+    struct Name : std::integral_constant<Type, Default> {};                                         \
+    template <class Op>                                                                             \
+    struct Name<Op, std::void_t<decltype(Op::Member)>> : std::integral_constant<Type, Op::Member> {};
+SPAI_OP_TRAIT(OpTree, kTree, bool, false)
+SPAI_OP_TRAIT(OpPublish, kPublish, int, 0)
+SPAI_OP_TRAIT(OpTake, kTake, int, 0)
+SPAI_OP_TRAIT(OpFold, kFold, bool, false)
+SPAI_OP_TRAIT(OpCommit, kCommit, bool, false)
+#undef SPAI_OP_TRAIT
+
+// The partial-sum group of `unit` (a unit of system s).
+struct UnitGroup {
+    int g;      // group index
+    int first;  // first unit of the group
+    int size;   // units in the group
+};
+__device__ __forceinline__ UnitGroup unit_group(const Ctx &c, int s, int unit) {
+    const int u0 = c.sys_units[s], nu = c.sys_units[s + 1] - u0;
+    const int G = group_units(nu);
+    const int local = (unit - u0) / G;
+    return {c.sys_groups[s] + local, u0 + local * G, min(G, nu - local * G)};
+}
+
+// This thread's share of system s's group partials of slots [slot, slot +
+// NT): groups g0 + t, g0 + t + blockDim.x, ... (PER of them: a system has
+// at most kMaxGroups groups).  Plain loads issued at once; summed where
+// part() is used.
+template <int NT, int PER = kTakePer>
+struct GroupShare {
+    double v[NT][PER];
+    __device__ __forceinline__ void load(const Ctx &c, int s, int slot) {
+        const int g0 = c.sys_groups[s], g1 = c.sys_groups[s + 1];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int g = g0 + (int)threadIdx.x + k * (int)blockDim.x;
+#pragma unroll
+            for (int q = 0; q < NT; ++q) v[q][k] = g < g1 ? c.gpart[(size_t)(slot + q) * c.ngroups + g] : 0.0;
+        }
+    }
+    __device__ __forceinline__ double part(int q) const {
+        double a = v[q][0];
+#pragma unroll
+        for (int k = 1; k < PER; ++k) a += v[q][k];
+        return a;
+    }
+};
+
+// Sum over the workgroup (W wavefronts) that every thread receives: wave
+// sums, then the wavefronts' sums in wavefront order.  Every workgroup of W
+// wavefronts forms the same value from the same inputs.
+template <int NT, int W>
+__device__ __forceinline__ void block_allsum(double (&v)[NT], double *shk) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    wave_sum<NT>(v);
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NT; ++q) shk[q * W + wave] = v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
+        double t = shk[q * W];
+#pragma unroll
+        for (int w = 1; w < W; ++w) t += shk[q * W + w];
+        v[q] = t;
+    }
+}
+
+// The scalars an op forms from the previous kernels' partials (every
+// workgroup alike), in two steps: take_issue requests the group partials
+// (and the committer's preloads) first thing, before the matrix stream, so
+// that waiting for them does not wait for the stream; take_finish sums them
+// once the op has read the system's state.
+template <class Op, int W>
+using TakeShare = GroupShare<(OpTake<Op>::value > 0 ? OpTake<Op>::value : 1), kMaxGroups / (W * 64)>;
+
+template <int W, class Op>
+__device__ __forceinline__ void take_issue(const Ctx &c, int s, int unit, Op &op, TakeShare<Op, W> &gs) {
+    if constexpr (OpTake<Op>::value > 0) gs.load(c, s, Op::kTakeSlot);
+    if constexpr (OpFold<Op>::value) {
+        if (unit == c.sys_units[s]) op.preload(c, s);
+    }
+}
+
+template <int W, class Op>
+__device__ __forceinline__ void take_finish(Op &op, const TakeShare<Op, W> &gs, double *shk) {
+    constexpr int NT = OpTake<Op>::value;
+    if constexpr (NT > 0) {
+        if (op.need_take()) {  // uniform over the workgroup
+            double v[NT];
+#pragma unroll
+            for (int q = 0; q < NT; ++q) v[q] = gs.part(q);
+            block_allsum<NT, W>(v, shk);
+            op.take(v);
+        }
+    }
+}
+
+// Sum of src[q * stride + t] over t in [first, last) by one wavefront, in a
+// fixed order.
+template <int NP>
+__device__ __forceinline__ void wave_sum_units(const double *src, size_t stride, int first, int last,
+                                               double (&acc)[NP]) {
+#pragma unroll
+    for (int q = 0; q < NP; ++q) acc[q] = 0.0;
+    for (int t = first + (int)(threadIdx.x & 63); t < last; t += 64) {
+#pragma unroll
+        for (int q = 0; q < NP; ++q) acc[q] += load_coherent(&src[(size_t)q * stride + t]);
+    }
+    wave_sum<NP>(acc);
+}
+
+// After the rows.  Tree ops: reduce_system.  Otherwise the workgroup's NRED
+// sums are reduced in the block (wavefront 0 keeps them), the first
+// kPublish go to the unit's group, and the system's first unit commits.
+template <int W, class Op>
+__device__ __forceinline__ void finish_unit(const Ctx &c, int unit, int s, double (&v)[Op::NRED > 0 ? Op::NRED : 1],
+                                            Op &op, double *sh) {
+    constexpr int NRED = Op::NRED;
+    if constexpr (OpTree<Op>::value) {
+        reduce_system<NRED, W>(c, unit, s, v, op, sh);
+    } else if constexpr (NRED > 0) {
+        constexpr int NP = OpPublish<Op>::value;
+        static_assert(NP <= NRED, "published values are the leading sums");
+        if constexpr (OpFold<Op>::value) op.fold(v);
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        wave_sum<NRED>(v);
+        if (lane == 0) {
+#pragma unroll
+            for (int q = 0; q < NRED; ++q) sh[q * W + wave] = v[q];
+        }
+        __syncthreads();
+        if (wave != 0) return;
+        double val[NRED];
+#pragma unroll
+        for (int q = 0; q < NRED; ++q) {
+            val[q] = 0.0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) val[q] += sh[q * W + w];
+        }
+        if constexpr (NP > 0) {
+            const UnitGroup ug = unit_group(c, s, unit);
+            if (ug.size == 1) {
+                if (lane == 0) {
+#pragma unroll
+                    for (int q = 0; q < NP; ++q) c.gpart[(size_t)(Op::kSlot + q) * c.ngroups + ug.g] = val[q];
+                }
+            } else {
+                double pub[NP];
+#pragma unroll
+                for (int q = 0; q < NP; ++q) pub[q] = val[q];
+                if (publish_and_count<NP>(&c.upart[(size_t)Op::kSlot * c.nunits + unit], pub, &c.gcount[ug.g],
+                                          ug.size, (size_t)c.nunits)) {
+                    double tot[NP];
+                    wave_sum_units<NP>(c.upart + (size_t)Op::kSlot * c.nunits, (size_t)c.nunits, ug.first,
+                                       ug.first + ug.size, tot);
+                    if (lane == 0) {
+#pragma unroll
+                        for (int q = 0; q < NP; ++q) c.gpart[(size_t)(Op::kSlot + q) * c.ngroups + ug.g] = tot[q];
+                    }
+                }
+            }
+        }
+        if constexpr (OpCommit<Op>::value) {
+            if (lane == 0 && unit == c.sys_units[s]) op.finish(c, s, val);
+        }
+    } else if constexpr (OpCommit<Op>::value) {
+        if (threadIdx.x == 0 && unit == c.sys_units[s]) op.finish(c, s, v);
+    }
 }
 
 // ------------------------------------------------------- unit kernels ----
@@ -424,10 +630,13 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
     __shared__ int rp[HAS_MATRIX ? kRowsPerTile + 1 : 1];
     __shared__ double rowsum[HAS_MATRIX ? kRowsPerTile : 1];  // sums of the long rows
     __shared__ double sh[(NRED > 0 ? NRED : 1) * 4];
+    __shared__ double shk[(OpTake<Op>::value > 0 ? OpTake<Op>::value : 1) * 4];
     const int unit = unit_of(uv);
     if (unit < 0) return;
     const int s = uv.sys[unit];
     if (!op.enabled(c.st[s])) return;
+    TakeShare<Op, kThreads / 64> gs;
+    take_issue<kThreads / 64>(c, s, unit, op, gs);
     op.load(c.st[s]);
     const int tid = threadIdx.x;
     double acc[NRED > 0 ? NRED : 1];
@@ -436,17 +645,12 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
     const int r0 = uv.row[unit], r1 = uv.row[unit + 1];
     const int nrows = r1 - r0;
     if (!HAS_MATRIX) {
+        take_finish<kThreads / 64>(op, gs, shk);
         for (int i = r0 + tid; i < r1; i += kThreads) op.row(i, 0.0, op.pre(i, acc), acc);
     } else {
         const int p0 = ptr[r0];
         const int cnt = ptr[r1] - p0;
         if (cnt <= kLdsProducts && nrows <= kRowsPerTile) {
-            typename Op::Pre pre[kRowsPerThread];
-#pragma unroll
-            for (int k = 0; k < kRowsPerThread; ++k) {
-                const int i = r0 + tid + k * kThreads;
-                if (i < r1) pre[k] = op.pre(i, acc);
-            }
             for (int k = tid; k <= nrows; k += kThreads) rp[k] = ptr[r0 + k] - p0;
             constexpr int U = kLdsProducts / kThreads;
             int jj[U];
@@ -456,6 +660,14 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
                 const int q = u * kThreads + tid;
                 jj[u] = q < cnt ? stream_load(&idx[p0 + q]) : 0;
                 vv[u] = q < cnt ? stream_load(&val[p0 + q]) : (MT)0;
+            }
+            // the scalars (pre and the gathers use them) while the tile streams in
+            take_finish<kThreads / 64>(op, gs, shk);
+            typename Op::Pre pre[kRowsPerThread];
+#pragma unroll
+            for (int k = 0; k < kRowsPerThread; ++k) {
+                const int i = r0 + tid + k * kThreads;
+                if (i < r1) pre[k] = op.pre(i, acc);
             }
             double gv[U];
 #pragma unroll
@@ -506,6 +718,7 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
         } else {
             // a row longer than the LDS buffer (it has a tile of its own):
             // the whole workgroup reduces it
+            take_finish<kThreads / 64>(op, gs, shk);
             for (int i = r0; i < r1; ++i) {
                 const int b = ptr[i], e = ptr[i + 1];
                 double part[1] = {0.0};
@@ -515,7 +728,7 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
             }
         }
     }
-    if constexpr (NRED > 0) reduce_system<NRED, kThreads / 64>(c, unit, s, acc, op, sh);
+    finish_unit<kThreads / 64>(c, unit, s, acc, op, sh);
 }
 
 // SELL-64.  Wavefront w of the workgroup takes kSellSlicesPerWave
@@ -585,10 +798,13 @@ __global__ __launch_bounds__(kSellThreads) SPAI_SELL_OCCUPANCY void k_sell(const
     constexpr int U = kSellUnroll;
     constexpr bool STAGE = HAS_MATRIX && sell_stage<Op>();
     __shared__ double sh[(NRED > 0 ? NRED : 1) * kSellWaves];
+    __shared__ double shk[(OpTake<Op>::value > 0 ? OpTake<Op>::value : 1) * kSellWaves];
     __shared__ double sv[STAGE ? kSlicesPerUnit * kSliceRows : 1];  // the unit's gathered values
     const int unit = unit_of(uv);
     if (unit < 0) return;
     const int s = uv.sys[unit];
+    TakeShare<Op, kSellWaves> gs;
+    take_issue<kSellWaves>(c, s, unit, op, gs);
     const int lane = threadIdx.x & (kSliceRows - 1);
     const int sl_end = uv.slice[unit + 1];
     const int sl0 = uv.slice[unit] + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kSliceRows)) * SPW;
@@ -628,6 +844,7 @@ __global__ __launch_bounds__(kSellThreads) SPAI_SELL_OCCUPANCY void k_sell(const
     if (HAS_MATRIX) load_cols(0);
     if (!op.enabled(c.st[s])) return;
     op.load(c.st[s]);
+    take_finish<kSellWaves>(op, gs, shk);
     double acc[NRED > 0 ? NRED : 1];
 #pragma unroll
     for (int q = 0; q < (NRED > 0 ? NRED : 1); ++q) acc[q] = 0.0;
@@ -692,40 +909,15 @@ __global__ __launch_bounds__(kSellThreads) SPAI_SELL_OCCUPANCY void k_sell(const
         for (int j = 0; j < SPW; ++j)
             if (live[j]) op.row(row[j], sum[j], pre[j], acc);
     }
-    if constexpr (NRED > 0) reduce_system<NRED, kSellWaves>(c, unit, s, acc, op, sh);
+    finish_unit<kSellWaves>(c, unit, s, acc, op, sh);
 }
 
 // ------------------------------------------------------------- the ops ----
-// Stop test after an iteration's update (the reference's loop tail,
-// bicgstab_solver.h:224-231): converged, then non-finite, then MAX_ITER.
-__device__ __forceinline__ void finish_iteration(const Ctx &c, int s, SysState &S, double rr) {
-    S.rr = rr;
-    S.iters += 1;
-    const double err = sqrt(rr);
-    if (c.hist && S.iters <= c.hist_stride) c.hist[(size_t)s * c.hist_stride + S.iters - 1] = err;
-    bool stop = false;
-    if (err < S.tol) {
-        S.status = kConverged;
-        stop = true;
-    } else if (!(err == err) || !isfinite(S.beta)) {
-        S.status = kBreakdown;
-        stop = true;
-    } else if (S.iters >= S.max_iter - 1) {
-        // the reference returns MAX_ITER after MAX_ITER - 1 iterations
-        S.status = kMaxIter;
-        S.iters = S.max_iter;
-        stop = true;
-    }
-    if (stop) {
-        S.active = 0;
-        atomicSub(&c.counter[c.nsys], 1);
-    }
-}
-
 // r = b - A x0, r* = r, s = r, p = a = t = y = z = 0; rho = rr = (r, r); tol.
 // With alpha = omega = beta = 0 the first K1 reproduces r and forms p' = r.
 struct OpInit {
     static constexpr int NRED = 2;
+    static constexpr bool kTree = true;
     struct Pre {
         double b;
     };
@@ -759,6 +951,7 @@ struct OpInit {
     __device__ void finish(const Ctx &c, int s_, const double (&tot)[2]) const {
         SysState &S = c.st[s_];
         S.rho = tot[0];
+        S.rho_next = tot[0];
         S.rr = tot[0];
         S.alpha = 0.0;
         S.omega = 0.0;
@@ -766,8 +959,11 @@ struct OpInit {
         S.alpha_prev = 0.0;
         S.omega_prev = 0.0;
         S.true_rr = 0.0;
-        S.pending = 0;
         S.xdefer = 0;
+        S.k1_count = 0;
+        S.k2_count = 0;
+        S.k1_defer = 0;
+        S.iters_base = restart ? S.iters : 0;
         if (restart) {
             S.restarts += 1;
             if (sqrt(tot[0]) < S.tol) {
@@ -797,7 +993,12 @@ struct OpInit {
 };
 
 // K1.  Per row: the previous iteration's x += alpha y + omega z and
-// r = s - omega t, (r, r) reduced; p' = beta p + r - beta omega a.  YMODE:
+// r = s - omega t, (r, r) published; p' = beta p + r - beta omega a.  omega,
+// rho' and beta come from K4's partials (every workgroup forms them alike):
+// omega = (t, s) / (t, t), rho' = (r*, s) - omega (r*, t) = (r*, r),
+// beta = (rho' / rho) (alpha / omega); the first K1 after OpInit has
+// alpha = omega = beta = 0, reproduces r and forms p' = r.  The system's
+// first unit commits omega, beta and rho' (K2 makes rho' the rho).  YMODE:
 //   kYProduct  y = M p' gathered from p'(j) formed on the fly (bit-identical
 //              to the stored p'): the fused K1, four gathered vectors
 //   kYCopy     y = p' (no preconditioner)
@@ -810,6 +1011,9 @@ enum { kXOne = 0, kXDefer = 1, kXTwo = 2 };
 template <class MT, int YMODE, int XMODE>
 struct OpMapplyP {
     static constexpr int NRED = 1;
+    static constexpr int kPublish = 1, kSlot = kSlotRR;
+    static constexpr int kTake = 4, kTakeSlot = kSlotOmega;
+    static constexpr bool kCommit = true;
     static constexpr bool kStage = SPAI_STAGE_K1 != 0;
     struct Pre {
         double p;
@@ -820,16 +1024,30 @@ struct OpMapplyP {
     // XMODE, the x update: kXOne the previous iteration's (every K1);
     // kXDefer none, x is left for the next K1 (even iterations of a batch);
     // kXTwo the two previous iterations' (odd iterations of a batch)
-    double alpha, omega, alpha_prev, omega_prev, nomega, beta, nbw;
+    double alpha, omega, alpha_prev, omega_prev, nomega, beta, nbw, rho, rho_new;
+    int first;  // no K2 since OpInit
     __device__ bool enabled(const SysState &S) const { return S.active; }
     __device__ void load(const SysState &S) {
         alpha = S.alpha;
-        omega = S.omega;
-        alpha_prev = S.alpha_prev;
-        omega_prev = S.omega_prev;
-        nomega = -S.omega;
-        beta = S.beta;
-        nbw = -S.beta * S.omega;
+        rho = S.rho;
+        first = S.k2_count == 0;
+        if constexpr (XMODE == kXTwo) {
+            alpha_prev = S.alpha_prev;
+            omega_prev = S.omega_prev;
+        }
+        omega = 0.0;
+        beta = 0.0;
+        rho_new = rho;
+        nomega = -omega;
+        nbw = -beta * omega;
+    }
+    __device__ bool need_take() const { return !first; }
+    __device__ void take(const double (&v)[4]) {
+        omega = wave_uniform(v[0] / v[1]);
+        rho_new = wave_uniform(v[2] - omega * v[3]);
+        beta = wave_uniform((rho_new / rho) * (alpha / omega));
+        nomega = -omega;
+        nbw = wave_uniform(-beta * omega);
     }
     __device__ double rval(int j) const { return s[j] + nomega * t[j]; }
     __device__ double pval(int j, double rj) const { return (beta * pin[j] + rj) + nbw * a[j]; }
@@ -858,20 +1076,17 @@ struct OpMapplyP {
         if (YMODE == kYProduct) y[i] = mx;
         else if (YMODE == kYCopy) y[i] = pr.p;
     }
-    __device__ void finish(const Ctx &c, int s_, const double (&tot)[1]) const {
+    __device__ void finish(const Ctx &c, int s_, const double (&)[1]) const {
         SysState &S = c.st[s_];
-        constexpr bool defer = XMODE == kXDefer;
-        if (defer) {  // every workgroup has loaded alpha and omega by now
-            S.alpha_prev = S.alpha;
-            S.omega_prev = S.omega;
+        S.omega = omega;
+        S.beta = beta;
+        S.rho_next = rho_new;
+        S.k1_count += 1;
+        S.k1_defer = XMODE == kXDefer;
+        if constexpr (XMODE == kXDefer) {  // the update this K1 leaves for the next
+            S.alpha_prev = alpha;
+            S.omega_prev = omega;
         }
-        if (!S.pending) {  // first K1 after OpInit: nothing to finish
-            S.pending = 1;
-            S.rr = tot[0];
-            return;
-        }
-        finish_iteration(c, s_, S, tot[0]);
-        if (defer && !S.active) S.xdefer = 1;  // OpFlushX applies it
     }
 };
 
@@ -919,41 +1134,105 @@ struct OpApplyM {
     __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
 
-struct OpSpmvAlpha {  // K2: a = A y, alpha = rho / (r*, a)
-    static constexpr int NRED = 1;
+// K2: a = A y, (r*, a) published.  The system's first unit also sums K1's
+// (r, r) partials (loaded before its rows, folded into its block sums) and
+// runs the stop test of the iteration K1 finished (the reference's loop
+// tail, bicgstab_solver.h:224-231): converged, then non-finite, then
+// MAX_ITER.  A stopping system's other workgroups still form a and their
+// partials, which nothing reads.
+struct OpSpmvAlpha {
+    static constexpr int NRED = 2;  // (r*, a); the committer's (r, r)
+    static constexpr int kPublish = 1, kSlot = kSlotRsA;
+    static constexpr bool kFold = true, kCommit = true;
     struct Pre {
         double rs;
     };
     const double *y, *rs;
     double *a;
+    double rrv[kTakePer];
     __device__ bool enabled(const SysState &S) const { return S.active; }
     __device__ void load(const SysState &) {}
-    __device__ Pre pre(int i, double (&)[1]) const { return {rs[i]}; }
+    // committer only (the other workgroups keep rrv = 0 from the launch)
+    __device__ void preload(const Ctx &c, int s_) {
+        GroupShare<1> gs;
+        gs.load(c, s_, kSlotRR);
+#pragma unroll
+        for (int k = 0; k < kTakePer; ++k) rrv[k] = gs.v[0][k];
+    }
+    __device__ void fold(double (&acc)[2]) const {
+        double t = rrv[0];
+#pragma unroll
+        for (int k = 1; k < kTakePer; ++k) t += rrv[k];
+        acc[1] = t;
+    }
+    __device__ Pre pre(int i, double (&)[2]) const { return {rs[i]}; }
     __device__ double gather(int j) const { return y[j]; }
     __device__ double stage(int i, const Pre &) const { return y[i]; }
-    __device__ void row(int i, double ax, const Pre &pr, double (&acc)[1]) const {
+    __device__ void row(int i, double ax, const Pre &pr, double (&acc)[2]) const {
         a[i] = ax;
         acc[0] += pr.rs * ax;
     }
-    __device__ void finish(const Ctx &c, int s, const double (&tot)[1]) const {
+    __device__ void finish(const Ctx &c, int s, const double (&tot)[2]) const {
         SysState &S = c.st[s];
-        S.alpha = S.rho / tot[0];
+        const double rr = tot[1];
+        S.rr = rr;
+        S.rho = S.rho_next;
+        S.k2_count += 1;
+        if (S.k1_count < 2) return;  // the first K1 finished no iteration
+        S.iters = S.iters_base + S.k1_count - 1;
+        const double err = sqrt(rr);
+        if (c.hist && S.iters <= c.hist_stride) c.hist[(size_t)s * c.hist_stride + S.iters - 1] = err;
+        bool stop = false;
+        if (err < S.tol) {
+            S.status = kConverged;
+            stop = true;
+        } else if (!(err == err) || !isfinite(S.beta)) {
+            S.status = kBreakdown;
+            stop = true;
+        } else if (S.iters >= S.max_iter - 1) {
+            // the reference returns MAX_ITER after MAX_ITER - 1 iterations
+            S.status = kMaxIter;
+            S.iters = S.max_iter;
+            stop = true;
+        }
+        if (stop) {
+            S.active = 0;
+            S.xdefer = S.k1_defer;  // OpFlushX applies the update that K1 left
+            atomicSub(&c.counter[c.nsys], 1);
+        }
     }
+};
+
+// alpha = rho / (r*, a) from K2's partials, formed by every workgroup of a
+// K3; the system's first unit commits it.
+struct TakeAlpha {
+    double rho, alpha, nalpha;
+    __device__ void load_rho(const SysState &S) { rho = S.rho; }
+    __device__ void take(const double (&v)[1]) {
+        alpha = wave_uniform(rho / v[0]);
+        nalpha = -alpha;
+    }
+    __device__ void commit(const Ctx &c, int s) const { c.st[s].alpha = alpha; }
 };
 
 template <class MT>
 struct OpMapplyS {  // K3: z = M s, s = r - alpha a (stored before the product; gathers read r and a)
     static constexpr int NRED = 0;
+    static constexpr int kTake = 1, kTakeSlot = kSlotRsA;
+    static constexpr bool kCommit = true;
     static constexpr bool kStage = true;
     struct Pre {
         double s;
     };
     const double *r, *a;
     double *s, *z;
-    double nalpha;
+    TakeAlpha ta;
     __device__ bool enabled(const SysState &S) const { return S.active; }
-    __device__ void load(const SysState &S) { nalpha = -S.alpha; }
-    __device__ double sval(int j) const { return r[j] + nalpha * a[j]; }
+    __device__ void load(const SysState &S) { ta.load_rho(S); }
+    __device__ bool need_take() const { return true; }
+    __device__ void take(const double (&v)[1]) { ta.take(v); }
+    __device__ void finish(const Ctx &c, int s_, const double (&)[1]) const { ta.commit(c, s_); }
+    __device__ double sval(int j) const { return r[j] + ta.nalpha * a[j]; }
     __device__ Pre pre(int i, double (&)[1]) const {
         const double si = sval(i);
         s[i] = si;
@@ -962,7 +1241,6 @@ struct OpMapplyS {  // K3: z = M s, s = r - alpha a (stored before the product; 
     __device__ double gather(int j) const { return sval(j); }
     __device__ double stage(int, const Pre &pr) const { return pr.s; }
     __device__ void row(int i, double mx, const Pre &, double (&)[1]) const { z[i] = mx; }
-    __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
 
 // K3 split in two for row tiles: s = r - alpha a as a streaming pass, then
@@ -972,24 +1250,29 @@ struct OpMapplyS {  // K3: z = M s, s = r - alpha a (stored before the product; 
 // formed by the same expression as OpMapplyS's, so z is the same.
 struct OpUpdateS {
     static constexpr int NRED = 0;
+    static constexpr int kTake = 1, kTakeSlot = kSlotRsA;
+    static constexpr bool kCommit = true;
     struct Pre {};
     const double *r, *a;
     double *s;
-    double nalpha;
+    TakeAlpha ta;
     __device__ bool enabled(const SysState &S) const { return S.active; }
-    __device__ void load(const SysState &S) { nalpha = -S.alpha; }
+    __device__ void load(const SysState &S) { ta.load_rho(S); }
+    __device__ bool need_take() const { return true; }
+    __device__ void take(const double (&v)[1]) { ta.take(v); }
+    __device__ void finish(const Ctx &c, int s_, const double (&)[1]) const { ta.commit(c, s_); }
     __device__ Pre pre(int i, double (&)[1]) const {
-        s[i] = r[i] + nalpha * a[i];
+        s[i] = r[i] + ta.nalpha * a[i];
         return {};
     }
     __device__ double gather(int) const { return 0.0; }
     __device__ double stage(int, const Pre &) const { return 0.0; }
     __device__ void row(int, double, const Pre &, double (&)[1]) const {}
-    __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
 
-struct OpSpmvOmega {  // K4: t = A z; omega, rho' = (r*, s) - omega (r*, t), beta
+struct OpSpmvOmega {  // K4: t = A z; (t, s), (t, t), (r*, s), (r*, t) published for the next K1
     static constexpr int NRED = 4;
+    static constexpr int kPublish = 4, kSlot = kSlotOmega;
     struct Pre {
         double s, rs;
     };
@@ -1007,17 +1290,11 @@ struct OpSpmvOmega {  // K4: t = A z; omega, rho' = (r*, s) - omega (r*, t), bet
         acc[2] += pr.rs * pr.s;
         acc[3] += pr.rs * ax;
     }
-    __device__ void finish(const Ctx &c, int s_, const double (&tot)[4]) const {
-        SysState &S = c.st[s_];
-        S.omega = tot[0] / tot[1];
-        const double rho_new = tot[2] - S.omega * tot[3];
-        S.beta = (rho_new / S.rho) * (S.alpha / S.omega);
-        S.rho = rho_new;
-    }
 };
 
 struct OpTrueResidual {  // ||b - A x||^2 per system
     static constexpr int NRED = 1;
+    static constexpr bool kTree = true;
     struct Pre {
         double b;
     };
@@ -1050,26 +1327,31 @@ struct OpSpmvPlain {  // y = A x (timing of the bare product)
 
 struct OpCopyS {  // K3 without a preconditioner: z = s
     static constexpr int NRED = 0;
+    static constexpr int kTake = 1, kTakeSlot = kSlotRsA;
+    static constexpr bool kCommit = true;
     struct Pre {
         double s;
     };
     const double *r, *a;
     double *s, *z;
-    double nalpha;
+    TakeAlpha ta;
     __device__ bool enabled(const SysState &S) const { return S.active; }
-    __device__ void load(const SysState &S) { nalpha = -S.alpha; }
+    __device__ void load(const SysState &S) { ta.load_rho(S); }
+    __device__ bool need_take() const { return true; }
+    __device__ void take(const double (&v)[1]) { ta.take(v); }
+    __device__ void finish(const Ctx &c, int s_, const double (&)[1]) const { ta.commit(c, s_); }
     __device__ Pre pre(int i, double (&)[1]) const {
-        const double si = r[i] + nalpha * a[i];
+        const double si = r[i] + ta.nalpha * a[i];
         s[i] = si;
         return {si};
     }
     __device__ double gather(int) const { return 0.0; }
     __device__ double stage(int, const Pre &) const { return 0.0; }
     __device__ void row(int i, double, const Pre &pr, double (&)[1]) const { z[i] = pr.s; }
-    __device__ void finish(const Ctx &, int, const double (&)[1]) const {}
 };
 
-__global__ void k_state_init(SysState *st, int nsys, int max_iter, int *counter, int nnodes) {
+// ncounters: the arrival counters after the two counts (tree nodes, groups)
+__global__ void k_state_init(SysState *st, int nsys, int max_iter, int *counter, int ncounters) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < nsys) {
         SysState z{};
@@ -1083,7 +1365,7 @@ __global__ void k_state_init(SysState *st, int nsys, int max_iter, int *counter,
         counter[nsys] = nsys;
         counter[nsys + 1] = 0;
     }
-    for (int b = s; b < nnodes; b += gridDim.x * blockDim.x) counter[nsys + 2 + b] = 0;
+    for (int b = s; b < ncounters; b += gridDim.x * blockDim.x) counter[nsys + 2 + b] = 0;
 }
 
 // systems whose recursive residual converged but whose true residual did not
@@ -1101,13 +1383,22 @@ __global__ void k_check_true(SysState *st, int nsys, int *counter) {
 }
 
 // re-enables every system (scalars kept) so single kernels can be timed
-__global__ void k_state_activate(SysState *st, int nsys, int *counter, int nnodes) {
+__global__ void k_state_activate(SysState *st, int nsys, int *counter, int ncounters) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < nsys) {
         st[s].active = 1;
         counter[s] = 0;
     }
-    for (int b = s; b < nnodes; b += gridDim.x * blockDim.x) counter[nsys + 2 + b] = 0;
+    for (int b = s; b < ncounters; b += gridDim.x * blockDim.x) counter[nsys + 2 + b] = 0;
+}
+
+// Single-kernel timing: the iteration partials all 1.0 and one K2 counted,
+// so that every K1 forms its scalars from K4's partials as in a solve (finite
+// values: omega = 1, rho' = 0, beta = 0)
+__global__ void k_seed_partials(SysState *st, int nsys, double *gpart, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nsys) st[i].k2_count = 1;
+    if (i < n) gpart[i] = 1.0;
 }
 
 // The graph loop's host check: active systems and active units into
@@ -1333,7 +1624,7 @@ void launch_k1(const OperatorView &O, const TileMap &tm, const Ctx &c, int xmode
     auto go = [&](auto xtag) {
         constexpr int XM = decltype(xtag)::value;
         OpMapplyP<MT, YMODE, XM> op{k.x, k.r, k.pout, k.y, k.s, k.t, k.pin, k.a, k.z, k.yb, k.zb,
-                                    0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                                    0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
         launch_units<HM, MT>(O, tm, c, op, list, grid, st, reverse);
     };
     if (xmode == kXOne) go(std::integral_constant<int, kXOne>{});
@@ -1556,6 +1847,14 @@ void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector
         snodes[(size_t)s + 1] = snodes[(size_t)s] + total;
     }
     tm.nnodes = snodes[(size_t)nsys];
+    // partial-sum groups of the iteration kernels
+    std::vector<int> sgroups((size_t)nsys + 1, 0);
+    for (int s = 0; s < nsys; ++s) {
+        const int nu = tm.h_sys_units[(size_t)s + 1] - tm.h_sys_units[(size_t)s];
+        const int G = group_units(nu);
+        sgroups[(size_t)s + 1] = sgroups[(size_t)s] + (nu + G - 1) / G;
+    }
+    tm.ngroups = sgroups[(size_t)nsys];
     auto up = [&](DeviceArray<int> &d, const std::vector<int> &h) {
         d.resize(std::max<size_t>(h.size(), 1));
         if (!h.empty()) SPAI_HIP(hipMemcpyAsync(d.get(), h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, st));
@@ -1566,6 +1865,7 @@ void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector
     up(tm.sys_units, tm.h_sys_units);
     up(tm.unit_slice, uslice);
     up(tm.sys_nodes, snodes);
+    up(tm.sys_groups, sgroups);
     tm.active.resize((size_t)std::max(tm.nunits, 1));
     tm.sys_base.resize((size_t)nsys);
     if (sell) {
@@ -1696,8 +1996,11 @@ void BiCGStab::forget_layout(unsigned long long generation) {
 }
 
 Ctx BiCGStab::make_ctx(const TileMap &tm, int nsys) {
-    return Ctx{st_, tm.sys_units.get(), tm.sys_nodes.get(), cnt_, part_,
-               nsys,         hist_ptr_,          hist_stride_};
+    return Ctx{st_,       tm.sys_units.get(),  tm.sys_nodes.get(),
+               cnt_,      part_,               nsys,
+               hist_ptr_, hist_stride_,        tm.sys_groups.get(),
+               gpart_,    upart_,              cnt_ + nsys + 2 + tm.nnodes,
+               tm.ngroups, tm.nunits};
 }
 
 namespace {
@@ -1721,12 +2024,21 @@ void carve_vectors(double *v, size_t N, double *&r, double *&rs, double *(&p)[2]
 size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
 }  // namespace
 
-size_t BiCGStab::workspace_bytes(int n, int nsys, int nnodes) {
+namespace {
+// doubles of the reduction partials: kMaxRed per tree node, then kSlots per
+// group and kSlots per unit
+size_t partial_doubles(const TileMap &tm) {
+    return (size_t)kMaxRed * (size_t)std::max(tm.nnodes, 1) + (size_t)kSlots * (size_t)std::max(tm.ngroups, 1) +
+           (size_t)kSlots * (size_t)std::max(tm.nunits, 1);
+}
+size_t counter_ints(const TileMap &tm, int nsys) { return (size_t)nsys + 2 + (size_t)tm.nnodes + (size_t)tm.ngroups; }
+}  // namespace
+
+size_t BiCGStab::workspace_bytes(int n, int nsys, const TileMap &tm) {
     const size_t N = (size_t)std::max(n, 1);
     // nine vectors (no second y and z: a bound workspace never defers x)
     return round256(sizeof(double) * 9 * N) + round256(sizeof(SysState) * (size_t)nsys) +
-           round256(sizeof(int) * ((size_t)nsys + 2 + (size_t)nnodes)) +
-           round256(sizeof(double) * kMaxRed * (size_t)std::max(nnodes, 1));
+           round256(sizeof(int) * counter_ints(tm, nsys)) + round256(sizeof(double) * partial_doubles(tm));
 }
 
 void BiCGStab::bind_workspace(void *base, size_t bytes) {
@@ -1737,14 +2049,21 @@ void BiCGStab::bind_workspace(void *base, size_t bytes) {
     n_ = 0;
     nsys_ = 0;
     nnodes_ = -1;
-    nunits_ = 0;
+    nunits_ = -1;
+    ngroups_ = -1;
 }
 
-void BiCGStab::ensure(int n, int nsys, int nunits, int nnodes) {
+void BiCGStab::ensure(int n, int nsys, const TileMap &tm) {
+    const int nunits = tm.nunits, nnodes = tm.nnodes, ngroups = tm.ngroups;
+    auto carve_partials = [&](double *p) {
+        part_ = p;
+        gpart_ = p + (size_t)kMaxRed * (size_t)std::max(nnodes, 1);
+        upart_ = gpart_ + (size_t)kSlots * (size_t)std::max(ngroups, 1);
+    };
     if (ws_base_) {
         // a bound workspace: carve, never allocate (the same pointers for
         // the same shape, so nothing is cached across shapes)
-        const size_t need = workspace_bytes(n, nsys, nnodes);
+        const size_t need = workspace_bytes(n, nsys, tm);
         if (need > ws_bytes_)
             throw Error(kErrUnsupported, "the solve needs " + std::to_string(need) + " bytes of workspace, " +
                                              std::to_string(ws_bytes_) + " are available");
@@ -1758,12 +2077,13 @@ void BiCGStab::ensure(int n, int nsys, int nunits, int nnodes) {
         st_ = (SysState *)p;
         p += round256(sizeof(SysState) * (size_t)nsys);
         cnt_ = (int *)p;
-        p += round256(sizeof(int) * ((size_t)nsys + 2 + (size_t)nnodes));
-        part_ = (double *)p;
+        p += round256(sizeof(int) * counter_ints(tm, nsys));
+        carve_partials((double *)p);
         n_ = n;
         nsys_ = nsys;
         nnodes_ = nnodes;
         nunits_ = nunits;
+        ngroups_ = ngroups;
         return;
     }
     if (n != n_) {
@@ -1772,20 +2092,18 @@ void BiCGStab::ensure(int n, int nsys, int nunits, int nnodes) {
         carve_vectors(vec_.get(), (size_t)std::max(n, 1), r_, rs_, p_, y_, a_, s_, z_, t_);
         n_ = n;
     }
-    if (nsys != nsys_ || nnodes != nnodes_) {
+    if (nsys != nsys_ || nnodes != nnodes_ || nunits != nunits_ || ngroups != ngroups_) {
         drop_graph();
         state_.resize((size_t)nsys);
-        counter_.resize((size_t)nsys + 2 + nnodes);
-        partials_.resize((size_t)kMaxRed * std::max(nnodes, 1));
+        counter_.resize(counter_ints(tm, nsys));
+        partials_.resize(partial_doubles(tm));
         st_ = state_.get();
         cnt_ = counter_.get();
-        part_ = partials_.get();
+        carve_partials(partials_.get());
         nsys_ = nsys;
         nnodes_ = nnodes;
-    }
-    if (nunits != nunits_) {
-        drop_graph();
         nunits_ = nunits;
+        ngroups_ = ngroups;
     }
 }
 
@@ -1833,16 +2151,16 @@ void BiCGStab::enqueue_k3(const OperatorView &M, const TileMap &tm, int parity, 
                           bool split, hipStream_t st) {
     Ctx c = make_ctx(tm, nsys_);
     if (split && tm.kind == kLayoutTiles) {
-        OpUpdateS us{r_, a_, s_, 0.0};
+        OpUpdateS us{r_, a_, s_, {}};
         launch_units<false, double>(M, tm, c, us, list, grid, st, flip());
         OpApplyM om{s_, zbuf(parity)};
         if (M.fp32) launch_units<true, float>(M, tm, c, om, list, grid, st, flip());
         else launch_units<true, double>(M, tm, c, om, list, grid, st, flip());
     } else if (M.fp32) {
-        OpMapplyS<float> op{r_, a_, s_, zbuf(parity), 0.0};
+        OpMapplyS<float> op{r_, a_, s_, zbuf(parity), {}};
         launch_units<true, float>(M, tm, c, op, list, grid, st, flip());
     } else {
-        OpMapplyS<double> op{r_, a_, s_, zbuf(parity), 0.0};
+        OpMapplyS<double> op{r_, a_, s_, zbuf(parity), {}};
         launch_units<true, double>(M, tm, c, op, list, grid, st, flip());
     }
 }
@@ -1855,13 +2173,13 @@ void BiCGStab::enqueue_iteration(const OperatorView &A, const OperatorView &M, c
     const int *list = grid > 0 ? tm.active.get() : nullptr;
     enqueue_k1(A, M, tm, parity, list, grid, split, st);
     {
-        OpSpmvAlpha op{ybuf(parity), rs_, a_};
+        OpSpmvAlpha op{ybuf(parity), rs_, a_, {}};
         launch_units<true, double>(A, tm, c, op, list, grid, st, flip());
     }
     if (M.val) {
         enqueue_k3(M, tm, parity, list, grid, split, st);
     } else {
-        OpCopyS op{r_, a_, s_, zbuf(parity), 0.0};
+        OpCopyS op{r_, a_, s_, zbuf(parity), {}};
         launch_units<false, double>(A, tm, c, op, list, grid, st, flip());
     }
     {
@@ -1883,7 +2201,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     const bool bound = ws_base_ != nullptr;
     require(!bound || (!opt.use_graph && !opt.record_history && states_out),
             "bicgstab: a bound workspace supports neither graphs nor histories and needs a states array");
-    ensure(n, nsys, tm.nunits, tm.nnodes);
+    ensure(n, nsys, tm);
     b_ = b;
     x_ = x;
     defer_x_ = nsys > 1 && !bound;
@@ -1904,7 +2222,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     }
     Ctx c = make_ctx(tm, nsys);
     hipLaunchKernelGGL(k_state_init, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, st_, nsys, opt.max_iter,
-                       cnt_, tm.nnodes);
+                       cnt_, tm.nnodes + tm.ngroups);
     {
         OpInit op{x,     b,         r_,          rs_,         p_[0],      p_[1], a_, s_, t_, y_[0], z_[0],
                   y1(),  z1(),      opt.rel_tol, opt.abs_tol, 0};
@@ -2149,7 +2467,7 @@ double *BiCGStab::seed_timing_state(const OperatorView &A, const TileMap &tm, in
     hipLaunchKernelGGL(k_fill_double, dim3(ceil_div(n, 256)), dim3(256), 0, st, tb, 1.0, (long long)n);
     hipLaunchKernelGGL(k_fill_double, dim3(ceil_div(n, 256)), dim3(256), 0, st, tx, 0.0, (long long)n);
     hipLaunchKernelGGL(k_state_init, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, st_, nsys, max_iter,
-                       cnt_, tm.nnodes);
+                       cnt_, tm.nnodes + tm.ngroups);
     Ctx c = make_ctx(tm, nsys);
     OpInit op{tx, tb, r_, rs_, p_[0], p_[1], a_, s_, t_, y_[0], z_[0], y1(), z1(), 0.0, 1e-300, 0};
     launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
@@ -2162,7 +2480,7 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
     require(which >= 0 && which <= 5, "time_kernel: which must be 0..5");
     require((which != 1 && which != 3 && which != 5) || M.val, "time_kernel: no preconditioner");
     defer_x_ = nsys > 1;  // as the solve runs it
-    ensure(n, nsys, tm.nunits, tm.nnodes);
+    ensure(n, nsys, tm);
     hist_ptr_ = nullptr;
     hist_stride_ = 0;
     Ctx c = make_ctx(tm, nsys);
@@ -2172,7 +2490,12 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
     // caller's); every system stays active.
     double *tx = seed_timing_state(A, tm, n, nsys, 1 << 30, st);
     hipLaunchKernelGGL(k_state_activate, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, st_, nsys,
-                       cnt_, tm.nnodes);
+                       cnt_, tm.nnodes + tm.ngroups);
+    {
+        const long long ng = (long long)kSlots * tm.ngroups;
+        hipLaunchKernelGGL(k_seed_partials, dim3(ceil_div(std::max<long long>(ng, nsys), 256)), dim3(256), 0, st,
+                           st_, nsys, gpart_, ng);
+    }
     hipEvent_t e0, e1;
     SPAI_HIP(hipEventCreate(&e0));
     SPAI_HIP(hipEventCreate(&e1));
@@ -2197,7 +2520,7 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
             }
         } break;
         case 2: {
-            OpSpmvAlpha op{ybuf(parity), rs_, a_};
+            OpSpmvAlpha op{ybuf(parity), rs_, a_, {}};
             launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
         } break;
         case 3:  // the form the solver uses with every unit active
@@ -2229,7 +2552,7 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
     require(variant >= -1 && variant <= 1, "time_iteration: variant must be -1, 0 or 1");
     const bool split = variant < 0 ? tm.nunits >= split_units(tm) : variant == 1;
     defer_x_ = nsys > 1;  // as the solve runs it
-    ensure(n, nsys, tm.nunits, tm.nnodes);
+    ensure(n, nsys, tm);
     hist_ptr_ = nullptr;
     hist_stride_ = 0;
     Ctx c = make_ctx(tm, nsys);
@@ -2265,7 +2588,7 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
             }
             SPAI_HIP(hipEventRecord(e[2], st));
             {
-                OpSpmvAlpha op{ybuf(parity), rs_, a_};
+                OpSpmvAlpha op{ybuf(parity), rs_, a_, {}};
                 launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
             }
             SPAI_HIP(hipEventRecord(e[3], st));

@@ -9,8 +9,9 @@
 
 namespace spai {
 
-// Per-system solver state kept in device memory; the fused kernels' last
-// workgroup of each system updates it, the host only reads it.
+// Per-system solver state kept in device memory; one workgroup of each
+// system (its first unit's) updates it, the host only reads it.  No kernel
+// reads a field that the same kernel writes.
 struct alignas(64) SysState {
     double rho;      // (r*, r)
     double alpha;
@@ -21,14 +22,19 @@ struct alignas(64) SysState {
     double err0;     // ||r_0||_2
     double true_rr;  // ||b - A x||^2 after the solve
     double alpha_prev, omega_prev;  // alpha, omega of the iteration whose x update is deferred
+    double rho_next;  // rho' formed by the last K1, the rho of the next K3 (K2 moves it to rho)
     int active;      // 1 while iterating
     int iters;       // the reference's iteration count once stopped
     int status;      // SolveStatus
     int max_iter;
     int restarts;    // restarts from the true residual (true_residual_restart)
-    int pending;     // 1 once an iteration's x/r update is left for the next K1
     int xdefer;      // 1: stopped with x missing the update held in the odd buffers
+    int iters_base;  // iterations before the last (re)start
+    int k1_count;    // K1 launches since the last (re)start
+    int k2_count;    // K2 launches since the last (re)start
+    int k1_defer;    // the last K1 left its x update for the next one
 };
+static_assert(sizeof(SysState) == 128, "two 64-byte lines per system");
 
 enum SolveStatus { kRunning = 0, kConverged = 1, kMaxIter = 2, kBreakdown = 3, kConvergedAtStart = 4 };
 
@@ -46,11 +52,21 @@ enum SolveStatus { kRunning = 0, kConverged = 1, kMaxIter = 2, kBreakdown = 3, k
 //    a tile of its own), over a CSR (or, for a transposed operator, CSC)
 //    pointer array.  Used for irregular rows and for operators given as CSR.
 //
-// Reducing kernels publish one partial per unit and combine them up a tree
-// per system (each node counts at most kFanIn arrivals).
+// The iteration kernels publish one partial sum per group of units of a
+// system (a unit alone when the system has at most kMaxGroups units), and the
+// next kernel that needs the total sums the system's group partials itself.
+// The set-up reductions combine per-unit partials up a tree per system (each
+// node counts at most kFanIn arrivals).
 // `active` lists the units of systems still iterating (rebuilt on the device
 // from the system states); launches that use it exit at once past its count.
 enum LayoutKind { kLayoutTiles = 0, kLayoutSell = 1 };
+#ifndef SPAI_MAX_GROUPS
+#define SPAI_MAX_GROUPS 512
+#endif
+constexpr int kMaxGroups = SPAI_MAX_GROUPS;  // partial-sum groups per system
+static_assert(kMaxGroups % 256 == 0 && kMaxGroups <= 1024, "kMaxGroups: a multiple of 256, at most 1024");
+// units per group of a system with `units` units
+inline __host__ __device__ int group_units(int units) { return units <= kMaxGroups ? 1 : (units + kMaxGroups - 1) / kMaxGroups; }
 // A unit is 8 slices (512 rows), one per wavefront of its workgroup: with
 // few units active (the end of a batched solve) eight wavefronts per unit hide
 // more latency than four wavefronts of two slices each.
@@ -92,6 +108,11 @@ struct TileMap {
     // kFanIn nodes per level): nodes of system s start at sys_nodes[s]
     int nnodes = 0;
     DeviceArray<int> sys_nodes;  // nsys + 1
+    // partial-sum groups of the iteration kernels: system s with U units has
+    // ceil(U / G) groups of G = ceil(U / kMaxGroups) consecutive units,
+    // starting at sys_groups[s]
+    int ngroups = 0;
+    DeviceArray<int> sys_groups;  // nsys + 1
     // active-unit list: [0, nunits) unit ids in XCD-blocked order;
     // compaction scratch per system
     DeviceArray<int> active;
@@ -201,9 +222,9 @@ public:
     // history, creates no events and never defers x updates (batches
     // included; the results are bit-identical to the deferring form).
     void bind_workspace(void *base, size_t bytes);
-    // Device bytes a bound workspace needs for a solve of n rows, nsys
-    // systems and nnodes reduction tree nodes.
-    static size_t workspace_bytes(int n, int nsys, int nnodes);
+    // Device bytes a bound workspace needs for a solve of n rows and nsys
+    // systems on the layout tm.
+    static size_t workspace_bytes(int n, int nsys, const TileMap &tm);
     // Drops the captured graphs of the layout `generation` (a system that
     // is destroyed or laid out again).
     void forget_layout(unsigned long long generation);
@@ -228,7 +249,7 @@ public:
                         int reps, int variant, hipStream_t st, float ms[5]);
 
 private:
-    void ensure(int n, int nsys, int nunits, int nnodes);
+    void ensure(int n, int nsys, const TileMap &tm);
     // b = 1 and x = 0 in tvec_, the states and work vectors of a solve from
     // them (OpInit); returns x
     double *seed_timing_state(const OperatorView &A, const TileMap &tm, int n, int nsys, int max_iter,
@@ -241,16 +262,17 @@ private:
                     int grid, bool split, hipStream_t st);
     void enqueue_k3(const OperatorView &M, const TileMap &tiles, int parity, const int *list, int grid, bool split,
                     hipStream_t st);
-    int n_ = 0, nsys_ = 0, nunits_ = 0, nnodes_ = -1;
+    int n_ = 0, nsys_ = 0, nunits_ = -1, nnodes_ = -1, ngroups_ = -1;
     DeviceArray<double> vec_;       // x-independent work vectors
-    DeviceArray<double> partials_;
+    DeviceArray<double> partials_;  // tree partials, then the iteration partials of the groups and the units
     DeviceArray<SysState> state_;
-    DeviceArray<int> counter_;      // [nsys] active systems, [nsys + 1] active units, [nsys + 2 + node] tree nodes
+    DeviceArray<int> counter_;      // [nsys] active systems, [nsys + 1] active units, [nsys + 2 + node] tree
+                                    // nodes, then one arrival counter per group
     DeviceArray<double> tvec_;      // b = 1 and x = 0 of time_iteration
     // what the kernels use: the arrays above, or a bound workspace
     SysState *st_ = nullptr;
     int *cnt_ = nullptr;
-    double *part_ = nullptr;
+    double *part_ = nullptr, *gpart_ = nullptr, *upart_ = nullptr;
     void *ws_base_ = nullptr;
     size_t ws_bytes_ = 0;
     std::vector<SysState> h_states_;
