@@ -227,6 +227,13 @@ def run_leg(D, O, ctx, torch, name, precision, schedule, steps, cpu_budget, with
     layout, _ = S.layout()
     solve_ms = float(np.median(solve))
     rows_it = row_iterations([A], [oc], D.SOLVE_MAX_ITER)
+    # the whole solve against the roofline: the iterations run x one
+    # iteration's algorithmic bytes (fused K1 count), over the solve's time
+    # (graph-replayed, so without the launch gaps of the per-kernel events)
+    its_run = rows_it // A.n
+    iter_bytes = sum(kernel_bytes(k, A.n, A.nnz, m_bytes, False)
+                     for k in ("update_mapply_p", "spmv_alpha", "mapply_s", "spmv_omega"))
+    solve_gbs = its_run * iter_bytes / (solve_ms * 1e-3) / 1e9
     leg = {"workload": desc, "precision": "mixed" if mixed else "fp64",
            "schedule": "balanced_nnz" if schedule == D.SCHEDULE_BALANCED_NNZ else "balanced_columns",
            "layout": layout, "n": A.n, "nnz": A.nnz, "steps": steps,
@@ -241,7 +248,10 @@ def run_leg(D, O, ctx, torch, name, precision, schedule, steps, cpu_budget, with
                         "launch_ms": round(kms[dom], 5),
                         "all_kernels_frac": {k: round(kernel_bytes(k, A.n, A.nnz, m_bytes, False) / (v * 1e-3) /
                                                       1e9 / HBM_PEAK_GBS, 4) for k, v in kms.items()},
-                        "all_kernels_ms": {k: round(v, 5) for k, v in kms.items()}},
+                        "all_kernels_ms": {k: round(v, 5) for k, v in kms.items()},
+                        "iteration_us": round(1e3 * solve_ms / max(its_run, 1), 2),
+                        "iteration_bytes": int(iter_bytes),
+                        "solve_achieved": round(solve_gbs, 1), "solve_frac": round(solve_gbs / HBM_PEAK_GBS, 4)},
            "spai_roofline": spai_roofline(S, [A], rb, mixed, float(np.median(spai))),
            "generate_s": round(gen_s, 2)}
     if adjoint:
@@ -398,6 +408,11 @@ def main():
         if kern:
             roofline["traffic"] = kern["traffic_bytes"]
             roofline["traffic_source"] = "profiles/batched64_traffic.md"
+            # the same launch time against the bytes the counters saw (the
+            # layout's own index data instead of CSR's)
+            tgbs = kern["traffic_bytes"] / (kernel_ms[dominant] * 1e-3) / 1e9
+            roofline["traffic_achieved"] = round(tgbs, 1)
+            roofline["traffic_frac"] = round(tgbs / HBM_PEAK_GBS, 4)
     copy_ms = ctx.calibration_copy(256 * 1024 * 1024, 10)
     roofline["copy_kernel_GBs"] = round(2 * 256 * 1024 * 1024 / (copy_ms * 1e-3) / 1e9, 1)
     # whole solve: every system's iterations x its per-iteration algorithmic

@@ -439,7 +439,7 @@ __device__ __forceinline__ void block_allsum(double (&v)[NT], double *shk) {
 // that waiting for them does not wait for the stream; take_finish sums them
 // once the op has read the system's state.
 template <class Op, int W>
-using TakeShare = GroupShare<(OpTake<Op>::value > 0 ? OpTake<Op>::value : 1), kMaxGroups / (W * 64)>;
+using TakeShare = GroupShare<(OpTake<Op>::value > 0 ? OpTake<Op>::value : 1), (kMaxGroups + W * 64 - 1) / (W * 64)>;
 
 template <int W, class Op>
 __device__ __forceinline__ void take_issue(const Ctx &c, int s, int unit, Op &op, TakeShare<Op, W> &gs) {

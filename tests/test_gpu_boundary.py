@@ -227,3 +227,44 @@ def test_graphs_not_replayed_on_a_new_system(ctx):
         Acsr = O.csc2csr(A.n, A.colptr, A.rowidx, vals)[:3]
         assert O.true_residual(Acsr, b, xg) / np.linalg.norm(b) <= 1.5e-8
         S.close()
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_systems_of_more_than_512_units(ctx, batched):
+    """A 650 x 650 conv-diff grid (422,500 rows: 826 SELL-64 units of 512
+    rows) has more units than partial-sum groups per system, so its units
+    publish through groups of two (the group level with arrival counters)
+    while a small system in the same batch publishes one partial per unit.
+    The first 10 iterates and residuals agree with the oracle's to 1e-8 and
+    the solve converges."""
+    big = D.Problem.convection_diffusion(650, 5.0).system(0)
+    mats = [big, D.Problem.convection_diffusion(40, 5.0).system(0)] if batched else [big]
+    rng = np.random.default_rng(11)
+    bs = [rng.uniform(size=A.n) for A in mats]
+    S = D.System(ctx, mats if batched else big)
+    layout, _ = S.layout()
+    assert layout.startswith("sell64") and big.n > 512 * 512
+    S.build(D.PRECISION_FP64, D.SCHEDULE_BALANCED_NNZ)
+    m_gpu = S.m_values()
+    b = np.concatenate(bs)
+    x10, _, oc10 = S.solve(b, rel_tol=0.0, abs_tol=1e-300, max_iter=11, record_history=True)
+    off = S.row_offsets
+    moff = np.concatenate([[0], np.cumsum([A.nnz for A in mats])])
+    for s, A in enumerate(mats):
+        Acsr = O.csc2csr(A.n, A.colptr, A.rowidx, A.val)[:3]
+        Mcsr = O.csc2csr(A.n, A.colptr, A.rowidx, m_gpu[moff[s]:moff[s + 1]])[:3]
+        xo, _, _, ho = O.bicgstab(Acsr, Mcsr, bs[s], np.zeros(A.n), 0.0, 11, history=True)
+        xs = x10[off[s]:off[s + 1]]
+        h = S.residual_history(s)
+        err = np.linalg.norm(xs - xo) / np.linalg.norm(xo)
+        rel = np.abs(h[:10] - ho[:10]) / ho[:10]
+        print(f"system {s} (n={A.n}): 10-iterate rel err {err:.2e}, history {rel.max():.2e}", flush=True)
+        assert oc10[s]["iterations"] == 11
+        assert err <= 1e-8 and rel.max() <= 1e-8, s
+    x, rep, oc = S.solve(b, rel_tol=1e-8, max_iter=3000, use_graph=True)
+    for s, A in enumerate(mats):
+        Acsr = O.csc2csr(A.n, A.colptr, A.rowidx, A.val)[:3]
+        true_rel = O.true_residual(Acsr, bs[s], x[off[s]:off[s + 1]]) / np.linalg.norm(bs[s])
+        print(f"system {s}: {oc[s]['iterations']} iterations, true rel {true_rel:.2e}", flush=True)
+        assert oc[s]["status"] == D.SOLVE_CONVERGED and true_rel <= 1.5e-8, s
+    S.close()
