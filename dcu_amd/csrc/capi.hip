@@ -196,6 +196,12 @@ struct spai_system {
     int m_precision = -1;
     TileMap tiles, tiles_t;
     bool tiles_t_ready = false;
+    // the adjoint's operators A^T and M^T on a SELL-64 tiles_t: values in
+    // its order, refreshed when A's values or M change
+    DeviceArray<double> at_lay, mt64_lay;
+    DeviceArray<float> mt32_lay;
+    bool at_ready = false;
+    unsigned long long m_version = 0, mt_version = 0;  // builds of M; the build mt*_lay holds
     DeviceArray<double> stage_b, stage_x, stage_g, stage_dA, stage_db;
     float setup_ms = 0.0f;
     int tile_shape[kTunedTiers] = {0, 0, 0};  // register-tier group shapes of the next builds
@@ -310,12 +316,31 @@ spai_system *create_from_host(spai_context *ctx, const std::vector<const CSC_Mat
     return S.release();
 }
 
+// The adjoint's layout over the CSR of A^T (A's CSC arrays): SELL-64 when M
+// has A's pattern (M^T then shares A^T's slices) and the rows suit it, row
+// tiles otherwise.
 void ensure_transposed_tiles(spai_system *S) {
     if (S->tiles_t_ready) return;
     const bool own = S->pattern != SPAI_PATTERN_A;
-    build_layout(S->tiles_t, S->colptr.get(), S->rowidx.get(), S->h_row_off, S->h_nnz_off, false, S->ctx->stream,
+    build_layout(S->tiles_t, S->colptr.get(), S->rowidx.get(), S->h_row_off, S->h_nnz_off, !own, S->ctx->stream,
                  own ? S->pcolptr.get() : nullptr);
     S->tiles_t_ready = true;
+    S->at_ready = false;
+    S->mt_version = 0;
+}
+
+// A^T for the adjoint solve in tiles_t's layout
+OperatorView at_view(spai_system *S) {
+    if (S->tiles_t.kind != kLayoutSell) return OperatorView{S->colptr.get(), S->rowidx.get(), S->cscval.get(), 0};
+    if (!S->at_ready) {
+        S->at_lay.resize((size_t)S->tiles_t.sell_len);
+        layout_values<double>(S->tiles_t, S->nnz, S->cscval.get(), S->at_lay.get(), S->ctx->stream);
+        S->at_ready = true;
+    }
+    OperatorView A{nullptr, S->tiles_t.sell_col.get(), S->at_lay.get(), 0};
+    A.off = S->tiles_t.sell_off.get();
+    A.diag = S->tiles_t.sell_diag.get();
+    return A;
 }
 
 SolveOptions to_options(const spai_solve_options *o) {
@@ -342,8 +367,29 @@ void fill_report(spai_solve_report *r, const SolveResult &res) {
     r->all_converged = res.all_converged;
 }
 
-// M in the solver's layout (transposed: CSC arrays = CSR of M^T, row tiles)
+// M in the solver's layout; transposed: M^T in tiles_t's (its CSC arrays =
+// the CSR of M^T for row tiles; on SELL-64 tiles_t, which only M with A's
+// pattern gets, A^T's slices with M's CSC values in their order)
 OperatorView m_view(spai_system *S, bool transposed) {
+    if (transposed && S->tiles_t.kind == kLayoutSell) {
+        const bool fp32 = S->m_precision == SPAI_PRECISION_MIXED;
+        if (S->mt_version != S->m_version) {
+            const size_t len = (size_t)S->tiles_t.sell_len;
+            if (fp32) {
+                S->mt32_lay.resize(len);
+                layout_values<float, float>(S->tiles_t, S->nnz, S->m32_csc.get(), S->mt32_lay.get(), S->ctx->stream);
+            } else {
+                S->mt64_lay.resize(len);
+                layout_values<double>(S->tiles_t, S->nnz, S->m64_csc.get(), S->mt64_lay.get(), S->ctx->stream);
+            }
+            S->mt_version = S->m_version;
+        }
+        OperatorView M{nullptr, S->tiles_t.sell_col.get(), nullptr, fp32 ? 1 : 0};
+        M.val = fp32 ? (const void *)S->mt32_lay.get() : (const void *)S->mt64_lay.get();
+        M.off = S->tiles_t.sell_off.get();
+        M.diag = S->tiles_t.sell_diag.get();
+        return M;
+    }
     OperatorView M;
     const bool sell = !transposed && S->tiles.kind == kLayoutSell;
     const bool own = S->pattern != SPAI_PATTERN_A;
@@ -557,6 +603,7 @@ int spai_system_set_values(spai_system *sys, const double *values, int values_on
                                 hipMemcpyHostToDevice, st));
         if (sys->tiles.kind == kLayoutSell)
             layout_values<double>(sys->tiles, sys->nnz, sys->csrval.get(), sys->sellval.get(), st);
+        sys->at_ready = false;  // the adjoint's A^T values follow on its next call
         SPAI_HIP(hipStreamSynchronize(st));
     });
 }
@@ -622,6 +669,7 @@ void build_m(spai_system *sys, int precision, int schedule, int col_begin, int c
     sys->ctx->builder.build(p, st, &stats);
     float ms = tm.stop();
     sys->m_precision = precision;
+    sys->m_version += 1;
     if (report) {
         report->ms = ms;
         for (int t = 0; t < 5; ++t) report->tier_columns[t] = stats.tier_columns[t];
@@ -1061,7 +1109,7 @@ int spai_system_adjoint(spai_system *sys, const double *x, const double *g, doub
             ddb = sys->stage_db.get();
         }
         fill(ddb, (long long)n, st);  // lambda_0 = 0
-        OperatorView At{sys->colptr.get(), sys->rowidx.get(), sys->cscval.get(), 0};
+        OperatorView At = at_view(sys);
         OperatorView Mt = opt->precondition ? m_view(sys, true) : OperatorView{};
         SolveResult res;
         sys->ctx->adjoint_solver.solve(At, Mt, sys->tiles_t, sys->n, sys->nsys, dg, ddb, to_options(opt), st, &res);

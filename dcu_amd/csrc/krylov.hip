@@ -1530,14 +1530,14 @@ __global__ void k_sell_fill(int nslices, const int *slice_row, const int *slice_
     if (!fits) atomicOr(wide, 1);
 }
 
-template <class MT_OUT>
-__global__ void k_scatter_values(long long nnz, const int *pos, const double *in, MT_OUT *out) {
+template <class MT_OUT, class MT_IN>
+__global__ void k_scatter_values(long long nnz, const int *pos, const MT_IN *in, MT_OUT *out) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p < nnz) out[pos[p]] = (MT_OUT)in[p];
 }
 
-template <class MT_OUT>
-__global__ void k_convert_values(long long nnz, const double *in, MT_OUT *out) {
+template <class MT_OUT, class MT_IN>
+__global__ void k_convert_values(long long nnz, const MT_IN *in, MT_OUT *out) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p < nnz) out[p] = (MT_OUT)in[p];
 }
@@ -1900,22 +1900,23 @@ void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector
     SPAI_HIP(hipStreamSynchronize(st));
 }
 
-template <class MT_OUT>
-void layout_values(const TileMap &tm, long long nnz, const double *in, MT_OUT *out, hipStream_t st) {
+template <class MT_OUT, class MT_IN>
+void layout_values(const TileMap &tm, long long nnz, const MT_IN *in, MT_OUT *out, hipStream_t st) {
     if (tm.kind == kLayoutSell) {
         hipLaunchKernelGGL(k_fill_zero<MT_OUT>, dim3((unsigned)((tm.sell_len + 255) / 256)), dim3(256), 0, st, out,
                            tm.sell_len);
         if (nnz > 0)
-            hipLaunchKernelGGL(k_scatter_values<MT_OUT>, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, nnz,
-                               tm.sell_pos.get(), in, out);
+            hipLaunchKernelGGL((k_scatter_values<MT_OUT, MT_IN>), dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0,
+                               st, nnz, tm.sell_pos.get(), in, out);
     } else if (nnz > 0) {
-        hipLaunchKernelGGL(k_convert_values<MT_OUT>, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, nnz, in,
-                           out);
+        hipLaunchKernelGGL((k_convert_values<MT_OUT, MT_IN>), dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st,
+                           nnz, in, out);
     }
     SPAI_CHECK_LAUNCH();
 }
-template void layout_values<double>(const TileMap &, long long, const double *, double *, hipStream_t);
-template void layout_values<float>(const TileMap &, long long, const double *, float *, hipStream_t);
+template void layout_values<double, double>(const TileMap &, long long, const double *, double *, hipStream_t);
+template void layout_values<float, double>(const TileMap &, long long, const double *, float *, hipStream_t);
+template void layout_values<float, float>(const TileMap &, long long, const float *, float *, hipStream_t);
 
 void build_sell_operator(const TileMap &tm, const int *rowptr, const int *colidx, int nnz, SellOperator &op,
                          hipStream_t st) {

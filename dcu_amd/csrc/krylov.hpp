@@ -140,9 +140,10 @@ void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector
                   const int *ptr2 = nullptr);
 
 // Values of a CSR-ordered array in the layout's order: out[sell_pos[p]] =
-// in[p] (SELL, padding 0) or a copy (tiles).  MT_OUT is double or float.
-template <class MT_OUT>
-void layout_values(const TileMap &tm, long long nnz, const double *in, MT_OUT *out, hipStream_t st);
+// in[p] (SELL, padding 0) or a copy (tiles).  (MT_OUT, MT_IN): (double,
+// double), (float, double) or (float, float).
+template <class MT_OUT, class MT_IN = double>
+void layout_values(const TileMap &tm, long long nnz, const MT_IN *in, MT_OUT *out, hipStream_t st);
 
 // perm_out[p] = position of CSC entry p in the layout's value order, from
 // perm_csr[p] = its CSR position.

@@ -158,3 +158,23 @@ def test_autograd_wrapper(ctx):
     dA, db, _ = S.adjoint(xh, xh - xt, rel_tol=1e-13, max_iter=5000)
     assert np.allclose(vals.grad.cpu().numpy(), dA, rtol=1e-10, atol=1e-14)
     assert np.allclose(bt.grad.cpu().numpy(), db, rtol=1e-10, atol=1e-14)
+
+
+def test_adjoint_follows_new_values_and_m(ctx):
+    """The adjoint keeps A^T and M^T in its own layout: after set_values and
+    a rebuild of M (in either precision) it solves with the new operators.
+    With A scaled by 2, lambda halves."""
+    P, A, S, b, xt = setup(ctx, m=16, seed=3)
+    L, x = loss_of(S, b, xt)
+    g = x - xt
+    _, db1, rep = S.adjoint(x, g, rel_tol=TOL, max_iter=5000)
+    assert rep["all_converged"]
+    S.set_values(2.0 * A.val)
+    for precision in (D.PRECISION_FP64, D.PRECISION_MIXED):
+        S.build(precision, D.SCHEDULE_BALANCED_NNZ)
+        dA2, db2, rep = S.adjoint(x, g, rel_tol=TOL, max_iter=5000)
+        assert rep["all_converged"]
+        err = np.linalg.norm(db2 - 0.5 * db1) / np.linalg.norm(db1)
+        print(f"precision {precision}: |db(2A) - db(A)/2| / |db(A)| = {err:.2e}", flush=True)
+        assert err <= 1e-10
+    S.set_values(A.val)
