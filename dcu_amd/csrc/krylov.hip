@@ -913,27 +913,38 @@ __global__ __launch_bounds__(kSellThreads) SPAI_SELL_OCCUPANCY void k_sell(const
 }
 
 // ------------------------------------------------------------- the ops ----
-// r = b - A x0, r* = r, s = r, p = a = t = y = z = 0; rho = rr = (r, r); tol.
-// With alpha = omega = beta = 0 the first K1 reproduces r and forms p' = r.
+// The shadow residual r* is stored as RT: double with an FP64 M, float with
+// an FP32 (mixed) M.  It is rounded once, when OpInit forms it, and rho and
+// every later inner product read the rounded value, so the solve is BiCGSTAB
+// with r* = fl32(r0), another valid shadow vector (its oracle iteration
+// counts spread as those of r* = r0).  Every other vector stays FP64.
+//
+// r = b - A x0, r* = r, s = r, p = a = t = y = z = 0; rho = (r*, r),
+// rr = (r, r); tol.  With alpha = omega = beta = 0 the first K1 reproduces r
+// and forms p' = r.
+template <class RT>
 struct OpInit {
-    static constexpr int NRED = 2;
+    static constexpr int NRED = 3;
     static constexpr bool kTree = true;
     struct Pre {
         double b;
     };
     const double *x, *b;
-    double *r, *rs, *p0, *p1, *a, *s, *t, *y, *z, *y1, *z1;
+    double *r;
+    RT *rs;
+    double *p0, *p1, *a, *s, *t, *y, *z, *y1, *z1;
     double rel_tol, abs_tol;
     int restart;  // 1: restart of an active system from its current x (iterations and tol kept)
     __device__ bool enabled(const SysState &S) const { return !restart || S.active; }
     __device__ void load(const SysState &) {}
-    __device__ Pre pre(int i, double (&)[2]) const { return {b[i]}; }
+    __device__ Pre pre(int i, double (&)[3]) const { return {b[i]}; }
     __device__ double gather(int j) const { return x[j]; }
     __device__ double stage(int i, const Pre &) const { return x[i]; }
-    __device__ void row(int i, double ax, const Pre &pr, double (&acc)[2]) const {
+    __device__ void row(int i, double ax, const Pre &pr, double (&acc)[3]) const {
         double ri = -ax + pr.b;
+        const RT rsi = (RT)ri;
         r[i] = ri;
-        rs[i] = ri;
+        rs[i] = rsi;
         s[i] = ri;
         p0[i] = 0.0;
         p1[i] = 0.0;
@@ -945,14 +956,15 @@ struct OpInit {
             y1[i] = 0.0;
             z1[i] = 0.0;
         }
-        acc[0] += ri * ri;
-        acc[1] += pr.b * pr.b;
+        acc[0] += (double)rsi * ri;  // (r*, r): (r, r) itself when RT is double
+        acc[1] += ri * ri;
+        acc[2] += pr.b * pr.b;
     }
-    __device__ void finish(const Ctx &c, int s_, const double (&tot)[2]) const {
+    __device__ void finish(const Ctx &c, int s_, const double (&tot)[3]) const {
         SysState &S = c.st[s_];
         S.rho = tot[0];
         S.rho_next = tot[0];
-        S.rr = tot[0];
+        S.rr = tot[1];
         S.alpha = 0.0;
         S.omega = 0.0;
         S.beta = 0.0;
@@ -966,15 +978,15 @@ struct OpInit {
         S.iters_base = restart ? S.iters : 0;
         if (restart) {
             S.restarts += 1;
-            if (sqrt(tot[0]) < S.tol) {
+            if (sqrt(tot[1]) < S.tol) {
                 S.status = kConverged;
                 S.active = 0;
                 atomicSub(&c.counter[c.nsys], 1);
             }
             return;
         }
-        S.err0 = sqrt(tot[0]);
-        S.tol = abs_tol > 0.0 ? abs_tol : rel_tol * sqrt(tot[1]);
+        S.err0 = sqrt(tot[1]);
+        S.tol = abs_tol > 0.0 ? abs_tol : rel_tol * sqrt(tot[2]);
         S.iters = 0;
         if (S.err0 < S.tol) {
             // r0 already below the threshold: the reference would divide 0/0
@@ -1140,6 +1152,7 @@ struct OpApplyM {
 // tail, bicgstab_solver.h:224-231): converged, then non-finite, then
 // MAX_ITER.  A stopping system's other workgroups still form a and their
 // partials, which nothing reads.
+template <class RT>
 struct OpSpmvAlpha {
     static constexpr int NRED = 2;  // (r*, a); the committer's (r, r)
     static constexpr int kPublish = 1, kSlot = kSlotRsA;
@@ -1147,7 +1160,8 @@ struct OpSpmvAlpha {
     struct Pre {
         double rs;
     };
-    const double *y, *rs;
+    const double *y;
+    const RT *rs;
     double *a;
     double rrv[kTakePer];
     __device__ bool enabled(const SysState &S) const { return S.active; }
@@ -1165,7 +1179,7 @@ struct OpSpmvAlpha {
         for (int k = 1; k < kTakePer; ++k) t += rrv[k];
         acc[1] = t;
     }
-    __device__ Pre pre(int i, double (&)[2]) const { return {rs[i]}; }
+    __device__ Pre pre(int i, double (&)[2]) const { return {(double)rs[i]}; }
     __device__ double gather(int j) const { return y[j]; }
     __device__ double stage(int i, const Pre &) const { return y[i]; }
     __device__ void row(int i, double ax, const Pre &pr, double (&acc)[2]) const {
@@ -1270,17 +1284,19 @@ struct OpUpdateS {
     __device__ void row(int, double, const Pre &, double (&)[1]) const {}
 };
 
+template <class RT>
 struct OpSpmvOmega {  // K4: t = A z; (t, s), (t, t), (r*, s), (r*, t) published for the next K1
     static constexpr int NRED = 4;
     static constexpr int kPublish = 4, kSlot = kSlotOmega;
     struct Pre {
         double s, rs;
     };
-    const double *z, *s, *rs;
+    const double *z, *s;
+    const RT *rs;
     double *t;
     __device__ bool enabled(const SysState &S) const { return S.active; }
     __device__ void load(const SysState &) {}
-    __device__ Pre pre(int i, double (&)[4]) const { return {s[i], rs[i]}; }
+    __device__ Pre pre(int i, double (&)[4]) const { return {s[i], (double)rs[i]}; }
     __device__ double gather(int j) const { return z[j]; }
     __device__ double stage(int i, const Pre &) const { return z[i]; }
     __device__ void row(int i, double ax, const Pre &pr, double (&acc)[4]) const {
@@ -1630,6 +1646,13 @@ void launch_k1(const OperatorView &O, const TileMap &tm, const Ctx &c, int xmode
     if (xmode == kXOne) go(std::integral_constant<int, kXOne>{});
     else if (xmode == kXDefer) go(std::integral_constant<int, kXDefer>{});
     else go(std::integral_constant<int, kXTwo>{});
+}
+
+// f(RT{}) with RT = float when r* is stored in FP32, else double.
+template <class F>
+void with_rt(bool rs32, F &&f) {
+    if (rs32) f(float{});
+    else f(double{});
 }
 
 }  // namespace
@@ -2166,6 +2189,42 @@ void BiCGStab::enqueue_k3(const OperatorView &M, const TileMap &tm, int parity, 
     }
 }
 
+// K2: a = A y, (r*, a).
+void BiCGStab::enqueue_k2(const OperatorView &A, const TileMap &tm, int parity, const int *list, int grid,
+                          hipStream_t st) {
+    with_rt(rs32_, [&](auto rt) {
+        using RT = decltype(rt);
+        Ctx c = make_ctx(tm, nsys_);
+        OpSpmvAlpha<RT> op{ybuf(parity), reinterpret_cast<const RT *>(rs_), a_, {}};
+        launch_units<true, double>(A, tm, c, op, list, grid, st, flip());
+    });
+}
+
+// K4: t = A z, (t, s), (t, t), (r*, s), (r*, t).
+void BiCGStab::enqueue_k4(const OperatorView &A, const TileMap &tm, int parity, const int *list, int grid,
+                          hipStream_t st) {
+    with_rt(rs32_, [&](auto rt) {
+        using RT = decltype(rt);
+        Ctx c = make_ctx(tm, nsys_);
+        OpSpmvOmega<RT> op{zbuf(parity), s_, reinterpret_cast<const RT *>(rs_), t_};
+        launch_units<true, double>(A, tm, c, op, list, grid, st, flip());
+    });
+}
+
+// OpInit over every unit: a solve's start (restart = 0) or a restart of the
+// systems still active from their current x (restart = 1).
+void BiCGStab::enqueue_init(const OperatorView &A, const TileMap &tm, const double *x, const double *b,
+                            double rel_tol, double abs_tol, int restart, hipStream_t st) {
+    with_rt(rs32_, [&](auto rt) {
+        using RT = decltype(rt);
+        Ctx c = make_ctx(tm, nsys_);
+        OpInit<RT> op{x,     b,     r_,    reinterpret_cast<RT *>(rs_), p_[0], p_[1],   a_,      s_,
+                      t_,    y_[0], z_[0], y1(),                        z1(),  rel_tol, abs_tol, restart};
+        launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
+    });
+    SPAI_CHECK_LAUNCH();
+}
+
 // One iteration: K1 (one or two launches), K2, K3 (one or two), K4.  grid > 0: walk the
 // first `grid` entries of the active-unit list; grid == 0: every unit.
 void BiCGStab::enqueue_iteration(const OperatorView &A, const OperatorView &M, const TileMap &tm, int parity,
@@ -2173,20 +2232,14 @@ void BiCGStab::enqueue_iteration(const OperatorView &A, const OperatorView &M, c
     Ctx c = make_ctx(tm, nsys_);
     const int *list = grid > 0 ? tm.active.get() : nullptr;
     enqueue_k1(A, M, tm, parity, list, grid, split, st);
-    {
-        OpSpmvAlpha op{ybuf(parity), rs_, a_, {}};
-        launch_units<true, double>(A, tm, c, op, list, grid, st, flip());
-    }
+    enqueue_k2(A, tm, parity, list, grid, st);
     if (M.val) {
         enqueue_k3(M, tm, parity, list, grid, split, st);
     } else {
         OpCopyS op{r_, a_, s_, zbuf(parity), {}};
         launch_units<false, double>(A, tm, c, op, list, grid, st, flip());
     }
-    {
-        OpSpmvOmega op{zbuf(parity), s_, rs_, t_};
-        launch_units<true, double>(A, tm, c, op, list, grid, st, flip());
-    }
+    enqueue_k4(A, tm, parity, list, grid, st);
     SPAI_CHECK_LAUNCH();
 }
 
@@ -2206,6 +2259,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     b_ = b;
     x_ = x;
     defer_x_ = nsys > 1 && !bound;
+    rs32_ = M.val && M.fp32;
     if (opt.record_history) {
         hist_.reserve((size_t)nsys * (size_t)opt.max_iter);
         hist_ptr_ = hist_.get();
@@ -2224,12 +2278,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     Ctx c = make_ctx(tm, nsys);
     hipLaunchKernelGGL(k_state_init, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, st_, nsys, opt.max_iter,
                        cnt_, tm.nnodes + tm.ngroups);
-    {
-        OpInit op{x,     b,         r_,          rs_,         p_[0],      p_[1], a_, s_, t_, y_[0], z_[0],
-                  y1(),  z1(),      opt.rel_tol, opt.abs_tol, 0};
-        launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
-    }
-    SPAI_CHECK_LAUNCH();
+    enqueue_init(A, tm, x, b, opt.rel_tol, opt.abs_tol, 0, st);
     // A batch walks the active-unit list, rebuilt after every host check; a
     // single system walks every unit.
     const bool use_list = nsys > 1;
@@ -2389,10 +2438,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                 true_current = true;
                 break;
             }
-            OpInit op{x,    b,    r_,          rs_,         p_[0], p_[1], a_, s_, t_, y_[0], z_[0],
-                      y1(), z1(), opt.rel_tol, opt.abs_tol, 1};
-            launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
-            SPAI_CHECK_LAUNCH();
+            enqueue_init(A, tm, x, b, opt.rel_tol, opt.abs_tol, 1, st);
             done_iters = 0;
             iterate();
             flush_x();
@@ -2469,10 +2515,7 @@ double *BiCGStab::seed_timing_state(const OperatorView &A, const TileMap &tm, in
     hipLaunchKernelGGL(k_fill_double, dim3(ceil_div(n, 256)), dim3(256), 0, st, tx, 0.0, (long long)n);
     hipLaunchKernelGGL(k_state_init, dim3(ceil_div(nsys, 256)), dim3(256), 0, st, st_, nsys, max_iter,
                        cnt_, tm.nnodes + tm.ngroups);
-    Ctx c = make_ctx(tm, nsys);
-    OpInit op{tx, tb, r_, rs_, p_[0], p_[1], a_, s_, t_, y_[0], z_[0], y1(), z1(), 0.0, 1e-300, 0};
-    launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
-    SPAI_CHECK_LAUNCH();
+    enqueue_init(A, tm, tx, tb, 0.0, 1e-300, 0, st);
     return tx;
 }
 
@@ -2481,6 +2524,7 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
     require(which >= 0 && which <= 5, "time_kernel: which must be 0..5");
     require((which != 1 && which != 3 && which != 5) || M.val, "time_kernel: no preconditioner");
     defer_x_ = nsys > 1;  // as the solve runs it
+    rs32_ = M.val && M.fp32;
     ensure(n, nsys, tm);
     hist_ptr_ = nullptr;
     hist_stride_ = 0;
@@ -2520,17 +2564,15 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
                 launch_k1<true, double, kYProduct>(M, tm, c, x_mode(1), k1, nullptr, 0, st, flip());
             }
         } break;
-        case 2: {
-            OpSpmvAlpha op{ybuf(parity), rs_, a_, {}};
-            launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
-        } break;
+        case 2:
+            enqueue_k2(A, tm, parity, nullptr, 0, st);
+            break;
         case 3:  // the form the solver uses with every unit active
             enqueue_k3(M, tm, parity, nullptr, 0, tm.nunits >= split_units(tm), st);
             break;
-        case 4: {
-            OpSpmvOmega op{zbuf(parity), s_, rs_, t_};
-            launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
-        } break;
+        case 4:
+            enqueue_k4(A, tm, parity, nullptr, 0, st);
+            break;
         }
     };
     once();
@@ -2553,6 +2595,7 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
     require(variant >= -1 && variant <= 1, "time_iteration: variant must be -1, 0 or 1");
     const bool split = variant < 0 ? tm.nunits >= split_units(tm) : variant == 1;
     defer_x_ = nsys > 1;  // as the solve runs it
+    rs32_ = M.val && M.fp32;
     ensure(n, nsys, tm);
     hist_ptr_ = nullptr;
     hist_stride_ = 0;
@@ -2588,17 +2631,11 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
                 SPAI_HIP(hipEventRecord(e[1], st));
             }
             SPAI_HIP(hipEventRecord(e[2], st));
-            {
-                OpSpmvAlpha op{ybuf(parity), rs_, a_, {}};
-                launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
-            }
+            enqueue_k2(A, tm, parity, nullptr, 0, st);
             SPAI_HIP(hipEventRecord(e[3], st));
             enqueue_k3(M, tm, parity, nullptr, 0, split, st);
             SPAI_HIP(hipEventRecord(e[4], st));
-            {
-                OpSpmvOmega op{zbuf(parity), s_, rs_, t_};
-                launch_units<true, double>(A, tm, c, op, nullptr, 0, st, flip());
-            }
+            enqueue_k4(A, tm, parity, nullptr, 0, st);
         }
         SPAI_HIP(hipEventRecord(ev.back(), st));
         SPAI_HIP(hipEventSynchronize(ev.back()));

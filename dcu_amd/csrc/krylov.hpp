@@ -261,8 +261,14 @@ private:
                            bool split, hipStream_t st);
     void enqueue_k1(const OperatorView &A, const OperatorView &M, const TileMap &tiles, int parity, const int *list,
                     int grid, bool split, hipStream_t st);
+    void enqueue_k2(const OperatorView &A, const TileMap &tiles, int parity, const int *list, int grid,
+                    hipStream_t st);
     void enqueue_k3(const OperatorView &M, const TileMap &tiles, int parity, const int *list, int grid, bool split,
                     hipStream_t st);
+    void enqueue_k4(const OperatorView &A, const TileMap &tiles, int parity, const int *list, int grid,
+                    hipStream_t st);
+    void enqueue_init(const OperatorView &A, const TileMap &tiles, const double *x, const double *b, double rel_tol,
+                      double abs_tol, int restart, hipStream_t st);
     int n_ = 0, nsys_ = 0, nunits_ = -1, nnodes_ = -1, ngroups_ = -1;
     DeviceArray<double> vec_;       // x-independent work vectors
     DeviceArray<double> partials_;  // tree partials, then the iteration partials of the groups and the units
@@ -306,6 +312,8 @@ private:
     // A batch updates x in every other K1 (fewer bytes where bandwidth
     // bounds the launches); a single system, latency-bound, in every K1.
     bool defer_x_ = false;
+    // r* stored in FP32 (an FP32 M); every other vector is FP64
+    bool rs32_ = false;
     int x_mode(int parity) const { return defer_x_ ? (parity == 0 ? 1 : 2) : 0; }
     // without deferral one y and one z serve every iteration (read for x,
     // then overwritten, row by row, as before the double buffers)

@@ -46,6 +46,8 @@ def lib() -> ctypes.CDLL:
         L.oracle_set_threads.restype = None
         L.oracle_set_rho_mode.argtypes = [i]
         L.oracle_set_rho_mode.restype = None
+        L.oracle_set_shadow_fp32.argtypes = [i]
+        L.oracle_set_shadow_fp32.restype = None
         _lib = L
     return _lib
 
@@ -90,18 +92,19 @@ def set_rho_mode(mode: int) -> None:
     lib().oracle_set_rho_mode(int(mode))
 
 
-def iteration_spread(a_csr, m_csr, b, tol: float, max_iter: int, threads=(1, 2, 3, 4, 5, 6, 7, 8, 12, 16), x0=None):
+def iteration_spread(a_csr, m_csr, b, tol: float, max_iter: int, threads=(1, 2, 3, 4, 5, 6, 7, 8, 12, 16), x0=None,
+                     shadow32: bool = False):
     """Oracle iteration counts under different reduction orders (thread
     counts): BiCGSTAB's count is a chaotic function of rounding on
     non-normal problems, so GPU parity is judged against this spread.
-    x0: the initial guess (zeros by default)."""
+    x0: the initial guess (zeros by default); shadow32 as in bicgstab."""
     before = num_threads()
     its = []
     start = np.zeros(len(b)) if x0 is None else x0
     try:
         for t in threads:
             set_threads(t)
-            its.append(bicgstab(a_csr, m_csr, b, start, tol, max_iter)[1])
+            its.append(bicgstab(a_csr, m_csr, b, start, tol, max_iter, shadow32=shadow32)[1])
     finally:
         set_threads(before)
     return its
@@ -157,9 +160,11 @@ def spmv(rp, ci, v, x) -> np.ndarray:
     return y
 
 
-def bicgstab(a_csr, m_csr, b, x0, tol: float, max_iter: int, history: bool = False):
+def bicgstab(a_csr, m_csr, b, x0, tol: float, max_iter: int, history: bool = False, shadow32: bool = False):
     """Returns (x, iterations, info) with info = (||r||, ||r0||, converged);
-    with history=True also the per-iteration ||r_k|| as a fourth value."""
+    with history=True also the per-iteration ||r_k|| as a fourth value.
+    shadow32=True takes r* = r0 rounded to FP32 (the mixed-precision solve's
+    storage of the shadow residual)."""
     rp, ci, v = (_i32(a_csr[0]), _i32(a_csr[1]), _f64(a_csr[2]))
     b = _f64(b)
     x = np.array(x0, dtype=np.float64, copy=True)
@@ -171,9 +176,14 @@ def bicgstab(a_csr, m_csr, b, x0, tol: float, max_iter: int, history: bool = Fal
         mrp, mci, mv = _i32(m_csr[0]), _i32(m_csr[1]), _f64(m_csr[2])
         mp = (mrp.ctypes.data, mci.ctypes.data, mv.ctypes.data)
     hist = np.zeros(max(int(max_iter), 1)) if history else None
-    it = lib().oracle_bicgstab_history(len(rp) - 1, rp.ctypes.data, ci.ctypes.data, v.ctypes.data, mp[0], mp[1],
+    L = lib()
+    L.oracle_set_shadow_fp32(1 if shadow32 else 0)
+    try:
+        it = L.oracle_bicgstab_history(len(rp) - 1, rp.ctypes.data, ci.ctypes.data, v.ctypes.data, mp[0], mp[1],
                                        mp[2], b.ctypes.data, x.ctypes.data, float(tol), int(max_iter),
                                        info.ctypes.data, None if hist is None else hist.ctypes.data)
+    finally:
+        L.oracle_set_shadow_fp32(0)
     if history:
         k = it if info[2] else it - 1
         return x, it, info, hist[:k]

@@ -31,6 +31,8 @@
  *   with x0 and TOL as parameters instead of the reference's fixed b = x0 = 1,
  *   TOL = 1e-7 (src/spai_multi_dcu.cpp:1920-1934).  b is never overwritten
  *   (the reference overwrites it with A*x at exit, bicgstab_solver.h:237-246).
+ *   A switch (oracle_set_shadow_fp32) takes r* = fl32(r0), as the GPU's
+ *   mixed-precision solve stores it.
  *
  *   CSC -> CSR with rows ordered by column (include/common/cuFormatConversion.h
  *   lines 453-585; include/common/assemble.h:34-75).
@@ -337,6 +339,14 @@ static double dot(int n, const double *a, const double *b) {
 static int g_rho_mode = 0;
 void oracle_set_rho_mode(int mode) { g_rho_mode = mode; }
 
+/* Shadow residual of the mixed-precision solve: 0 (default) r* = r0 as the
+ * reference (bicgstab_solver.h:117); 1 r* = r0 rounded to FP32, every inner
+ * product with r* (rho and the alpha denominator) reading the rounded value.
+ * This is how the GPU solver stores r* next to an FP32 M; every other vector
+ * stays FP64. */
+static int g_shadow32 = 0;
+void oracle_set_shadow_fp32(int on) { g_shadow32 = on; }
+
 /* Right-preconditioned BiCGSTAB, restating bicgstab_solver.h:106-232.
  *   x      : in = x0, out = solution
  *   tol    : absolute threshold on the recursive residual ||r||_2 (strict <)
@@ -373,6 +383,8 @@ int oracle_bicgstab_history(int n, const int *arp, const int *aci, const double 
     for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
     memcpy(rs, r, sz);
     memcpy(p, r, sz);
+    if (g_shadow32)
+        for (int i = 0; i < n; ++i) rs[i] = (double)(float)rs[i];
     double err0 = sqrt(dot(n, r, r));
     double rho = dot(n, rs, r);
     double err = err0;

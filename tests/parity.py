@@ -25,12 +25,13 @@ from oracle import binding as O
 ORDERS = (1, 2, 3, 4, 5, 6, 7, 8, 12, 16)
 
 
-def oracle_counts(a_csr, m_csr, b, tol, max_iter, threads=ORDERS, x0=None):
+def oracle_counts(a_csr, m_csr, b, tol, max_iter, threads=ORDERS, x0=None, shadow32=False):
     counts = []
     try:
         for mode in (0, 1):
             O.set_rho_mode(mode)
-            counts += list(O.iteration_spread(a_csr, m_csr, b, tol, max_iter, threads=threads, x0=x0))
+            counts += list(O.iteration_spread(a_csr, m_csr, b, tol, max_iter, threads=threads, x0=x0,
+                                              shadow32=shadow32))
     finally:
         O.set_rho_mode(0)
     return counts

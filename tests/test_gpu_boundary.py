@@ -150,7 +150,8 @@ def test_residual_history_batch(ctx):
         bs = b[off[s]:off[s + 1]]
         Acsr = O.csc2csr(A.n, A.colptr, A.rowidx, A.val)[:3]
         Mcsr = O.csc2csr(A.n, A.colptr, A.rowidx, m_gpu[moff[s]:moff[s + 1]])[:3]
-        _, _, _, ho = O.bicgstab(Acsr, Mcsr, bs, np.zeros(A.n), 0.0, 11, history=True)
+        # the mixed solve stores r* in FP32: the oracle's shadow32 mode
+        _, _, _, ho = O.bicgstab(Acsr, Mcsr, bs, np.zeros(A.n), 0.0, 11, history=True, shadow32=True)
         rel = np.abs(h[:10] - ho[:10]) / ho[:10]
         assert rel.max() <= 1e-8, (s, rel.max())
     with pytest.raises(D.SpaiError):

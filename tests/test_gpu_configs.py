@@ -81,7 +81,8 @@ def test_batched_config_mixed(ctx):
         assert oc[s]["status"] == D.SOLVE_CONVERGED, (s, oc[s])
         assert true < 1e-8, (s, true)
     # Every system against the oracle run with the GPU's own FP32 M (widened
-    # to FP64): the first 10 iterates agree to 1e-8, and the count passes
+    # to FP64) and r* rounded to FP32 as the mixed solve stores it
+    # (shadow32): the first 10 iterates agree to 1e-8, and the count passes
     # count_ok (tests/parity.py: +-1 where every oracle sample agrees).  The
     # conv-diff systems converge irregularly: the oracle's own count moves by
     # up to +-50 between orders.
@@ -94,9 +95,10 @@ def test_batched_config_mixed(ctx):
         bs = b[off[s]:off[s + 1]]
         Acsr = O.csc2csr(A.n, A.colptr, A.rowidx, A.val)[:3]
         Mcsr = O.csc2csr(A.n, A.colptr, A.rowidx, m_gpu[moff[s]:moff[s + 1]])[:3]
-        x10 = O.bicgstab(Acsr, Mcsr, bs, np.zeros(A.n), 0.0, 11)[0]
+        x10 = O.bicgstab(Acsr, Mcsr, bs, np.zeros(A.n), 0.0, 11, shadow32=True)[0]
         e10 = np.linalg.norm(x10g[off[s]:off[s + 1]] - x10) / np.linalg.norm(x10)
-        spread = oracle_counts(Acsr, Mcsr, bs, 1e-8 * np.linalg.norm(bs), 1000, threads=(1, 2, 4, 8, 16))
+        spread = oracle_counts(Acsr, Mcsr, bs, 1e-8 * np.linalg.norm(bs), 1000, threads=(1, 2, 4, 8, 16),
+                               shadow32=True)
         print(f"cfg4 system {s} (n={A.n}): gpu {oc[s]['iterations']} oracle spread {sorted(spread)} "
               f"10-iterate rel diff {e10:.2e}", flush=True)
         assert oc10[s]["iterations"] == 11
@@ -123,13 +125,14 @@ def test_power_law_config(ctx):
     assert oc[0]["status"] == D.SOLVE_MAX_ITER and info[2] == 0
     assert 0.1 < g / o < 10.0
     # The row-tile solver path at full size, iterate by iterate: the first
-    # 10 iterates against the oracle run with the GPU's own M (FP32, widened),
-    # and the residual history of those iterations
+    # 10 iterates against the oracle run with the GPU's own M (FP32, widened)
+    # and r* rounded to FP32 (shadow32), and the residual history of those
+    # iterations
     m_gpu = S.m_values()
     Mg = O.csc2csr(A.n, A.colptr, A.rowidx, m_gpu)[:3]
     assert S.layout()[0] == "row_tiles"
     xg, _, og = S.solve(b, rel_tol=0.0, abs_tol=1e-300, max_iter=11, record_history=True)
-    xo, ito, _, ho = O.bicgstab(Acsr, Mg, b, np.zeros(A.n), 0.0, 11, history=True)
+    xo, ito, _, ho = O.bicgstab(Acsr, Mg, b, np.zeros(A.n), 0.0, 11, history=True, shadow32=True)
     hg = S.residual_history(0)
     e10 = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
     hrel = np.max(np.abs(hg - ho) / ho)

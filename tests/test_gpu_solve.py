@@ -230,7 +230,7 @@ def test_batch_of_more_than_1024_systems(ctx):
         assert true_rel < 1e-8, (s, true_rel)
         if s % 100 == 0:  # counts on a sample, against the oracle with the GPU's M
             Mcsr = O.csc2csr(A.n, A.colptr, A.rowidx, m_gpu[moff[s]:moff[s + 1]])[:3]
-            spread = oracle_counts(Acsr, Mcsr, bs, 1e-8 * np.linalg.norm(bs), 1000, threads=(1, 2))
+            spread = oracle_counts(Acsr, Mcsr, bs, 1e-8 * np.linalg.norm(bs), 1000, threads=(1, 2), shadow32=True)
             assert count_ok(oc[s]["iterations"], spread), (s, oc[s]["iterations"], sorted(spread))
 
 

@@ -13,7 +13,11 @@ count) fails it:
   - CSC -> CSR vs scipy and the reference's CSC2CSR (include/common/assemble.h);
   - BiCGSTAB iterate-by-iterate vs scipy.sparse.linalg.bicgstab, M = A^-1
     converging in one iteration, a 2x2 case by hand, the MAX_ITER semantics
-    of bicgstab_solver.h:132,231 and a planted mistake that must be caught.
+    of bicgstab_solver.h:132,231 and a planted mistake that must be caught;
+  - the FP32 shadow residual (shadow32) by a 2x2 iteration by hand with
+    r~ = fl32(r0), bit-identity with the FP64 solve when r0 is exactly
+    representable in FP32, and convergence with the recursive residual still
+    tracking b - A x otherwise.
 """
 import ctypes
 import os
@@ -308,6 +312,69 @@ def test_bicgstab_two_by_two_by_hand():
     x2, it, info = O.bicgstab((A.indptr, A.indices, A.data), None, b, np.zeros(2), 0.0, 3)
     assert np.allclose(A.toarray() @ x1 - b, [1.0 / 16, 1.0 / 16], rtol=0, atol=1e-15)
     assert np.all(np.isnan(x2))
+
+
+def test_shadow32_two_by_two_by_hand():
+    # shadow32: r* = fl32(r0).  One iteration by hand (van der Vorst's
+    # BiCGSTAB with a shadow vector r~ other than r0): rho0 = (r~, r0),
+    # alpha = rho0 / (r~, A r0), s = r0 - alpha A r0, omega = (As, s)/(As, As),
+    # x1 = alpha r0 + omega s.  b = (0.1, 0.3) round to FP32 with different
+    # relative errors (1.5e-8, 4.0e-8), so r~ is not a multiple of r0 and
+    # alpha moves by ~1e-8 if rho0 or (r~, A r0) used r0 instead of r~.
+    A = sp.csr_matrix(np.array([[2.0, 1.0], [0.0, 3.0]]))
+    Ad = A.toarray()
+    b = np.array([0.1, 0.3])
+    rt = b.astype(np.float32).astype(np.float64)
+    v = Ad @ b
+    alpha = (rt @ b) / (rt @ v)
+    s = b - alpha * v
+    t = Ad @ s
+    omega = (t @ s) / (t @ t)
+    x_hand = alpha * b + omega * s
+    x1, it, info = O.bicgstab((A.indptr, A.indices, A.data), None, b, np.zeros(2), 0.0, 2, shadow32=True)
+    assert np.allclose(x1, x_hand, rtol=1e-15, atol=0)
+    # the FP64 shadow differs in alpha at the 1e-8 level
+    alpha64 = (b @ b) / (b @ v)
+    assert abs(alpha - alpha64) > 1e-10 * abs(alpha64)
+    x64, _, _ = O.bicgstab((A.indptr, A.indices, A.data), None, b, np.zeros(2), 0.0, 2)
+    assert not np.allclose(x1, x64, rtol=1e-12, atol=0)
+
+
+def test_shadow32_rounds_only_the_shadow_residual():
+    # With r0 = b exactly representable in FP32 (x0 = 0, b of small binary
+    # fractions), fl32(r0) = r0 and the shadow32 solve is bit-identical to the
+    # FP64 one; were p, y or z rounded too, the iterates would differ.  With
+    # a general b both converge, the counts within 1 on these regularly
+    # converging systems, and the recursive residual still tracks b - A x.
+    # One thread: OpenMP combines the threads' partial sums in no fixed order.
+    before = O.num_threads()
+    O.set_threads(1)
+    try:
+        _shadow32_checks()
+    finally:
+        O.set_threads(before)
+
+
+def _shadow32_checks():
+    for name, n, cp, ri, va, m, b, xs, _ in _golden_systems():
+        A = O.csc2csr(n, cp, ri, va)[:3]
+        M = O.csc2csr(n, cp, ri, m)[:3]
+        bq = np.round(b * 64.0) / 64.0
+        assert np.array_equal(bq.astype(np.float32).astype(np.float64), bq)
+        xa, ita, _ = O.bicgstab(A, M, bq, np.zeros(n), 0.0, 7)
+        xb, itb, _ = O.bicgstab(A, M, bq, np.zeros(n), 0.0, 7, shadow32=True)
+        assert ita == itb and np.array_equal(xa, xb), name
+        bg = b + 1e-3 * np.sin(np.arange(n))  # entries not representable in FP32
+        assert not np.array_equal(bg.astype(np.float32).astype(np.float64), bg)
+        x5a, _, _ = O.bicgstab(A, M, bg, np.zeros(n), 0.0, 6)
+        x5b, _, _ = O.bicgstab(A, M, bg, np.zeros(n), 0.0, 6, shadow32=True)
+        d = np.linalg.norm(x5a - x5b) / np.linalg.norm(x5a)
+        assert 0.0 < d < 1e-5, (name, d)
+        tol = 1e-10 * np.linalg.norm(bg)
+        xa, ita, ia = O.bicgstab(A, M, bg, np.zeros(n), tol, 1000)
+        xb, itb, ib = O.bicgstab(A, M, bg, np.zeros(n), tol, 1000, shadow32=True)
+        assert ia[2] == 1 and ib[2] == 1 and abs(ita - itb) <= 1, (name, ita, itb)
+        assert abs(O.true_residual(A, bg, xb) - ib[0]) <= 1e-12 * np.linalg.norm(bg), name
 
 
 def test_bicgstab_max_iter_semantics():
