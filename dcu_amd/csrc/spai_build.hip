@@ -78,16 +78,22 @@ struct RegTier {
 // whose largest T_k and |J_k| hold it.  Each bucket has a list of candidate
 // group shapes (the first is the default); the autotuner times them on a
 // system's own columns and keeps the fastest per bucket.  Every candidate of
-// a bucket gathers exactly the bucket's T_k (TMAX), holds its |J_k| (NJ, one
-// lane per x entry in the refinement: W >= NJ) and, when its rows fall short
-// of T_k, hands a wider column to the next bucket, whose every candidate
-// holds the rows.  The shape changes the order of the group sums, so M
-// differs between shapes in the last bits (well inside the parity bars).
+// a bucket gathers exactly the bucket's T_k (TMAX) and has a lane per x entry
+// in the refinement (W >= NJ).  A candidate's NJ may be below the bucket's
+// largest |J_k| (5 columns for the 2D 5-point stencil, 7 for the 3D 7-point:
+// the QR, the solve and the refinement then do not run over padded columns);
+// a column with more than NJ columns, or with more rows than the candidate
+// holds, goes to the next bucket, whose every candidate holds its rows (the
+// last bucket's candidates hold its full |J_k|).  The shape changes the order
+// of the group sums, so M differs between shapes in the last bits (well
+// inside the parity bars).
 constexpr int kRegTiers = kTunedTiers;
 constexpr int kBucketT[kRegTiers] = {32, 64, 128};
 constexpr int kBucketNJ[kRegTiers] = {8, 8, 16};
-using Bucket0 = std::tuple<RegTier<8, 4, 8, 2>, RegTier<16, 2, 8, 1>, RegTier<8, 4, 8, 4>, RegTier<16, 2, 8, 2>>;
-using Bucket1 = std::tuple<RegTier<16, 4, 8, 2>, RegTier<32, 2, 8, 1>, RegTier<16, 4, 8, 4>, RegTier<8, 8, 8, 4>>;
+using Bucket0 = std::tuple<RegTier<8, 4, 8, 2>, RegTier<16, 2, 8, 1>, RegTier<8, 4, 8, 4>, RegTier<16, 2, 8, 2>,
+                           RegTier<8, 4, 5, 2>>;
+using Bucket1 = std::tuple<RegTier<16, 4, 8, 2>, RegTier<32, 2, 8, 1>, RegTier<16, 4, 8, 4>, RegTier<8, 8, 8, 4>,
+                           RegTier<8, 8, 7, 4>, RegTier<16, 4, 7, 2>>;
 using Bucket2 = std::tuple<RegTier<64, 2, 16, 2>, RegTier<32, 4, 16, 4>>;
 template <int B>
 using BucketOf = std::tuple_element_t<B, std::tuple<Bucket0, Bucket1, Bucket2>>;
@@ -95,7 +101,8 @@ using BucketOf = std::tuple_element_t<B, std::tuple<Bucket0, Bucket1, Bucket2>>;
 constexpr bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 template <int B, class TR>
 constexpr bool shape_ok() {
-    return TR::TMAX == kBucketT[B] && TR::NJ == kBucketNJ[B] && TR::W >= TR::NJ && pow2(TR::W) && TR::W <= 64 &&
+    return TR::TMAX == kBucketT[B] && TR::NJ <= kBucketNJ[B] && (B < kRegTiers - 1 || TR::NJ == kBucketNJ[B]) &&
+           TR::W >= TR::NJ && pow2(TR::W) && TR::W <= 64 &&
            pow2(TR::EPL) && kBlock % TR::W == 0 && (B == 0 || TR::ROWS >= kBucketT[B - 1]) &&
            (B < kRegTiers - 1 || TR::ROWS >= TR::TMAX);
 }
@@ -108,8 +115,8 @@ constexpr bool bucket_ok() {
     return bucket_ok<B>(std::make_index_sequence<std::tuple_size_v<BucketOf<B>>>{});
 }
 static_assert(bucket_ok<0>() && bucket_ok<1>() && bucket_ok<2>(),
-              "a candidate must gather its bucket's T_k, hold its |J_k|, and an overflowed column must fit the "
-              "next bucket's rows");
+              "a candidate must gather its bucket's T_k, the last bucket's must hold its |J_k|, and an overflowed "
+              "column must fit the next bucket's rows");
 
 // f(shape) for candidate c of bucket B (c is range-checked by the caller)
 template <int B, class F, size_t... I>
@@ -276,6 +283,14 @@ __device__ __forceinline__ void bitonic_sort(KT (&key)[EPL], int lane) {
     }
 }
 
+// Reciprocal and square root of the register-tier Householder: the hardware
+// instructions in FP32 (mixed mode, whose FP64 refinement sweep corrects the
+// FP32 solution), IEEE operations in FP64.
+__device__ __forceinline__ float qr_rcp(float v) { return __builtin_amdgcn_rcpf(v); }
+__device__ __forceinline__ double qr_rcp(double v) { return 1.0 / v; }
+__device__ __forceinline__ float qr_sqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
+__device__ __forceinline__ double qr_sqrt(double v) { return sqrt(v); }
+
 template <class MT>
 __device__ __forceinline__ void store_m(const ColumnArgs &a, int pos, double x) {
     MT *csc = (MT *)a.m_csc;
@@ -293,8 +308,11 @@ __device__ __forceinline__ void store_m(const ColumnArgs &a, int pos, double x) 
 #ifndef SPAI_TIER_ENTRY_TABLE
 #define SPAI_TIER_ENTRY_TABLE 1
 #endif
+// Column k with J_k = prowidx[c0 .. c0 + n2) and, in lane j < n2, jr = the
+// row index J_k[j] (both loaded by the caller, see spai_reg_tier_kernel).
 template <class TR, class QT, bool REFINE, class MT, bool K32>
-__device__ __forceinline__ void solve_column_reg(const ColumnArgs &a, int k, int lane, double *lds) {
+__device__ __forceinline__ void solve_column_reg(const ColumnArgs &a, int k, int c0, int n2, int jr, int lane,
+                                                 double *lds) {
     constexpr int W = TR::W, EPL = TR::EPL, NJ = TR::NJ, TS = TR::TS;
     using SK = SortKey<K32, log2_exact(TR::TMAX)>;
     using KT = typename SK::T;
@@ -308,14 +326,15 @@ __device__ __forceinline__ void solve_column_reg(const ColumnArgs &a, int k, int
     unsigned char *jslot = (unsigned char *)lds + L::JS_OFF;  //   their J indices
     int *rstart = (int *)((char *)lds + L::RST_OFF);          //   first entry of each row, n1 + 1
 
-    const int c0 = a.pcolptr[k];
-    const int n2 = a.pcolptr[k + 1] - c0;
     if (n2 <= 0) return;
+    if (n2 > NJ) {  // a shape narrower than its bucket: the next bucket takes it
+        if (lane == 0) a.ovf_list[atomicAdd(a.ovf_count, 1)] = k;
+        return;
+    }
 
-    // J_k and the extent of each column A(:, J_k[j])
+    // the extent of each column A(:, J_k[j])
     int cs = 0, cl = 0;
     if (lane < n2) {
-        int jr = a.prowidx[c0 + lane];
         cs = a.colptr[jr];
         cl = a.colptr[jr + 1] - cs;
     }
@@ -457,10 +476,10 @@ __device__ __forceinline__ void solve_column_reg(const ColumnArgs &a, int k, int
             QT sigma = group_allsum<W>(sl);
             QT beta = alpha, t = 0, scale = 0;
             if (sigma != (QT)0) {
-                QT nrm = sqrt(alpha * alpha + sigma);
+                QT nrm = qr_sqrt(alpha * alpha + sigma);
                 beta = alpha >= (QT)0 ? -nrm : nrm;
-                t = (beta - alpha) / beta;
-                scale = (QT)1 / (alpha - beta);
+                t = (beta - alpha) * qr_rcp(beta);
+                scale = qr_rcp(alpha - beta);
             }
             tau[j] = t;
             QT v[RPL];
@@ -635,50 +654,108 @@ __device__ __forceinline__ void solve_column_reg(const ColumnArgs &a, int k, int
 #ifndef SPAI_TIER_MIN_WAVES
 #define SPAI_TIER_MIN_WAVES 0
 #endif
-#if SPAI_TIER_MIN_WAVES > 0
-#define SPAI_TIER_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SPAI_TIER_MIN_WAVES)))
-#else
-#define SPAI_TIER_OCCUPANCY
+// Without it, the FP32 shapes of at most 8 columns ask for 4 wavefronts per
+// SIMD (128 registers): the 3D 7-column shape otherwise lands 2 registers
+// above that and runs 3.
+template <class TR, class QT>
+constexpr int tier_min_waves() {
+    return SPAI_TIER_MIN_WAVES > 0 ? SPAI_TIER_MIN_WAVES : (sizeof(QT) == 4 && TR::NJ <= 8 ? 4 : 1);
+}
+#define SPAI_TIER_OCCUPANCY __attribute__((amdgpu_waves_per_eu(tier_min_waves<TR, QT>())))
+// A wavefront takes its columns in batches of CH per group (G * CH per
+// wavefront).  The dependent index loads that precede a column's gather (the
+// list entry k, then pcolptr[k], then J_k = prowidx[...]) are issued for the
+// whole batch at once, one round trip per level for CH columns, and parked in
+// a small per-group LDS area; the columns then run one after another.
+//
+// Dynamic schedule: most of a tier's list is split statically into equal
+// contiguous shares, one per wavefront of the persistent grid (columns of one
+// bucket cost about the same), and only the rest goes through the atomic
+// queue, so the queue head, a single address every wavefront increments, sees
+// few atomics.
+#ifndef SPAI_TIER_STATIC_SHARE
+#define SPAI_TIER_STATIC_SHARE 3  // quarters of the list split statically
 #endif
 template <class TR, class QT, bool REFINE, class MT, bool DYNAMIC, bool K32>
 __global__ __launch_bounds__(kBlock) SPAI_TIER_OCCUPANCY void spai_reg_tier_kernel(ColumnArgs a) {
     constexpr int W = TR::W;
     constexpr int G = 64 / W;    // groups per wavefront
-    constexpr int CH = SPAI_TIER_GRAB;  // columns per group per queue grab
+    constexpr int CH = SPAI_TIER_GRAB;  // columns per group per batch
+    constexpr int HEAD = CH * (W + 3);  // per group: k, c0, n2 and each lane's J_k entry, per batch column
     __shared__ double lds_all[TR::GROUPS * TierLds<TR, QT, REFINE>::DOUBLES];
+    __shared__ int head_all[TR::GROUPS * HEAD];
     const int tid = threadIdx.x;
     const int lane = tid % W;
     const int grp = tid / W;
     const int wgrp = (tid % 64) / W;  // group index inside the wavefront
     double *lds = lds_all + grp * TierLds<TR, QT, REFINE>::DOUBLES;
+    int *hk = head_all + grp * HEAD, *hc0 = hk + CH, *hn2 = hc0 + CH, *hjr = hn2 + CH;
 
+    // one batch: column_of(it) gives the group's it-th column (or -1)
+    auto batch = [&](auto column_of) {
+        int kk[CH], c0v[CH], n2v[CH];
+#pragma unroll
+        for (int it = 0; it < CH; ++it) kk[it] = column_of(it);
+#pragma unroll
+        for (int it = 0; it < CH; ++it) {
+            c0v[it] = kk[it] >= 0 ? a.pcolptr[kk[it]] : 0;
+            n2v[it] = kk[it] >= 0 ? a.pcolptr[kk[it] + 1] - c0v[it] : 0;
+        }
+#pragma unroll
+        for (int it = 0; it < CH; ++it) {
+            // lanes past the tier's NJ load nothing (such a column moves on)
+            hjr[it * W + lane] = (lane < n2v[it] && n2v[it] <= TR::NJ) ? a.prowidx[c0v[it] + lane] : 0;
+            if (lane == 0) {
+                hk[it] = kk[it];
+                hc0[it] = c0v[it];
+                hn2[it] = n2v[it];
+            }
+        }
+        wave_lds_fence();
+        for (int it = 0; it < CH; ++it) {
+            const int k = hk[it];
+            if (k >= 0) solve_column_reg<TR, QT, REFINE, MT, K32>(a, k, hc0[it], hn2[it], hjr[it * W + lane], lane, lds);
+        }
+        wave_lds_fence();  // the head area is rewritten by the next batch
+    };
+
+    const int nwaves = gridDim.x * (kBlock / 64);
+    const int wave = blockIdx.x * (kBlock / 64) + tid / 64;
     if (DYNAMIC) {
         // the tier's list, then the columns the previous tier handed on
         const int count = *a.count;
         const int total = count + (a.ovf_in_count ? *a.ovf_in_count : 0);
+        if (total == 0) return;
+        // static shares, whole batches each; then the rest from the queue
+        // (its head counts from the end of the shares).  One loop, so that
+        // the column code is inlined once.
+        const int share = (int)((long long)total * SPAI_TIER_STATIC_SHARE / 4 / nwaves / (G * CH)) * (G * CH);
+        const int rest0 = share * nwaves;
+        int base = wave * share;
+        const int share_end = base + share;
         for (;;) {
-            int base = 0;
-            if ((tid & 63) == 0) base = atomicAdd(a.head, G * CH);
-            base = __shfl(base, 0, 64);
-            if (base >= total) break;
-            for (int it = 0; it < CH; ++it) {
-                int idx = base + it * G + wgrp;
-                if (idx < total)
-                    solve_column_reg<TR, QT, REFINE, MT, K32>(a, idx < count ? a.list[idx] : a.ovf_in_list[idx - count],
-                                                         lane, lds);
+            if (base >= share_end) {
+                int b = 0;
+                if ((tid & 63) == 0) b = atomicAdd(a.head, G * CH);
+                base = __shfl(b, 0, 64) + rest0;
+                if (base >= total) break;
             }
+            batch([&](int it) {
+                const int idx = base + it * G + wgrp;
+                return idx < total ? (idx < count ? a.list[idx] : a.ovf_in_list[idx - count]) : -1;
+            });
+            base = base + G * CH < share_end ? base + G * CH : share_end;
         }
     } else {
-        const int nwaves = gridDim.x * (kBlock / 64);
-        const int wave = blockIdx.x * (kBlock / 64) + tid / 64;
         const int ncols = a.col_end - a.col_begin;
         const int chunk = (ncols + nwaves - 1) / nwaves;
         const int start = a.col_begin + wave * chunk;
         const int stop = min(start + chunk, a.col_end);
-        for (int base = start; base < stop; base += G) {
-            int k = base + wgrp;
-            if (k < stop) solve_column_reg<TR, QT, REFINE, MT, K32>(a, k, lane, lds);
-        }
+        for (int base = start; base < stop; base += G * CH)
+            batch([&](int it) {
+                const int k = base + it * G + wgrp;
+                return k < stop ? k : -1;
+            });
     }
 }
 

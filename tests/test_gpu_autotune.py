@@ -4,8 +4,9 @@ Every candidate of every size bucket is pinned against the oracle on
 columns that reach it: M within the usual bars (FP64 1e-10, mixed 4 * 2^-23,
 relative per column), |I_k| bit-exact, and the columns counted in the tier
 that solved them, which follows from the oracle's (T_k, |J_k|, |I_k|) and
-the candidate's rows per group (a column whose |I_k| exceeds them is solved
-by the next bucket).  The autotuner must return candidates that exist, time
+the candidate's rows and columns per group (a column whose |I_k| exceeds the
+rows, or whose |J_k| exceeds the columns of a candidate narrower than its
+bucket, is solved by the next bucket).  The autotuner must return candidates that exist, time
 every candidate of a populated bucket, leave M built with its choice (bit
 for bit what a pinned build with that choice gives) and keep the choice for
 later builds.
@@ -27,8 +28,10 @@ TOL = {D.PRECISION_FP64: 1e-10, D.PRECISION_MIXED: 4 * 2.0 ** -23}
 def expected_tiers(n1, n2, t, shape):
     """Columns solved per tier under `shape` (balanced_nnz): the first bucket
     that holds (T_k, |J_k|), then on to the next while |I_k| exceeds the
-    candidate's rows; 3 the LDS tier, 4 the global tier."""
+    candidate's rows or |J_k| its columns; 3 the LDS tier, 4 the global
+    tier."""
     rows = [D.tile_shape(b, shape[b])["lanes"] * D.tile_shape(b, shape[b])["rows_per_lane"] for b in range(3)]
+    cols = [D.tile_shape(b, shape[b])["max_cols"] for b in range(3)]
     out = np.zeros(5, dtype=np.int64)
     for a, b_, c in zip(n1, n2, t):
         if b_ == 0:
@@ -37,7 +40,7 @@ def expected_tiers(n1, n2, t, shape):
         if tier is None:
             need = c * b_ + b_ * b_ + b_
             tier = 3 if (b_ <= 64 and c <= 1024 and need <= 7168) else 4
-        while tier < 2 and a > rows[tier]:
+        while tier < 2 and (a > rows[tier] or b_ > cols[tier]):
             tier += 1
         out[tier] += 1
     return out.tolist()
@@ -97,6 +100,49 @@ def test_overflow_follows_the_rows(ctx, schedule, c0):
     assert err <= 1e-10
     assert rep["tier_columns"][1] == int((n1_ref > rows).sum())
     assert rep["tier_columns"][0] == int((n1_ref <= rows).sum())
+    S.close()
+
+
+def wide_short_columns(n=3000, seed=5):
+    """About a third of the columns hold the diagonal and 5-7 random rows,
+    the rest the diagonal and 1-2: columns with |J_k| of 6 to 8 whose T_k
+    stays within 32 or 64, which the 5- and 7-column candidates hand on."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for k in range(n):
+        m = int(rng.integers(5, 8)) if rng.uniform() < 0.33 else int(rng.integers(1, 3))
+        rs = set(rng.choice(n, m, replace=False).tolist()) | {k}
+        rows += sorted(rs)
+        cols += [k] * len(rs)
+    A = sp.csc_matrix((rng.uniform(-1, 1, len(rows)), (rows, cols)), shape=(n, n))
+    A.setdiag(10.0)
+    A.sort_indices()
+    return D.CSC(n, A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64))
+
+
+@pytest.mark.parametrize("schedule", [D.SCHEDULE_BALANCED_COLUMNS, D.SCHEDULE_BALANCED_NNZ])
+@pytest.mark.parametrize("shape", [[4, 4, 0], [4, 5, 0], [0, 4, 0]])
+def test_narrow_shapes_hand_wide_columns_on(ctx, schedule, shape):
+    # candidates narrower than their bucket (5 or 7 columns): a column with
+    # more columns moves to the next bucket (twice for |J_k| = 8 under a
+    # 5-column then a 7-column candidate), under both schedules
+    A = wide_short_columns()
+    n1_ref, n2_ref, t_ref, _ = O.column_sets(A.n, A.colptr, A.rowidx)
+    assert ((t_ref <= 32) & (n2_ref > 5)).sum() > 50
+    assert ((t_ref > 32) & (t_ref <= 64) & (n2_ref == 8)).sum() > 20
+    m_ref = O.spai(A.n, A.colptr, A.rowidx, A.val, 0)
+    S = D.System(ctx, A)
+    S.set_tile_shapes(shape)
+    for precision in (D.PRECISION_FP64, D.PRECISION_MIXED):
+        rep = S.build(precision, schedule)
+        err = column_rel_err(A, S.m_values(), m_ref)
+        print(f"shape {shape} sched {schedule} prec {precision}: tiers {rep['tier_columns']} err {err:.3e}",
+              flush=True)
+        assert err <= TOL[precision], (shape, err)
+        assert np.array_equal(S.row_set_sizes(), n1_ref)
+        if schedule == D.SCHEDULE_BALANCED_NNZ:
+            assert rep["tier_columns"] == expected_tiers(n1_ref, n2_ref, t_ref, shape)
     S.close()
 
 

@@ -40,10 +40,12 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_tile_shape_candidates():
-    # each size bucket's candidates gather exactly its T_k, hold its |J_k|
-    # with a lane per x entry, and a column handed on by a bucket (|I_k| <=
-    # T_k of that bucket) fits every candidate of the next; the last bucket
-    # never hands on; candidate 0 is the default
+    # each size bucket's candidates gather exactly its T_k, hold at most its
+    # |J_k| with a lane per x entry (the last bucket's hold all of it), and a
+    # column handed on by a bucket (|I_k| <= T_k of that bucket) fits every
+    # candidate of the next; the last bucket never hands on; candidate 0 is
+    # the default; the 2D 5-point and 3D 7-point stencils have candidates of
+    # their own width
     bucket_t, bucket_nj = (32, 64, 128), (8, 8, 16)
     assert [D.tile_shape_count(t) for t in range(4)][3] == 0
     for t in range(3):
@@ -53,13 +55,16 @@ def test_tile_shape_candidates():
         assert len({tuple(s.values()) for s in shapes}) == n
         for s in shapes:
             lanes, epl = s["lanes"], s["entries_per_lane"]
-            assert lanes * epl == bucket_t[t] and s["max_cols"] == bucket_nj[t]
+            assert lanes * epl == bucket_t[t] and s["max_cols"] <= bucket_nj[t]
+            assert t < 2 or s["max_cols"] == bucket_nj[t]
             assert lanes >= s["max_cols"] and 64 % lanes == 0 and epl & (epl - 1) == 0
             rows = lanes * s["rows_per_lane"]
             if t > 0:
                 assert rows >= bucket_t[t - 1]
             if t == 2:
                 assert rows >= bucket_t[t]
+    assert any(D.tile_shape(0, c)["max_cols"] == 5 for c in range(D.tile_shape_count(0)))
+    assert any(D.tile_shape(1, c)["max_cols"] == 7 for c in range(D.tile_shape_count(1)))
     assert [D.tile_shape(t, 0) for t in range(3)] == [
         {"lanes": 8, "entries_per_lane": 4, "max_cols": 8, "rows_per_lane": 2},
         {"lanes": 16, "entries_per_lane": 4, "max_cols": 8, "rows_per_lane": 2},
