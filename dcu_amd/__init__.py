@@ -645,12 +645,15 @@ class System:
         return [float(v) for v in ms]
 
     def layout(self) -> Tuple[str, int]:
-        """("sell64" | "sell64_off16" | "sell64_diag" | "row_tiles", stored
-        entries per operator).  sell64_off16 stores 16-bit column - row
-        offsets, sell64_diag one offset per column block of a slice."""
+        """("sell64" | "sell64_off16" | "sell64_diag" | "row_tiles" |
+        "row_tiles_blocked", stored entries per operator).  sell64_off16
+        stores 16-bit column - row offsets, sell64_diag one offset per column
+        block of a slice; row_tiles_blocked splits the products by column
+        blocks."""
         kind, entries = ctypes.c_int(), ctypes.c_longlong()
         _check(self._lib.spai_system_layout(self._h, ctypes.byref(kind), ctypes.byref(entries)))
-        return {0: "row_tiles", 1: "sell64", 2: "sell64_off16", 3: "sell64_diag"}[kind.value], int(entries.value)
+        names = {0: "row_tiles", 1: "sell64", 2: "sell64_off16", 3: "sell64_diag", 4: "row_tiles_blocked"}
+        return names[kind.value], int(entries.value)
 
 
 def _report(rep: SolveReport) -> dict:

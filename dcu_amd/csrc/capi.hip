@@ -223,7 +223,7 @@ void finish_setup(spai_system *S, hipStream_t st) {
     S->perm.resize((size_t)std::max(S->nnz, 1));
     csc_to_csr_device(S->n, S->n, S->nnz, S->colptr.get(), S->rowidx.get(), S->cscval.get(), S->rowptr.get(),
                       S->colidx.get(), S->csrval.get(), S->perm.get(), st);
-    build_layout(S->tiles, S->rowptr.get(), S->colidx.get(), S->h_row_off, S->h_nnz_off, true, st);
+    build_layout(S->tiles, S->rowptr.get(), S->colidx.get(), S->h_row_off, S->h_nnz_off, true, st, nullptr, true);
     S->sell_layout = S->tiles.kind == kLayoutSell;
     S->m_nnz = S->nnz;
     S->perm_lay.resize((size_t)std::max(S->nnz, 1));
@@ -794,7 +794,7 @@ int spai_system_set_pattern(spai_system *sys, int pattern) {
             sys->m_nnz = nnz;
             if (!sys->sell_layout)
                 build_layout(sys->tiles, sys->rowptr.get(), sys->colidx.get(), sys->h_row_off, sys->h_nnz_off, false,
-                             st);
+                             st, nullptr, true);
             layout_perm(sys->tiles, nnz, sys->perm.get(), sys->perm_lay.get(), st);
             SPAI_HIP(hipStreamSynchronize(st));
             return;
@@ -1150,7 +1150,9 @@ int spai_system_time_iteration(spai_system *sys, int reps, int variant, float *m
 int spai_system_layout(const spai_system *sys, int *kind, long long *entries) {
     if (!sys) return fail(SPAI_ERR_INVALID, "sys is NULL");
     if (kind)
-        *kind = sys->tiles.kind != kLayoutSell ? SPAI_LAYOUT_ROW_TILES
+        *kind = sys->tiles.kind != kLayoutSell
+                    ? (sys->tiles.cblocks > 0 && sys->pattern == SPAI_PATTERN_A ? SPAI_LAYOUT_ROW_TILES_BLOCKED
+                                                                                  : SPAI_LAYOUT_ROW_TILES)
                 : sys->tiles.sell_diag.get()   ? SPAI_LAYOUT_SELL64_DIAG
                 : sys->tiles.sell_off.get()    ? SPAI_LAYOUT_SELL64_OFF16
                                                : SPAI_LAYOUT_SELL64;

@@ -94,6 +94,22 @@ constexpr int kFanIn = SPAI_REDUCE_FANIN;  // children per reduction tree node
 #define SPAI_SPLIT_UNITS_TILES 1024
 #endif
 constexpr int kSplitUnits = SPAI_SPLIT_UNITS;
+// Column blocks of large one-system row-tile layouts: blocks (dividing the 8
+// XCDs; each block's slice of an FP64 vector is n / kColBlocks * 8 bytes) and
+// the smallest system that gets them (its FP64 vector, 4 MiB, fills an XCD's
+// L2).
+#ifndef SPAI_COLUMN_BLOCKS
+#define SPAI_COLUMN_BLOCKS 4
+#endif
+constexpr int kColBlocks = SPAI_COLUMN_BLOCKS;
+static_assert(kColBlocks >= 1 && 8 % kColBlocks == 0, "column blocks divide the 8 XCDs");
+constexpr int kColBlockMinRows = 1 << 19;
+// Rows per tile of a column block (a block holds about 1 / kColBlocks of a
+// row's entries, so its tiles take more rows for the same products).
+#ifndef SPAI_PART_TILE_ROWS
+#define SPAI_PART_TILE_ROWS 512
+#endif
+constexpr int kRowsPerPartTile = SPAI_PART_TILE_ROWS;
 constexpr int kSplitUnitsTiles = SPAI_SPLIT_UNITS_TILES;
 #ifndef SPAI_COMPACT_EVERY
 #define SPAI_COMPACT_EVERY 8
@@ -547,6 +563,8 @@ struct UnitView {
     const int *count;         // number of entries of the list
     const int *sys_row;       // nsys + 1 row boundaries of the systems
     int reverse;              // each XCD walks its range of units from the end
+    const double *psum;       // column-block partial sums (k_tiles with PS > 0): block c of row i at psum[c * pstride + i]
+    int pstride;
 };
 
 // The workgroup's unit, or -1 past the end of the active list.
@@ -621,8 +639,10 @@ constexpr int kTileShortRow = SPAI_TILE_SHORT_ROW;
 // wavefront: lane l sums entries l, l + 64, ... in order, then a fixed DPP
 // tree adds the lanes, so that no thread walks a long row alone while the
 // workgroup waits.  Every order is fixed: results are reproducible.
-// HAS_MATRIX = false skips the product (streaming ops).
-template <bool HAS_MATRIX, class MT, class Op>
+// HAS_MATRIX = false skips the product (streaming ops); with PS > 0 the
+// product of row i is instead the sum of PS column-block partials
+// uv.psum[c * uv.pstride + i], added in block order (see k_tiles_part).
+template <bool HAS_MATRIX, class MT, class Op, int PS = 0>
 __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr, const int *__restrict__ idx,
                                                     const MT *__restrict__ val, UnitView uv, Ctx c, Op op) {
     constexpr int NRED = Op::NRED;
@@ -646,7 +666,16 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
     const int nrows = r1 - r0;
     if (!HAS_MATRIX) {
         take_finish<kThreads / 64>(op, gs, shk);
-        for (int i = r0 + tid; i < r1; i += kThreads) op.row(i, 0.0, op.pre(i, acc), acc);
+        for (int i = r0 + tid; i < r1; i += kThreads) {
+            const typename Op::Pre pr = op.pre(i, acc);
+            double prod = 0.0;
+            if constexpr (PS > 0) {
+                prod = uv.psum[i];
+#pragma unroll
+                for (int q = 1; q < PS; ++q) prod += uv.psum[(size_t)q * uv.pstride + i];
+            }
+            op.row(i, prod, pr, acc);
+        }
     } else {
         const int p0 = ptr[r0];
         const int cnt = ptr[r1] - p0;
@@ -729,6 +758,97 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
         }
     }
     finish_unit<kThreads / 64>(c, unit, s, acc, op, sh);
+}
+
+// Column-blocked products (row tiles of one large system, TileMap::cblocks =
+// C): workgroup b runs on XCD b % 8, which serves column block (b % 8) % C,
+// and the block's tiles are dealt to its 8 / C XCDs in order.  A tile forms,
+// for each of its rows, the sum of the products over the block's columns
+// (the gathered values lie in the block's 1/C of the vector, which stays in
+// the XCD's L2) and stores it as part[c * n + i]; every row of every block is
+// written (0.0 for a row with no entry in the block).  The row sums follow
+// k_tiles: storage order per row, long rows by a wavefront in a fixed tree.
+// Only ops whose gather reads one stored vector use it (no state, no sums).
+template <class MT, class Op, int C>
+__global__ __launch_bounds__(kThreads) void k_tiles_part(const int *__restrict__ ptr, const int *__restrict__ idx,
+                                                         const MT *__restrict__ val, const int *__restrict__ unit_row,
+                                                         const int *__restrict__ unit_off, int n,
+                                                         double *__restrict__ part, Ctx c, Op op) {
+    __shared__ double prod[kLdsProducts];  // a long row's sum replaces its first product
+    __shared__ int rp[kRowsPerPartTile + 1];
+    __shared__ double sh[4];
+    const int b = (int)blockIdx.x, x = b & 7, blk = x % C;
+    const int q = (b >> 3) * (8 / C) + x / C;  // the tile among its block's
+    const int u0 = unit_off[blk], nu = unit_off[blk + 1] - u0 - 1;
+    if (q >= nu) return;
+    if (!op.enabled(c.st[0])) return;
+    const int tid = threadIdx.x;
+    const int r0 = unit_row[u0 + q], r1 = unit_row[u0 + q + 1];
+    const int nrows = r1 - r0;
+    const int *bp = ptr + (size_t)blk * n;  // bp[i]: first entry of row i in the block; bp[n] ends the block
+    double *out = part + (size_t)blk * n;
+    const int p0 = bp[r0];
+    const int cnt = bp[r1] - p0;
+    if (cnt <= kLdsProducts && nrows <= kRowsPerPartTile) {
+        for (int k = tid; k <= nrows; k += kThreads) rp[k] = bp[r0 + k] - p0;
+        constexpr int U = kLdsProducts / kThreads;
+        int jj[U];
+        MT vv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int qq = u * kThreads + tid;
+            jj[u] = qq < cnt ? stream_load(&idx[p0 + qq]) : 0;
+            vv[u] = qq < cnt ? stream_load(&val[p0 + qq]) : (MT)0;
+        }
+        double gv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int qq = u * kThreads + tid;
+            gv[u] = qq < cnt ? op.gather(jj[u]) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int qq = u * kThreads + tid;
+            if (qq < cnt) prod[qq] = (double)vv[u] * gv[u];
+        }
+        __syncthreads();
+        {
+            const int lane = tid & 63;
+            for (int base = tid - lane; base < nrows; base += kThreads) {
+                const int li = base + lane;
+                const bool lng = li < nrows && rp[li + 1] - rp[li] > kTileShortRow;
+                unsigned long long m = __ballot(lng);
+                while (m) {
+                    const int j = __ffsll((long long)m) - 1;
+                    m &= m - 1;
+                    const int bb = rp[base + j], e = rp[base + j + 1];
+                    double s = 0.0;
+                    for (int qq = bb + lane; qq < e; qq += 64) s += prod[qq];
+                    s = group_allsum<64>(s);  // every lane has read the row's products
+                    if (lane == 0) prod[bb] = s;
+                }
+            }
+        }
+        __syncthreads();
+        for (int li = tid; li < nrows; li += kThreads) {
+            const int bb = rp[li], e = rp[li + 1];
+            double s = 0.0;
+            if (e - bb > kTileShortRow) {
+                s = prod[bb];
+            } else {
+                for (int qq = bb; qq < e; ++qq) s += prod[qq];
+            }
+            __builtin_nontemporal_store(s, &out[r0 + li]);
+        }
+    } else {
+        // a row longer than the staging buffer (a tile of its own)
+        for (int i = r0; i < r1; ++i) {
+            double s[1] = {0.0};
+            for (int qq = bp[i] + tid; qq < bp[i + 1]; qq += kThreads) s[0] += (double)val[qq] * op.gather(idx[qq]);
+            block_reduce<1>(s, sh);
+            if (tid == 0) out[i] = s[0];
+        }
+    }
 }
 
 // SELL-64.  Wavefront w of the workgroup takes kSellSlicesPerWave
@@ -1648,6 +1768,26 @@ void launch_k1(const OperatorView &O, const TileMap &tm, const Ctx &c, int xmode
     else go(std::integral_constant<int, kXTwo>{});
 }
 
+// Column-blocked product of one op (k_tiles_part) into part, and the launch
+// of the op that consumes the partials (k_tiles, PS = kColBlocks).
+template <class MT, class Op>
+void launch_part(const TileMap &tm, const Ctx &c, const Op &op, const MT *val, double *part, int n, hipStream_t st) {
+    constexpr int per = 8 / kColBlocks;  // XCDs per block
+    const int grid = 8 * ((tm.cb_units_max + per - 1) / per);
+    hipLaunchKernelGGL((k_tiles_part<MT, Op, kColBlocks>), dim3(grid), dim3(kThreads), 0, st, tm.cb_ptr.get(),
+                       tm.cb_idx.get(), val, tm.cb_unit_row.get(), tm.cb_unit_off.get(), n, part, c, op);
+}
+template <class Op>
+void launch_summed(const TileMap &tm, const Ctx &c, const Op &op, const double *part, int n, hipStream_t st,
+                   bool reverse) {
+    UnitView uv = unit_view(tm, false, c.counter + c.nsys + 1);
+    uv.reverse = reverse ? 1 : 0;
+    uv.psum = part;
+    uv.pstride = n;
+    hipLaunchKernelGGL((k_tiles<false, double, Op, kColBlocks>), dim3(tm.nunits), dim3(kThreads), 0, st,
+                       (const int *)nullptr, (const int *)nullptr, (const double *)nullptr, uv, c, op);
+}
+
 // f(RT{}) with RT = float when r* is stored in FP32, else double.
 template <class F>
 void with_rt(bool rs32, F &&f) {
@@ -1754,8 +1894,90 @@ static void fill_sell(int nslices, const int *slice_row, const int *slice_blk, c
     if (h_wide) off.release();
 }
 
+// Row tiles over the rows [r0, r1) of one operator given by its row lengths
+// (hp: prefix, n + 1): at most `target` entries and max_rows rows per tile,
+// a row longer than the staging buffer alone.
+static void cut_tiles(const std::vector<int> &hp, int r0, int r1, int target, std::vector<int> &urow,
+                      int max_rows = kRowsPerTile) {
+    int i = r0;
+    while (i < r1) {
+        const int start = i;
+        long long nz = 0;
+        if (hp[(size_t)i + 1] - hp[(size_t)i] > kLdsProducts) {
+            ++i;  // a row longer than the staging buffer gets a tile of its own
+        } else {
+            while (i < r1 && i - start < max_rows) {
+                const int len = hp[(size_t)i + 1] - hp[(size_t)i];
+                if (len > kLdsProducts || (i > start && nz + len > target)) break;
+                nz += len;
+                ++i;
+            }
+        }
+        urow.push_back(start);
+    }
+}
+
+static bool column_blocks_enabled() {
+    const char *env = std::getenv("SPAI_MI355X_COLUMN_BLOCKS");
+    return !(env && std::atoi(env) == 0);
+}
+
+// Column blocks of a one-system row-tile layout (see TileMap): entries
+// reordered block by block on the host, each block cut into tiles of its own.
+static void build_column_blocks(TileMap &tm, int n, const std::vector<int> &hp, const int *idx, long long total,
+                                int target, hipStream_t st) {
+    constexpr int C = kColBlocks;
+    std::vector<int> hidx((size_t)std::max<long long>(total, 1));
+    if (total > 0)
+        SPAI_HIP(hipMemcpyAsync(hidx.data(), idx, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost, st));
+    SPAI_HIP(hipStreamSynchronize(st));
+    const long long width = ((long long)n + C - 1) / C;  // columns per block
+    auto block_of = [&](int col) { return (int)std::min<long long>(col / width, C - 1); };
+    // counts per (block, row), block-major, then their prefix
+    std::vector<int> bp((size_t)C * n + 1, 0);
+    for (int i = 0; i < n; ++i)
+        for (int p = hp[(size_t)i]; p < hp[(size_t)i + 1]; ++p) ++bp[(size_t)block_of(hidx[(size_t)p]) * n + i + 1];
+    for (size_t q = 1; q < bp.size(); ++q) bp[q] += bp[q - 1];
+    std::vector<int> bidx((size_t)std::max<long long>(total, 1)), bpos((size_t)std::max<long long>(total, 1));
+    {
+        std::vector<int> cur(bp.begin(), bp.end() - 1);
+        for (int i = 0; i < n; ++i)
+            for (int p = hp[(size_t)i]; p < hp[(size_t)i + 1]; ++p) {
+                const int q = cur[(size_t)block_of(hidx[(size_t)p]) * n + i]++;
+                bidx[(size_t)q] = hidx[(size_t)p];
+                bpos[(size_t)p] = q;
+            }
+    }
+    // each block's tiles, on the block's own row lengths
+    std::vector<int> urow, uoff(1, 0);
+    int umax = 0;
+    for (int c = 0; c < C; ++c) {
+        std::vector<int> lens((size_t)n + 1);
+        for (int i = 0; i <= n; ++i) lens[(size_t)i] = bp[(size_t)c * n + i] - bp[(size_t)c * n];
+        const size_t before = urow.size();
+        cut_tiles(lens, 0, n, kLdsProducts, urow, kRowsPerPartTile);
+        umax = std::max(umax, (int)(urow.size() - before));
+        urow.push_back(n);
+        uoff.push_back((int)urow.size());
+    }
+    auto up = [&](DeviceArray<int> &d, const std::vector<int> &h) {
+        d.resize(std::max<size_t>(h.size(), 1));
+        SPAI_HIP(hipMemcpyAsync(d.get(), h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, st));
+    };
+    up(tm.cb_ptr, bp);
+    up(tm.cb_idx, bidx);
+    up(tm.cb_pos, bpos);
+    up(tm.cb_unit_row, urow);
+    up(tm.cb_unit_off, uoff);
+    SPAI_HIP(hipStreamSynchronize(st));  // the host vectors end with this function
+    tm.cblocks = C;
+    tm.cb_units_max = umax;
+    tm.cb_nnz = total;
+}
+
 void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector<int> &h_row_off,
-                  const std::vector<long long> &h_nnz_off, bool allow_sell, hipStream_t st, const int *ptr2) {
+                  const std::vector<long long> &h_nnz_off, bool allow_sell, hipStream_t st, const int *ptr2,
+                  bool allow_blocks) {
     const int nsys = (int)h_row_off.size() - 1;
     require(nsys >= 1, "build_layout: no systems");
     static std::atomic<unsigned long long> generations{0};
@@ -1831,27 +2053,9 @@ void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector
         tm.target_nnz = target;
         for (int s = 0; s < nsys; ++s) {
             const int r0 = h_row_off[(size_t)s], r1 = h_row_off[(size_t)s + 1];
-            int i = r0;
-            if (r0 == r1) {  // empty system: one empty tile keeps the reductions uniform
-                urow.push_back(r0);
-                usys.push_back(s);
-            }
-            while (i < r1) {
-                const int start = i;
-                long long nz = 0;
-                if (hp[(size_t)i + 1] - hp[(size_t)i] > kLdsProducts) {
-                    ++i;  // a row longer than the staging buffer gets a tile of its own
-                } else {
-                    while (i < r1 && i - start < kRowsPerTile) {
-                        const int len = hp[(size_t)i + 1] - hp[(size_t)i];
-                        if (len > kLdsProducts || (i > start && nz + len > target)) break;
-                        nz += len;
-                        ++i;
-                    }
-                }
-                urow.push_back(start);
-                usys.push_back(s);
-            }
+            if (r0 == r1) urow.push_back(r0);  // empty system: one empty tile keeps the reductions uniform
+            cut_tiles(hp, r0, r1, target, urow);
+            usys.resize(urow.size(), s);
             tm.h_sys_units[(size_t)s + 1] = (int)usys.size();
         }
         uslice.assign(usys.size() + 1, 0);
@@ -1919,6 +2123,17 @@ void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector
     } else {
         tm.slice_row.resize(1);
         tm.slice_blk.resize(1);
+    }
+    tm.cblocks = 0;
+    tm.cb_units_max = 0;
+    if (!sell && allow_blocks && !ptr2 && nsys == 1 && n >= kColBlockMinRows && column_blocks_enabled())
+        build_column_blocks(tm, n, hp, idx, total, tm.target_nnz, st);
+    else {
+        tm.cb_ptr.release();
+        tm.cb_idx.release();
+        tm.cb_pos.release();
+        tm.cb_unit_row.release();
+        tm.cb_unit_off.release();
     }
     SPAI_HIP(hipStreamSynchronize(st));
 }
@@ -2131,6 +2346,30 @@ void BiCGStab::ensure(int n, int nsys, const TileMap &tm) {
     }
 }
 
+void BiCGStab::prepare_blocks(const OperatorView &A, const OperatorView &M, const TileMap &tm, int n, int nsys,
+                              hipStream_t st) {
+    cb_on_ = tm.kind == kLayoutTiles && tm.cblocks == kColBlocks && nsys == 1 && A.ptr && A.idx && A.val &&
+             (!M.val || (M.ptr == A.ptr && M.idx == A.idx));
+    if (!cb_on_) return;
+    const long long nnz = tm.cb_nnz;
+    const unsigned g = (unsigned)((std::max<long long>(nnz, 1) + 255) / 256);
+    // grow-only: a captured graph keeps these addresses
+    cb_part_.reserve((size_t)kColBlocks * (size_t)n);
+    cb_a_.reserve((size_t)std::max<long long>(nnz, 1));
+    hipLaunchKernelGGL((k_scatter_values<double, double>), dim3(g), dim3(256), 0, st, nnz, tm.cb_pos.get(),
+                       (const double *)A.val, cb_a_.get());
+    if (M.val && M.fp32) {
+        cb_m32_.reserve((size_t)std::max<long long>(nnz, 1));
+        hipLaunchKernelGGL((k_scatter_values<float, float>), dim3(g), dim3(256), 0, st, nnz, tm.cb_pos.get(),
+                           (const float *)M.val, cb_m32_.get());
+    } else if (M.val) {
+        cb_m64_.reserve((size_t)std::max<long long>(nnz, 1));
+        hipLaunchKernelGGL((k_scatter_values<double, double>), dim3(g), dim3(256), 0, st, nnz, tm.cb_pos.get(),
+                           (const double *)M.val, cb_m64_.get());
+    }
+    SPAI_CHECK_LAUNCH();
+}
+
 // Rebuilds the active-unit list from the system states (stream-ordered).
 void BiCGStab::compact(const TileMap &tm, int nsys, hipStream_t st, int *pub) {
     if (nsys <= kCompactBlock) {
@@ -2158,9 +2397,7 @@ void BiCGStab::enqueue_k1(const OperatorView &A, const OperatorView &M, const Ti
         launch_k1<false, double, kYCopy>(A, tm, c, x_mode(parity), k1, list, grid, st, flip());
     } else if (split) {
         launch_k1<false, double, kYNone>(A, tm, c, x_mode(parity), k1, list, grid, st, flip());
-        OpApplyM om{pout, ybuf(parity)};
-        if (M.fp32) launch_units<true, float>(M, tm, c, om, list, grid, st, flip());
-        else launch_units<true, double>(M, tm, c, om, list, grid, st, flip());
+        enqueue_mapply_p(M, tm, parity, list, grid, st);
     } else if (M.fp32) {
         launch_k1<true, float, kYProduct>(M, tm, c, x_mode(parity), k1, list, grid, st, flip());
     } else {
@@ -2168,9 +2405,25 @@ void BiCGStab::enqueue_k1(const OperatorView &A, const OperatorView &M, const Ti
     }
 }
 
+// Second half of the split K1: y = M p', column-blocked when the solve is.
+void BiCGStab::enqueue_mapply_p(const OperatorView &M, const TileMap &tm, int parity, const int *list, int grid,
+                                hipStream_t st) {
+    Ctx c = make_ctx(tm, nsys_);
+    OpApplyM om{p_[parity ^ 1], ybuf(parity)};
+    if (cb_on_) {
+        if (M.fp32) launch_part<float>(tm, c, om, cb_m32_.get(), cb_part_.get(), n_, st);
+        else launch_part<double>(tm, c, om, cb_m64_.get(), cb_part_.get(), n_, st);
+        launch_summed(tm, c, om, cb_part_.get(), n_, st, flip());
+    } else if (M.fp32) {
+        launch_units<true, float>(M, tm, c, om, list, grid, st, flip());
+    } else {
+        launch_units<true, double>(M, tm, c, om, list, grid, st, flip());
+    }
+}
+
 // K3: z = M s with s = r - alpha a, fused (s formed while gathering) or, for
 // row tiles while K1 is split too, a streaming s pass and an M-apply of the
-// stored s.
+// stored s (column-blocked when the solve is).
 void BiCGStab::enqueue_k3(const OperatorView &M, const TileMap &tm, int parity, const int *list, int grid,
                           bool split, hipStream_t st) {
     Ctx c = make_ctx(tm, nsys_);
@@ -2178,8 +2431,15 @@ void BiCGStab::enqueue_k3(const OperatorView &M, const TileMap &tm, int parity, 
         OpUpdateS us{r_, a_, s_, {}};
         launch_units<false, double>(M, tm, c, us, list, grid, st, flip());
         OpApplyM om{s_, zbuf(parity)};
-        if (M.fp32) launch_units<true, float>(M, tm, c, om, list, grid, st, flip());
-        else launch_units<true, double>(M, tm, c, om, list, grid, st, flip());
+        if (cb_on_) {
+            if (M.fp32) launch_part<float>(tm, c, om, cb_m32_.get(), cb_part_.get(), n_, st);
+            else launch_part<double>(tm, c, om, cb_m64_.get(), cb_part_.get(), n_, st);
+            launch_summed(tm, c, om, cb_part_.get(), n_, st, flip());
+        } else if (M.fp32) {
+            launch_units<true, float>(M, tm, c, om, list, grid, st, flip());
+        } else {
+            launch_units<true, double>(M, tm, c, om, list, grid, st, flip());
+        }
     } else if (M.fp32) {
         OpMapplyS<float> op{r_, a_, s_, zbuf(parity), {}};
         launch_units<true, float>(M, tm, c, op, list, grid, st, flip());
@@ -2196,7 +2456,12 @@ void BiCGStab::enqueue_k2(const OperatorView &A, const TileMap &tm, int parity, 
         using RT = decltype(rt);
         Ctx c = make_ctx(tm, nsys_);
         OpSpmvAlpha<RT> op{ybuf(parity), reinterpret_cast<const RT *>(rs_), a_, {}};
-        launch_units<true, double>(A, tm, c, op, list, grid, st, flip());
+        if (cb_on_) {
+            launch_part<double>(tm, c, op, cb_a_.get(), cb_part_.get(), n_, st);
+            launch_summed(tm, c, op, cb_part_.get(), n_, st, flip());
+        } else {
+            launch_units<true, double>(A, tm, c, op, list, grid, st, flip());
+        }
     });
 }
 
@@ -2207,7 +2472,12 @@ void BiCGStab::enqueue_k4(const OperatorView &A, const TileMap &tm, int parity, 
         using RT = decltype(rt);
         Ctx c = make_ctx(tm, nsys_);
         OpSpmvOmega<RT> op{zbuf(parity), s_, reinterpret_cast<const RT *>(rs_), t_};
-        launch_units<true, double>(A, tm, c, op, list, grid, st, flip());
+        if (cb_on_) {
+            launch_part<double>(tm, c, op, cb_a_.get(), cb_part_.get(), n_, st);
+            launch_summed(tm, c, op, cb_part_.get(), n_, st, flip());
+        } else {
+            launch_units<true, double>(A, tm, c, op, list, grid, st, flip());
+        }
     });
 }
 
@@ -2260,6 +2530,8 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     x_ = x;
     defer_x_ = nsys > 1 && !bound;
     rs32_ = M.val && M.fp32;
+    if (bound) cb_on_ = false;
+    else prepare_blocks(A, M, tm, n, nsys, st);
     if (opt.record_history) {
         hist_.reserve((size_t)nsys * (size_t)opt.max_iter);
         hist_ptr_ = hist_.get();
@@ -2526,6 +2798,7 @@ float BiCGStab::time_kernel(int which, const OperatorView &A, const OperatorView
     defer_x_ = nsys > 1;  // as the solve runs it
     rs32_ = M.val && M.fp32;
     ensure(n, nsys, tm);
+    prepare_blocks(A, M, tm, n, nsys, st);
     hist_ptr_ = nullptr;
     hist_stride_ = 0;
     Ctx c = make_ctx(tm, nsys);
@@ -2597,6 +2870,7 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
     defer_x_ = nsys > 1;  // as the solve runs it
     rs32_ = M.val && M.fp32;
     ensure(n, nsys, tm);
+    prepare_blocks(A, M, tm, n, nsys, st);
     hist_ptr_ = nullptr;
     hist_stride_ = 0;
     Ctx c = make_ctx(tm, nsys);
@@ -2619,9 +2893,7 @@ void BiCGStab::time_iteration(const OperatorView &A, const OperatorView &M, cons
             if (split) {
                 launch_k1<false, double, kYNone>(A, tm, c, x_mode(parity), k1, nullptr, 0, st, flip());
                 SPAI_HIP(hipEventRecord(e[1], st));
-                OpApplyM om{pout, ybuf(parity)};
-                if (M.fp32) launch_units<true, float>(M, tm, c, om, nullptr, 0, st, flip());
-                else launch_units<true, double>(M, tm, c, om, nullptr, 0, st, flip());
+                enqueue_mapply_p(M, tm, parity, nullptr, 0, st);
             } else {
                 if (M.fp32) {
                     launch_k1<true, float, kYProduct>(M, tm, c, x_mode(parity), k1, nullptr, 0, st, flip());

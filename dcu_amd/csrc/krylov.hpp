@@ -119,6 +119,19 @@ struct TileMap {
     DeviceArray<int> sys_base;
     // entries of an operator's value array in this layout
     long long value_len(long long nnz) const { return kind == kLayoutSell ? sell_len : nnz; }
+    // Column blocks (row tiles of one system whose vectors exceed an XCD's
+    // L2): the products of the split iteration are formed per block of
+    // n / cblocks columns, on the XCDs the block is assigned to, as partial
+    // row sums that the consuming kernel adds.  The entries are stored block
+    // by block (row-major inside a block, CSR order inside a row).
+    int cblocks = 0;                  // 0: none
+    int cb_units_max = 0;             // most tiles of any block
+    long long cb_nnz = 0;             // entries
+    DeviceArray<int> cb_ptr;          // cblocks * n + 1: entries of (block c, row i) start at cb_ptr[c * n + i]
+    DeviceArray<int> cb_idx;          // nnz column indices, block by block
+    DeviceArray<int> cb_pos;          // nnz: CSR position -> block position
+    DeviceArray<int> cb_unit_row;     // per block its tiles' row boundaries (tiles + 1 entries, the last n)
+    DeviceArray<int> cb_unit_off;     // cblocks + 1: block c's boundaries start at cb_unit_row[cb_unit_off[c]]
 };
 
 // Device CSC -> CSR of an m x n matrix (m rows: rowptr holds m + 1 entries;
@@ -135,9 +148,11 @@ void csc_to_csr_device(int m, int n, int nnz, const int *colptr, const int *rowi
 // pattern); otherwise, or when its padding is too large, row tiles.  ptr2
 // (optional) is a second operator's row pointers over the same rows (M with
 // its own pattern): tiles are cut so that neither operator overflows one.
+// allow_blocks: a row-tile layout of one system with at least
+// kColBlockMinRows rows gets column blocks (unless SPAI_MI355X_COLUMN_BLOCKS=0).
 void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector<int> &h_row_off,
                   const std::vector<long long> &h_nnz_off, bool allow_sell, hipStream_t st,
-                  const int *ptr2 = nullptr);
+                  const int *ptr2 = nullptr, bool allow_blocks = false);
 
 // Values of a CSR-ordered array in the layout's order: out[sell_pos[p]] =
 // in[p] (SELL, padding 0) or a copy (tiles).  (MT_OUT, MT_IN): (double,
@@ -269,6 +284,18 @@ private:
                     hipStream_t st);
     void enqueue_init(const OperatorView &A, const TileMap &tiles, const double *x, const double *b, double rel_tol,
                       double abs_tol, int restart, hipStream_t st);
+    // y = M p' of the split K1
+    void enqueue_mapply_p(const OperatorView &M, const TileMap &tiles, int parity, const int *list, int grid,
+                          hipStream_t st);
+    // Column-blocked products for this solve (a one-system row-tile layout
+    // with column blocks, M on A's pattern, no bound workspace): the
+    // operators' values in the blocks' order, refreshed here, before any
+    // launch of the solve.
+    void prepare_blocks(const OperatorView &A, const OperatorView &M, const TileMap &tiles, int n, int nsys,
+                        hipStream_t st);
+    bool cb_on_ = false;
+    DeviceArray<double> cb_a_, cb_m64_, cb_part_;
+    DeviceArray<float> cb_m32_;
     int n_ = 0, nsys_ = 0, nunits_ = -1, nnodes_ = -1, ngroups_ = -1;
     DeviceArray<double> vec_;       // x-independent work vectors
     DeviceArray<double> partials_;  // tree partials, then the iteration partials of the groups and the units

@@ -292,11 +292,16 @@ int spai_system_time_iteration(spai_system *sys, int reps, int variant, float *m
  * SPAI_LAYOUT_SELL64_DIAG (one column - row offset per column block of a
  * slice, shared by its 64 rows, when a slice's distinct offsets add at most
  * 10% to the row-length padding: stencil matrices) or SPAI_LAYOUT_ROW_TILES;
- * `entries` = stored entries per operator (nnz plus padding, or nnz). */
+ * SPAI_LAYOUT_ROW_TILES_BLOCKED is row tiles whose products split by column
+ * blocks (one system of at least 2^19 rows: each block's slice of the
+ * vector stays in the L2 of the XCDs it runs on; SPAI_MI355X_COLUMN_BLOCKS=0
+ * at creation turns them off).  `entries` = stored entries per operator (nnz
+ * plus padding, or nnz). */
 #define SPAI_LAYOUT_ROW_TILES 0
 #define SPAI_LAYOUT_SELL64 1
 #define SPAI_LAYOUT_SELL64_OFF16 2
 #define SPAI_LAYOUT_SELL64_DIAG 3
+#define SPAI_LAYOUT_ROW_TILES_BLOCKED 4
 int spai_system_layout(const spai_system *sys, int *kind, long long *entries);
 
 /* Average device time (ms) of a plain copy kernel that reads and writes

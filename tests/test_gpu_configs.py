@@ -130,7 +130,7 @@ def test_power_law_config(ctx):
     # iterations
     m_gpu = S.m_values()
     Mg = O.csc2csr(A.n, A.colptr, A.rowidx, m_gpu)[:3]
-    assert S.layout()[0] == "row_tiles"
+    assert S.layout()[0] == "row_tiles_blocked"
     xg, _, og = S.solve(b, rel_tol=0.0, abs_tol=1e-300, max_iter=11, record_history=True)
     xo, ito, _, ho = O.bicgstab(Acsr, Mg, b, np.zeros(A.n), 0.0, 11, history=True, shadow32=True)
     hg = S.residual_history(0)

@@ -335,3 +335,55 @@ def test_batch_of_odd_sizes(ctx, use_graph):
         off_x += A.n
         off_m += A.nnz
     S.close()
+
+
+@pytest.mark.parametrize("precision", [D.PRECISION_FP64, D.PRECISION_MIXED])
+def test_column_blocks_match_plain_tiles(ctx, monkeypatch, precision):
+    # One power-law system of more than 2^19 rows: its row-tile layout forms
+    # the split iteration's products by column blocks (partial row sums per
+    # block, added in block order by the consuming kernel).  The same system
+    # created with SPAI_MI355X_COLUMN_BLOCKS=0 runs the plain row tiles.  M
+    # is the same; the first 10 iterates and residuals agree with the plain
+    # tiles' to 1e-12 (the row sums are added in another order) and with the
+    # oracle's (run with the GPU's M) to 1e-8; a longer solve (which stagnates
+    # on this matrix, as the oracle does) ends alike, at a comparable
+    # residual (rounding differences grow over many unconverged iterations).
+    P = D.Problem.power_law(600000, 11)
+    A = P.system(0)
+    b = P.rhs(0)
+    S1 = D.System(ctx, A)
+    monkeypatch.setenv("SPAI_MI355X_COLUMN_BLOCKS", "0")
+    S2 = D.System(ctx, A)
+    monkeypatch.delenv("SPAI_MI355X_COLUMN_BLOCKS")
+    assert S1.layout()[0] == "row_tiles_blocked" and S2.layout()[0] == "row_tiles"
+    for S in (S1, S2):
+        S.set_tile_shapes([0, 0, 0])
+        S.build(precision, D.SCHEDULE_BALANCED_NNZ)
+    m = S1.m_values()
+    assert np.array_equal(m, S2.m_values())
+    out = []
+    for S in (S1, S2):
+        x, _, oc = S.solve(b, rel_tol=0.0, abs_tol=1e-300, max_iter=11, record_history=True, use_graph=True)
+        out.append((x, S.residual_history(0), oc[0]))
+    (x1, h1, o1), (x2, h2, o2) = out
+    assert o1["iterations"] == o2["iterations"] == 11
+    d = np.linalg.norm(x1 - x2) / np.linalg.norm(x2)
+    dh = np.max(np.abs(h1 - h2) / h2)
+    Acsr = O.csc2csr(A.n, A.colptr, A.rowidx, A.val)[:3]
+    Mcsr = O.csc2csr(A.n, A.colptr, A.rowidx, m)[:3]
+    xo, _, _, ho = O.bicgstab(Acsr, Mcsr, b, np.zeros(A.n), 0.0, 11, history=True,
+                              shadow32=precision == D.PRECISION_MIXED)
+    eo = np.linalg.norm(x1 - xo) / np.linalg.norm(xo)
+    ho_err = np.max(np.abs(h1 - ho) / ho)
+    print(f"prec {precision}: blocked vs plain 10 iterates {d:.2e}, history {dh:.2e}; vs oracle {eo:.2e}, "
+          f"history {ho_err:.2e}", flush=True)
+    assert d <= 1e-12 and dh <= 1e-12
+    assert eo <= 1e-8 and ho_err <= 1e-8
+    r1 = S1.solve(b, rel_tol=1e-8, max_iter=120, use_graph=True)[2][0]
+    r2 = S2.solve(b, rel_tol=1e-8, max_iter=120, use_graph=True)[2][0]
+    print(f"120 iterations: blocked {r1['iterations']} res {r1['residual']:.6e}, plain {r2['iterations']} "
+          f"res {r2['residual']:.6e}", flush=True)
+    assert r1["status"] == r2["status"] and r1["iterations"] == r2["iterations"]
+    assert 0.5 < r1["residual"] / r2["residual"] < 2.0
+    S1.close()
+    S2.close()
