@@ -69,11 +69,13 @@ def kernel_bytes(kind, n, nnz, m_bytes, batched=True):
     """Algorithmic HBM bytes of one launch of an iteration kernel with every
     system active, in the CSR terms of the reference's operators: per
     nonzero a 4 B column index and the value (A: 8 B, M: m_bytes), 4 B per
-    row pointer, and each FP64 vector read or written once.  The solver's
+    row pointer, and each FP64 vector read or written once (the shadow
+    residual r*, stored in FP32 next to an FP32 M, at m_bytes).  The solver's
     SELL-64 layouts store less index data (16-bit offsets, or one offset per
     column block of a slice) and some padding; the measured traffic is the
     `traffic` figure."""
     rp = 4 * (n + 1)
+    rs = m_bytes * n  # r*
     # x: a batch's K1 updates x every other iteration from two iterations' y
     # and z (one launch reads x, y, z, y', z' and writes x, the next touches
     # none of them: 3 vectors per launch on average); a single system's K1
@@ -86,11 +88,11 @@ def kernel_bytes(kind, n, nnz, m_bytes, batched=True):
     if kind == "mapply_p":        # split K1, y = M p'; reads M, p' (gather, once); writes y
         return (m_bytes + 4) * nnz + rp + 8 * n * 2
     if kind == "spmv_alpha":      # a = A y; reads A, y (gather, once), r*; writes a
-        return 12 * nnz + rp + 8 * n * 3
+        return 12 * nnz + rp + 8 * n * 2 + rs
     if kind == "mapply_s":        # z = M s; reads M, r, a; writes s, z
         return (m_bytes + 4) * nnz + rp + 8 * n * 4
     if kind == "spmv_omega":      # t = A z; reads A, z, s, r*; writes t
-        return 12 * nnz + rp + 8 * n * 4
+        return 12 * nnz + rp + 8 * n * 3 + rs
     raise ValueError(kind)
 
 
