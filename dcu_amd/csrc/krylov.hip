@@ -761,93 +761,178 @@ __global__ __launch_bounds__(kThreads) void k_tiles(const int *__restrict__ ptr,
 }
 
 // Column-blocked products (row tiles of one large system, TileMap::cblocks =
-// C): workgroup b runs on XCD b % 8, which serves column block (b % 8) % C,
-// and the block's tiles are dealt to its 8 / C XCDs in order.  A tile forms,
-// for each of its rows, the sum of the products over the block's columns
-// (the gathered values lie in the block's 1/C of the vector, which stays in
-// the XCD's L2) and stores it as part[c * n + i]; every row of every block is
-// written (0.0 for a row with no entry in the block).  The row sums follow
-// k_tiles: storage order per row, long rows by a wavefront in a fixed tree.
-// Only ops whose gather reads one stored vector use it (no state, no sums).
+// C): workgroup b runs on XCD b % 8, which serves column block (b % 8) % C.
+// A tile forms, for each of its rows, the sum of the products over the
+// block's columns (the gathered values lie in the block's 1/C of the vector,
+// which stays in the XCD's L2) and stores it as part[c * n + i]; every row of
+// every block is written (0.0 for a row with no entry in the block).  The row
+// sums follow k_tiles: storage order per row, long rows by a wavefront in a
+// fixed tree.  Only ops whose gather reads one stored vector use it (no
+// state, no sums).
+//
+// A workgroup's first load is its tile's descriptor (rows and entries,
+// host-built), then the row pointers, column indices and values together,
+// then the gathers: three dependent round trips instead of five (block
+// offsets, tile bounds, row pointers, entries, gathers; power-law products
+// 6% faster).  With kPartPipe the launch is persistent: the block's
+// workgroups take its tiles in a fixed stride and request the next tile's
+// operands behind the current tile's gathers.
+struct PartTiles {
+    const int4 *tile;     // per tile: first row, end row, first entry, end entry
+    int off[kColBlocks + 1];  // block c's tiles are [off[c], off[c + 1])
+};
+#ifndef SPAI_PART_WG_PER_CU
+#define SPAI_PART_WG_PER_CU 4
+#endif
+constexpr int kPartWgPerCu = SPAI_PART_WG_PER_CU;  // persistent workgroups per CU (32 CUs per XCD)
+// Products staged per tile of a column block.  A block's rows hold few
+// entries each (a quarter of a row's), so its tiles are cut by rows
+// (kRowsPerPartTile) long before they fill the buffer; 1024 keeps fewer idle
+// slots in registers and measured the same (power-law: 278 against 275 ms
+// for 1000 iterations).
+#ifndef SPAI_PART_PRODUCTS
+#define SPAI_PART_PRODUCTS 2048
+#endif
+constexpr int kPartProducts = SPAI_PART_PRODUCTS;
+static_assert(kPartProducts % kThreads == 0 && kPartProducts <= kLdsProducts, "part tile products");
+// 1: persistent workgroups that request the next tile's operands behind the
+// current tile's gathers; 0: one workgroup per tile.  On the power-law
+// matrix (1000 iterations) one workgroup per tile took 275 ms, the
+// persistent form 278-294 ms (4, 6 or 8 workgroups per CU): the products
+// are bound by their gathers, not by the dependent round trips.
+#ifndef SPAI_PART_PIPE
+#define SPAI_PART_PIPE 0
+#endif
+constexpr bool kPartPipe = SPAI_PART_PIPE != 0;
+constexpr int kPartRp = (kRowsPerPartTile + 1 + kThreads - 1) / kThreads;  // row pointers per thread
+
+template <class MT>
+struct PartOperands {
+    static constexpr int U = kPartProducts / kThreads;
+    int j[U];
+    MT v[U];
+    int rp[kPartRp];
+};
+
+// Requests a tile's operands (nothing for a tile beyond the staging buffer).
+template <class MT>
+__device__ __forceinline__ void part_load(const int4 d, const int *__restrict__ bp, const int *__restrict__ idx,
+                                          const MT *__restrict__ val, int tid, PartOperands<MT> &o) {
+    constexpr int U = PartOperands<MT>::U;
+    const int nrows = d.y - d.x, cnt = d.w - d.z;
+    const bool fits = cnt <= kPartProducts && nrows <= kRowsPerPartTile;
+#pragma unroll
+    for (int k = 0; k < kPartRp; ++k) {
+        const int kk = k * kThreads + tid;
+        o.rp[k] = fits && kk <= nrows ? bp[d.x + kk] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int q = u * kThreads + tid;
+        const bool in = fits && q < cnt;
+        o.j[u] = in ? stream_load(&idx[d.z + q]) : 0;
+        o.v[u] = in ? stream_load(&val[d.z + q]) : (MT)0;
+    }
+}
+
 template <class MT, class Op, int C>
 __global__ __launch_bounds__(kThreads) void k_tiles_part(const int *__restrict__ ptr, const int *__restrict__ idx,
-                                                         const MT *__restrict__ val, const int *__restrict__ unit_row,
-                                                         const int *__restrict__ unit_off, int n,
+                                                         const MT *__restrict__ val, PartTiles pt, int n,
                                                          double *__restrict__ part, Ctx c, Op op) {
-    __shared__ double prod[kLdsProducts];  // a long row's sum replaces its first product
+    constexpr int U = PartOperands<MT>::U;
+    __shared__ double prod[kPartProducts];  // a long row's sum replaces its first product
     __shared__ int rp[kRowsPerPartTile + 1];
     __shared__ double sh[4];
     const int b = (int)blockIdx.x, x = b & 7, blk = x % C;
-    const int q = (b >> 3) * (8 / C) + x / C;  // the tile among its block's
-    const int u0 = unit_off[blk], nu = unit_off[blk + 1] - u0 - 1;
-    if (q >= nu) return;
+    constexpr int per = 8 / C;                           // XCDs per block
+    const int w = (b >> 3) * per + x / C;                // the workgroup among its block's
+    const int nw = ((int)gridDim.x >> 3) * per;          // workgroups per block (the grid is a multiple of 8)
+    int t0 = 0, t1 = 0;
+#pragma unroll
+    for (int k = 0; k < C; ++k)
+        if (k == blk) {
+            t0 = pt.off[k];
+            t1 = pt.off[k + 1];
+        }
+    int t = t0 + w;
+    if (t >= t1) return;
     if (!op.enabled(c.st[0])) return;
     const int tid = threadIdx.x;
-    const int r0 = unit_row[u0 + q], r1 = unit_row[u0 + q + 1];
-    const int nrows = r1 - r0;
     const int *bp = ptr + (size_t)blk * n;  // bp[i]: first entry of row i in the block; bp[n] ends the block
     double *out = part + (size_t)blk * n;
-    const int p0 = bp[r0];
-    const int cnt = bp[r1] - p0;
-    if (cnt <= kLdsProducts && nrows <= kRowsPerPartTile) {
-        for (int k = tid; k <= nrows; k += kThreads) rp[k] = bp[r0 + k] - p0;
-        constexpr int U = kLdsProducts / kThreads;
-        int jj[U];
-        MT vv[U];
+    int4 d = pt.tile[t];
+    PartOperands<MT> cur;
+    part_load(d, bp, idx, val, tid, cur);
+    for (;;) {
+        const int tn = t + nw;
+        const bool more = kPartPipe && tn < t1;
+        int4 dn = d;
+        if (more) dn = pt.tile[tn];
+        const int r0 = d.x, nrows = d.y - d.x, p0 = d.z, cnt = d.w - d.z;
+        PartOperands<MT> nxt;
+        if (cnt <= kPartProducts && nrows <= kRowsPerPartTile) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int qq = u * kThreads + tid;
-            jj[u] = qq < cnt ? stream_load(&idx[p0 + qq]) : 0;
-            vv[u] = qq < cnt ? stream_load(&val[p0 + qq]) : (MT)0;
-        }
-        double gv[U];
+            for (int k = 0; k < kPartRp; ++k) {
+                const int kk = k * kThreads + tid;
+                if (kk <= nrows) rp[kk] = cur.rp[k] - p0;
+            }
+            double gv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int qq = u * kThreads + tid;
-            gv[u] = qq < cnt ? op.gather(jj[u]) : 0.0;
-        }
+            for (int u = 0; u < U; ++u) {
+                const int q = u * kThreads + tid;
+                gv[u] = q < cnt ? op.gather(cur.j[u]) : 0.0;
+            }
+            // the next tile's operands, behind this tile's gathers
+            if (more) part_load(dn, bp, idx, val, tid, nxt);
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int qq = u * kThreads + tid;
-            if (qq < cnt) prod[qq] = (double)vv[u] * gv[u];
-        }
-        __syncthreads();
-        {
-            const int lane = tid & 63;
-            for (int base = tid - lane; base < nrows; base += kThreads) {
-                const int li = base + lane;
-                const bool lng = li < nrows && rp[li + 1] - rp[li] > kTileShortRow;
-                unsigned long long m = __ballot(lng);
-                while (m) {
-                    const int j = __ffsll((long long)m) - 1;
-                    m &= m - 1;
-                    const int bb = rp[base + j], e = rp[base + j + 1];
-                    double s = 0.0;
-                    for (int qq = bb + lane; qq < e; qq += 64) s += prod[qq];
-                    s = group_allsum<64>(s);  // every lane has read the row's products
-                    if (lane == 0) prod[bb] = s;
+            for (int u = 0; u < U; ++u) {
+                const int q = u * kThreads + tid;
+                if (q < cnt) prod[q] = (double)cur.v[u] * gv[u];
+            }
+            __syncthreads();
+            {
+                const int lane = tid & 63;
+                for (int base = tid - lane; base < nrows; base += kThreads) {
+                    const int li = base + lane;
+                    const bool lng = li < nrows && rp[li + 1] - rp[li] > kTileShortRow;
+                    unsigned long long m = __ballot(lng);
+                    while (m) {
+                        const int j = __ffsll((long long)m) - 1;
+                        m &= m - 1;
+                        const int bb = rp[base + j], e = rp[base + j + 1];
+                        double s = 0.0;
+                        for (int qq = bb + lane; qq < e; qq += 64) s += prod[qq];
+                        s = group_allsum<64>(s);  // every lane has read the row's products
+                        if (lane == 0) prod[bb] = s;
+                    }
                 }
             }
-        }
-        __syncthreads();
-        for (int li = tid; li < nrows; li += kThreads) {
-            const int bb = rp[li], e = rp[li + 1];
-            double s = 0.0;
-            if (e - bb > kTileShortRow) {
-                s = prod[bb];
-            } else {
-                for (int qq = bb; qq < e; ++qq) s += prod[qq];
+            __syncthreads();
+            for (int li = tid; li < nrows; li += kThreads) {
+                const int bb = rp[li], e = rp[li + 1];
+                double s = 0.0;
+                if (e - bb > kTileShortRow) {
+                    s = prod[bb];
+                } else {
+                    for (int qq = bb; qq < e; ++qq) s += prod[qq];
+                }
+                __builtin_nontemporal_store(s, &out[r0 + li]);
             }
-            __builtin_nontemporal_store(s, &out[r0 + li]);
+        } else {
+            // a row longer than the staging buffer (a tile of its own)
+            if (more) part_load(dn, bp, idx, val, tid, nxt);
+            for (int i = d.x; i < d.y; ++i) {
+                double s[1] = {0.0};
+                for (int qq = bp[i] + tid; qq < bp[i + 1]; qq += kThreads) s[0] += (double)val[qq] * op.gather(idx[qq]);
+                block_reduce<1>(s, sh);
+                if (tid == 0) out[i] = s[0];
+            }
         }
-    } else {
-        // a row longer than the staging buffer (a tile of its own)
-        for (int i = r0; i < r1; ++i) {
-            double s[1] = {0.0};
-            for (int qq = bp[i] + tid; qq < bp[i + 1]; qq += kThreads) s[0] += (double)val[qq] * op.gather(idx[qq]);
-            block_reduce<1>(s, sh);
-            if (tid == 0) out[i] = s[0];
-        }
+        if (!more) break;
+        __syncthreads();  // the LDS is refilled for the next tile
+        t = tn;
+        d = dn;
+        cur = nxt;
     }
 }
 
@@ -1773,9 +1858,14 @@ void launch_k1(const OperatorView &O, const TileMap &tm, const Ctx &c, int xmode
 template <class MT, class Op>
 void launch_part(const TileMap &tm, const Ctx &c, const Op &op, const MT *val, double *part, int n, hipStream_t st) {
     constexpr int per = 8 / kColBlocks;  // XCDs per block
-    const int grid = 8 * ((tm.cb_units_max + per - 1) / per);
-    hipLaunchKernelGGL((k_tiles_part<MT, Op, kColBlocks>), dim3(grid), dim3(kThreads), 0, st, tm.cb_ptr.get(),
-                       tm.cb_idx.get(), val, tm.cb_unit_row.get(), tm.cb_unit_off.get(), n, part, c, op);
+    // persistent: kPartWgPerCu workgroups on each of an XCD's 32 CUs, fewer
+    // when the largest block has fewer tiles
+    const int per_xcd = std::min(kPartPipe ? 32 * kPartWgPerCu : INT32_MAX, (tm.cb_units_max + per - 1) / per);
+    PartTiles pt;
+    pt.tile = reinterpret_cast<const int4 *>(tm.cb_tile.get());
+    for (int k = 0; k <= kColBlocks; ++k) pt.off[k] = tm.cb_toff[k];
+    hipLaunchKernelGGL((k_tiles_part<MT, Op, kColBlocks>), dim3(8 * per_xcd), dim3(kThreads), 0, st,
+                       tm.cb_ptr.get(), tm.cb_idx.get(), val, pt, n, part, c, op);
 }
 template <class Op>
 void launch_summed(const TileMap &tm, const Ctx &c, const Op &op, const double *part, int n, hipStream_t st,
@@ -1898,17 +1988,17 @@ static void fill_sell(int nslices, const int *slice_row, const int *slice_blk, c
 // (hp: prefix, n + 1): at most `target` entries and max_rows rows per tile,
 // a row longer than the staging buffer alone.
 static void cut_tiles(const std::vector<int> &hp, int r0, int r1, int target, std::vector<int> &urow,
-                      int max_rows = kRowsPerTile) {
+                      int max_rows = kRowsPerTile, int cap = kLdsProducts) {
     int i = r0;
     while (i < r1) {
         const int start = i;
         long long nz = 0;
-        if (hp[(size_t)i + 1] - hp[(size_t)i] > kLdsProducts) {
+        if (hp[(size_t)i + 1] - hp[(size_t)i] > cap) {
             ++i;  // a row longer than the staging buffer gets a tile of its own
         } else {
             while (i < r1 && i - start < max_rows) {
                 const int len = hp[(size_t)i + 1] - hp[(size_t)i];
-                if (len > kLdsProducts || (i > start && nz + len > target)) break;
+                if (len > cap || (i > start && nz + len > target)) break;
                 nz += len;
                 ++i;
             }
@@ -1948,17 +2038,23 @@ static void build_column_blocks(TileMap &tm, int n, const std::vector<int> &hp, 
                 bpos[(size_t)p] = q;
             }
     }
-    // each block's tiles, on the block's own row lengths
-    std::vector<int> urow, uoff(1, 0);
+    // each block's tiles, on the block's own row lengths, as descriptors
+    // (first row, end row, first entry, end entry in the block numbering)
+    std::vector<int> tiles;
     int umax = 0;
+    tm.cb_toff[0] = 0;
     for (int c = 0; c < C; ++c) {
-        std::vector<int> lens((size_t)n + 1);
+        std::vector<int> lens((size_t)n + 1), urow;
         for (int i = 0; i <= n; ++i) lens[(size_t)i] = bp[(size_t)c * n + i] - bp[(size_t)c * n];
-        const size_t before = urow.size();
-        cut_tiles(lens, 0, n, kLdsProducts, urow, kRowsPerPartTile);
-        umax = std::max(umax, (int)(urow.size() - before));
+        cut_tiles(lens, 0, n, kPartProducts, urow, kRowsPerPartTile, kPartProducts);
         urow.push_back(n);
-        uoff.push_back((int)urow.size());
+        const int nt = (int)urow.size() - 1;
+        umax = std::max(umax, nt);
+        for (int k = 0; k < nt; ++k) {
+            const int r0 = urow[(size_t)k], r1 = urow[(size_t)k + 1];
+            tiles.insert(tiles.end(), {r0, r1, bp[(size_t)c * n + r0], bp[(size_t)c * n + r1]});
+        }
+        tm.cb_toff[c + 1] = tm.cb_toff[c] + nt;
     }
     auto up = [&](DeviceArray<int> &d, const std::vector<int> &h) {
         d.resize(std::max<size_t>(h.size(), 1));
@@ -1967,8 +2063,7 @@ static void build_column_blocks(TileMap &tm, int n, const std::vector<int> &hp, 
     up(tm.cb_ptr, bp);
     up(tm.cb_idx, bidx);
     up(tm.cb_pos, bpos);
-    up(tm.cb_unit_row, urow);
-    up(tm.cb_unit_off, uoff);
+    up(tm.cb_tile, tiles);
     SPAI_HIP(hipStreamSynchronize(st));  // the host vectors end with this function
     tm.cblocks = C;
     tm.cb_units_max = umax;
@@ -2132,8 +2227,7 @@ void build_layout(TileMap &tm, const int *ptr, const int *idx, const std::vector
         tm.cb_ptr.release();
         tm.cb_idx.release();
         tm.cb_pos.release();
-        tm.cb_unit_row.release();
-        tm.cb_unit_off.release();
+        tm.cb_tile.release();
     }
     SPAI_HIP(hipStreamSynchronize(st));
 }

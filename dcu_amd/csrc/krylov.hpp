@@ -130,8 +130,8 @@ struct TileMap {
     DeviceArray<int> cb_ptr;          // cblocks * n + 1: entries of (block c, row i) start at cb_ptr[c * n + i]
     DeviceArray<int> cb_idx;          // nnz column indices, block by block
     DeviceArray<int> cb_pos;          // nnz: CSR position -> block position
-    DeviceArray<int> cb_unit_row;     // per block its tiles' row boundaries (tiles + 1 entries, the last n)
-    DeviceArray<int> cb_unit_off;     // cblocks + 1: block c's boundaries start at cb_unit_row[cb_unit_off[c]]
+    DeviceArray<int> cb_tile;         // 4 per tile: first row, end row, first entry, end entry (block numbering)
+    int cb_toff[9] = {};              // cblocks + 1: block c's tiles are [cb_toff[c], cb_toff[c + 1])
 };
 
 // Device CSC -> CSR of an m x n matrix (m rows: rowptr holds m + 1 entries;
