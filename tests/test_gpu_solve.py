@@ -387,3 +387,62 @@ def test_column_blocks_match_plain_tiles(ctx, monkeypatch, precision):
     assert 0.5 < r1["residual"] / r2["residual"] < 2.0
     S1.close()
     S2.close()
+
+
+def _long_row_system(n, long_rows, long_len, seed):
+    # n x n, three random off-diagonal entries per row, and rows `long_rows`
+    # with `long_len` entries each; a_ii = 1.2 * sum_j |a_ij| (row dominant)
+    rng = np.random.default_rng(seed)
+    rows = [np.repeat(np.arange(n), 3)]
+    cols = [rng.integers(0, n, 3 * n)]
+    for r in long_rows:
+        rows.append(np.full(long_len, r))
+        cols.append(rng.choice(n, long_len, replace=False))
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    keep = r != c
+    r, c = r[keep], c[keep]
+    v = rng.uniform(-1.0, 1.0, r.size)
+    A = sp.csr_matrix((v, (r, c)), shape=(n, n))
+    A.sum_duplicates()
+    d = 1.2 * np.asarray(abs(A).sum(axis=1)).ravel() + 1.0
+    A = (A + sp.diags(d)).tocsc()
+    A.sort_indices()
+    return D.CSC(n, A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64))
+
+
+def test_column_blocks_rows_beyond_the_staging_buffer(ctx, monkeypatch):
+    # Rows with more entries than a tile stages (2048): with column blocks a
+    # row of 12000 entries holds about 3000 in each quarter of the columns,
+    # so its partial products take the one-row tile path of k_tiles_part;
+    # the plain row tiles reduce it with the whole workgroup.  Both agree
+    # with each other and with the oracle (run with the GPU's M) over 10
+    # iterations.
+    A = _long_row_system(600000, [12345, 400000], 12000, 5)
+    b = np.random.default_rng(6).uniform(0.0, 1.0, A.n)
+    S1 = D.System(ctx, A)
+    monkeypatch.setenv("SPAI_MI355X_COLUMN_BLOCKS", "0")
+    S2 = D.System(ctx, A)
+    monkeypatch.delenv("SPAI_MI355X_COLUMN_BLOCKS")
+    assert S1.layout()[0] == "row_tiles_blocked" and S2.layout()[0] == "row_tiles"
+    for S in (S1, S2):
+        S.build(D.PRECISION_FP64, D.SCHEDULE_BALANCED_NNZ)
+    m = S1.m_values()
+    assert np.array_equal(m, S2.m_values())
+    out = []
+    for S in (S1, S2):
+        x, _, oc = S.solve(b, rel_tol=0.0, abs_tol=1e-300, max_iter=11, record_history=True, use_graph=True)
+        out.append((x, S.residual_history(0), oc[0]))
+    (x1, h1, o1), (x2, h2, o2) = out
+    assert o1["iterations"] == o2["iterations"] == 11
+    Acsr = O.csc2csr(A.n, A.colptr, A.rowidx, A.val)[:3]
+    Mcsr = O.csc2csr(A.n, A.colptr, A.rowidx, m)[:3]
+    xo, _, _, ho = O.bicgstab(Acsr, Mcsr, b, np.zeros(A.n), 0.0, 11, history=True)
+    d = np.linalg.norm(x1 - x2) / np.linalg.norm(x2)
+    eo = max(np.linalg.norm(x - xo) / np.linalg.norm(xo) for x in (x1, x2))
+    ho_err = max(np.max(np.abs(h - ho) / ho) for h in (h1, h2))
+    print(f"long rows: blocked vs plain 10 iterates {d:.2e}; vs oracle {eo:.2e}, history {ho_err:.2e}", flush=True)
+    assert d <= 1e-12
+    assert eo <= 1e-8 and ho_err <= 1e-8
+    S1.close()
+    S2.close()
