@@ -179,3 +179,59 @@ def test_tier_overflow(ctx, schedule, precision, tol):
     # the overflowed columns are counted where they were solved
     assert rep["tier_columns"][1] == int((n1_ref > 16).sum())
     assert rep["tier_columns"][0] == int((n1_ref <= 16).sum())
+
+
+def _stencil_with_breaks(dims, frac, seed):
+    # 5-point (2D) or 7-point (3D) stencil matrix with random values, then a
+    # fraction `frac` of the columns perturbed: an extra entry at a random
+    # row, or an off-diagonal entry removed.  The interior columns gather
+    # their rows at the same offsets from k, the perturbed ones and their
+    # neighbours do not.
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(dims))
+    idx = np.arange(n).reshape(dims[::-1])
+    rows, cols = [], []
+    for ax in range(len(dims)):
+        a = np.swapaxes(idx, ax, -1)
+        rows += [a[..., :-1].ravel(), a[..., 1:].ravel()]
+        cols += [a[..., 1:].ravel(), a[..., :-1].ravel()]
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    keep = np.ones(r.size, bool)
+    drop = rng.choice(r.size, int(frac * n / 2), replace=False)
+    keep[drop] = False
+    r, c = r[keep], c[keep]
+    extra_c = rng.choice(n, int(frac * n / 2), replace=False)
+    extra_r = rng.integers(0, n, extra_c.size)
+    ok = extra_r != extra_c
+    r = np.concatenate([r, extra_r[ok]])
+    c = np.concatenate([c, extra_c[ok]])
+    v = rng.uniform(-1.0, -0.1, r.size)
+    A = sp.csc_matrix((v, (r, c)), shape=(n, n))
+    A.sum_duplicates()
+    d = 1.5 * np.asarray(abs(A).sum(axis=0)).ravel() + 1.0
+    A = (A + sp.diags(d)).tocsc()
+    A.sort_indices()
+    return D.CSC(n, A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64))
+
+
+@pytest.mark.parametrize("dims", [(180, 150), (34, 30, 28)])
+def test_broken_translation_invariance(ctx, dims):
+    # The register tiers reuse a column's sorted (row, entry) order for the
+    # next column of the same group when every gathered row sits at the same
+    # offset from its column; a wavefront skips the sort only when all its
+    # groups match.  On stencils whose structure is broken at random columns
+    # (1% of them), M must still equal the oracle's column for column, with
+    # the default and the stencil-width tile shapes, in both precisions.
+    A = _stencil_with_breaks(dims, 0.01, 7)
+    m_ref = O.spai(A.n, A.colptr, A.rowidx, A.val, 0)
+    S = D.System(ctx, A)
+    for shapes in ([0, 0, 0], [4, 4, 0]):
+        S.set_tile_shapes(shapes)
+        for prec, tol in ((D.PRECISION_FP64, 1e-10), (D.PRECISION_MIXED, 4 * 2.0 ** -23)):
+            rep = S.build(prec, D.SCHEDULE_BALANCED_NNZ)
+            err = column_rel_err(A, S.m_values(), m_ref)
+            print(f"dims={dims} shapes={shapes} prec={prec} tiers={rep['tier_columns']} err={err:.3e}", flush=True)
+            assert err <= tol, (shapes, prec, err)
+    S.close()
