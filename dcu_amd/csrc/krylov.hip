@@ -2316,6 +2316,7 @@ BiCGStab::~BiCGStab() {
     for (auto &e : check_ev_)
         if (e) (void)hipEventDestroy(e);
     if (h_pub_) (void)hipHostFree(h_pub_);
+    if (h_states_pin_) (void)hipHostFree(h_states_pin_);
 }
 
 void BiCGStab::drop_graph() {
@@ -2840,8 +2841,18 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
         SPAI_HIP(hipStreamSynchronize(st));
     }
     if (!bound) {
-        h_states_.resize((size_t)nsys);
-        SPAI_HIP(hipMemcpy(h_states_.data(), st_, sizeof(SysState) * nsys, hipMemcpyDeviceToHost));
+        // through pinned staging: a copy into pageable memory costs the
+        // caller tens of microseconds per solve
+        if (h_states_cap_ < nsys) {
+            if (h_states_pin_) SPAI_HIP(hipHostFree(h_states_pin_));
+            h_states_pin_ = nullptr;
+            h_states_cap_ = 0;
+            SPAI_HIP(hipHostMalloc((void **)&h_states_pin_, sizeof(SysState) * nsys, hipHostMallocDefault));
+            h_states_cap_ = nsys;
+        }
+        SPAI_HIP(hipMemcpyAsync(h_states_pin_, st_, sizeof(SysState) * nsys, hipMemcpyDeviceToHost, st));
+        SPAI_HIP(hipStreamSynchronize(st));
+        h_states_.assign(h_states_pin_, h_states_pin_ + nsys);
         states = h_states_.data();
     }
     h_hist_.clear();

@@ -329,6 +329,8 @@ private:
     const void *graph_key_[kGraphKey] = {};
     hipEvent_t check_ev_[2] = {nullptr, nullptr};  // host checks of the graph loop
     int *h_pub_ = nullptr, *d_pub_ = nullptr;      // counts published by the graphs (host-coherent)
+    SysState *h_states_pin_ = nullptr;               // pinned staging of the states read back after a solve
+    int h_states_cap_ = 0;
     void drop_graph();
     // Consecutive unit launches walk the units in alternating directions, so
     // that a kernel starts on the rows the one before it touched last (still

@@ -1812,6 +1812,8 @@ SpaiBuilder::SpaiBuilder() = default;
 SpaiBuilder::~SpaiBuilder() {
     for (auto &e : ev_)
         if (e) (void)hipEventDestroy(e);
+    if (h_counters_) (void)hipHostFree(h_counters_);
+    if (h_stats_) (void)hipHostFree(h_stats_);
 }
 
 void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats *stats) {
@@ -1848,10 +1850,14 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
     SPAI_HIP(hipEventRecord(ev_[1], st));
     int hc[kCounters] = {0};
     unsigned long long h64[3] = {0, 0, 0};
+    if (!h_counters_) SPAI_HIP(hipHostMalloc((void **)&h_counters_, sizeof(int) * kCounters, hipHostMallocDefault));
+    if (!h_stats_) SPAI_HIP(hipHostMalloc((void **)&h_stats_, sizeof(h64), hipHostMallocDefault));
     auto fetch = [&]() {
-        SPAI_HIP(hipMemcpyAsync(hc, counters_.get(), sizeof(int) * nc, hipMemcpyDeviceToHost, st));
-        SPAI_HIP(hipMemcpyAsync(h64, stats64_.get(), sizeof(h64), hipMemcpyDeviceToHost, st));
+        SPAI_HIP(hipMemcpyAsync(h_counters_, counters_.get(), sizeof(int) * nc, hipMemcpyDeviceToHost, st));
+        SPAI_HIP(hipMemcpyAsync(h_stats_, stats64_.get(), sizeof(h64), hipMemcpyDeviceToHost, st));
         SPAI_HIP(hipStreamSynchronize(st));
+        std::memcpy(hc, h_counters_, sizeof(int) * nc);
+        std::memcpy(h64, h_stats_, sizeof(h64));
     };
     ColumnArgs a{};
     a.colptr = p.colptr;

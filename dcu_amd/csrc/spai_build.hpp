@@ -79,6 +79,8 @@ class SpaiBuilder {
 public:
     SpaiBuilder();
     ~SpaiBuilder();
+    SpaiBuilder(const SpaiBuilder &) = delete;  // owns pinned host buffers
+    SpaiBuilder &operator=(const SpaiBuilder &) = delete;
     // Enqueues the build on `st` and waits for it (the planner's counters are
     // read back at the end); throws spai::Error on failure.
     void build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats *stats);
@@ -91,6 +93,10 @@ private:
     DeviceArray<uint64_t> glob_keys_;
     DeviceArray<int> glob_ints_;
     int cus_ = 0;
+    // pinned host copies of the planner's counters and statistics (read back
+    // mid-build; a copy into pageable memory stalls the stream far longer)
+    int *h_counters_ = nullptr;
+    unsigned long long *h_stats_ = nullptr;
     // phase boundaries: planner start/end, tiers start, after the register
     // tiers, after the LDS tier, global tier start/end
     hipEvent_t ev_[7] = {};
