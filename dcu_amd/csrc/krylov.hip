@@ -2323,6 +2323,7 @@ void BiCGStab::drop_graph() {
     for (auto &g : graphs_) (void)hipGraphExecDestroy(g.exec);
     graphs_.clear();
     for (auto &k : graph_key_) k = nullptr;
+    pred_launches_ = 0;
 }
 
 void BiCGStab::forget_layout(unsigned long long generation) {
@@ -2673,6 +2674,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
     // every system stops by then: the K1 of iteration k finishes iteration k - 1
     const int max_launch_iters = opt.max_iter + 1;
     int done_iters = 0;
+    int passes = 0;  // iteration loops run (a restart runs another)
     try {
         // `active` holds nsys, or the count read after the last restart
         // check; the unit count starts at its bound (the list is sized by
@@ -2680,6 +2682,10 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
         auto iterate = [&]() {
             if (use_list) compact(tm, nsys, st);
             units = tm.nunits;
+            struct PassCount {
+                int &p;
+                ~PassCount() { ++p; }
+            } pass_count{passes};
             if (opt.use_graph) {
                 const int gi = std::max(2, opt.graph_iters + (opt.graph_iters & 1));
                 const void *key[kGraphKey] = {A.ptr, A.val, A.idx, A.off, A.diag, M.ptr, M.val, M.idx, M.off,
@@ -2711,19 +2717,19 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                     }
                     return g;
                 };
-                auto graph_for = [&](int grid, bool split) {
+                auto graph_for = [&](int grid, bool split, int giv) {
                     for (auto &g : graphs_)
-                        if (g.iters == gi && g.grid == grid && g.compact == (int)use_list && g.split == (int)split)
+                        if (g.iters == giv && g.grid == grid && g.compact == (int)use_list && g.split == (int)split)
                             return g.exec;
                     hipStream_t cap;
                     hipGraph_t graph = nullptr;
                     hipGraphExec_t exec = nullptr;
                     SPAI_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
                     SPAI_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-                    for (int k = 0; k < gi; ++k) {
+                    for (int k = 0; k < giv; ++k) {
                         enqueue_iteration(A, M, tm, k & 1, grid, split, cap);
                         // long graphs also rebuild the list every kCompactEvery iterations
-                        if (use_list && (k + 1) % kCompactEvery == 0 && k + 1 < gi) compact(tm, nsys, cap);
+                        if (use_list && (k + 1) % kCompactEvery == 0 && k + 1 < giv) compact(tm, nsys, cap);
                     }
                     if (use_list) compact(tm, nsys, cap, d_pub_);
                     else
@@ -2734,7 +2740,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                     const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
                     (void)hipGraphDestroy(graph);
                     SPAI_HIP(e);
-                    graphs_.push_back(GraphEntry{gi, grid, (int)use_list, (int)split, exec});
+                    graphs_.push_back(GraphEntry{giv, grid, (int)use_list, (int)split, exec});
                     return exec;
                 };
                 int slot = 0, pending = 0;
@@ -2743,12 +2749,37 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                     active = __atomic_load_n(&h_pub_[0], __ATOMIC_ACQUIRE);
                     units = use_list ? __atomic_load_n(&h_pub_[1], __ATOMIC_ACQUIRE) : tm.nunits;
                 };
+                // The first loop of a solve whose operands and stop rule match
+                // the previous graph-replayed solve's ends with shorter graphs
+                // (even lengths keep the iteration parity) sized to the
+                // iterations that solve launched, instead of running most of
+                // a last full graph after its systems stopped.  Only launch
+                // counts change; a wrong guess costs launches, not results.
+                // Once the planned iterations are queued, the host waits for
+                // them before queueing more (instead of one graph ahead); if
+                // systems are still active the plan is dropped.
+                int planned = (passes == 0 && pred_launches_ > 0 && pred_max_iter_ == opt.max_iter &&
+                               pred_rel_ == opt.rel_tol && pred_abs_ == opt.abs_tol)
+                                  ? pred_launches_
+                                  : 0;
                 try {
                     while (active > 0 && done_iters < max_launch_iters) {
+                        if (planned > 0 && done_iters >= planned && pending > 0) {
+                            SPAI_HIP(hipEventSynchronize(check_ev_[slot ^ 1]));  // the latest graph's
+                            pending = 0;
+                            read_published();
+                            planned = 0;
+                            continue;
+                        }
                         const int grid = use_list ? rung(units) : 0;
                         const bool split = M.val && units >= split_units(tm);
-                        SPAI_HIP(hipGraphLaunch(graph_for(grid, split), st));
-                        done_iters += gi;
+                        int giv = gi;
+                        if (planned > done_iters && planned - done_iters < gi) {
+                            giv = 2;
+                            while (giv < planned - done_iters) giv <<= 1;
+                        }
+                        SPAI_HIP(hipGraphLaunch(graph_for(grid, split, giv), st));
+                        done_iters += giv;
                         SPAI_HIP(hipEventRecord(check_ev_[slot], st));
                         slot ^= 1;
                         if (++pending == 2) {
@@ -2854,6 +2885,19 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
         SPAI_HIP(hipStreamSynchronize(st));
         h_states_.assign(h_states_pin_, h_states_pin_ + nsys);
         states = h_states_.data();
+        if (opt.use_graph) {
+            // the launches this solve's single iteration loop needed: the K2
+            // of iteration k + 1 stops a system after k iterations
+            pred_launches_ = 0;
+            if (passes == 1) {
+                int nmax = 0;
+                for (int q = 0; q < nsys; ++q) nmax = std::max(nmax, states[q].iters);
+                pred_launches_ = std::min(nmax + 1, max_launch_iters);
+                pred_max_iter_ = opt.max_iter;
+                pred_rel_ = opt.rel_tol;
+                pred_abs_ = opt.abs_tol;
+            }
+        }
     }
     h_hist_.clear();
     if (hist_ptr_) {

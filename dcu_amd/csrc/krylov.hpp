@@ -331,6 +331,11 @@ private:
     int *h_pub_ = nullptr, *d_pub_ = nullptr;      // counts published by the graphs (host-coherent)
     SysState *h_states_pin_ = nullptr;               // pinned staging of the states read back after a solve
     int h_states_cap_ = 0;
+    // iterations the last graph-replayed solve on the current graph operands
+    // launched before its systems stopped, with its stop rule (0: unknown);
+    // the next such solve shortens its final graphs to match
+    int pred_launches_ = 0, pred_max_iter_ = 0;
+    double pred_rel_ = 0.0, pred_abs_ = 0.0;
     void drop_graph();
     // Consecutive unit launches walk the units in alternating directions, so
     // that a kernel starts on the rows the one before it touched last (still

@@ -60,6 +60,42 @@ def test_graph_and_loop_agree_bitwise(ctx):
     assert np.array_equal(x1, x2) and np.array_equal(x2, x3)
 
 
+@pytest.mark.parametrize("nsys", [1, 3])
+def test_planned_final_graphs(ctx, nsys):
+    # A graph-replayed solve sizes its last graphs from the iterations the
+    # previous solve on the same operands and stop rule launched.  The plan
+    # changes launch counts only: right-hand sides that need fewer or more
+    # iterations than the previous one (a wrong plan) give the same x and
+    # counts, bit for bit, as a System that has solved nothing before.
+    mats = [D.Problem.convection_diffusion(48 + 8 * s, 5.0 + 3.0 * s).system(0) for s in range(nsys)]
+    n = sum(A.n for A in mats)
+    rng = np.random.default_rng(3)
+    rhs = [np.ones(n), rng.standard_normal(n), rng.uniform(0.0, 1.0, n), np.ones(n)]
+    # references first: a context's systems share one solver, whose plan is
+    # dropped whenever another system's operands come in, so each new System
+    # solves unplanned
+    refs = []
+    for b in rhs:
+        F = D.System(ctx, mats if nsys > 1 else mats[0])
+        F.build(D.PRECISION_FP64, D.SCHEDULE_BALANCED_NNZ)
+        xf, _, of = F.solve(b, rel_tol=1e-8, max_iter=1000, use_graph=True)
+        F.close()
+        refs.append((xf, of))
+    # then one System solves them in a row: each solve after the first is
+    # planned from the one before
+    S = D.System(ctx, mats if nsys > 1 else mats[0])
+    S.build(D.PRECISION_FP64, D.SCHEDULE_BALANCED_NNZ)
+    counts = []
+    for b, (xf, of) in zip(rhs, refs):
+        x, _, oc = S.solve(b, rel_tol=1e-8, max_iter=1000, use_graph=True)
+        counts.append([o["iterations"] for o in oc])
+        assert [o["iterations"] for o in oc] == [o["iterations"] for o in of]
+        assert [o["status"] for o in oc] == [o["status"] for o in of]
+        assert np.array_equal(x, xf)
+    print(f"nsys={nsys}: iterations per right-hand side {counts}", flush=True)
+    S.close()
+
+
 def test_unpreconditioned_matches_oracle(ctx):
     # Unpreconditioned BiCGSTAB on this conv-diff converges erratically: the
     # oracle alone needs 101..105 iterations depending on its OpenMP reduction
