@@ -2717,10 +2717,13 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                     }
                     return g;
                 };
-                auto graph_for = [&](int grid, bool split, int giv) {
+                auto find_graph = [&](int grid, bool split, int giv) -> hipGraphExec_t {
                     for (auto &g : graphs_)
                         if (g.iters == giv && g.grid == grid && g.compact == (int)use_list && g.split == (int)split)
                             return g.exec;
+                    return nullptr;
+                };
+                auto capture_graph = [&](int grid, bool split, int giv) {
                     hipStream_t cap;
                     hipGraph_t graph = nullptr;
                     hipGraphExec_t exec = nullptr;
@@ -2742,6 +2745,17 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                     SPAI_HIP(e);
                     graphs_.push_back(GraphEntry{giv, grid, (int)use_list, (int)split, exec});
                     return exec;
+                };
+                // A full-length graph is captured together with the shorter
+                // even lengths that a planned solve ends with (4, 2), so that
+                // a planned solve meets no capture of its own.
+                auto graph_for = [&](int grid, bool split, int giv) {
+                    if (hipGraphExec_t e = find_graph(grid, split, giv)) return e;
+                    hipGraphExec_t e = capture_graph(grid, split, giv);
+                    if (giv == gi)
+                        for (int g2 = 2; g2 < gi; g2 <<= 1)
+                            if (!find_graph(grid, split, g2)) capture_graph(grid, split, g2);
+                    return e;
                 };
                 int slot = 0, pending = 0;
                 auto read_published = [&]() {
@@ -2777,6 +2791,7 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
                         if (planned > done_iters && planned - done_iters < gi) {
                             giv = 2;
                             while (giv < planned - done_iters) giv <<= 1;
+                            giv = std::min(giv, gi);
                         }
                         SPAI_HIP(hipGraphLaunch(graph_for(grid, split, giv), st));
                         done_iters += giv;
