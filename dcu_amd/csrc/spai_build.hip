@@ -1852,10 +1852,11 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
     unsigned long long h64[3] = {0, 0, 0};
     if (!h_counters_) SPAI_HIP(hipHostMalloc((void **)&h_counters_, sizeof(int) * kCounters, hipHostMallocDefault));
     if (!h_stats_) SPAI_HIP(hipHostMalloc((void **)&h_stats_, sizeof(h64), hipHostMallocDefault));
-    auto fetch = [&]() {
+    auto copy_counters = [&]() {
         SPAI_HIP(hipMemcpyAsync(h_counters_, counters_.get(), sizeof(int) * nc, hipMemcpyDeviceToHost, st));
         SPAI_HIP(hipMemcpyAsync(h_stats_, stats64_.get(), sizeof(h64), hipMemcpyDeviceToHost, st));
-        SPAI_HIP(hipStreamSynchronize(st));
+    };
+    auto take_counters = [&]() {
         std::memcpy(hc, h_counters_, sizeof(int) * nc);
         std::memcpy(h64, h_stats_, sizeof(h64));
     };
@@ -1876,8 +1877,17 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
         // the reference reads n2max back before choosing its kernels
         // (src/spai_single_dcu.cpp:1245-1252); static mode does the same for
         // the largest column's tier
-        fetch();
+        copy_counters();
+        SPAI_HIP(hipStreamSynchronize(st));
+        take_counters();
         static_tier = classify_host(hc[2 * kTiers], hc[2 * kTiers + 1]);
+    } else if (ce > cb) {
+        // the global tier's columns and tile bounds are final once the
+        // planner has run (the register tiers hand columns on only among
+        // themselves): the copy is read after the tiers are queued, so the
+        // device does not wait for the host in between
+        copy_counters();
+        SPAI_HIP(hipEventRecord(ev_[7], st));
     }
     for (int t = 0; t < kTunedTiers; ++t)
         require(p.shape[t] >= 0 && p.shape[t] < tile_shape_count(t), "spai build: no such tile shape");
@@ -1893,8 +1903,10 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
         SPAI_HIP(hipEventRecord(ev_[3], st));
     }
     SPAI_HIP(hipEventRecord(ev_[4], st));
-    fetch();
-    if (p.tiers_ms) SPAI_HIP(hipEventElapsedTime(p.tiers_ms, ev_[2], ev_[4]));
+    if (dynamic && ce > cb) {
+        SPAI_HIP(hipEventSynchronize(ev_[7]));
+        take_counters();
+    }
     SPAI_HIP(hipEventRecord(ev_[5], st));
     const int global_cols = dynamic ? hc[kTiers - 1] : (static_tier == kTiers - 1 ? ce - cb : 0);
     if (global_cols > 0) {
@@ -1949,8 +1961,12 @@ void SpaiBuilder::build(const SpaiBuildParams &p, hipStream_t st, SpaiBuildStats
         else launch(double{});
         SPAI_CHECK_LAUNCH();
     }
+    // the final counters (the tiers' hand-on counts) for the report
+    copy_counters();
     SPAI_HIP(hipEventRecord(ev_[6], st));
     SPAI_HIP(hipEventSynchronize(ev_[6]));
+    take_counters();
+    if (p.tiers_ms) SPAI_HIP(hipEventElapsedTime(p.tiers_ms, ev_[2], ev_[4]));
     if (stats) {
         const int from[4] = {0, 2, 3, 5}, to[4] = {1, 3, 4, 6};
         for (int q = 0; q < 4; ++q) SPAI_HIP(hipEventElapsedTime(&stats->phase_ms[q], ev_[from[q]], ev_[to[q]]));

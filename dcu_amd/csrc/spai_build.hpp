@@ -98,8 +98,9 @@ private:
     int *h_counters_ = nullptr;
     unsigned long long *h_stats_ = nullptr;
     // phase boundaries: planner start/end, tiers start, after the register
-    // tiers, after the LDS tier, global tier start/end
-    hipEvent_t ev_[7] = {};
+    // tiers, after the LDS tier, global tier start/end; then the planner
+    // counters' copy to the host
+    hipEvent_t ev_[8] = {};
 };
 
 }  // namespace spai
