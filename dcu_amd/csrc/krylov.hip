@@ -2863,10 +2863,17 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
         }
         if (!bound) {
             SPAI_HIP(hipEventRecord(e1, st));
-            SPAI_HIP(hipEventSynchronize(e1));
-        } else {
-            SPAI_HIP(hipStreamSynchronize(st));
+            // the states' copy rides behind the end event (one wait for both)
+            if (h_states_cap_ < nsys) {
+                if (h_states_pin_) SPAI_HIP(hipHostFree(h_states_pin_));
+                h_states_pin_ = nullptr;
+                h_states_cap_ = 0;
+                SPAI_HIP(hipHostMalloc((void **)&h_states_pin_, sizeof(SysState) * nsys, hipHostMallocDefault));
+                h_states_cap_ = nsys;
+            }
+            SPAI_HIP(hipMemcpyAsync(h_states_pin_, st_, sizeof(SysState) * nsys, hipMemcpyDeviceToHost, st));
         }
+        SPAI_HIP(hipStreamSynchronize(st));
     } catch (...) {
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
@@ -2887,17 +2894,8 @@ void BiCGStab::solve(const OperatorView &A, const OperatorView &M, const TileMap
         SPAI_HIP(hipStreamSynchronize(st));
     }
     if (!bound) {
-        // through pinned staging: a copy into pageable memory costs the
-        // caller tens of microseconds per solve
-        if (h_states_cap_ < nsys) {
-            if (h_states_pin_) SPAI_HIP(hipHostFree(h_states_pin_));
-            h_states_pin_ = nullptr;
-            h_states_cap_ = 0;
-            SPAI_HIP(hipHostMalloc((void **)&h_states_pin_, sizeof(SysState) * nsys, hipHostMallocDefault));
-            h_states_cap_ = nsys;
-        }
-        SPAI_HIP(hipMemcpyAsync(h_states_pin_, st_, sizeof(SysState) * nsys, hipMemcpyDeviceToHost, st));
-        SPAI_HIP(hipStreamSynchronize(st));
+        // copied through pinned staging behind the end event (a copy into
+        // pageable memory costs the caller tens of microseconds per solve)
         h_states_.assign(h_states_pin_, h_states_pin_ + nsys);
         states = h_states_.data();
         if (opt.use_graph) {
